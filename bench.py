@@ -1,0 +1,258 @@
+"""Benchmark: Mrays/s of the software BVH ray query on one MI355X.
+
+A step is one pass of the hot path over one frame of synthetic input, all on
+the device: primary rays -> closest hit -> sun-shadow and cosine GI rays from
+every hit -> any-hit shadow visibility and closest-hit GI. `value` is the
+rays traced per second over the K timed steps (inputs resident in HBM).
+
+Workload (default): 1M-triangle seeded scene (5,000 lognormal-edge clusters in
+a 200 m cube plus ground), 1920x1080 primary + 1 sun shadow + 1 GI bounce per
+primary hit, traced through the 8-wide quantized layout; the plain FP32 layout
+is timed on the same rays for the quantized-vs-plain ratio.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c0]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # name: (scene kind, meshes, triangles, width, height, gi bounces, description)
+    "c0": (0, 0, 0, 960, 640, 0, "2,000 boxes + 8 UV spheres + ground (31,682 tris), 960x640 primary + sun shadow"),
+    "c1": (1, 5000, 1_000_000, 1920, 1080, 1,
+           "1M-tri seeded scene (5,000 lognormal clusters, 200 m cube, ground), 1920x1080 primary + shadow + 1 GI"),
+    "c2": (1, 5000, 4_000_000, 3840, 2160, 2,
+           "4M-tri seeded scene (5,000 lognormal clusters, 200 m cube, ground), 3840x2160 primary + shadow + 2 GI"),
+}
+SUN = np.array([0.3, 0.8, 0.2], np.float32) / np.float32(np.linalg.norm([0.3, 0.8, 0.2]))
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Frame:
+    """Device buffers of one frame and the launches of one step."""
+
+    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream):
+        self.rq, self.scene, self.w, self.h, self.n_gi, self.seed = rq, scene, w, h, n_gi, seed
+        n = w * h
+        self.n = n
+        self.inv = rq.camera(kind, seed, w, h)
+        self.prim = rq.DeviceBuffer(n * 32)
+        self.hits = rq.DeviceBuffer(n * 48)
+        self.shadow = rq.DeviceBuffer(n * 32)
+        self.gi = rq.DeviceBuffer(max(n_gi, 1) * n * 32)
+        self.gi_hits = rq.DeviceBuffer(max(n_gi, 1) * n * 48)
+        self.vis = rq.DeviceBuffer(((n + 31) // 32) * 4)
+        self.counts = rq.DeviceBuffer(4 * (n_gi + 1))
+        self.stream = stream
+        self.counts_host = None
+
+    def step(self, layout, events=None):
+        rq, s = self.rq, self.stream
+        rec = (lambda name: events.mark(name)) if events else (lambda name: None)
+        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=s)
+        rec("raygen")
+        self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=s)
+        rec("primary")
+        rq.gen_secondary(self.prim, self.hits, self.n, SUN, self.seed, self.n_gi, self.shadow,
+                         self.gi if self.n_gi else None, self.counts, stream=s)
+        rec("secondary_gen")
+        c = self.counts_host
+        self.scene.trace(layout, self.shadow, int(c[0]), rq.ANY, vis=self.vis, stream=s)
+        rec("shadow")
+        for b in range(self.n_gi):
+            ptr = self.gi.ptr.value + b * self.n * 32
+            hp = self.gi_hits.ptr.value + b * self.n * 48
+            self.scene.trace(layout, ptr, int(c[1 + b]), rq.CLOSEST, hits=hp, stream=s)
+            rec(f"gi{b}")
+
+    def prime(self, layout):
+        """One untimed pass that fixes the secondary ray counts (deterministic)."""
+        rq = self.rq
+        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=self.stream)
+        self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=self.stream)
+        rq.gen_secondary(self.prim, self.hits, self.n, SUN, self.seed, self.n_gi, self.shadow,
+                         self.gi if self.n_gi else None, self.counts, stream=self.stream)
+        self.stream.synchronize()
+        self.counts_host = self.counts.download(np.uint32)
+        return self.rays_per_step()
+
+    def rays_per_step(self):
+        return int(self.n + sum(int(x) for x in self.counts_host))
+
+
+class Marks:
+    """HIP events between launches on the benchmark stream."""
+
+    def __init__(self, rq, stream):
+        self.rq, self.stream = rq, stream
+        self.events = []
+
+    def start(self):
+        e = self.rq.Event()
+        e.record(self.stream)
+        self.events.append(("start", e))
+
+    def mark(self, name):
+        e = self.rq.Event()
+        e.record(self.stream)
+        self.events.append((name, e))
+
+    def durations(self):
+        out = {}
+        for (_, a), (name, b) in zip(self.events, self.events[1:]):
+            if name == "start":
+                continue
+            out.setdefault(name, []).append(a.elapsed_ms(b))
+        return out
+
+
+def time_layout(rq, frame, layout, steps, warmup):
+    for _ in range(warmup):
+        frame.step(layout)
+    frame.stream.synchronize()
+    marks = Marks(rq, frame.stream)
+    t0 = time.perf_counter()
+    marks.start()
+    for _ in range(steps):
+        frame.step(layout, marks)
+        marks.mark("start")
+    frame.stream.synchronize()
+    wall = time.perf_counter() - t0
+    per = marks.durations()
+    total_ms = sum(sum(v) for v in per.values())
+    return total_ms / steps, wall, {k: float(np.mean(v)) for k, v in per.items()}
+
+
+def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
+    """The oracle traversal (scalar C port of gi_traverse) on this host's cores, over a bounded sample."""
+    import oracle
+    threads = os.cpu_count() or 1
+    sample = []
+    for mode, rays in rays_sets:
+        sample.append((mode, rays[: max(1, min(len(rays), 20000))]))
+    # calibrate, then scale the sample to the budget
+    t0 = time.perf_counter()
+    n0 = 0
+    for mode, r in sample:
+        oracle.traverse(scene_words, r, mode, threads=threads, want_hits=False, want_vis=True)
+        n0 += len(r)
+    dt = time.perf_counter() - t0
+    scale = max(1.0, budget_s / max(dt, 1e-6))
+    total, secs = 0, 0.0
+    for mode, rays in rays_sets:
+        k = int(min(len(rays), max(1, int(20000 * scale))))
+        t0 = time.perf_counter()
+        oracle.traverse(scene_words, rays[:k], mode, threads=threads, want_hits=False, want_vis=True)
+        secs += time.perf_counter() - t0
+        total += k
+    return total / secs / 1e6, threads, total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--layout", default="cw8", choices=["ref", "plain", "cw8"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the plain layout")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ray query shards by ray "
+              "batch with the scene replicated per GPU; no multi-GPU figure is produced)", file=sys.stderr)
+        sys.exit(2)
+
+    from renderlab_amd import rq
+    kind, meshes, tris, w, h, n_gi, desc = CONFIGS[args.config]
+    t0 = time.perf_counter()
+    scene = rq.Scene.generate(kind, 1, meshes, tris)
+    st = scene.stats()
+    log(f"scene {args.config}: {st} built in {time.perf_counter() - t0:.1f}s")
+    layouts = [args.layout] + ([] if args.no_compare or args.layout == "plain" else ["plain"])
+    t0 = time.perf_counter()
+    scene.upload(tuple(set(layouts) | {"plain"}))
+    log(f"uploaded {layouts} in {time.perf_counter() - t0:.1f}s; bytes " +
+        ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
+    stream = rq.Stream()
+    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream)
+    rays = frame.prime(args.layout)
+    log(f"rays per step: {rays} (primary {frame.n}, secondary {list(frame.counts_host)})")
+
+    results = {}
+    for layout in layouts:
+        frame.prime(layout)
+        ms, wall, per = time_layout(rq, frame, layout, args.steps, args.warmup)
+        results[layout] = (ms, wall, per)
+        log(f"{layout}: {ms:.3f} ms/step, {rays / ms / 1e3:.1f} Mrays/s, per launch {per}")
+    ms, wall, per = results[args.layout]
+    value = rays / (ms * 1e-3) / 1e6
+
+    # Dominant trace launch and its algorithmic bytes (counted on the same rays).
+    trace_names = [k for k in per if k in ("primary", "shadow") or k.startswith("gi")]
+    dom = max(trace_names, key=lambda k: per[k])
+    counts = frame.counts_host
+    dom_rays = frame.n if dom == "primary" else int(counts[0]) if dom == "shadow" else int(counts[1 + int(dom[2:])])
+    dom_mode = rq.ANY if dom == "shadow" else rq.CLOSEST
+    dom_ptr = {"primary": frame.prim.ptr.value, "shadow": frame.shadow.ptr.value}.get(
+        dom, frame.gi.ptr.value + (int(dom[2:]) if dom.startswith("gi") else 0) * frame.n * 32)
+    nbytes = scene.count_bytes(args.layout, dom_ptr, dom_rays, dom_mode)
+    achieved = nbytes / (per[dom] * 1e-3)
+    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": None, "kernel": f"trace {dom}",
+                "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4)}
+
+    # FP64 verification of the GI hits (outside the timed region).
+    verify = None
+    if n_gi:
+        vr = rq.DeviceBuffer.__new__(rq.DeviceBuffer)
+        vr.ptr, vr.nbytes = rq._P(frame.gi.ptr.value), int(counts[1]) * 32
+        vh = rq.DeviceBuffer.__new__(rq.DeviceBuffer)
+        vh.ptr, vh.nbytes = rq._P(frame.gi_hits.ptr.value), int(counts[1]) * 48
+        verify = scene.verify_fp64(vr, vh, int(counts[1]))
+        vr.ptr = vh.ptr = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        prim = frame.prim.download(rq.RAY_DTYPE, frame.n)
+        sh = frame.shadow.download(rq.RAY_DTYPE, int(counts[0]))
+        sets = [(0, prim), (1, sh)]
+        if n_gi:
+            sets.append((0, frame.gi.download(rq.RAY_DTYPE, int(counts[1]))))
+        mrays, threads, sampled = cpu_baseline(scene.words, sets)
+        cpu = {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first rays of each set of the same frame, {sampled} rays, oracle gi_traverse on "
+                         f"{threads} threads"}
+
+    line = {
+        "metric": "Mrays/s (primary+shadow+GI) on 1 MI355X, quantized vs plain; p50/p99 job latency",
+        "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded scene and camera)",
+        "config": {"workload": desc, "layout": args.layout, "rays_per_step": rays, "triangles": st["triangles"],
+                   "instances": st["instances"], "parallelism": "rays across one GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    if "plain" in results and args.layout != "plain":
+        pm = results["plain"][0]
+        line["plain_f32"] = {"value": round(rays / (pm * 1e-3) / 1e6, 2), "ms_per_step": round(pm, 4),
+                             "quantized_speedup": round(pm / ms, 3)}
+    if verify:
+        line["fp64_verify"] = verify
+    line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+/*
+ * rq: MI355X software BVH ray query (renderlab non-RT path), C ABI.
+ *
+ * The reference has no FFI: its host packs one u32 storage buffer
+ * (src/gi_bvh.rs:151-274, src/instance_bvh.rs:162-220) and uploads it with
+ * wgpu's queue.write_buffer (src/gi_cache.rs:4351-4361, 4481-4485); its device
+ * side is the WGSL seam gi_closest / gi_visible / gi_visible_within
+ * (shaders/gi_rays_software.wgsl:280-291). Each entry point below replaces one
+ * of those pieces and cites it. Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every call returns rq_status; nothing aborts. rq_last_error() gives the
+ *    calling thread's last message.
+ *  - rq_scene owns all device memory it uploads; ray, hit and bit buffers are
+ *    caller-owned DEVICE pointers unless a parameter says host.
+ *  - Build functions run on the host and are re-entrant. One scene may be
+ *    traced from several streams at once (read-only device data).
+ *  - A ray with a non-finite origin or direction, a NaN or negative tmin, a
+ *    NaN tmax or tmax > 1e30 is a miss with flag RQ_HIT_INVALID.
+ */
+#ifndef RQ_H
+#define RQ_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rq_status {
+    RQ_OK = 0,
+    RQ_ERR_INVALID_INPUT = 1, /* reference: Error::InvalidInput (src/gi_bvh.rs:33-36, 236-241) */
+    RQ_ERR_OUT_OF_MEMORY = 2,
+    RQ_ERR_DEVICE = 3,        /* HIP error, or no GPU present */
+    RQ_ERR_UNSUPPORTED = 4,
+    RQ_ERR_BUDGET = 5,        /* serving: admission refused by the HBM budget */
+    RQ_ERR_NOT_READY = 6
+} rq_status;
+
+/* Device layouts of one scene. */
+typedef enum rq_layout {
+    RQ_LAYOUT_REF_BVH4Q8 = 0, /* the reference gi_geometry words, verbatim */
+    RQ_LAYOUT_PLAIN_F32 = 1,  /* BVH4, child boxes pre-decoded to f32, triangles pre-gathered */
+    RQ_LAYOUT_CW8_Q8 = 2      /* 8-wide nodes, 8-bit outward-quantized child boxes, f32 triangles */
+} rq_layout;
+
+/* gi_traverse modes (shaders/gi_rays_software.wgsl:214-216). */
+typedef enum rq_mode {
+    RQ_MODE_CLOSEST = 0,    /* gi_closest */
+    RQ_MODE_ANY = 1,        /* gi_visible: any hit, sky visibility */
+    RQ_MODE_ANY_OPAQUE = 2  /* gi_visible_within: opaque casters only */
+} rq_mode;
+
+/* One ray, 32 bytes: origin, tmin, direction (need not be unit), tmax. */
+typedef struct rq_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} rq_ray;
+
+#define RQ_HIT_HIT 1u
+#define RQ_HIT_INVALID 2u
+/* GiHit (shaders/gi_rays_software.wgsl:19) as a buffer record, 48 bytes.
+   t is tmax and inst 0xffffffff on a miss. tri holds the hit triangle's
+   position indices, or (primitive, 0, 0) for closest hits on meshes built
+   with keep_primitives (:147-148, 161-162). slot is the triangle's index in
+   its mesh's BVH order (the FP64 verification reads it). */
+typedef struct rq_hit {
+    float t;
+    uint32_t inst;
+    uint32_t tri[3];
+    float u, v;
+    float normal[3];
+    uint32_t flags;
+    uint32_t slot;
+} rq_hit;
+
+/* One mesh: a triangle list. keep_primitives asks for the primitive map of
+   Blas::with_primitives (src/gi_bvh.rs:82-87, 108-136). */
+typedef struct rq_mesh {
+    const float* positions; /* host, 3 floats per vertex */
+    uint32_t vertex_count;
+    const uint32_t* indices; /* host, 3 per triangle */
+    uint32_t index_count;
+    uint32_t keep_primitives;
+} rq_mesh;
+
+/* One placement (src/gi_cache.rs:4400-4431): object-to-world model matrix,
+   column-major as glam's Mat4::to_cols_array. flags bit 0: casts no shadow. */
+typedef struct rq_instance {
+    float model[16];
+    uint32_t mesh;
+    uint32_t flags;
+    uint32_t identity;
+    uint32_t placement;
+} rq_instance;
+
+typedef struct rq_scene rq_scene;
+typedef struct rq_bvh rq_bvh;
+
+const char* rq_last_error(void);
+/* Number of visible GPUs (0 on a host without one). */
+int rq_device_count(void);
+
+/* ---- BLAS build: crates/asset/src/near_field.rs:172-190 (build_bvh, build_bvh_sah) */
+rq_status rq_bvh_build(const float* positions, uint32_t vertex_count, const uint32_t* indices,
+                       uint32_t index_count, uint32_t sah, rq_bvh** out);
+/* near_field.rs:445-456 (refit_bvh): same topology, new positions. */
+rq_status rq_bvh_refit(rq_bvh* bvh, const float* positions, uint32_t vertex_count);
+/* Counts: [0] nodes, [1] leaves, [2] vertices, [3] indices, [4] root tag, [5] depth. */
+rq_status rq_bvh_info(const rq_bvh* bvh, uint32_t info[6]);
+/* Runtime records (src/gi_bvh.rs:191-216): 16 words a node, 2 a leaf, 3 a
+   position, then the indices. Each output may be NULL. */
+rq_status rq_bvh_records(const rq_bvh* bvh, uint32_t* nodes, uint32_t* leaves, float* positions,
+                         uint32_t* indices);
+void rq_bvh_destroy(rq_bvh* bvh);
+
+/* ---- Top level: src/instance_bvh.rs:346-412 (build, build_with) over
+   instance boxes (lo xyz, hi xyz per box). Writes up to reserve(count)
+   words and returns the count written. sah=0 takes median splits. */
+uint32_t rq_instance_bvh_reserve(uint32_t count);
+rq_status rq_instance_bvh_build(const float* boxes, uint32_t count, uint32_t sah, uint32_t* nodes,
+                                uint32_t* written);
+/* src/instance_bvh.rs:224-275 (refit, refit_wide) */
+rq_status rq_instance_bvh_refit(uint32_t* nodes, uint32_t node_words, const float* boxes,
+                                uint32_t count);
+/* src/instance_bvh.rs:162-220 (place) without the worker thread: appends the
+   top level over `words`' records after `records_words` words (header
+   `header`, `stride` words a record), sets header words 1-3, reusing or
+   refitting `previous` (may be NULL) as the reference does. `out` holds
+   records_words + reserve(slots) words. `force`: -1 automatic (>= 40
+   instances), 0 never, 1 always. */
+rq_status rq_instance_bvh_place(const uint32_t* words, uint32_t records_words, uint32_t header,
+                                uint32_t stride, uint32_t slots, const uint32_t* previous,
+                                uint32_t previous_words, int force, uint32_t* out);
+
+/* ---- Scene: the whole gi_geometry buffer.
+   rq_scene_build replaces set_indirect_geometry + prepare_gi_geometry
+   (src/gi_cache.rs:3188-3196, 4246-4487): SAH BLAS per mesh, layout,
+   mesh_words, instance_words, extra words 24-31 and place. */
+rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                         uint32_t instance_count, int top_level, rq_scene** out);
+/* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
+   would upload, src/gi_cache.rs:4361). */
+rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* The packed reference words (host memory owned by the scene). */
+rq_status rq_scene_gi_words(const rq_scene* scene, const uint32_t** words, size_t* word_count);
+/* [0] instances, [1] meshes, [2] triangles (over instances), [3] unique triangles,
+   [4] BLAS nodes, [5] top-level root tag, [6] max BLAS depth */
+rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]);
+/* Uploads the requested layouts (bit 1<<rq_layout) to the current device. */
+rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
+rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes);
+void rq_scene_destroy(rq_scene* scene);
+
+/* ---- Trace: gi_closest / gi_visible / gi_visible_within over a batch.
+   rays, hits and vis_bits are device pointers; hits and vis_bits may each be
+   NULL. vis_bits gets bit i set when ray i is unoccluded (gi_visible's
+   result); the caller zeroes nothing, the call writes every word it covers.
+   stream: a hipStream_t, or NULL for the default stream. */
+rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                   uint32_t mode, rq_hit* hits, uint32_t* vis_bits, void* stream);
+
+/* The same traversal with a tally of the global bytes it moves (rays,
+   records, nodes, triangles, stack spills, hit records): the algorithmic
+   bytes of one launch for the roofline. Synchronous; not for timing. */
+rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes);
+
+/* Launch parameters of the traversal kernels (the autotuner's knobs). */
+typedef struct rq_tuning {
+    uint32_t block;        /* threads per workgroup: 64, 128, 256 */
+    uint32_t stack;        /* LDS stack entries per lane: 8 .. 48 */
+    uint32_t persistent;   /* 0: one lane per ray; 1: persistent, per-lane refill */
+    uint32_t waves_per_eu; /* occupancy hint 0 (none), 1 .. 8 */
+    uint32_t grid_per_cu;  /* persistent workgroups per CU (0: auto) */
+    uint32_t reserved[3];
+} rq_tuning;
+rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out);
+rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* tuning);
+
+/* ---- FP64 verification: the only FP64 on the GPU. For each hit ray,
+   re-intersects its reported triangle in f64 (object-space ray computed in
+   f64 from the f32 world ray) and reduces max |t-t64|/|t64| and max
+   |u-u64|, |v-v64|. out (host): [0] max rel t error, [1] max abs bary error,
+   [2] hits checked. */
+rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,
+                         double out[3], void* stream);
+
+/* ---- Ray generation (device). Formulas: shaders/primary_rays.wgsl:50-66
+   (primary, reversed Z), shaders/reference_path_tracer.wgsl:149-167,
+   237-256, 466-470 (offset, sun shadow, cosine bounce, pcg). */
+/* inverse: column-major inverse view-projection (host). */
+rq_status rq_gen_primary(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays,
+                         void* stream);
+/* Secondary rays of `count` primary rays and their closest hits: compacted
+   in pixel order into shadow (one per lit hit) and gi (n_gi per hit, one
+   array per bounce, each compacted); counts (device, n_gi+1 words) receive
+   the lengths. src_pixel (may be NULL) receives each ray's pixel, array by
+   array. */
+rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t count,
+                           const float sun[3], uint32_t seed, uint32_t n_gi, rq_ray* shadow,
+                           rq_ray* gi, uint32_t* counts, void* stream);
+
+/* ---- Device helpers (HIP runtime of this library; no torch types). */
+rq_status rq_malloc(size_t bytes, void** ptr);
+rq_status rq_free(void* ptr);
+rq_status rq_memcpy_h2d(void* dst, const void* src, size_t bytes);
+rq_status rq_memcpy_d2h(void* dst, const void* src, size_t bytes);
+rq_status rq_memset(void* dst, int value, size_t bytes);
+rq_status rq_synchronize(void);
+rq_status rq_stream_create(void** stream);
+rq_status rq_stream_destroy(void* stream);
+rq_status rq_stream_synchronize(void* stream);
+rq_status rq_event_create(void** event);
+rq_status rq_event_destroy(void* event);
+rq_status rq_event_record(void* event, void* stream);
+rq_status rq_event_synchronize(void* event);
+rq_status rq_event_elapsed_ms(void* start, void* stop, float* ms);
+rq_status rq_event_query(void* event); /* RQ_OK when complete, RQ_ERR_NOT_READY otherwise */
+
+/* ---- Seeded synthetic scenes (the benchmark configurations).
+   kind 0: boxes + UV spheres + ground quad (2,000 boxes, 8 spheres);
+   kind 1: lognormal-edge triangle clusters in a 200 m cube plus ground,
+           `meshes` meshes and about `triangles` triangles in all. */
+rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                       rq_scene** out);
+/* Camera on a seeded orbit around the scene: column-major inverse of the
+   reversed-Z view-projection (60 degree vertical field of view). */
+rq_status rq_gen_camera(uint32_t kind, uint64_t seed, uint32_t width, uint32_t height,
+                        float inverse[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,790 @@
+/*
+ * CPU oracle for the software BVH ray query -- TEST INFRASTRUCTURE, not product
+ * code. Only the tests, the smoke check and the benchmark's CPU-baseline leg
+ * load it, and only as the checker. Reference citations are per function; the
+ * arithmetic convention is stated in oracle.h.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <pthread.h>
+#include <string.h>
+#include <unistd.h>
+
+#define LEAF 0x80000000u
+#define EMPTY 0xffffffffu
+#define STACK 48u
+#define IW 32u
+#define FMAXV 3.4028235e38f
+
+/* ---------------------------------------------------------------- scalars */
+static inline float bf(uint32_t w) { float f; memcpy(&f, &w, 4); return f; }
+static inline uint32_t fb(float f) { uint32_t w; memcpy(&w, &f, 4); return w; }
+/* IEEE 754-2019 minimum / maximum: NaN propagates, -0 orders below +0. */
+static inline float mn(float a, float b) {
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+static inline float mx(float a, float b) {
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+
+typedef struct v3 { float x, y, z; } v3;
+static inline v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 sub3(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul3(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross3(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+/* --------------------------------------------------------- counted loads */
+typedef struct ctx {
+    const uint32_t* g;
+    size_t n;
+    orc_counts c;
+    int bad;
+} ctx;
+static inline uint32_t W(ctx* x, uint64_t i) {
+    x->c.words++;
+    if (i >= x->n) { x->bad = 1; return 0; }
+    return x->g[i];
+}
+static inline float F(ctx* x, uint64_t i) { return bf(W(x, i)); }
+static inline v3 GV(ctx* x, uint64_t i) { return mk(F(x, i), F(x, i + 1), F(x, i + 2)); }
+
+/* gi_rays_software.wgsl:27-30 */
+static inline float safe_inv1(float d) {
+    float tiny = d >= 0.0f ? 1e-30f : -1e-30f;
+    return 1.0f / (fabsf(d) < 1e-30f ? tiny : d);
+}
+static inline v3 safe_inverse(v3 d) { return mk(safe_inv1(d.x), safe_inv1(d.y), safe_inv1(d.z)); }
+
+/* gi_rays_software.wgsl:35-40 */
+static inline float slab_near(v3 o, v3 inv, v3 lo, v3 hi, float tmin, float tmax) {
+    v3 a = mul3(sub3(lo, o), inv);
+    v3 b = mul3(sub3(hi, o), inv);
+    float near = mx(mx(mx(mn(a.x, b.x), mn(a.y, b.y)), mn(a.z, b.z)), tmin);
+    float far = mn(mn(mn(mx(a.x, b.x), mx(a.y, b.y)), mx(a.z, b.z)), tmax);
+    return near <= far ? near : FMAXV;
+}
+
+/* gi_rays_software.wgsl:42-58: Moller-Trumbore, both faces; t<0 is a miss. */
+static inline v3 tri_test(v3 o, v3 d, v3 a, v3 b, v3 c, float tmin, float tmax) {
+    v3 none = mk(-1.0f, 0.0f, 0.0f);
+    v3 e1 = sub3(b, a), e2 = sub3(c, a);
+    v3 p = cross3(d, e2);
+    float det = dot3(e1, p);
+    if (fabsf(det) < 1e-30f) return none;
+    float inv = 1.0f / det;
+    v3 s = sub3(o, a);
+    float u = dot3(s, p) * inv;
+    if (u < 0.0f || u > 1.0f) return none;
+    v3 q = cross3(s, e1);
+    float v = dot3(d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return none;
+    float t = dot3(e2, q) * inv;
+    if (t > tmin && t < tmax) return mk(t, u, v);
+    return none;
+}
+
+/* shaders/scene.wgsl:86-90 */
+static inline v3 unit_normal(v3 n) {
+    float scale = mx(mx(fabsf(n.x), fabsf(n.y)), fabsf(n.z));
+    float s = mx(scale, 1e-30f);
+    v3 sc = mk(n.x / s, n.y / s, n.z / s);
+    float k = 1.0f / sqrtf(mx(dot3(sc, sc), 1e-30f));
+    return mk(sc.x * k, sc.y * k, sc.z * k);
+}
+
+typedef struct best_t {
+    int hit;
+    float t;
+    v3 n;
+    uint32_t inst;
+    uint32_t tri[3];
+    float u, v;
+    uint32_t slot;
+} best_t;
+
+/* gi_rays_software.wgsl:68-70 */
+static inline float limit_of(const best_t* b, uint32_t i) {
+    return (b->hit && i < b->inst) ? bf(fb(b->t) + 1u) : b->t;
+}
+
+/* Hit normal: mat3x3(r0,r1,r2) * cross(b-a, c-a) (gi_rays_software.wgsl:142). */
+static inline v3 hit_normal(v3 r0, v3 r1, v3 r2, v3 a, v3 b, v3 c) {
+    v3 x = cross3(sub3(b, a), sub3(c, a));
+    v3 n = mk(r0.x * x.x + r1.x * x.y + r2.x * x.z, r0.y * x.x + r1.y * x.y + r2.y * x.z,
+              r0.z * x.x + r1.z * x.y + r2.z * x.z);
+    return unit_normal(n);
+}
+
+/* gi_rays_software.wgsl:98-207 (opaque instances; see-through off). */
+static int gi_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, float tmin, int any_hit,
+                       best_t* best, uint32_t* stack) {
+    uint64_t r = 16u + (uint64_t)i * IW;
+    x->c.inst_tests++;
+    if (any_hit && (W(x, r + 27) & 1u) != 0u) return 0;
+    if (!(slab_near(origin, world_inv, GV(x, r + 12), GV(x, r + 15), tmin, limit_of(best, i)) <=
+          limit_of(best, i)))
+        return 0;
+    x->c.inst_entries++;
+    v3 r0 = GV(x, r), r1 = GV(x, r + 4), r2 = GV(x, r + 8);
+    float w0 = F(x, r + 3), w1 = F(x, r + 7), w2 = F(x, r + 11);
+    v3 o = mk(dot3(r0, origin) + w0 * 1.0f, dot3(r1, origin) + w1 * 1.0f, dot3(r2, origin) + w2 * 1.0f);
+    v3 d = mk(dot3(r0, dir), dot3(r1, dir), dot3(r2, dir));
+    v3 inv = safe_inverse(d);
+    uint32_t nodes = W(x, r + 18), leaves = W(x, r + 19);
+    uint32_t positions = W(x, r + 20), indices = W(x, r + 21);
+    uint32_t prim_base = W(x, r + 28);
+    stack[0] = W(x, r + 22);
+    uint32_t sp = 1;
+    while (sp > 0 && !x->bad) {
+        sp -= 1;
+        uint32_t child = stack[sp];
+        if (child & LEAF) {
+            x->c.leaves++;
+            uint64_t l = leaves + (uint64_t)(child & ~LEAF) * 2u;
+            uint32_t first = W(x, l), count = W(x, l + 1);
+            for (uint32_t k = 0; k < count && !x->bad; k += 6) {
+                x->c.tri_steps++;
+                uint64_t xi = (uint64_t)indices + first + k;
+                int two = k + 3 < count;
+                uint64_t yi = two ? xi + 3 : xi;
+                uint32_t ia = W(x, xi), ib = W(x, xi + 1), ic = W(x, xi + 2);
+                uint32_t ja = W(x, yi), jb = W(x, yi + 1), jc = W(x, yi + 2);
+                v3 a = GV(x, positions + (uint64_t)ia * 3u), b = GV(x, positions + (uint64_t)ib * 3u),
+                   c = GV(x, positions + (uint64_t)ic * 3u);
+                v3 a2 = GV(x, positions + (uint64_t)ja * 3u), b2 = GV(x, positions + (uint64_t)jb * 3u),
+                   c2 = GV(x, positions + (uint64_t)jc * 3u);
+                x->c.tri_tests++;
+                v3 h = tri_test(o, d, a, b, c, tmin, limit_of(best, i));
+                if (h.x >= 0.0f) {
+                    best->hit = 1; best->t = h.x; best->u = h.y; best->v = h.z; best->inst = i;
+                    best->n = hit_normal(r0, r1, r2, a, b, c);
+                    best->tri[0] = ia; best->tri[1] = ib; best->tri[2] = ic;
+                    best->slot = (first + k) / 3u;
+                    if (!any_hit && prim_base != 0u) {
+                        best->tri[0] = W(x, prim_base + (uint64_t)(first + k) / 3u);
+                        best->tri[1] = 0; best->tri[2] = 0;
+                    }
+                    if (any_hit) return 1;
+                }
+                if (two) {
+                    x->c.tri_tests++;
+                    v3 h2 = tri_test(o, d, a2, b2, c2, tmin, limit_of(best, i));
+                    if (h2.x >= 0.0f) {
+                        best->hit = 1; best->t = h2.x; best->u = h2.y; best->v = h2.z; best->inst = i;
+                        best->n = hit_normal(r0, r1, r2, a2, b2, c2);
+                        best->tri[0] = ja; best->tri[1] = jb; best->tri[2] = jc;
+                        best->slot = (first + k) / 3u + 1u;
+                        if (!any_hit && prim_base != 0u) {
+                            best->tri[0] = W(x, prim_base + (uint64_t)(first + k) / 3u + 1u);
+                            best->tri[1] = 0; best->tri[2] = 0;
+                        }
+                        if (any_hit) return 1;
+                    }
+                }
+            }
+            continue;
+        }
+        x->c.blas_nodes++;
+        uint64_t nb = nodes + (uint64_t)child * 16u;
+        v3 oq = GV(x, nb), st = GV(x, nb + 3);
+        float t[4] = {FMAXV, FMAXV, FMAXV, FMAXV};
+        uint32_t c[4] = {W(x, nb + 6), W(x, nb + 7), W(x, nb + 8), W(x, nb + 9)};
+        uint32_t lx = W(x, nb + 10), ly = W(x, nb + 11), lz = W(x, nb + 12);
+        uint32_t hx = W(x, nb + 13), hy = W(x, nb + 14), hz = W(x, nb + 15);
+        for (uint32_t s = 0; s < 4; ++s) {
+            if (c[s] == EMPTY) continue;
+            uint32_t sh = 8u * s;
+            v3 lq = mk((float)((lx >> sh) & 0xffu), (float)((ly >> sh) & 0xffu), (float)((lz >> sh) & 0xffu));
+            v3 hq = mk((float)((hx >> sh) & 0xffu), (float)((hy >> sh) & 0xffu), (float)((hz >> sh) & 0xffu));
+            v3 lo = mk(oq.x + st.x * lq.x, oq.y + st.y * lq.y, oq.z + st.z * lq.z);
+            v3 hi = mk(oq.x + st.x * hq.x, oq.y + st.y * hq.y, oq.z + st.z * hq.z);
+            t[s] = slab_near(o, inv, lo, hi, tmin, limit_of(best, i));
+        }
+        /* Sorting network on (bits(t), child), descending (:189-197). */
+        uint32_t k0[2] = {fb(t[0]), c[0]}, k1[2] = {fb(t[1]), c[1]}, k2[2] = {fb(t[2]), c[2]},
+                 k3[2] = {fb(t[3]), c[3]}, tmp[2];
+#define SWP(A, B) do { memcpy(tmp, A, 8); memcpy(A, B, 8); memcpy(B, tmp, 8); } while (0)
+        if (k0[0] < k1[0]) SWP(k0, k1);
+        if (k2[0] < k3[0]) SWP(k2, k3);
+        if (k0[0] < k2[0]) SWP(k0, k2);
+        if (k1[0] < k3[0]) SWP(k1, k3);
+        if (k1[0] < k2[0]) SWP(k1, k2);
+#undef SWP
+        uint32_t* e[4] = {k0, k1, k2, k3};
+        for (int s = 0; s < 4; ++s) {
+            if (bf(e[s][0]) <= limit_of(best, i) && sp < STACK) stack[sp++] = e[s][1];
+        }
+    }
+    return 0;
+}
+
+static inline int ray_invalid(const orc_ray* r) {
+    for (int a = 0; a < 3; ++a)
+        if (!isfinite(r->o[a]) || !isfinite(r->d[a])) return 1;
+    if (!isfinite(r->tmin) || r->tmin < 0.0f || r->tmax != r->tmax || r->tmax > 1e30f) return 1;
+    return 0;
+}
+
+static void write_hit(const best_t* b, orc_hit* h) {
+    h->t = b->t;
+    h->inst = b->inst;
+    memcpy(h->tri, b->tri, sizeof h->tri);
+    h->u = b->u; h->v = b->v;
+    h->n[0] = b->n.x; h->n[1] = b->n.y; h->n[2] = b->n.z;
+    h->flags = b->hit ? 1u : 0u;
+    h->slot = b->hit ? b->slot : 0u;
+}
+
+static void init_best(best_t* b, float tmax) {
+    memset(b, 0, sizeof *b);
+    b->t = tmax;
+    b->inst = 0xffffffffu;
+}
+
+/* gi_rays_software.wgsl:217-279 (GI_TOP_WIDE, see-through off). */
+int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mode, orc_hit* hit,
+                    orc_counts* counts) {
+    ctx x;
+    memset(&x, 0, sizeof x);
+    x.g = g; x.n = n;
+    best_t best;
+    init_best(&best, ray->tmax);
+    if (ray_invalid(ray)) {
+        write_hit(&best, hit);
+        hit->flags = 2u;
+        if (counts) *counts = x.c;
+        return 0;
+    }
+    int any_hit = mode != 0u;
+    uint32_t stack[STACK];
+    v3 origin = mk(ray->o[0], ray->o[1], ray->o[2]);
+    v3 dir = mk(ray->d[0], ray->d[1], ray->d[2]);
+    float tmin = ray->tmin;
+    v3 world_inv = safe_inverse(dir);
+    uint32_t instances = W(&x, 0), top = W(&x, 1);
+    if (top == 0u) {
+        for (uint32_t i = 0; i < instances && !x.bad; ++i)
+            if (gi_instance(&x, i, origin, dir, world_inv, tmin, any_hit, &best, stack)) break;
+    } else {
+        uint32_t nodes[STACK];
+        nodes[0] = W(&x, 2);
+        uint32_t sp = 1;
+        while (sp > 0 && !x.bad) {
+            sp -= 1;
+            uint32_t item = nodes[sp];
+            if (item & LEAF) {
+                if (gi_instance(&x, item & ~LEAF, origin, dir, world_inv, tmin, any_hit, &best, stack))
+                    break;
+                continue;
+            }
+            x.c.top_nodes++;
+            uint64_t nb = top + (uint64_t)item * 32u;
+            float bound = best.hit ? bf(fb(best.t) + 1u) : best.t;
+            float dist[4];
+            uint32_t ids[4];
+            for (int l = 0; l < 4; ++l) {
+                float ax = (F(&x, nb + l) - origin.x) * world_inv.x;
+                float bx = (F(&x, nb + 12 + l) - origin.x) * world_inv.x;
+                float ay = (F(&x, nb + 4 + l) - origin.y) * world_inv.y;
+                float by = (F(&x, nb + 16 + l) - origin.y) * world_inv.y;
+                float az = (F(&x, nb + 8 + l) - origin.z) * world_inv.z;
+                float bz = (F(&x, nb + 20 + l) - origin.z) * world_inv.z;
+                float near = mx(mx(mx(mn(ax, bx), mn(ay, by)), mn(az, bz)), tmin);
+                float far = mn(mn(mn(mx(ax, bx), mx(ay, by)), mx(az, bz)), bound);
+                ids[l] = W(&x, nb + 24 + l);
+                dist[l] = (near <= far && ids[l] != EMPTY) ? near : FMAXV;
+            }
+#define SW(A, B) do { float td = dist[A]; dist[A] = dist[B]; dist[B] = td; \
+                      uint32_t ti = ids[A]; ids[A] = ids[B]; ids[B] = ti; } while (0)
+            if (dist[1] < dist[0]) SW(0, 1);
+            if (dist[3] < dist[2]) SW(2, 3);
+            if (dist[2] < dist[0]) SW(0, 2);
+            if (dist[3] < dist[1]) SW(1, 3);
+            if (dist[2] < dist[1]) SW(1, 2);
+#undef SW
+            for (int l = 3; l >= 0; --l)
+                if (dist[l] <= bound && sp < STACK) nodes[sp++] = ids[l];
+        }
+    }
+    write_hit(&best, hit);
+    if (x.bad) hit->flags |= 4u;
+    if (counts) *counts = x.c;
+    return x.bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------- batching */
+typedef struct job {
+    const uint32_t* g; size_t n; const orc_ray* rays; uint32_t count; uint32_t mode;
+    orc_hit* hits; uint32_t* vis; orc_counts sum; int fails;
+    uint32_t next; /* shared cursor */
+    pthread_mutex_t lock;
+} job;
+typedef struct worker { job* j; orc_counts c; int fails; } worker;
+
+static void* run_worker(void* arg) {
+    worker* w = (worker*)arg;
+    job* j = w->j;
+    for (;;) {
+        uint32_t start = __atomic_fetch_add(&j->next, 256u, __ATOMIC_RELAXED);
+        if (start >= j->count) break;
+        uint32_t end = start + 256u < j->count ? start + 256u : j->count;
+        for (uint32_t i = start; i < end; ++i) {
+            orc_hit h;
+            orc_counts c;
+            if (orc_gi_traverse(j->g, j->n, &j->rays[i], j->mode, &h, &c)) w->fails++;
+            if (j->hits) j->hits[i] = h;
+            if (j->vis && !(h.flags & 1u)) __atomic_fetch_or(&j->vis[i / 32u], 1u << (i % 32u), __ATOMIC_RELAXED);
+            w->c.words += c.words; w->c.top_nodes += c.top_nodes; w->c.inst_tests += c.inst_tests;
+            w->c.inst_entries += c.inst_entries; w->c.blas_nodes += c.blas_nodes; w->c.leaves += c.leaves;
+            w->c.tri_steps += c.tri_steps; w->c.tri_tests += c.tri_tests;
+        }
+    }
+    return NULL;
+}
+
+int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                          orc_hit* hits, uint32_t* vis_bits, orc_counts* counts, uint32_t threads) {
+    if (threads == 0) {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        threads = c > 0 ? (uint32_t)c : 1u;
+    }
+    if (threads > 256) threads = 256;
+    job j;
+    memset(&j, 0, sizeof j);
+    j.g = g; j.n = n; j.rays = rays; j.count = count; j.mode = mode; j.hits = hits; j.vis = vis_bits;
+    if (vis_bits) memset(vis_bits, 0, ((size_t)count + 31u) / 32u * 4u);
+    worker ws[256];
+    pthread_t th[256];
+    for (uint32_t t = 0; t < threads; ++t) {
+        memset(&ws[t], 0, sizeof ws[t]);
+        ws[t].j = &j;
+        pthread_create(&th[t], NULL, run_worker, &ws[t]);
+    }
+    orc_counts sum;
+    memset(&sum, 0, sizeof sum);
+    int fails = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        pthread_join(th[t], NULL);
+        fails += ws[t].fails;
+        sum.words += ws[t].c.words; sum.top_nodes += ws[t].c.top_nodes; sum.inst_tests += ws[t].c.inst_tests;
+        sum.inst_entries += ws[t].c.inst_entries; sum.blas_nodes += ws[t].c.blas_nodes;
+        sum.leaves += ws[t].c.leaves; sum.tri_steps += ws[t].c.tri_steps; sum.tri_tests += ws[t].c.tri_tests;
+    }
+    if (counts) *counts = sum;
+    return fails;
+}
+
+/* -------------------------------------------------- flat f32 loop (no tree) */
+int orc_flat_f32(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                 orc_hit* hits) {
+    int fails = 0;
+    for (uint32_t q = 0; q < count; ++q) {
+        ctx x;
+        memset(&x, 0, sizeof x);
+        x.g = g; x.n = n;
+        const orc_ray* ray = &rays[q];
+        best_t best;
+        init_best(&best, ray->tmax);
+        if (ray_invalid(ray)) { write_hit(&best, &hits[q]); hits[q].flags = 2u; continue; }
+        int any_hit = mode != 0u;
+        v3 origin = mk(ray->o[0], ray->o[1], ray->o[2]);
+        v3 dir = mk(ray->d[0], ray->d[1], ray->d[2]);
+        uint32_t instances = W(&x, 0);
+        int done = 0;
+        for (uint32_t i = 0; i < instances && !done && !x.bad; ++i) {
+            uint64_t r = 16u + (uint64_t)i * IW;
+            if (any_hit && (W(&x, r + 27) & 1u)) continue;
+            v3 r0 = GV(&x, r), r1 = GV(&x, r + 4), r2 = GV(&x, r + 8);
+            v3 o = mk(dot3(r0, origin) + F(&x, r + 3) * 1.0f, dot3(r1, origin) + F(&x, r + 7) * 1.0f,
+                      dot3(r2, origin) + F(&x, r + 11) * 1.0f);
+            v3 d = mk(dot3(r0, dir), dot3(r1, dir), dot3(r2, dir));
+            uint32_t leaves = W(&x, r + 19), positions = W(&x, r + 20), indices = W(&x, r + 21);
+            uint32_t prim_base = W(&x, r + 28), nodes = W(&x, r + 18);
+            /* The mesh's index count: leaves run contiguously up to the last leaf's end. */
+            uint32_t leaf_count = (positions - leaves) / 2u;
+            (void)nodes;
+            uint32_t total = leaf_count ? W(&x, leaves + 2u * (leaf_count - 1u)) +
+                                              W(&x, leaves + 2u * (leaf_count - 1u) + 1u)
+                                        : 0u;
+            for (uint32_t k = 0; k < total && !x.bad; k += 3) {
+                uint32_t ia = W(&x, indices + k), ib = W(&x, indices + k + 1), ic = W(&x, indices + k + 2);
+                v3 a = GV(&x, positions + (uint64_t)ia * 3u), b = GV(&x, positions + (uint64_t)ib * 3u),
+                   c = GV(&x, positions + (uint64_t)ic * 3u);
+                v3 h = tri_test(o, d, a, b, c, ray->tmin, limit_of(&best, i));
+                if (h.x >= 0.0f) {
+                    best.hit = 1; best.t = h.x; best.u = h.y; best.v = h.z; best.inst = i;
+                    best.n = hit_normal(r0, r1, r2, a, b, c);
+                    best.tri[0] = ia; best.tri[1] = ib; best.tri[2] = ic;
+                    best.slot = k / 3u;
+                    if (!any_hit && prim_base != 0u) {
+                        best.tri[0] = W(&x, prim_base + k / 3u);
+                        best.tri[1] = 0; best.tri[2] = 0;
+                    }
+                    if (any_hit) { done = 1; break; }
+                }
+            }
+        }
+        write_hit(&best, &hits[q]);
+        if (x.bad) { hits[q].flags |= 4u; fails++; }
+    }
+    return fails;
+}
+
+/* ------------------------------------------------------------ f64 checks */
+typedef struct d3 { double x, y, z; } d3;
+static inline d3 dk(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 dsub(d3 a, d3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 dcross(d3 a, d3 b) {
+    return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline d3 dv(const uint32_t* g, uint64_t i) { return dk(bf(g[i]), bf(g[i + 1]), bf(g[i + 2])); }
+
+/* Object-space f64 ray of instance record r (world-to-object rows, widened). */
+static void obj_ray64(const uint32_t* g, uint64_t r, const orc_ray* ray, d3* o, d3* d) {
+    double ro[3], rd[3];
+    for (int k = 0; k < 3; ++k) {
+        uint64_t w = r + 4u * (uint64_t)k;
+        ro[k] = (double)bf(g[w]) * ray->o[0] + (double)bf(g[w + 1]) * ray->o[1] +
+                (double)bf(g[w + 2]) * ray->o[2] + (double)bf(g[w + 3]);
+        rd[k] = (double)bf(g[w]) * ray->d[0] + (double)bf(g[w + 1]) * ray->d[1] +
+                (double)bf(g[w + 2]) * ray->d[2];
+    }
+    *o = dk(ro[0], ro[1], ro[2]);
+    *d = dk(rd[0], rd[1], rd[2]);
+}
+
+/* reference.rs:121-158 arithmetic; returns 1 with (t,u,v) on a hit. */
+static int tri64(d3 o, d3 d, d3 a, d3 b, d3 c, double* t, double* u, double* v) {
+    d3 ab = dsub(b, a), ac = dsub(c, a);
+    d3 p = dcross(d, ac);
+    double det = ddot(ab, p);
+    if (det == 0.0) return 0;
+    d3 s = dsub(o, a);
+    double uu = ddot(s, p) / det;
+    if (!(uu >= 0.0 && uu <= 1.0)) return 0;
+    d3 q = dcross(s, ab);
+    double vv = ddot(d, q) / det;
+    if (vv < 0.0 || uu + vv > 1.0) return 0;
+    *t = ddot(ac, q) / det;
+    *u = uu;
+    *v = vv;
+    return 1;
+}
+
+static int mesh_index_count(const uint32_t* g, size_t n, uint64_t r, uint32_t* total) {
+    uint32_t leaves = g[r + 19], positions = g[r + 20];
+    uint32_t leaf_count = (positions - leaves) / 2u;
+    if (leaf_count == 0) { *total = 0; return 0; }
+    uint64_t l = leaves + 2u * (uint64_t)(leaf_count - 1u);
+    if (l + 1 >= n) return -1;
+    *total = g[l] + g[l + 1];
+    return 0;
+}
+
+typedef struct bjob {
+    const uint32_t* g; size_t n; const orc_ray* rays; uint32_t count; uint32_t mode;
+    orc_hit64* hits; uint32_t next;
+} bjob;
+
+static void brute_one(const bjob* j, uint32_t q) {
+    const uint32_t* g = j->g;
+    const orc_ray* ray = &j->rays[q];
+    orc_hit64 best;
+    memset(&best, 0, sizeof best);
+    best.t = -1.0;
+    best.inst = 0xffffffffu;
+    if (ray_invalid(ray)) { j->hits[q] = best; return; }
+    int any_hit = j->mode != 0u;
+    uint32_t instances = g[0];
+    for (uint32_t i = 0; i < instances; ++i) {
+        uint64_t r = 16u + (uint64_t)i * IW;
+        if (r + IW > j->n) break;
+        if (any_hit && (g[r + 27] & 1u)) continue;
+        d3 o, d;
+        obj_ray64(g, r, ray, &o, &d);
+        uint32_t positions = g[r + 20], indices = g[r + 21], total;
+        if (mesh_index_count(g, j->n, r, &total)) continue;
+        for (uint32_t k = 0; k + 2 < total; k += 3) {
+            double t, u, v;
+            d3 a = dv(g, positions + 3u * (uint64_t)g[indices + k]);
+            d3 b = dv(g, positions + 3u * (uint64_t)g[indices + k + 1]);
+            d3 c = dv(g, positions + 3u * (uint64_t)g[indices + k + 2]);
+            if (!tri64(o, d, a, b, c, &t, &u, &v)) continue;
+            if (!(t > (double)ray->tmin && t < (double)ray->tmax)) continue;
+            if (!best.hit || t < best.t) {
+                best.hit = 1; best.t = t; best.u = u; best.v = v; best.inst = i; best.slot = k / 3u;
+            }
+            if (any_hit) { j->hits[q] = best; return; }
+        }
+    }
+    j->hits[q] = best;
+}
+
+static void* brute_worker(void* arg) {
+    bjob* j = (bjob*)arg;
+    for (;;) {
+        uint32_t q = __atomic_fetch_add(&j->next, 1u, __ATOMIC_RELAXED);
+        if (q >= j->count) break;
+        brute_one(j, q);
+    }
+    return NULL;
+}
+
+int orc_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                  orc_hit64* hits, uint32_t threads) {
+    if (threads == 0) {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        threads = c > 0 ? (uint32_t)c : 1u;
+    }
+    if (threads > 256) threads = 256;
+    bjob j = {g, n, rays, count, mode, hits, 0};
+    pthread_t th[256];
+    for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, brute_worker, &j);
+    for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+int orc_reintersect_f64(const uint32_t* g, size_t n, const orc_ray* rays, const orc_hit* hits,
+                        uint32_t count, orc_hit64* out) {
+    int fails = 0;
+    for (uint32_t q = 0; q < count; ++q) {
+        orc_hit64 h;
+        memset(&h, 0, sizeof h);
+        h.t = -1.0;
+        h.inst = hits[q].inst;
+        h.slot = hits[q].slot;
+        if (hits[q].flags & 1u) {
+            uint64_t r = 16u + (uint64_t)hits[q].inst * IW;
+            if (r + IW > n) { fails++; out[q] = h; continue; }
+            d3 o, d;
+            obj_ray64(g, r, &rays[q], &o, &d);
+            uint32_t positions = g[r + 20], indices = g[r + 21];
+            uint64_t k = (uint64_t)indices + 3u * (uint64_t)hits[q].slot;
+            if (k + 2 >= n) { fails++; out[q] = h; continue; }
+            d3 a = dv(g, positions + 3u * (uint64_t)g[k]);
+            d3 b = dv(g, positions + 3u * (uint64_t)g[k + 1]);
+            d3 c = dv(g, positions + 3u * (uint64_t)g[k + 2]);
+            /* The plane solve without the inside test: the reported triangle is given. */
+            d3 ab = dsub(b, a), ac = dsub(c, a), p = dcross(d, ac);
+            double det = ddot(ab, p);
+            if (det != 0.0) {
+                d3 s = dsub(o, a), qq = dcross(s, ab);
+                h.u = ddot(s, p) / det;
+                h.v = ddot(d, qq) / det;
+                h.t = ddot(ac, qq) / det;
+                h.hit = 1;
+            }
+        }
+        out[q] = h;
+    }
+    return fails;
+}
+
+/* ------------------------------------------------------ Bvh::validate */
+static inline float dec(float o, float s, uint32_t q, int fused) {
+    return fused ? fmaf(s, (float)q, o) : o + s * (float)q;
+}
+
+int orc_validate_blas(const uint32_t* g, size_t n, uint32_t nodes_at, uint32_t node_count,
+                      uint32_t leaves_at, uint32_t leaf_count, uint32_t positions_at,
+                      uint32_t vertex_count, uint32_t indices_at, uint32_t index_count, uint32_t root,
+                      uint32_t* depth_out) {
+    if ((uint64_t)nodes_at + 16ull * node_count > n || (uint64_t)leaves_at + 2ull * leaf_count > n ||
+        (uint64_t)positions_at + 3ull * vertex_count > n || (uint64_t)indices_at + index_count > n)
+        return -1;
+    if (node_count >= LEAF || leaf_count == 0) return -2;
+    for (uint32_t v = 0; v < 3u * vertex_count; ++v)
+        if (!isfinite(bf(g[positions_at + v]))) return -3;
+    for (uint32_t k = 0; k < index_count; ++k)
+        if (g[indices_at + k] >= vertex_count) return -4;
+    uint32_t end = 0;
+    for (uint32_t l = 0; l < leaf_count; ++l) {
+        uint32_t off = g[leaves_at + 2 * l], cnt = g[leaves_at + 2 * l + 1];
+        if (off != end || cnt == 0 || cnt % 3 != 0 || (uint64_t)off + cnt > index_count) return -5;
+        end = off + cnt;
+    }
+    if (end != index_count) return -6;
+    /* Per node: its f64 enclosure box, built children-first (post-order ids). */
+    double* boxes = (double*)calloc((size_t)node_count * 6u + 6u, sizeof(double));
+    uint32_t* depth = (uint32_t*)calloc((size_t)node_count + 1u, sizeof(uint32_t));
+    if (!boxes || !depth) { free(boxes); free(depth); return -7; }
+    int rc = 0;
+    for (uint32_t i = 0; i < node_count && !rc; ++i) {
+        const uint32_t* w = g + nodes_at + 16ull * i;
+        float o[3] = {bf(w[0]), bf(w[1]), bf(w[2])}, s[3] = {bf(w[3]), bf(w[4]), bf(w[5])};
+        int nonempty = 0;
+        for (int a = 0; a < 3; ++a)
+            if (!isfinite(o[a]) || !isfinite(s[a]) || !(s[a] > 0.0f)) rc = -8;
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t dmax = 0;
+        for (uint32_t slot = 0; slot < 4 && !rc; ++slot) {
+            uint32_t child = w[6 + slot];
+            uint32_t ql[3], qh[3];
+            for (int a = 0; a < 3; ++a) {
+                ql[a] = (w[10 + a] >> (8 * slot)) & 0xffu;
+                qh[a] = (w[13 + a] >> (8 * slot)) & 0xffu;
+            }
+            if (child == EMPTY) {
+                for (int a = 0; a < 3; ++a)
+                    if (ql[a] || qh[a]) rc = -9;
+                continue;
+            }
+            nonempty = 1;
+            double clo[3], chi[3];
+            if (child & LEAF) {
+                uint32_t id = child & ~LEAF;
+                if (id >= leaf_count) { rc = -10; break; }
+                uint32_t off = g[leaves_at + 2 * id], cnt = g[leaves_at + 2 * id + 1];
+                for (int a = 0; a < 3; ++a) { clo[a] = INFINITY; chi[a] = -INFINITY; }
+                for (uint32_t k = off; k < off + cnt; ++k) {
+                    uint32_t vi = g[indices_at + k];
+                    for (int a = 0; a < 3; ++a) {
+                        double p = bf(g[positions_at + 3 * vi + a]);
+                        if (p < clo[a]) clo[a] = p;
+                        if (p > chi[a]) chi[a] = p;
+                    }
+                }
+            } else {
+                if (child >= i) { rc = -11; break; }
+                for (int a = 0; a < 3; ++a) { clo[a] = boxes[6 * child + a]; chi[a] = boxes[6 * child + 3 + a]; }
+                if (depth[child] > dmax) dmax = depth[child];
+            }
+            for (int a = 0; a < 3; ++a) {
+                double la = dec(o[a], s[a], ql[a], 0), ha = dec(o[a], s[a], qh[a], 0);
+                double lf = dec(o[a], s[a], ql[a], 1), hf = dec(o[a], s[a], qh[a], 1);
+                if (!isfinite(la) || !isfinite(ha) || !isfinite(lf) || !isfinite(hf) || la > clo[a] ||
+                    ha < chi[a] || lf > clo[a] || hf < chi[a]) {
+                    rc = -12;
+                    break;
+                }
+                double l = la < lf ? la : lf, h = ha > hf ? ha : hf;
+                if (l < lo[a]) lo[a] = l;
+                if (h > hi[a]) hi[a] = h;
+            }
+        }
+        if (!rc && !nonempty) rc = -13;
+        for (int a = 0; a < 3; ++a) { boxes[6 * i + a] = lo[a]; boxes[6 * i + 3 + a] = hi[a]; }
+        depth[i] = dmax + 1u;
+    }
+    /* Reachability: every node and leaf exactly once from the root. */
+    if (!rc) {
+        uint8_t* seen_n = (uint8_t*)calloc(node_count + 1u, 1);
+        uint8_t* seen_l = (uint8_t*)calloc(leaf_count, 1);
+        uint32_t* st = (uint32_t*)malloc(((size_t)node_count * 4u + 8u) * sizeof(uint32_t));
+        size_t sp = 0;
+        st[sp++] = root;
+        size_t visits = 0;
+        while (sp && !rc) {
+            uint32_t c = st[--sp];
+            if (++visits > (size_t)node_count + leaf_count) { rc = -14; break; }
+            if (c == EMPTY) { rc = -15; break; }
+            if (c & LEAF) {
+                uint32_t id = c & ~LEAF;
+                if (id >= leaf_count || seen_l[id]) { rc = -16; break; }
+                seen_l[id] = 1;
+            } else {
+                if (c >= node_count || seen_n[c]) { rc = -17; break; }
+                seen_n[c] = 1;
+                for (int s = 0; s < 4; ++s) {
+                    uint32_t ch = g[nodes_at + 16ull * c + 6 + s];
+                    if (ch != EMPTY) st[sp++] = ch;
+                }
+            }
+        }
+        for (uint32_t i = 0; i < node_count && !rc; ++i)
+            if (!seen_n[i]) rc = -18;
+        for (uint32_t l = 0; l < leaf_count && !rc; ++l)
+            if (!seen_l[l]) rc = -18;
+        free(seen_n); free(seen_l); free(st);
+    }
+    if (depth_out) *depth_out = (root & LEAF) ? 0u : (root < node_count ? depth[root] : 0u);
+    free(boxes);
+    free(depth);
+    return rc;
+}
+
+/* ------------------------------------------------------ ray generation */
+/* shaders/primary_rays.wgsl:50-66, reversed Z, explicit fma; `inverse` is
+   column-major (column c at inverse[4c..4c+3]). Direction = far - near. */
+void orc_primary_ray(const float inverse[16], uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                     orc_ray* out) {
+    float nx = fmaf(((float)px + 0.5f) / (float)w, 2.0f, -1.0f);
+    float ny = fmaf(((float)py + 0.5f) / (float)h, -2.0f, 1.0f);
+    float xy[4], a[4], b[4];
+    for (int k = 0; k < 4; ++k) {
+        xy[k] = fmaf(inverse[k], nx, inverse[4 + k] * ny);
+        a[k] = (xy[k] + inverse[8 + k]) + inverse[12 + k];
+        b[k] = xy[k] + inverse[12 + k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        float near = a[k] / a[3], far = b[k] / b[3];
+        out->o[k] = near;
+        out->d[k] = far - near;
+    }
+    out->tmin = 0.0f;
+    out->tmax = 1e20f;
+}
+
+/* reference_path_tracer.wgsl:149-152 (Jarzynski and Olano 2020). */
+uint32_t orc_pcg(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+static inline float rnd(uint32_t* s) {
+    *s = orc_pcg(*s);
+    return (float)(*s >> 8u) * (1.0f / 16777216.0f);
+}
+
+uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const float sun[3], uint32_t pixel,
+                            uint32_t seed, uint32_t n_gi, orc_ray* out) {
+    if (!(hit->flags & 1u)) return 0u;
+    float t = hit->t;
+    v3 o = mk(primary->o[0], primary->o[1], primary->o[2]);
+    v3 d = mk(primary->d[0], primary->d[1], primary->d[2]);
+    v3 p = mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    v3 ng = mk(hit->n[0], hit->n[1], hit->n[2]);
+    if (dot3(ng, d) > 0.0f) ng = mk(-ng.x, -ng.y, -ng.z);
+    /* reference_path_tracer.wgsl:237-240 */
+    float m = mx(mx(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
+    float k = 1e-4f + 4e-7f * m;
+    v3 start = mk(p.x + ng.x * k, p.y + ng.y * k, p.z + ng.z * k);
+    uint32_t valid = 0;
+    v3 s = mk(sun[0], sun[1], sun[2]);
+    orc_ray* r = &out[0];
+    r->o[0] = start.x; r->o[1] = start.y; r->o[2] = start.z;
+    r->d[0] = s.x; r->d[1] = s.y; r->d[2] = s.z;
+    r->tmin = 0.0f; r->tmax = 1e20f;
+    if (dot3(ng, s) > 0.0f) valid |= 1u;
+    /* reference_path_tracer.wgsl:160-167 (Duff et al. 2017) and :466-470. */
+    float sg = ng.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sg + ng.z);
+    float b = ng.x * ng.y * a;
+    v3 c0 = mk(1.0f + sg * ng.x * ng.x * a, sg * b, -sg * ng.x);
+    v3 c1 = mk(b, sg + ng.y * ng.y * a, -ng.y);
+    v3 c2 = ng;
+    uint32_t state = pixel ^ (seed * 0x9E3779B9u);
+    for (uint32_t gi = 0; gi < n_gi; ++gi) {
+        float rr = sqrtf(rnd(&state));
+        float phi = 6.2831855f * rnd(&state);
+        v3 l = mk(rr * cosf(phi), rr * sinf(phi), sqrtf(mx(0.0f, 1.0f - rr * rr)));
+        v3 wi = mk(c0.x * l.x + c1.x * l.y + c2.x * l.z, c0.y * l.x + c1.y * l.y + c2.y * l.z,
+                   c0.z * l.x + c1.z * l.y + c2.z * l.z);
+        orc_ray* q = &out[1 + gi];
+        q->o[0] = start.x; q->o[1] = start.y; q->o[2] = start.z;
+        q->d[0] = wi.x; q->d[1] = wi.y; q->d[2] = wi.z;
+        q->tmin = 0.0f; q->tmax = 1e20f;
+        if (dot3(ng, wi) > 0.0f) valid |= 1u << (1u + gi);
+    }
+    return valid;
+}

@@ -1,0 +1,111 @@
+/*
+ * CPU oracle for the software BVH ray query. TEST INFRASTRUCTURE ONLY: the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load this
+ * library as the checker. The product library (renderlab_amd/librq.so) never
+ * links, loads or calls it.
+ *
+ * What it restates (reference files, read as text):
+ *   orc_gi_traverse      shaders/gi_rays_software.wgsl:217-279 (gi_traverse),
+ *                        :98-207 (gi_instance), :27-70 (helpers), with
+ *                        shaders/scene.wgsl:86-90 (unit_normal).
+ *   orc_flat_f32         the flat instance loop of the same file (:225-228)
+ *                        with every triangle of every instance tested in leaf
+ *                        order: the tree-free answer gi_traverse must equal.
+ *   orc_brute_f64        crates/asset/src/cluster_bvh/reference.rs:121-158,
+ *                        184-194 (f64 Moller-Trumbore, exact det==0 rejection,
+ *                        lexicographic tie rule) over every instance.
+ *   orc_reintersect_f64  the same f64 test applied to one reported hit.
+ *   orc_validate_blas    crates/asset/src/cluster_bvh.rs:170-333 (Bvh::validate)
+ *                        on one mesh's runtime records.
+ *   orc_primary_ray      shaders/primary_rays.wgsl:50-66 (primary_ray).
+ *   orc_secondary_rays   shaders/reference_path_tracer.wgsl:149-167, 237-256,
+ *                        466-470 (pcg, rand, basis, offset, cosine bounce).
+ *
+ * Arithmetic convention (fixed here because WGSL leaves it to the driver):
+ * no contraction (built with -ffp-contract=off), IEEE division and square
+ * root, min/max are IEEE 754-2019 minimum/maximum (-0 < +0), dot products
+ * sum x, y, then z.
+ */
+#ifndef RQ_ORACLE_H
+#define RQ_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ray {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} orc_ray;
+
+/* Same 48-byte shape as rq_hit in include/rq.h. */
+typedef struct orc_hit {
+    float t;
+    uint32_t inst;
+    uint32_t tri[3];
+    float u, v;
+    float n[3];
+    uint32_t flags; /* bit 0 hit, bit 1 invalid input */
+    uint32_t slot;  /* BVH-order triangle index inside the hit mesh */
+} orc_hit;
+
+/* Visit and load counters of one traversal (words = u32 loads). */
+typedef struct orc_counts {
+    uint64_t words;
+    uint64_t top_nodes;
+    uint64_t inst_tests;
+    uint64_t inst_entries;
+    uint64_t blas_nodes;
+    uint64_t leaves;
+    uint64_t tri_steps;
+    uint64_t tri_tests;
+} orc_counts;
+
+/* One ray through gi_traverse. Returns 0, or -1 if a read left the buffer. */
+int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mode,
+                    orc_hit* hit, orc_counts* counts);
+/* A batch on `threads` host threads (0: all). hits and vis_bits may be NULL;
+   vis_bits gets bit i set when ray i is unoccluded (gi_visible's sense).
+   counts (may be NULL) receives the sum over rays. Returns failures. */
+int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count,
+                          uint32_t mode, orc_hit* hits, uint32_t* vis_bits,
+                          orc_counts* counts, uint32_t threads);
+int orc_flat_f32(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count,
+                 uint32_t mode, orc_hit* hits);
+
+typedef struct orc_hit64 {
+    double t, u, v;
+    uint32_t inst;
+    uint32_t slot;
+    uint32_t hit;
+    uint32_t pad;
+} orc_hit64;
+int orc_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count,
+                  uint32_t mode, orc_hit64* hits, uint32_t threads);
+/* f64 re-intersection of each reported hit (inst, slot); misses get t=-1. */
+int orc_reintersect_f64(const uint32_t* g, size_t n, const orc_ray* rays, const orc_hit* hits,
+                        uint32_t count, orc_hit64* out);
+
+/* Bvh::validate on one mesh of the packed buffer (node/leaf/position/index word
+   offsets, root tag, counts). Returns 0 when valid, else a negative code. */
+int orc_validate_blas(const uint32_t* g, size_t n, uint32_t nodes_at, uint32_t node_count,
+                      uint32_t leaves_at, uint32_t leaf_count, uint32_t positions_at,
+                      uint32_t vertex_count, uint32_t indices_at, uint32_t index_count,
+                      uint32_t root, uint32_t* depth_out);
+
+void orc_primary_ray(const float inverse[16], uint32_t px, uint32_t py, uint32_t w, uint32_t h,
+                     orc_ray* out);
+/* Secondary rays of one primary hit: out[0] shadow toward `sun`, out[1..n_gi]
+   cosine bounces. valid bit k of the return value says out[k] is a ray. */
+uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const float sun[3],
+                            uint32_t pixel, uint32_t seed, uint32_t n_gi, orc_ray* out);
+uint32_t orc_pcg(uint32_t v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+// Host-side data structures shared by the builder, the packers, the layouts and
+// the device launchers. Names follow the reference's record vocabulary (nodes,
+// leaves, positions, indices, instance records, top level).
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rq.h"
+
+namespace rq {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;  // crates/asset/src/cluster_bvh.rs:12
+constexpr uint32_t EMPTY = 0xffffffffu;     // crates/asset/src/cluster_bvh.rs:13
+constexpr uint32_t LEAF_TRIANGLES = 8;      // crates/asset/src/near_field.rs:23
+constexpr uint32_t MAX_SAH_DEPTH = 15;      // crates/asset/src/near_field.rs:178
+constexpr uint32_t HEADER_WORDS = 16;       // src/gi_bvh.rs:19
+constexpr uint32_t INSTANCE_WORDS = 32;     // src/gi_bvh.rs:25
+constexpr uint32_t NODE_WORDS = 16;         // src/gi_bvh.rs:26
+constexpr uint32_t TOP_WORDS = 32;          // src/instance_bvh.rs:29 (wide node)
+constexpr uint32_t TOP_BIN_WORDS = 16;      // src/instance_bvh.rs:28 (binary node)
+constexpr uint32_t TOP_STACK = 48;          // src/instance_bvh.rs:32
+constexpr uint32_t MIN_INSTANCES = 40;      // src/instance_bvh.rs:72
+constexpr uint32_t REFITS = 64;             // src/instance_bvh.rs:94
+
+inline float bits_f(uint32_t w) { float f; std::memcpy(&f, &w, 4); return f; }
+inline uint32_t f_bits(float f) { uint32_t w; std::memcpy(&w, &f, 4); return w; }
+// Keep-the-first-on-ties min/max (Rust's f32::min/max agree on every non-zero,
+// non-NaN pair; the builder never sees NaN).
+inline float mn(float a, float b) { return b < a ? b : a; }
+inline float mx(float a, float b) { return b > a ? b : a; }
+// f32::total_cmp as an integer key.
+inline int32_t total_key(float f) {
+    int32_t k = (int32_t)f_bits(f);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+// `as usize` of an f32: saturating, NaN to 0.
+inline uint32_t sat_index(float f, uint32_t cap) {
+    if (!(f > 0.0f)) return 0;
+    if (f >= (float)cap) return cap;
+    return (uint32_t)f;
+}
+
+// The runtime BVH4 node record: exactly the 16 words of src/gi_bvh.rs:191-202
+// (origin, step, four tagged children, axis-major byte planes; child c is byte c).
+struct Node {
+    float origin[3];
+    float step[3];
+    uint32_t children[4];
+    uint8_t lower[3][4];
+    uint8_t upper[3][4];
+};
+static_assert(sizeof(Node) == 64, "node record is 16 words");
+
+struct Bvh {
+    std::vector<Node> nodes;
+    std::vector<uint32_t> leaf_offset;  // index units
+    std::vector<uint32_t> leaf_count;   // index units
+    std::vector<float> positions;       // 3 per vertex
+    std::vector<uint32_t> indices;
+    uint32_t root = 0;
+};
+
+struct Bounds {
+    float lo[3];
+    float hi[3];
+};
+inline Bounds empty_bounds() {
+    const float inf = __builtin_inff();
+    return Bounds{{inf, inf, inf}, {-inf, -inf, -inf}};
+}
+inline Bounds merge(const Bounds& a, const Bounds& b) {
+    Bounds r;
+    for (int i = 0; i < 3; ++i) {
+        r.lo[i] = mn(a.lo[i], b.lo[i]);
+        r.hi[i] = mx(a.hi[i], b.hi[i]);
+    }
+    return r;
+}
+
+// builder.cpp
+bool build_bvh(const float* positions, uint32_t vertex_count, const uint32_t* indices,
+               uint32_t index_count, bool sah, Bvh& out, std::string& err);
+bool build_bvh_sah(const float* positions, uint32_t vertex_count, const uint32_t* indices,
+                   uint32_t index_count, Bvh& out, std::string& err);
+bool refit_bvh(Bvh& bvh, const float* positions, uint32_t vertex_count, std::string& err);
+uint32_t bvh_depth(const Bvh& bvh, uint32_t child);
+void decode_child(const Node& n, int slot, bool fused, float lo[3], float hi[3]);
+
+// toplevel.cpp
+uint32_t top_reserve(uint32_t count);
+std::vector<uint32_t> top_build(const std::vector<Bounds>& boxes, bool sah);
+void top_refit(uint32_t* nodes, size_t words, const Bounds* boxes, size_t count);
+void top_place(std::vector<uint32_t>& words, uint32_t header, uint32_t stride, uint32_t slots,
+               const uint32_t* previous, size_t previous_words, int force);
+
+// pack.cpp
+bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* source, uint32_t n,
+                    std::vector<uint32_t>& out);
+bool mat4_inverse(const float m[16], float inv[16]);
+
+// Per-mesh facts recovered from an instance record of packed words.
+struct MeshView {
+    uint32_t nodes, leaves, positions, indices;  // word offsets (record words 18-21)
+    uint32_t node_count, leaf_count, vertex_count, index_count;
+};
+bool mesh_view(const std::vector<uint32_t>& words, uint32_t record, MeshView& out, std::string& err);
+
+// Device-side layouts (device.hip); opaque here.
+struct DeviceLayout {
+    bool ready = false;
+    void* base = nullptr;  // one allocation
+    size_t bytes = 0;
+    // word/element offsets into `base`, layout specific
+    uint64_t off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct HostLayout {
+    // Plain and CW8 host images (uploaded verbatim).
+    std::vector<uint32_t> inst;    // 32 words per instance
+    std::vector<uint32_t> top;     // top-level wide nodes (32 words)
+    std::vector<uint32_t> nodes;   // BLAS nodes (plain: 32 words, cw8: 20 words)
+    std::vector<float> tris;       // 12 floats per triangle (a, b, c, pad)
+    std::vector<uint32_t> tri_ids; // 4 words per triangle: ia, ib, ic, primitive
+    uint32_t top_root = 0;
+    uint32_t top_on = 0;
+    uint32_t instance_count = 0;
+};
+
+bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::string& err);
+bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out,
+               std::string& err);
+
+void set_error(const std::string& msg);
+
+}  // namespace rq
+
+struct rq_bvh {
+    rq::Bvh bvh;
+};
+
+struct rq_scene {
+    std::vector<uint32_t> words;  // the reference gi_geometry words
+    uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    rq::HostLayout plain, cw8;
+    rq::DeviceLayout dev[3];
+    int device = -1;
+};

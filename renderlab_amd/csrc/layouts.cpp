@@ -1,0 +1,384 @@
+// Device relayouts of a packed gi_geometry buffer. Both keep the reference's
+// hit semantics: the same instance records (rows and world boxes bit for
+// bit), the same top level, the same triangles in the same leaf order.
+//
+// PLAIN_F32: each BLAS node becomes a 32-word node of f32 child boxes, the
+//   axis-major format the top level already uses (src/instance_bvh.rs:10-18);
+//   each box is the unfused decode origin + step*q of the reference record
+//   (shaders/gi_rays_software.wgsl:184-186), so every slab test sees the same
+//   floats as the reference traversal. Triangles are gathered from the
+//   index/position arrays into 48-byte records (a, b, c, BVH-order slot,
+//   primitive id) so a leaf costs no dependent index load; the position
+//   indices move to a side array read once per hit.
+// CW8_Q8: 8-wide nodes with 8-bit child boxes on a power-of-two grid (after
+//   Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
+//   Through Compressed Wide BVHs", HPG 2017), collapsed from the BVH4. A
+//   child's box encloses the PLAIN box of every subtree it holds, rounded
+//   outward, and decodes exactly as fma(2^e, q, origin) = origin + 2^e*q.
+//   Leaves and their triangle order are the BVH4's.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace rq {
+namespace {
+
+constexpr uint32_t LEAF_COUNT_SHIFT = 27;   // leaf tag: LEAF | (n-1)<<27 | first triangle
+constexpr uint32_t MAX_LEAF_TRIS = 16;
+constexpr uint32_t MAX_FIRST_TRI = 1u << 27;
+
+struct MeshMap {
+    uint32_t nodes_at;  // key: the mesh's node word offset
+    MeshView view;
+    uint32_t node_base = 0;
+    uint32_t tri_base = 0;
+    uint32_t root = 0;
+    bool prims = false;
+    uint32_t prim_base = 0;
+};
+
+bool collect_meshes(const std::vector<uint32_t>& words, std::vector<MeshMap>& meshes,
+                    std::vector<uint32_t>& instance_mesh, std::string& err) {
+    uint32_t count = words[0];
+    uint32_t total = std::max(words[0], words[4]);
+    (void)total;
+    instance_mesh.assign(count, 0);
+    for (uint32_t i = 0; i < count; ++i) {
+        MeshView v;
+        if (!mesh_view(words, i, v, err)) return false;
+        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        uint32_t root = words[r + 22];
+        uint32_t prim_base = words[r + 28];
+        size_t m = 0;
+        for (; m < meshes.size(); ++m)
+            if (meshes[m].nodes_at == v.nodes && meshes[m].root == root) break;
+        if (m == meshes.size()) {
+            MeshMap mm;
+            mm.nodes_at = v.nodes;
+            mm.view = v;
+            mm.root = root;
+            meshes.push_back(mm);
+        }
+        if (prim_base) {
+            meshes[m].prims = true;
+            meshes[m].prim_base = prim_base;
+        }
+        instance_mesh[i] = (uint32_t)m;
+    }
+    return true;
+}
+
+uint32_t leaf_tag(uint32_t first_tri, uint32_t n, std::string& err) {
+    if (n == 0 || n > MAX_LEAF_TRIS || first_tri >= MAX_FIRST_TRI) {
+        err = "leaf too large for the relayout (more than 16 triangles or 2^27 triangles)";
+        return EMPTY;
+    }
+    return LEAF_BIT | ((n - 1) << LEAF_COUNT_SHIFT) | first_tri;
+}
+
+void decode_unfused(const uint32_t* w, int slot, float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        float o = bits_f(w[a]), s = bits_f(w[3 + a]);
+        float ql = (float)((w[10 + a] >> (8 * slot)) & 0xffu);
+        float qh = (float)((w[13 + a] >> (8 * slot)) & 0xffu);
+        lo[a] = o + s * ql;
+        hi[a] = o + s * qh;
+    }
+}
+
+void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint32_t slot_first,
+                      uint32_t n, std::vector<float>& tris, std::vector<uint32_t>& ids) {
+    for (uint32_t k = slot_first; k < slot_first + n; ++k) {
+        for (int v = 0; v < 3; ++v) {
+            uint32_t vi = words[m.view.indices + 3ull * k + v];
+            for (int a = 0; a < 3; ++a) tris.push_back(bits_f(words[m.view.positions + 3ull * vi + a]));
+            ids.push_back(vi);
+        }
+        // Record words 9-11: BVH-order slot, primitive id, zero (read with c.z).
+        tris.push_back(bits_f(k));
+        tris.push_back(bits_f(m.prims ? words[m.prim_base + k] : 0u));
+        tris.push_back(0.0f);
+        ids.push_back(0u);
+    }
+}
+
+void instance_records(const std::vector<uint32_t>& words, const std::vector<MeshMap>& meshes,
+                      const std::vector<uint32_t>& instance_mesh, const std::vector<uint32_t>& roots,
+                      std::vector<uint32_t>& inst) {
+    uint32_t count = words[0];
+    inst.assign((size_t)count * INSTANCE_WORDS, 0u);
+    for (uint32_t i = 0; i < count; ++i) {
+        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        uint32_t* o = &inst[(size_t)i * INSTANCE_WORDS];
+        for (int k = 0; k < 18; ++k) o[k] = words[r + k];
+        const MeshMap& m = meshes[instance_mesh[i]];
+        o[18] = roots[instance_mesh[i]];
+        o[19] = m.tri_base;
+        o[20] = (words[r + 27] & 1u) | (words[r + 28] ? 2u : 0u);
+        o[21] = instance_mesh[i];
+        o[22] = i;
+    }
+}
+
+void copy_top(const std::vector<uint32_t>& words, HostLayout& out) {
+    uint32_t count = words[0];
+    out.instance_count = count;
+    out.top_on = words[1] != 0;
+    out.top_root = words[2];
+    out.top.clear();
+    if (out.top_on) {
+        size_t base = words[1];
+        size_t n = std::min<size_t>(top_reserve(count), words.size() - std::min(words.size(), base));
+        out.top.assign(words.begin() + base, words.begin() + base + n);
+    }
+    if (out.top.empty()) out.top.assign(TOP_WORDS, 0u);
+}
+
+}  // namespace
+
+bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::string& err) {
+    out = HostLayout();
+    std::vector<MeshMap> meshes;
+    std::vector<uint32_t> instance_mesh;
+    if (!collect_meshes(words, meshes, instance_mesh, err)) return false;
+    uint32_t node_total = 0, tri_total = 0;
+    for (auto& m : meshes) {
+        m.node_base = node_total;
+        m.tri_base = tri_total;
+        node_total += m.view.node_count;
+        tri_total += m.view.index_count / 3;
+    }
+    if (node_total >= LEAF_BIT || tri_total >= MAX_FIRST_TRI) {
+        err = "scene too large for the relayout";
+        return false;
+    }
+    out.nodes.assign((size_t)node_total * TOP_WORDS, 0u);
+    out.tris.reserve((size_t)tri_total * 12);
+    out.tri_ids.reserve((size_t)tri_total * 4);
+    std::vector<uint32_t> roots(meshes.size());
+    for (size_t mi = 0; mi < meshes.size(); ++mi) {
+        const MeshMap& m = meshes[mi];
+        auto tag = [&](uint32_t child) -> uint32_t {
+            if (child == EMPTY) return EMPTY;
+            if (child & LEAF_BIT) {
+                uint32_t l = child & ~LEAF_BIT;
+                if (l >= m.view.leaf_count) return EMPTY;
+                uint32_t first = words[m.view.leaves + 2ull * l], cnt = words[m.view.leaves + 2ull * l + 1];
+                return leaf_tag(m.tri_base + first / 3, cnt / 3, err);
+            }
+            if (child >= m.view.node_count) return EMPTY;
+            return m.node_base + child;
+        };
+        for (uint32_t n = 0; n < m.view.node_count; ++n) {
+            const uint32_t* w = &words[m.view.nodes + (size_t)n * NODE_WORDS];
+            uint32_t* o = &out.nodes[(size_t)(m.node_base + n) * TOP_WORDS];
+            for (int s = 0; s < 4; ++s) {
+                uint32_t c = w[6 + s];
+                if (c == EMPTY) {
+                    o[24 + s] = EMPTY;
+                    continue;
+                }
+                float lo[3], hi[3];
+                decode_unfused(w, s, lo, hi);
+                for (int a = 0; a < 3; ++a) {
+                    o[a * 4 + s] = f_bits(lo[a]);
+                    o[12 + a * 4 + s] = f_bits(hi[a]);
+                }
+                o[24 + s] = tag(c);
+                if (o[24 + s] == EMPTY) {
+                    if (err.empty()) err = "node child out of range";
+                    return false;
+                }
+            }
+        }
+        roots[mi] = tag(m.root);
+        if (roots[mi] == EMPTY) {
+            if (err.empty()) err = "mesh root out of range";
+            return false;
+        }
+        gather_triangles(words, m, 0, m.view.index_count / 3, out.tris, out.tri_ids);
+    }
+    instance_records(words, meshes, instance_mesh, roots, out.inst);
+    copy_top(words, out);
+    return true;
+}
+
+namespace {
+
+struct Cand {
+    uint32_t child;  // PLAIN tag: node index or leaf tag
+    Bounds box;
+};
+
+float half_area(const Bounds& b) {
+    float e[3];
+    for (int a = 0; a < 3; ++a) e[a] = mx(b.hi[a] - b.lo[a], 0.0f);
+    return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+}
+
+std::vector<Cand> plain_children(const HostLayout& plain, uint32_t node) {
+    std::vector<Cand> c;
+    const uint32_t* w = &plain.nodes[(size_t)node * TOP_WORDS];
+    for (int s = 0; s < 4; ++s) {
+        if (w[24 + s] == EMPTY) continue;
+        Cand k;
+        k.child = w[24 + s];
+        for (int a = 0; a < 3; ++a) {
+            k.box.lo[a] = bits_f(w[a * 4 + s]);
+            k.box.hi[a] = bits_f(w[12 + a * 4 + s]);
+        }
+        c.push_back(k);
+    }
+    return c;
+}
+
+struct Cw8Builder {
+    const HostLayout* plain;
+    std::vector<uint32_t>* nodes;  // 20 words a node
+    std::vector<float>* tris;
+    std::vector<uint32_t>* ids;
+    std::string* err;
+
+    // Quantizes `cands` into node `index` and recurses into inner children.
+    bool fill(uint32_t index, std::vector<Cand> cands) {
+        // Open the largest inner candidate while its children fit in 8 slots.
+        for (;;) {
+            int pick = -1;
+            for (int i = 0; i < (int)cands.size(); ++i) {
+                if (cands[i].child & LEAF_BIT) continue;
+                auto kids = plain_children(*plain, cands[i].child);
+                if (cands.size() - 1 + kids.size() > 8) continue;
+                if (pick < 0 || half_area(cands[i].box) > half_area(cands[pick].box)) pick = i;
+            }
+            if (pick < 0) break;
+            auto kids = plain_children(*plain, cands[pick].child);
+            cands.erase(cands.begin() + pick);
+            cands.insert(cands.begin() + pick, kids.begin(), kids.end());
+        }
+        Bounds all = empty_bounds();
+        for (auto& c : cands) all = merge(all, c.box);
+        uint32_t w[20] = {};
+        int exps[3];
+        for (int a = 0; a < 3; ++a) {
+            float p = all.lo[a];
+            float top = all.hi[a];
+            float ext = top - p;
+            int e = -126;
+            if (ext > 0.0f) {
+                int ex;
+                std::frexp(ext / 255.0f, &ex);  // ext/255 < 2^ex
+                e = std::max(ex - 1, -126);
+            }
+            for (;;) {
+                float s = std::ldexp(1.0f, e);
+                if (p + s * 255.0f >= top) break;
+                ++e;
+            }
+            if (e > 127) {
+                *err = "box too large to quantize";
+                return false;
+            }
+            exps[a] = e;
+            w[a] = f_bits(p);
+        }
+        uint32_t imask = 0, valid = 0, counts = 0;
+        uint32_t inner_n = 0;
+        for (auto& c : cands)
+            if (!(c.child & LEAF_BIT)) inner_n++;
+        uint32_t child_base = (uint32_t)(nodes->size() / 20);
+        nodes->resize(nodes->size() + (size_t)inner_n * 20, 0u);
+        uint32_t tri_base = (uint32_t)(tris->size() / 12);
+        std::vector<std::pair<uint32_t, std::vector<Cand>>> todo;
+        uint32_t rank = 0;
+        for (size_t s = 0; s < cands.size(); ++s) {
+            const Cand& c = cands[s];
+            valid |= 1u << s;
+            for (int a = 0; a < 3; ++a) {
+                float p = bits_f(w[a]);
+                float sc = std::ldexp(1.0f, exps[a]);
+                float fl = std::floor((c.box.lo[a] - p) / sc);
+                uint32_t ql = (uint32_t)mn(mx(fl, 0.0f), 255.0f);
+                while (ql > 0 && p + sc * (float)ql > c.box.lo[a]) ql--;
+                float fh = std::ceil((c.box.hi[a] - p) / sc);
+                uint32_t qh = (uint32_t)mn(mx(fh, 0.0f), 255.0f);
+                while (qh < 255 && p + sc * (float)qh < c.box.hi[a]) qh++;
+                if (p + sc * (float)ql > c.box.lo[a] || p + sc * (float)qh < c.box.hi[a]) {
+                    *err = "quantization failed to enclose";
+                    return false;
+                }
+                // planes: w[8..9] lo x, [10..11] lo y, [12..13] lo z, [14..15] hi x, ...
+                w[8 + 2 * a + s / 4] |= ql << (8 * (s % 4));
+                w[14 + 2 * a + s / 4] |= qh << (8 * (s % 4));
+            }
+            if (c.child & LEAF_BIT) {
+                uint32_t n = ((c.child >> LEAF_COUNT_SHIFT) & 0xfu) + 1u;
+                if (n > 15) {
+                    *err = "leaf too large for an 8-wide node";
+                    return false;
+                }
+                counts |= n << (4 * s);
+                uint32_t first = c.child & (MAX_FIRST_TRI - 1);
+                for (uint32_t t = first; t < first + n; ++t) {
+                    for (int k = 0; k < 12; ++k) tris->push_back(plain->tris[12ull * t + k]);
+                    for (int k = 0; k < 4; ++k) ids->push_back(plain->tri_ids[4ull * t + k]);
+                }
+            } else {
+                imask |= 1u << s;
+                todo.push_back({child_base + rank, plain_children(*plain, c.child)});
+                rank++;
+            }
+        }
+        w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) |
+               ((uint32_t)(exps[2] + 127) << 16) | (imask << 24);
+        w[4] = child_base;
+        w[5] = tri_base;
+        w[6] = counts;
+        w[7] = valid;
+        std::copy(w, w + 20, nodes->begin() + (size_t)index * 20);
+        for (auto& t : todo)
+            if (!fill(t.first, std::move(t.second))) return false;
+        return true;
+    }
+};
+
+}  // namespace
+
+bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
+    out = HostLayout();
+    std::vector<MeshMap> meshes;
+    std::vector<uint32_t> instance_mesh;
+    if (!collect_meshes(words, meshes, instance_mesh, err)) return false;
+    std::vector<uint32_t> roots(meshes.size());
+    out.tris.reserve(plain.tris.size());
+    out.tri_ids.reserve(plain.tri_ids.size());
+    // PLAIN roots per mesh, from any instance of it.
+    std::vector<uint32_t> plain_root(meshes.size(), EMPTY);
+    for (uint32_t i = 0; i < words[0]; ++i) plain_root[instance_mesh[i]] = plain.inst[(size_t)i * INSTANCE_WORDS + 18];
+    Cw8Builder b{&plain, &out.nodes, &out.tris, &out.tri_ids, &err};
+    for (size_t mi = 0; mi < meshes.size(); ++mi) {
+        uint32_t pr = plain_root[mi];
+        meshes[mi].tri_base = (uint32_t)(out.tris.size() / 12);
+        if (pr & LEAF_BIT) {
+            uint32_t n = ((pr >> LEAF_COUNT_SHIFT) & 0xfu) + 1u, first = pr & (MAX_FIRST_TRI - 1);
+            uint32_t nf = (uint32_t)(out.tris.size() / 12);
+            for (uint32_t t = first; t < first + n; ++t) {
+                for (int k = 0; k < 12; ++k) out.tris.push_back(plain.tris[12ull * t + k]);
+                for (int k = 0; k < 4; ++k) out.tri_ids.push_back(plain.tri_ids[4ull * t + k]);
+            }
+            roots[mi] = leaf_tag(nf, n, err);
+            continue;
+        }
+        uint32_t index = (uint32_t)(out.nodes.size() / 20);
+        out.nodes.resize(out.nodes.size() + 20, 0u);
+        roots[mi] = index;
+        if (!b.fill(index, plain_children(plain, pr))) return false;
+    }
+    // The hit's BVH-order slot is recovered from the side array, so tri_base
+    // in the instance record is the mesh's first triangle in this layout.
+    instance_records(words, meshes, instance_mesh, roots, out.inst);
+    copy_top(words, out);
+    return true;
+}
+
+}  // namespace rq

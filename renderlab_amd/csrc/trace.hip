@@ -1,0 +1,1392 @@
+// Device side: the traversal kernels for the three layouts, ray generation,
+// FP64 verification, and the C entry points that launch them.
+//
+// Traversal semantics are those of gi_traverse / gi_instance
+// (shaders/gi_rays_software.wgsl:98-279): a top level of 4-wide float nodes
+// (or the flat instance list when header word 1 is 0), per-instance
+// world-box test against gi_limit, world-to-object rows, a 48-entry stack
+// per level, child boxes sorted by entry distance and pushed far to near,
+// Moller-Trumbore in leaf order, first hit returned in any-hit modes.
+//
+// MI355X mapping (gfx950, wave64):
+//  - one lane per ray inside a persistent grid sized to the device's
+//    occupancy; a wave refills its idle lanes from a global atomic ray cursor
+//    (one atomic per refill, lane ranks by mbcnt), so finished rays stop
+//    costing SIMD slots;
+//  - the two per-level stacks share one per-lane stack: a ring of S entries in
+//    LDS laid out [entry][lane] (bank-conflict free), spilling the oldest
+//    entries to a per-lane global area when deeper than S;
+//  - min/max are v_minimum3/v_maximum3 (IEEE 754-2019), division and square
+//    root are correctly rounded, no contraction (-ffp-contract=off), so the
+//    kernels compute the oracle's arithmetic bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "internal.h"
+
+namespace rq {
+namespace dev {
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr float FMAXV = 3.4028235e38f;
+// Per-lane stack words: 48 top-level entries (the reference's bound) plus
+// the BLAS bound: 48 for the 4-wide layouts (depth <= 15 pushes at most
+// 1 + 3*15), 112 for CW8 (at most 1 + 7*15; a CW8 tree is never deeper
+// than the BVH4 it is collapsed from).
+constexpr uint32_t SPILL_WORDS = 160;
+constexpr uint32_t STACK_CAP = 48;
+constexpr uint32_t CW8_CAP = 112;
+constexpr uint32_t LEAF_MASK = (1u << 27) - 1u;
+
+enum : int { REF = 0, PLAIN = 1, CW8 = 2 };
+
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+__device__ __forceinline__ float bf(uint32_t w) { return __uint_as_float(w); }
+__device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 sub3(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float safe_inv1(float d) {
+    float tiny = d >= 0.0f ? 1e-30f : -1e-30f;
+    return 1.0f / (fabsf(d) < 1e-30f ? tiny : d);
+}
+__device__ __forceinline__ V3 safe_inverse(V3 d) { return mk(safe_inv1(d.x), safe_inv1(d.y), safe_inv1(d.z)); }
+
+// gi_slab_near (:35-40)
+__device__ __forceinline__ float slab_near(V3 o, V3 inv, V3 lo, V3 hi, float tmin, float tmax) {
+    float ax = (lo.x - o.x) * inv.x, bx = (hi.x - o.x) * inv.x;
+    float ay = (lo.y - o.y) * inv.y, by = (hi.y - o.y) * inv.y;
+    float az = (lo.z - o.z) * inv.z, bz = (hi.z - o.z) * inv.z;
+    float near = fmx(fmx(fmx(fmn(ax, bx), fmn(ay, by)), fmn(az, bz)), tmin);
+    float far = fmn(fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz)), tmax);
+    return near <= far ? near : FMAXV;
+}
+
+// gi_triangle (:42-58); returns t < 0 on a miss.
+__device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 b, V3 c, float tmin, float tmax, float& u_out,
+                                          float& v_out) {
+    V3 e1 = sub3(b, a), e2 = sub3(c, a);
+    V3 p = cross3(d, e2);
+    float det = dot3(e1, p);
+    if (fabsf(det) < 1e-30f) return -1.0f;
+    float inv = 1.0f / det;
+    V3 s = sub3(o, a);
+    float u = dot3(s, p) * inv;
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    V3 q = cross3(s, e1);
+    float v = dot3(d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    float t = dot3(e2, q) * inv;
+    if (t > tmin && t < tmax) {
+        u_out = u;
+        v_out = v;
+        return t;
+    }
+    return -1.0f;
+}
+
+// shaders/scene.wgsl:86-90 applied to mat3x3(r0,r1,r2) * cross(b-a, c-a)
+__device__ __forceinline__ V3 hit_normal(V3 r0, V3 r1, V3 r2, V3 a, V3 b, V3 c) {
+    V3 x = cross3(sub3(b, a), sub3(c, a));
+    V3 n = mk(r0.x * x.x + r1.x * x.y + r2.x * x.z, r0.y * x.x + r1.y * x.y + r2.y * x.z,
+              r0.z * x.x + r1.z * x.y + r2.z * x.z);
+    float scale = fmx(fmx(fabsf(n.x), fabsf(n.y)), fabsf(n.z));
+    float s = fmx(scale, 1e-30f);
+    V3 sc = mk(n.x / s, n.y / s, n.z / s);
+    float k = 1.0f / sqrtf(fmx(dot3(sc, sc), 1e-30f));
+    return mk(sc.x * k, sc.y * k, sc.z * k);
+}
+
+__device__ __forceinline__ bool ray_invalid(float4 a, float4 b) {
+    bool bad = !isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z) || !isfinite(b.x) || !isfinite(b.y) ||
+               !isfinite(b.z);
+    bad = bad || !isfinite(a.w) || a.w < 0.0f || b.w != b.w || b.w > 1e30f;
+    return bad;
+}
+
+struct TraceArgs {
+    const uint32_t* words;  // REF: the packed words
+    const uint32_t* inst;   // PLAIN/CW8 instance records (32 words)
+    const uint32_t* top;    // top-level wide nodes
+    const uint32_t* nodes;  // PLAIN 32 words, CW8 20 words a node
+    const float4* tris;     // 3 float4 a triangle
+    const uint32_t* tri_ids;
+    const rq_ray* rays;
+    rq_hit* hits;
+    uint32_t* vis;
+    uint32_t* counter;
+    uint32_t* spill;
+    unsigned long long* bytes;  // COUNT instantiations: global bytes moved
+    uint32_t count, n_inst, top_on, top_root, refill_min, pad;
+};
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <int L>
+struct Loader;
+template <>
+struct Loader<REF> {
+    static __device__ __forceinline__ uint32_t w(const uint32_t* p, uint64_t i) { return p[i]; }
+};
+
+// 8-key sorting network, descending on key (Batcher's odd-even merge sort, 19 comparators).
+__device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& kb, uint32_t& vb) {
+    bool sw = ka < kb;
+    uint32_t k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+    uint32_t v0 = sw ? vb : va, v1 = sw ? va : vb;
+    ka = k0; kb = k1; va = v0; vb = v1;
+}
+
+// COUNT: the same traversal, also tallying the bytes every global load and
+// store moves (rays, records, nodes, triangles, spills, hit records); used
+// once, untimed, to give the roofline its algorithmic bytes per ray.
+template <int LAYOUT, int BLOCK, int S, bool ANY, bool COUNT = false>
+__global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
+    static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
+    __shared__ uint32_t lds[S * BLOCK];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint32_t* st = lds + tid;
+    uint32_t* spill = a.spill + ((uint64_t)blockIdx.x * BLOCK + tid) * SPILL_WORDS;
+
+    // Lane state.
+    uint32_t ray = NONE;
+    V3 wo{0, 0, 0}, wd{0, 0, 0}, winv{0, 0, 0}, oo{0, 0, 0}, od{0, 0, 0}, oinv{0, 0, 0};
+    float tmin = 0.0f, bt = 0.0f, bu = 0.0f, bv = 0.0f;
+    uint32_t binst = NONE, btri = 0, cur = 0;
+    bool bhit = false, invalid = false, in_blas = false;
+    uint32_t sp = 0, lo = 0, floor_ = 0, flat_next = 0;
+    uint32_t m_nodes = 0, m_leaves = 0, m_pos = 0, m_idx = 0;
+    bool exhausted = false;
+    unsigned long long nb = 0;
+
+    auto push = [&](uint32_t v) {
+        if (sp - lo == (uint32_t)S) {
+            spill[lo] = st[(lo & (S - 1)) * BLOCK];
+            lo++;
+            if (COUNT) nb += 4;
+        }
+        st[(sp & (S - 1)) * BLOCK] = v;
+        sp++;
+    };
+    auto pop = [&]() -> uint32_t {
+        sp--;
+        if (sp < lo) {
+            lo = sp;
+            if (COUNT) nb += 4;
+            return spill[sp];
+        }
+        return st[(sp & (S - 1)) * BLOCK];
+    };
+    auto limit = [&](uint32_t i) -> float { return (bhit && i < binst) ? bf(fb(bt) + 1u) : bt; };
+
+    for (;;) {
+        // ---- refill idle lanes from the global cursor (wave-uniform decision)
+        const uint64_t idle = __ballot(ray == NONE);
+        if (!exhausted && idle != 0ull && ((uint32_t)__popcll(idle) >= a.refill_min || idle == ~0ull)) {
+            uint32_t n = (uint32_t)__popcll(idle);
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(a.counter, n);
+            base = __shfl(base, 0);
+            if (base + n >= a.count) exhausted = true;
+            if (ray == NONE) {
+                uint32_t r = base + lane_rank(idle);
+                if (r < a.count) {
+                    ray = r;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays + r);
+                    float4 r0 = rp[0], r1 = rp[1];
+                    if (COUNT) nb += 32;
+                    wo = mk(r0.x, r0.y, r0.z);
+                    tmin = r0.w;
+                    wd = mk(r1.x, r1.y, r1.z);
+                    bt = r1.w;
+                    invalid = ray_invalid(r0, r1);
+                    winv = safe_inverse(wd);
+                    binst = NONE;
+                    bhit = false;
+                    btri = 0;
+                    bu = 0.0f;
+                    bv = 0.0f;
+                    sp = 0;
+                    lo = 0;
+                    in_blas = false;
+                    flat_next = 0;
+                    if (!invalid && a.top_on) push(a.top_root);
+                }
+            }
+        }
+        if (exhausted && __ballot(ray != NONE) == 0ull) break;
+        if (ray == NONE) continue;
+
+        // ---- one traversal step
+        bool fin = invalid;
+        if (!fin) {
+            if (in_blas && sp == floor_) in_blas = false;
+            if (!in_blas) {
+                uint32_t item = 0;
+                bool have;
+                if (!a.top_on) {
+                    have = flat_next < a.n_inst;
+                    item = LEAF_BIT | flat_next;
+                    flat_next++;
+                } else {
+                    have = sp > 0;
+                    if (have) item = pop();
+                }
+                if (!have) {
+                    fin = true;
+                } else if (item & LEAF_BIT) {
+                    // gi_instance prologue (:100-113)
+                    const uint32_t i = item & ~LEAF_BIT;
+                    const uint32_t* r = LAYOUT == REF ? a.words + 16ull + (uint64_t)i * 32u : a.inst + (uint64_t)i * 32u;
+                    uint32_t rw[24];
+                    if (LAYOUT == REF) {
+#pragma unroll
+                        for (int k = 0; k < 18; ++k) rw[k] = r[k];
+                        rw[20] = ANY ? r[27] : 0u;
+                    } else {
+                        const uint4* r4 = reinterpret_cast<const uint4*>(r);
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) {
+                            uint4 q = r4[k];
+                            rw[4 * k] = q.x; rw[4 * k + 1] = q.y; rw[4 * k + 2] = q.z; rw[4 * k + 3] = q.w;
+                        }
+                    }
+                    if (COUNT) nb += LAYOUT == REF ? (ANY ? 76 : 72) : 96;
+                    bool skip = ANY && (rw[20] & 1u);
+                    float lim = limit(i);
+                    if (!skip && slab_near(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
+                                           mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim) {
+                        V3 r0 = mk(bf(rw[0]), bf(rw[1]), bf(rw[2])), r1 = mk(bf(rw[4]), bf(rw[5]), bf(rw[6])),
+                           r2 = mk(bf(rw[8]), bf(rw[9]), bf(rw[10]));
+                        oo = mk(dot3(r0, wo) + bf(rw[3]) * 1.0f, dot3(r1, wo) + bf(rw[7]) * 1.0f,
+                                dot3(r2, wo) + bf(rw[11]) * 1.0f);
+                        od = mk(dot3(r0, wd), dot3(r1, wd), dot3(r2, wd));
+                        oinv = safe_inverse(od);
+                        cur = i;
+                        uint32_t root;
+                        if (LAYOUT == REF) {
+                            m_nodes = r[18];
+                            m_leaves = r[19];
+                            m_pos = r[20];
+                            m_idx = r[21];
+                            root = r[22];
+                            if (COUNT) nb += 20;
+                        } else {
+                            root = rw[18];
+                        }
+                        floor_ = sp;
+                        in_blas = true;
+                        push(root);
+                    }
+                } else {
+                    // top-level wide node (:240-261)
+                    const uint32_t* w = a.top + (uint64_t)item * 32u;
+                    uint32_t tw[28];
+                    if (LAYOUT == REF) {
+#pragma unroll
+                        for (int k = 0; k < 28; ++k) tw[k] = w[k];
+                    } else {
+                        const uint4* w4 = reinterpret_cast<const uint4*>(w);
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) {
+                            uint4 q = w4[k];
+                            tw[4 * k] = q.x; tw[4 * k + 1] = q.y; tw[4 * k + 2] = q.z; tw[4 * k + 3] = q.w;
+                        }
+                    }
+                    if (COUNT) nb += 112;
+                    float bound = bhit ? bf(fb(bt) + 1u) : bt;
+                    float dist[4];
+                    uint32_t ids[4];
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        float ax = (bf(tw[l]) - wo.x) * winv.x, bx = (bf(tw[12 + l]) - wo.x) * winv.x;
+                        float ay = (bf(tw[4 + l]) - wo.y) * winv.y, by = (bf(tw[16 + l]) - wo.y) * winv.y;
+                        float az = (bf(tw[8 + l]) - wo.z) * winv.z, bz = (bf(tw[20 + l]) - wo.z) * winv.z;
+                        float near = fmx(fmx(fmx(fmn(ax, bx), fmn(ay, by)), fmn(az, bz)), tmin);
+                        float far = fmn(fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz)), bound);
+                        ids[l] = tw[24 + l];
+                        dist[l] = (near <= far && ids[l] != EMPTY) ? near : FMAXV;
+                    }
+#define RQ_SW(A, B)                \
+    {                              \
+        float td = dist[A];        \
+        dist[A] = dist[B];         \
+        dist[B] = td;              \
+        uint32_t ti = ids[A];      \
+        ids[A] = ids[B];           \
+        ids[B] = ti;               \
+    }
+                    if (dist[1] < dist[0]) RQ_SW(0, 1);
+                    if (dist[3] < dist[2]) RQ_SW(2, 3);
+                    if (dist[2] < dist[0]) RQ_SW(0, 2);
+                    if (dist[3] < dist[1]) RQ_SW(1, 3);
+                    if (dist[2] < dist[1]) RQ_SW(1, 2);
+#undef RQ_SW
+#pragma unroll
+                    for (int l = 3; l >= 0; --l)
+                        if (dist[l] <= bound && sp < STACK_CAP) push(ids[l]);
+                }
+            } else {
+                const uint32_t e = pop();
+                const float lim = limit(cur);
+                if (e & LEAF_BIT) {
+                    // leaf (:117-169)
+                    if (LAYOUT == REF) {
+                        uint64_t l = m_leaves + (uint64_t)(e & ~LEAF_BIT) * 2u;
+                        uint32_t first = a.words[l], count = a.words[l + 1];
+                        if (COUNT) nb += 8;
+                        float lim_t = lim;
+                        for (uint32_t k = 0; k < count; k += 3) {
+                            uint64_t x = (uint64_t)m_idx + first + k;
+                            uint32_t ia = a.words[x], ib = a.words[x + 1], ic = a.words[x + 2];
+                            const uint32_t* P = a.words + m_pos;
+                            V3 va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
+                            V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
+                            V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
+                            if (COUNT) nb += 48;
+                            float u, v;
+                            float t = tri_test(oo, od, va, vb, vc, tmin, lim_t, u, v);
+                            if (t >= 0.0f) {
+                                bt = t; bu = u; bv = v; binst = cur; bhit = true;
+                                btri = (first + k) / 3u;
+                                lim_t = limit(cur);
+                                if (ANY) { fin = true; break; }
+                            }
+                        }
+                    } else {
+                        uint32_t first = e & LEAF_MASK, n = ((e >> 27) & 0xfu) + 1u;
+                        float lim_t = lim;
+                        for (uint32_t j = 0; j < n; ++j) {
+                            const float4* tp = a.tris + 3ull * (first + j);
+                            float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                            if (COUNT) nb += 48;
+                            float u, v;
+                            float t = tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t0.w, t1.x, t1.y),
+                                               mk(t1.z, t1.w, t2.x), tmin, lim_t, u, v);
+                            if (t >= 0.0f) {
+                                bt = t; bu = u; bv = v; binst = cur; bhit = true;
+                                btri = first + j;
+                                lim_t = limit(cur);
+                                if (ANY) { fin = true; break; }
+                            }
+                        }
+                    }
+                } else if (LAYOUT == REF || LAYOUT == PLAIN) {
+                    // BLAS node (:171-204)
+                    float t[4];
+                    uint32_t c[4];
+                    if (LAYOUT == REF) {
+                        const uint32_t* w = a.words + m_nodes + (uint64_t)e * 16u;
+                        uint32_t nw[16];
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) nw[k] = w[k];
+                        if (COUNT) nb += 64;
+                        V3 oq = mk(bf(nw[0]), bf(nw[1]), bf(nw[2])), stp = mk(bf(nw[3]), bf(nw[4]), bf(nw[5]));
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            c[s] = nw[6 + s];
+                            t[s] = FMAXV;
+                            if (c[s] != EMPTY) {
+                                uint32_t sh = 8u * s;
+                                V3 lq = mk((float)((nw[10] >> sh) & 0xffu), (float)((nw[11] >> sh) & 0xffu),
+                                           (float)((nw[12] >> sh) & 0xffu));
+                                V3 hq = mk((float)((nw[13] >> sh) & 0xffu), (float)((nw[14] >> sh) & 0xffu),
+                                           (float)((nw[15] >> sh) & 0xffu));
+                                V3 lo3 = mk(oq.x + stp.x * lq.x, oq.y + stp.y * lq.y, oq.z + stp.z * lq.z);
+                                V3 hi3 = mk(oq.x + stp.x * hq.x, oq.y + stp.y * hq.y, oq.z + stp.z * hq.z);
+                                t[s] = slab_near(oo, oinv, lo3, hi3, tmin, lim);
+                            }
+                        }
+                    } else {
+                        const uint4* w4 = reinterpret_cast<const uint4*>(a.nodes + (uint64_t)e * 32u);
+                        uint32_t nw[28];
+                        if (COUNT) nb += 112;
+#pragma unroll
+                        for (int k = 0; k < 7; ++k) {
+                            uint4 q = w4[k];
+                            nw[4 * k] = q.x; nw[4 * k + 1] = q.y; nw[4 * k + 2] = q.z; nw[4 * k + 3] = q.w;
+                        }
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) {
+                            c[s] = nw[24 + s];
+                            t[s] = FMAXV;
+                            if (c[s] != EMPTY)
+                                t[s] = slab_near(oo, oinv, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
+                                                 mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
+                        }
+                    }
+                    uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
+                    cas_desc(k0, c[0], k1, c[1]);
+                    cas_desc(k2, c[2], k3, c[3]);
+                    cas_desc(k0, c[0], k2, c[2]);
+                    cas_desc(k1, c[1], k3, c[3]);
+                    cas_desc(k1, c[1], k2, c[2]);
+                    if (bf(k0) <= lim && sp - floor_ < STACK_CAP) push(c[0]);
+                    if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
+                    if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
+                    if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
+                } else {
+                    // CW8 node: 20 words
+                    const uint4* w4 = reinterpret_cast<const uint4*>(a.nodes + (uint64_t)e * 20u);
+                    uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4];
+                    if (COUNT) nb += 80;
+                    V3 p = mk(bf(q0.x), bf(q0.y), bf(q0.z));
+                    uint32_t ex = q0.w & 0xffu, ey = (q0.w >> 8) & 0xffu, ez = (q0.w >> 16) & 0xffu;
+                    uint32_t imask = q0.w >> 24;
+                    uint32_t child_base = q1.x, tri_base = q1.y, counts = q1.z, valid = q1.w & 0xffu;
+                    V3 sc = mk(bf(ex << 23), bf(ey << 23), bf(ez << 23));
+                    uint32_t plane[12] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w};
+                    uint32_t kk[8], cc[8];
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        kk[s] = fb(FMAXV);
+                        cc[s] = EMPTY;
+                        if ((valid >> s) & 1u) {
+                            const int wi = s >> 2, sh = 8 * (s & 3);
+                            V3 lq = mk((float)((plane[0 + wi] >> sh) & 0xffu), (float)((plane[2 + wi] >> sh) & 0xffu),
+                                       (float)((plane[4 + wi] >> sh) & 0xffu));
+                            V3 hq = mk((float)((plane[6 + wi] >> sh) & 0xffu), (float)((plane[8 + wi] >> sh) & 0xffu),
+                                       (float)((plane[10 + wi] >> sh) & 0xffu));
+                            V3 lo3 = mk(__builtin_fmaf(sc.x, lq.x, p.x), __builtin_fmaf(sc.y, lq.y, p.y),
+                                        __builtin_fmaf(sc.z, lq.z, p.z));
+                            V3 hi3 = mk(__builtin_fmaf(sc.x, hq.x, p.x), __builtin_fmaf(sc.y, hq.y, p.y),
+                                        __builtin_fmaf(sc.z, hq.z, p.z));
+                            kk[s] = fb(slab_near(oo, oinv, lo3, hi3, tmin, lim));
+                            const uint32_t below = (1u << s) - 1u;
+                            if ((imask >> s) & 1u) {
+                                cc[s] = child_base + (uint32_t)__popc(imask & below);
+                            } else {
+                                uint32_t x = counts & ((1u << (4 * s)) - 1u);
+                                if (s == 0) x = 0u;
+                                uint32_t y = (x & 0x0f0f0f0fu) + ((x >> 4) & 0x0f0f0f0fu);
+                                uint32_t off = (y * 0x01010101u) >> 24;
+                                uint32_t nt = (counts >> (4 * s)) & 0xfu;
+                                cc[s] = LEAF_BIT | ((nt - 1u) << 27) | (tri_base + off);
+                            }
+                        }
+                    }
+                    // Batcher odd-even merge network for 8, descending.
+                    cas_desc(kk[0], cc[0], kk[1], cc[1]); cas_desc(kk[2], cc[2], kk[3], cc[3]);
+                    cas_desc(kk[4], cc[4], kk[5], cc[5]); cas_desc(kk[6], cc[6], kk[7], cc[7]);
+                    cas_desc(kk[0], cc[0], kk[2], cc[2]); cas_desc(kk[1], cc[1], kk[3], cc[3]);
+                    cas_desc(kk[4], cc[4], kk[6], cc[6]); cas_desc(kk[5], cc[5], kk[7], cc[7]);
+                    cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[5], cc[5], kk[6], cc[6]);
+                    cas_desc(kk[0], cc[0], kk[4], cc[4]); cas_desc(kk[1], cc[1], kk[5], cc[5]);
+                    cas_desc(kk[2], cc[2], kk[6], cc[6]); cas_desc(kk[3], cc[3], kk[7], cc[7]);
+                    cas_desc(kk[2], cc[2], kk[4], cc[4]); cas_desc(kk[3], cc[3], kk[5], cc[5]);
+                    cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[3], cc[3], kk[4], cc[4]);
+                    cas_desc(kk[5], cc[5], kk[6], cc[6]);
+#pragma unroll
+                    for (int s = 0; s < 8; ++s)
+                        if (bf(kk[s]) <= lim && sp - floor_ < CW8_CAP) push(cc[s]);
+                }
+            }
+        }
+        if (!fin) continue;
+
+        // ---- hit writeback
+        uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
+        V3 n = mk(0.0f, 0.0f, 0.0f);
+        if (bhit) {
+            V3 r0, r1, r2, va, vb, vc;
+            if (LAYOUT == REF) {
+                const uint32_t* r = a.words + 16ull + (uint64_t)binst * 32u;
+                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
+                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
+                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
+                uint32_t pos = r[20], idx = r[21], prim_base = r[28];
+                uint64_t x = (uint64_t)idx + 3ull * btri;
+                tri0 = a.words[x];
+                tri1 = a.words[x + 1];
+                tri2 = a.words[x + 2];
+                const uint32_t* P = a.words + pos;
+                va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
+                vb = mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2]));
+                vc = mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2]));
+                slot = btri;
+                if (!ANY && prim_base != 0u) {
+                    tri0 = a.words[prim_base + (uint64_t)btri];
+                    tri1 = 0;
+                    tri2 = 0;
+                }
+            } else {
+                const uint32_t* r = a.inst + (uint64_t)binst * 32u;
+                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
+                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
+                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
+                const float4* tp = a.tris + 3ull * btri;
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                va = mk(t0.x, t0.y, t0.z);
+                vb = mk(t0.w, t1.x, t1.y);
+                vc = mk(t1.z, t1.w, t2.x);
+                slot = fb(t2.y);
+                const uint32_t* ids = a.tri_ids + 4ull * btri;
+                tri0 = ids[0];
+                tri1 = ids[1];
+                tri2 = ids[2];
+                if (!ANY && (r[20] & 2u)) {
+                    tri0 = fb(t2.z);
+                    tri1 = 0;
+                    tri2 = 0;
+                }
+            }
+            n = hit_normal(r0, r1, r2, va, vb, vc);
+            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!ANY ? 4 : 0)) : (36 + 48 + 12);
+        }
+        if (a.hits) {
+            uint4* hp = reinterpret_cast<uint4*>(a.hits + ray);
+            uint32_t flags = invalid ? 2u : (bhit ? 1u : 0u);
+            hp[0] = make_uint4(fb(bt), bhit ? binst : NONE, tri0, tri1);
+            hp[1] = make_uint4(tri2, fb(bhit ? bu : 0.0f), fb(bhit ? bv : 0.0f), fb(n.x));
+            hp[2] = make_uint4(fb(n.y), fb(n.z), flags, slot);
+        }
+        if (COUNT) nb += (a.hits ? 48 : 0) + (a.vis && !bhit ? 4 : 0);
+        if (a.vis && !bhit) atomicOr(a.vis + (ray >> 5), 1u << (ray & 31u));
+        ray = NONE;
+    }
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off);
+        if (lane == 0) atomicAdd(a.bytes, nb);
+    }
+}
+
+__global__ void fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ----------------------------------------------------------------- raygen
+struct Mat16 {
+    float m[16];
+};
+
+// shaders/primary_rays.wgsl:50-66
+__global__ void primary_kernel(Mat16 inv, uint32_t w, uint32_t h, rq_ray* rays) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t px = i % w, py = i / w;
+    float nx = __builtin_fmaf(((float)px + 0.5f) / (float)w, 2.0f, -1.0f);
+    float ny = __builtin_fmaf(((float)py + 0.5f) / (float)h, -2.0f, 1.0f);
+    float xy[4], A[4], B[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xy[k] = __builtin_fmaf(inv.m[k], nx, inv.m[4 + k] * ny);
+        A[k] = (xy[k] + inv.m[8 + k]) + inv.m[12 + k];
+        B[k] = xy[k] + inv.m[12 + k];
+    }
+    float near[3], far[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        near[k] = A[k] / A[3];
+        far[k] = B[k] / B[3];
+    }
+    float4* rp = reinterpret_cast<float4*>(rays + i);
+    rp[0] = make_float4(near[0], near[1], near[2], 0.0f);
+    rp[1] = make_float4(far[0] - near[0], far[1] - near[1], far[2] - near[2], 1e20f);
+}
+
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// reference_path_tracer.wgsl:149-167, 237-256, 466-470. Returns the valid
+// mask (bit 0 shadow, bit 1+g bounce g) and writes the rays asked for.
+__device__ __forceinline__ uint32_t secondary(const rq_ray* pr, const rq_hit* ph, V3 sun, uint32_t pixel, uint32_t seed,
+                                              uint32_t n_gi, rq_ray* out_shadow, rq_ray* const* out_gi) {
+    const float4* hp = reinterpret_cast<const float4*>(ph);
+    float4 h0 = hp[0], h1 = hp[1], h2 = hp[2];
+    if (!(fb(h2.z) & 1u)) return 0u;
+    float t = h0.x;
+    const float4* rp = reinterpret_cast<const float4*>(pr);
+    float4 r0 = rp[0], r1 = rp[1];
+    V3 o = mk(r0.x, r0.y, r0.z), d = mk(r1.x, r1.y, r1.z);
+    V3 p = mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+    V3 ng = mk(h1.w, h2.x, h2.y);
+    if (dot3(ng, d) > 0.0f) ng = mk(-ng.x, -ng.y, -ng.z);
+    float m = fmx(fmx(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
+    float k = 1e-4f + 4e-7f * m;
+    V3 start = mk(p.x + ng.x * k, p.y + ng.y * k, p.z + ng.z * k);
+    uint32_t valid = dot3(ng, sun) > 0.0f ? 1u : 0u;
+    if (out_shadow && valid) {
+        float4* q = reinterpret_cast<float4*>(out_shadow);
+        q[0] = make_float4(start.x, start.y, start.z, 0.0f);
+        q[1] = make_float4(sun.x, sun.y, sun.z, 1e20f);
+    }
+    float sg = ng.z >= 0.0f ? 1.0f : -1.0f;
+    float aa = -1.0f / (sg + ng.z);
+    float b = ng.x * ng.y * aa;
+    V3 c0 = mk(1.0f + sg * ng.x * ng.x * aa, sg * b, -sg * ng.x);
+    V3 c1 = mk(b, sg + ng.y * ng.y * aa, -ng.y);
+    V3 c2 = ng;
+    uint32_t state = pixel ^ (seed * 0x9E3779B9u);
+    for (uint32_t g = 0; g < n_gi; ++g) {
+        state = pcg(state);
+        float rr = sqrtf((float)(state >> 8u) * (1.0f / 16777216.0f));
+        state = pcg(state);
+        float phi = 6.2831855f * ((float)(state >> 8u) * (1.0f / 16777216.0f));
+        V3 l = mk(rr * cosf(phi), rr * sinf(phi), sqrtf(fmx(0.0f, 1.0f - rr * rr)));
+        V3 wi = mk(c0.x * l.x + c1.x * l.y + c2.x * l.z, c0.y * l.x + c1.y * l.y + c2.y * l.z,
+                   c0.z * l.x + c1.z * l.y + c2.z * l.z);
+        bool ok = dot3(ng, wi) > 0.0f;
+        if (ok) valid |= 1u << (1u + g);
+        if (out_gi && out_gi[g] && ok) {
+            float4* q = reinterpret_cast<float4*>(out_gi[g]);
+            q[0] = make_float4(start.x, start.y, start.z, 0.0f);
+            q[1] = make_float4(wi.x, wi.y, wi.z, 1e20f);
+        }
+    }
+    return valid;
+}
+
+constexpr uint32_t MAX_GI = 4;
+constexpr uint32_t SEC_BLOCK = 256;
+
+struct SecArgs {
+    const rq_ray* primary;
+    const rq_hit* hits;
+    uint32_t count;
+    float sx, sy, sz;
+    uint32_t seed, n_gi;
+    uint32_t* block_counts;  // [cat][block]
+    uint32_t* counts;        // [cat] totals
+    rq_ray* shadow;
+    rq_ray* gi;
+    uint32_t nblocks;
+};
+
+// Block-local exclusive rank of `flag` among the block's threads, and the
+// block total; uses wave ballots and a small LDS table.
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* wave_tot, uint32_t& total) {
+    const uint64_t m = __ballot(flag);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
+    __syncthreads();
+    if (lane == 0) wave_tot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < SEC_BLOCK / 64u; ++w) {
+        if (w < wave) before += wave_tot[w];
+        all += wave_tot[w];
+    }
+    total = all;
+    return before + lane_rank(m);
+}
+
+__global__ __launch_bounds__(SEC_BLOCK) void secondary_count(SecArgs a) {
+    __shared__ uint32_t wave_tot[SEC_BLOCK / 64u];
+    uint32_t i = blockIdx.x * SEC_BLOCK + threadIdx.x;
+    uint32_t valid = 0;
+    if (i < a.count) valid = secondary(a.primary + i, a.hits + i, mk(a.sx, a.sy, a.sz), i, a.seed, a.n_gi, nullptr, nullptr);
+    for (uint32_t c = 0; c <= a.n_gi; ++c) {
+        uint32_t total;
+        block_rank((valid >> c) & 1u, wave_tot, total);
+        if (threadIdx.x == 0) a.block_counts[c * a.nblocks + blockIdx.x] = total;
+    }
+}
+
+// One workgroup per category: exclusive scan of the block counts in place.
+__global__ __launch_bounds__(1024) void secondary_scan(SecArgs a) {
+    __shared__ uint32_t part[1024];
+    uint32_t c = blockIdx.x;
+    uint32_t* bc = a.block_counts + c * a.nblocks;
+    uint32_t per = (a.nblocks + 1023u) / 1024u;
+    uint32_t b0 = threadIdx.x * per, sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < a.nblocks) sum += bc[b0 + k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < a.nblocks) {
+            uint32_t v = bc[b0 + k];
+            bc[b0 + k] = run;
+            run += v;
+        }
+    if (threadIdx.x == 1023) a.counts[c] = part[1023];
+}
+
+__global__ __launch_bounds__(SEC_BLOCK) void secondary_write(SecArgs a) {
+    __shared__ uint32_t wave_tot[SEC_BLOCK / 64u];
+    uint32_t i = blockIdx.x * SEC_BLOCK + threadIdx.x;
+    uint32_t valid = 0;
+    V3 sun = mk(a.sx, a.sy, a.sz);
+    if (i < a.count) valid = secondary(a.primary + i, a.hits + i, sun, i, a.seed, a.n_gi, nullptr, nullptr);
+    rq_ray* dst_shadow = nullptr;
+    rq_ray* dst_gi[MAX_GI] = {nullptr, nullptr, nullptr, nullptr};
+    for (uint32_t c = 0; c <= a.n_gi; ++c) {
+        uint32_t total;
+        uint32_t rank = block_rank((valid >> c) & 1u, wave_tot, total);
+        uint32_t at = a.block_counts[c * a.nblocks + blockIdx.x] + rank;
+        if ((valid >> c) & 1u) {
+            if (c == 0)
+                dst_shadow = a.shadow + at;
+            else
+                dst_gi[c - 1] = a.gi + (uint64_t)(c - 1) * a.count + at;
+        }
+    }
+    if (valid) secondary(a.primary + i, a.hits + i, sun, i, a.seed, a.n_gi, dst_shadow, dst_gi);
+}
+
+// ------------------------------------------------------------ FP64 verify
+struct VerifyArgs {
+    const uint32_t* words;  // REF words, or null
+    const uint32_t* inst;   // PLAIN instance records, or null
+    const float4* tris;     // PLAIN triangles
+    const rq_ray* rays;
+    const rq_hit* hits;
+    uint32_t count;
+    unsigned long long* out;  // [0] max rel t err bits, [1] max bary err bits, [2] checked
+};
+
+__global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
+    __shared__ double st[256], sb[256];
+    __shared__ uint32_t sn[256];
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    double et = 0.0, eb = 0.0;
+    uint32_t n = 0;
+    if (i < a.count) {
+        const rq_hit h = a.hits[i];
+        if (h.flags & 1u) {
+            const rq_ray r = a.rays[i];
+            double ro[3], rd[3], va[3], vb[3], vc[3];
+            const uint32_t* rec = a.inst ? a.inst + (uint64_t)h.inst * 32u : a.words + 16ull + (uint64_t)h.inst * 32u;
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t* w = rec + 4 * k;
+                ro[k] = (double)bf(w[0]) * r.origin[0] + (double)bf(w[1]) * r.origin[1] +
+                        (double)bf(w[2]) * r.origin[2] + (double)bf(w[3]);
+                rd[k] = (double)bf(w[0]) * r.direction[0] + (double)bf(w[1]) * r.direction[1] +
+                        (double)bf(w[2]) * r.direction[2];
+            }
+            if (a.inst) {
+                const float4* tp = a.tris + 3ull * (rec[19] + h.slot);
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                va[0] = t0.x; va[1] = t0.y; va[2] = t0.z;
+                vb[0] = t0.w; vb[1] = t1.x; vb[2] = t1.y;
+                vc[0] = t1.z; vc[1] = t1.w; vc[2] = t2.x;
+            } else {
+                uint32_t pos = rec[20], idx = rec[21];
+                uint64_t x = (uint64_t)idx + 3ull * h.slot;
+                for (int k = 0; k < 3; ++k) {
+                    va[k] = bf(a.words[pos + 3ull * a.words[x] + k]);
+                    vb[k] = bf(a.words[pos + 3ull * a.words[x + 1] + k]);
+                    vc[k] = bf(a.words[pos + 3ull * a.words[x + 2] + k]);
+                }
+            }
+            double ab[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+            double ac[3] = {vc[0] - va[0], vc[1] - va[1], vc[2] - va[2]};
+            double p[3] = {rd[1] * ac[2] - rd[2] * ac[1], rd[2] * ac[0] - rd[0] * ac[2], rd[0] * ac[1] - rd[1] * ac[0]};
+            double det = ab[0] * p[0] + ab[1] * p[1] + ab[2] * p[2];
+            if (det != 0.0) {
+                double s[3] = {ro[0] - va[0], ro[1] - va[1], ro[2] - va[2]};
+                double q[3] = {s[1] * ab[2] - s[2] * ab[1], s[2] * ab[0] - s[0] * ab[2], s[0] * ab[1] - s[1] * ab[0]};
+                double u = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) / det;
+                double v = (rd[0] * q[0] + rd[1] * q[1] + rd[2] * q[2]) / det;
+                double t = (ac[0] * q[0] + ac[1] * q[1] + ac[2] * q[2]) / det;
+                et = fabs((double)h.t - t) / fmax(fabs(t), 1e-300);
+                eb = fmax(fabs((double)h.u - u), fabs((double)h.v - v));
+                n = 1;
+            }
+        }
+    }
+    st[threadIdx.x] = et;
+    sb[threadIdx.x] = eb;
+    sn[threadIdx.x] = n;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + off]);
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + off]);
+            sn[threadIdx.x] += sn[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(a.out, (unsigned long long)__double_as_longlong(st[0]));
+        atomicMax(a.out + 1, (unsigned long long)__double_as_longlong(sb[0]));
+        atomicAdd(a.out + 2, (unsigned long long)sn[0]);
+    }
+}
+
+}  // namespace dev
+
+// ------------------------------------------------------------- host side
+namespace {
+
+struct Tune {
+    uint32_t block, stack, refill_min, grid_per_cu;
+};
+
+struct Ctx {
+    std::mutex mu;
+    int device = -1;
+    int cus = 0;
+    uint32_t* counters = nullptr;  // ring
+    uint32_t counter_next = 0;
+    uint32_t* spill = nullptr;
+    size_t spill_lanes = 0;
+    uint32_t spill_slots = 0;
+    uint32_t spill_next = 0;
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    rq_tuning tuning[3][2];
+    bool tuned = false;
+};
+Ctx g_ctx;
+constexpr uint32_t COUNTER_RING = 4096;
+constexpr uint32_t SPILL_SLOTS = 8;
+
+rq_status hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RQ_ERR_OUT_OF_MEMORY : RQ_ERR_DEVICE;
+}
+#define HIP_TRY(x, what)                         \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+void default_tuning(rq_tuning& t, int layout, int any) {
+    std::memset(&t, 0, sizeof t);
+    t.block = 256;
+    t.stack = 16;
+    t.persistent = 1;
+    t.waves_per_eu = 0;
+    t.grid_per_cu = 0;
+    t.reserved[0] = any ? 16u : 16u;  // refill threshold (lanes)
+    (void)layout;
+}
+
+rq_status ensure_ctx() {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (g_ctx.device == dev) return RQ_OK;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    g_ctx.device = dev;
+    g_ctx.cus = prop.multiProcessorCount;
+    HIP_TRY(hipMalloc(&g_ctx.counters, COUNTER_RING * sizeof(uint32_t)), "hipMalloc counters");
+    HIP_TRY(hipMemset(g_ctx.counters, 0, COUNTER_RING * sizeof(uint32_t)), "hipMemset counters");
+    if (!g_ctx.tuned) {
+        for (int l = 0; l < 3; ++l)
+            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
+        g_ctx.tuned = true;
+    }
+    return RQ_OK;
+}
+
+template <int L, int B, int S, bool ANY, bool C = false>
+void* kernel_ptr() {
+    return reinterpret_cast<void*>(&dev::trace_kernel<L, B, S, ANY, C>);
+}
+
+typedef void (*Launch)(dim3, dim3, hipStream_t, const dev::TraceArgs&);
+template <int L, int B, int S, bool ANY, bool C = false>
+void launch_one(dim3 g, dim3 b, hipStream_t s, const dev::TraceArgs& a) {
+    hipLaunchKernelGGL((dev::trace_kernel<L, B, S, ANY, C>), g, b, 0, s, a);
+}
+
+struct KernelEntry {
+    uint32_t block, stack;
+    Launch fn[3][2];
+    void* ptr[3][2];
+};
+
+template <int B, int S>
+KernelEntry make_entry() {
+    KernelEntry e;
+    e.block = B;
+    e.stack = S;
+    e.fn[0][0] = &launch_one<dev::REF, B, S, false>;
+    e.fn[0][1] = &launch_one<dev::REF, B, S, true>;
+    e.fn[1][0] = &launch_one<dev::PLAIN, B, S, false>;
+    e.fn[1][1] = &launch_one<dev::PLAIN, B, S, true>;
+    e.fn[2][0] = &launch_one<dev::CW8, B, S, false>;
+    e.fn[2][1] = &launch_one<dev::CW8, B, S, true>;
+    e.ptr[0][0] = kernel_ptr<dev::REF, B, S, false>();
+    e.ptr[0][1] = kernel_ptr<dev::REF, B, S, true>();
+    e.ptr[1][0] = kernel_ptr<dev::PLAIN, B, S, false>();
+    e.ptr[1][1] = kernel_ptr<dev::PLAIN, B, S, true>();
+    e.ptr[2][0] = kernel_ptr<dev::CW8, B, S, false>();
+    e.ptr[2][1] = kernel_ptr<dev::CW8, B, S, true>();
+    return e;
+}
+
+const std::vector<KernelEntry>& kernels() {
+    static std::vector<KernelEntry> k = {make_entry<128, 8>(),  make_entry<128, 16>(), make_entry<128, 32>(),
+                                         make_entry<256, 8>(),  make_entry<256, 16>(), make_entry<256, 32>(),
+                                         make_entry<64, 16>(),  make_entry<64, 32>()};
+    return k;
+}
+
+const KernelEntry& count_kernel() {
+    static KernelEntry e = [] {
+        KernelEntry k;
+        k.block = 256;
+        k.stack = 16;
+        k.fn[0][0] = &launch_one<dev::REF, 256, 16, false, true>;
+        k.fn[0][1] = &launch_one<dev::REF, 256, 16, true, true>;
+        k.fn[1][0] = &launch_one<dev::PLAIN, 256, 16, false, true>;
+        k.fn[1][1] = &launch_one<dev::PLAIN, 256, 16, true, true>;
+        k.fn[2][0] = &launch_one<dev::CW8, 256, 16, false, true>;
+        k.fn[2][1] = &launch_one<dev::CW8, 256, 16, true, true>;
+        k.ptr[0][0] = kernel_ptr<dev::REF, 256, 16, false, true>();
+        k.ptr[0][1] = kernel_ptr<dev::REF, 256, 16, true, true>();
+        k.ptr[1][0] = kernel_ptr<dev::PLAIN, 256, 16, false, true>();
+        k.ptr[1][1] = kernel_ptr<dev::PLAIN, 256, 16, true, true>();
+        k.ptr[2][0] = kernel_ptr<dev::CW8, 256, 16, false, true>();
+        k.ptr[2][1] = kernel_ptr<dev::CW8, 256, 16, true, true>();
+        return k;
+    }();
+    return e;
+}
+
+const KernelEntry* find_kernel(uint32_t block, uint32_t stack) {
+    for (const auto& k : kernels())
+        if (k.block == block && k.stack == stack) return &k;
+    return nullptr;
+}
+
+rq_status ensure_spill(size_t lanes) {
+    if (g_ctx.spill && g_ctx.spill_lanes >= lanes) return RQ_OK;
+    if (g_ctx.spill) {
+        HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        HIP_TRY(hipFree(g_ctx.spill), "hipFree spill");
+        g_ctx.spill = nullptr;
+    }
+    size_t bytes = lanes * dev::SPILL_WORDS * sizeof(uint32_t) * SPILL_SLOTS;
+    HIP_TRY(hipMalloc(&g_ctx.spill, bytes), "hipMalloc spill");
+    g_ctx.spill_lanes = lanes;
+    g_ctx.spill_slots = SPILL_SLOTS;
+    return RQ_OK;
+}
+
+rq_status ensure_scratch(size_t bytes) {
+    if (g_ctx.scratch && g_ctx.scratch_bytes >= bytes) return RQ_OK;
+    if (g_ctx.scratch) {
+        HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        HIP_TRY(hipFree(g_ctx.scratch), "hipFree scratch");
+        g_ctx.scratch = nullptr;
+    }
+    HIP_TRY(hipMalloc(&g_ctx.scratch, bytes), "hipMalloc scratch");
+    g_ctx.scratch_bytes = bytes;
+    return RQ_OK;
+}
+
+template <typename T>
+rq_status upload(void* base, uint64_t& at, const std::vector<T>& v) {
+    if (!v.empty()) HIP_TRY(hipMemcpy((char*)base + at, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy scene");
+    at += (v.size() * sizeof(T) + 255) & ~(size_t)255;
+    return RQ_OK;
+}
+template <typename T>
+size_t padded(const std::vector<T>& v) {
+    return (v.size() * sizeof(T) + 255) & ~(size_t)255;
+}
+
+}  // namespace
+}  // namespace rq
+
+using namespace rq;
+
+extern "C" int rq_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
+    if (!scene) {
+        set_error("rq_scene_upload: null scene");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status s = ensure_ctx();
+    if (s != RQ_OK) return s;
+    std::string err;
+    if ((layout_mask & 6u) && scene->plain.inst.empty() && scene->words[0] > 0) {
+        if (!build_plain(scene->words, scene->plain, err)) {
+            set_error("plain relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    if ((layout_mask & 4u) && scene->cw8.inst.empty() && scene->words[0] > 0) {
+        if (!build_cw8(scene->words, scene->plain, scene->cw8, err)) {
+            set_error("cw8 relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    for (int l = 0; l < 3; ++l) {
+        if (!(layout_mask & (1u << l)) || scene->dev[l].ready) continue;
+        DeviceLayout& d = scene->dev[l];
+        if (l == 0) {
+            d.bytes = padded(scene->words);
+            HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
+            uint64_t at = 0;
+            s = upload(d.base, at, scene->words);
+            if (s != RQ_OK) return s;
+        } else {
+            HostLayout& h = l == 1 ? scene->plain : scene->cw8;
+            if (h.inst.empty()) h.inst.assign(32, 0u);
+            if (h.nodes.empty()) h.nodes.assign(32, 0u);
+            if (h.tris.empty()) h.tris.assign(12, 0.0f);
+            if (h.tri_ids.empty()) h.tri_ids.assign(4, 0u);
+            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.tris) + padded(h.tri_ids);
+            HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
+            uint64_t at = 0;
+            d.off[0] = at; if ((s = upload(d.base, at, h.inst)) != RQ_OK) return s;
+            d.off[1] = at; if ((s = upload(d.base, at, h.top)) != RQ_OK) return s;
+            d.off[2] = at; if ((s = upload(d.base, at, h.nodes)) != RQ_OK) return s;
+            d.off[3] = at; if ((s = upload(d.base, at, h.tris)) != RQ_OK) return s;
+            d.off[4] = at; if ((s = upload(d.base, at, h.tri_ids)) != RQ_OK) return s;
+        }
+        d.ready = true;
+    }
+    scene->device = g_ctx.device;
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout > 2) {
+        set_error("rq_scene_device_bytes: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    *bytes = scene->dev[layout].ready ? scene->dev[layout].bytes : 0;
+    return RQ_OK;
+}
+
+extern "C" void rq_scene_destroy(rq_scene* scene) {
+    if (!scene) return;
+    for (auto& d : scene->dev)
+        if (d.base) (void)hipFree(d.base);
+    delete scene;
+}
+
+extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out) {
+    if (!out || (int)layout < 0 || (int)layout > 2) {
+        set_error("rq_get_tuning: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    if (!g_ctx.tuned) {
+        for (int l = 0; l < 3; ++l)
+            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
+        g_ctx.tuned = true;
+    }
+    *out = g_ctx.tuning[layout][mode ? 1 : 0];
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* t) {
+    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->reserved[0] == 0 ||
+        t->reserved[0] > 64) {
+        set_error("rq_set_tuning: unsupported block/stack/refill combination");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    if (!g_ctx.tuned) {
+        for (int l = 0; l < 3; ++l)
+            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
+        g_ctx.tuned = true;
+    }
+    g_ctx.tuning[layout][mode ? 1 : 0] = *t;
+    return RQ_OK;
+}
+
+namespace rq {
+namespace {
+rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
+                       rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes) {
+    if (!scene || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays)) {
+        set_error("rq_trace: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (!scene->dev[layout].ready) {
+        set_error("rq_trace: layout not uploaded (rq_scene_upload)");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status st = ensure_ctx();
+    if (st != RQ_OK) return st;
+    if (scene->device != g_ctx.device) {
+        set_error("rq_trace: scene lives on another device");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (vis_bits && count) {
+        uint64_t words = (count + 31u) / 32u;
+        hipLaunchKernelGGL(dev::fill_u32, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 4096)), dim3(256), 0, s,
+                           vis_bits, 0u, words);
+    }
+    if (count == 0) return RQ_OK;
+    const int any = mode != 0 ? 1 : 0;
+    const rq_tuning& t = g_ctx.tuning[layout][any];
+    const KernelEntry* k = bytes ? &count_kernel() : find_kernel(t.block, t.stack);
+    if (!k) {
+        set_error("rq_trace: no kernel for the tuning");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    int per_cu = (int)t.grid_per_cu;
+    if (per_cu == 0) {
+        int blocks = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k->ptr[layout][any], (int)k->block, 0),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu = std::max(blocks, 1);
+    }
+    uint64_t grid = (uint64_t)g_ctx.cus * per_cu;
+    uint64_t need = ((uint64_t)count + k->block - 1) / k->block;
+    grid = std::max<uint64_t>(1, std::min(grid, need));
+    st = ensure_spill((size_t)(grid * k->block));
+    if (st != RQ_OK) return st;
+    uint32_t slot = g_ctx.counter_next;
+    g_ctx.counter_next = (g_ctx.counter_next + 1) % COUNTER_RING;
+    uint32_t spill_slot = g_ctx.spill_next;
+    g_ctx.spill_next = (g_ctx.spill_next + 1) % g_ctx.spill_slots;
+    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, g_ctx.counters + slot, 0u, (uint64_t)1);
+
+    dev::TraceArgs a;
+    std::memset(&a, 0, sizeof a);
+    const DeviceLayout& d = scene->dev[layout];
+    if (layout == RQ_LAYOUT_REF_BVH4Q8) {
+        a.words = reinterpret_cast<const uint32_t*>(d.base);
+        a.top = a.words + scene->words[1];
+        a.n_inst = scene->words[0];
+        a.top_on = scene->words[1] != 0;
+        a.top_root = scene->words[2];
+    } else {
+        const HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
+        char* b = reinterpret_cast<char*>(d.base);
+        a.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
+        a.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
+        a.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
+        a.tris = reinterpret_cast<const float4*>(b + d.off[3]);
+        a.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
+        a.n_inst = h.instance_count;
+        a.top_on = h.top_on;
+        a.top_root = h.top_root;
+    }
+    a.rays = rays;
+    a.hits = hits;
+    a.vis = vis_bits;
+    a.count = count;
+    a.counter = g_ctx.counters + slot;
+    a.spill = g_ctx.spill + (size_t)spill_slot * g_ctx.spill_lanes * dev::SPILL_WORDS;
+    a.refill_min = t.reserved[0] ? t.reserved[0] : 1u;
+    a.bytes = bytes;
+    k->fn[layout][any](dim3((unsigned)grid), dim3(k->block), s, a);
+    HIP_TRY(hipGetLastError(), "trace launch");
+    return RQ_OK;
+}
+}  // namespace
+}  // namespace rq
+
+extern "C" rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
+                              rq_hit* hits, uint32_t* vis_bits, void* stream) {
+    return launch_trace(scene, layout, rays, count, mode, hits, vis_bits, stream, nullptr);
+}
+
+extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays) ||
+        !scene->dev[layout].ready) {
+        set_error("rq_trace_count_bytes: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    unsigned long long* acc = nullptr;
+    HIP_TRY(hipMalloc(&acc, sizeof(unsigned long long)), "hipMalloc count");
+    HIP_TRY(hipMemset(acc, 0, sizeof(unsigned long long)), "hipMemset count");
+    rq_status st = launch_trace(scene, layout, rays, count, mode, hits, vis_bits, nullptr, acc);
+    unsigned long long host = 0;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&host, acc, sizeof host, hipMemcpyDeviceToHost);
+    (void)hipFree(acc);
+    if (st != RQ_OK) return st;
+    if (e != hipSuccess) return hip_fail(e, "count");
+    *bytes = host;
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,
+                                    double out[3], void* stream) {
+    if (!scene || !out || (count && (!rays || !hits))) {
+        set_error("rq_verify_fp64: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (!scene->dev[1].ready && !scene->dev[0].ready) {
+        set_error("rq_verify_fp64: needs the REF or PLAIN layout uploaded");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    unsigned long long* acc = nullptr;
+    HIP_TRY(hipMalloc(&acc, 3 * sizeof(unsigned long long)), "hipMalloc verify");
+    HIP_TRY(hipMemsetAsync(acc, 0, 3 * sizeof(unsigned long long), s), "hipMemset verify");
+    dev::VerifyArgs a;
+    std::memset(&a, 0, sizeof a);
+    if (scene->dev[1].ready) {
+        char* b = reinterpret_cast<char*>(scene->dev[1].base);
+        a.inst = reinterpret_cast<const uint32_t*>(b + scene->dev[1].off[0]);
+        a.tris = reinterpret_cast<const float4*>(b + scene->dev[1].off[3]);
+    } else {
+        a.words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
+    }
+    a.rays = rays;
+    a.hits = hits;
+    a.count = count;
+    a.out = acc;
+    if (count) hipLaunchKernelGGL(dev::verify_kernel, dim3((count + 255) / 256), dim3(256), 0, s, a);
+    unsigned long long host[3];
+    hipError_t e = hipMemcpyAsync(host, acc, sizeof host, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(acc);
+    if (e != hipSuccess) return hip_fail(e, "verify");
+    double t, b;
+    std::memcpy(&t, &host[0], 8);
+    std::memcpy(&b, &host[1], 8);
+    out[0] = t;
+    out[1] = b;
+    out[2] = (double)host[2];
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_gen_primary(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays,
+                                    void* stream) {
+    if (!inverse || !rays || width == 0 || height == 0) {
+        set_error("rq_gen_primary: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    dev::Mat16 m;
+    std::memcpy(m.m, inverse, sizeof m.m);
+    uint64_t n = (uint64_t)width * height;
+    hipLaunchKernelGGL(dev::primary_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), m, width, height, rays);
+    HIP_TRY(hipGetLastError(), "primary launch");
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t count, const float sun[3],
+                                      uint32_t seed, uint32_t n_gi, rq_ray* shadow, rq_ray* gi, uint32_t* counts,
+                                      void* stream) {
+    if (!primary || !hits || !sun || !counts || n_gi > dev::MAX_GI || !shadow || (n_gi && !gi)) {
+        set_error("rq_gen_secondary: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (count == 0) return RQ_OK;
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status st = ensure_ctx();
+    if (st != RQ_OK) return st;
+    uint32_t nblocks = (count + dev::SEC_BLOCK - 1) / dev::SEC_BLOCK;
+    st = ensure_scratch((size_t)(n_gi + 1) * nblocks * sizeof(uint32_t));
+    if (st != RQ_OK) return st;
+    dev::SecArgs a;
+    a.primary = primary;
+    a.hits = hits;
+    a.count = count;
+    a.sx = sun[0];
+    a.sy = sun[1];
+    a.sz = sun[2];
+    a.seed = seed;
+    a.n_gi = n_gi;
+    a.block_counts = reinterpret_cast<uint32_t*>(g_ctx.scratch);
+    a.counts = counts;
+    a.shadow = shadow;
+    a.gi = gi;
+    a.nblocks = nblocks;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(dev::secondary_count, dim3(nblocks), dim3(dev::SEC_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(dev::secondary_scan, dim3(n_gi + 1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(dev::secondary_write, dim3(nblocks), dim3(dev::SEC_BLOCK), 0, s, a);
+    HIP_TRY(hipGetLastError(), "secondary launch");
+    return RQ_OK;
+}
+
+// ---- device helpers
+extern "C" rq_status rq_malloc(size_t bytes, void** ptr) {
+    if (!ptr) return RQ_ERR_INVALID_INPUT;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+    return RQ_OK;
+}
+extern "C" rq_status rq_free(void* ptr) {
+    HIP_TRY(hipFree(ptr), "hipFree");
+    return RQ_OK;
+}
+extern "C" rq_status rq_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy h2d");
+    return RQ_OK;
+}
+extern "C" rq_status rq_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy d2h");
+    return RQ_OK;
+}
+extern "C" rq_status rq_memset(void* dst, int value, size_t bytes) {
+    HIP_TRY(hipMemset(dst, value, bytes), "hipMemset");
+    return RQ_OK;
+}
+extern "C" rq_status rq_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return RQ_OK;
+}
+extern "C" rq_status rq_stream_create(void** stream) {
+    if (!stream) return RQ_ERR_INVALID_INPUT;
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    *stream = s;
+    return RQ_OK;
+}
+extern "C" rq_status rq_stream_destroy(void* stream) {
+    HIP_TRY(hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)), "hipStreamDestroy");
+    return RQ_OK;
+}
+extern "C" rq_status rq_stream_synchronize(void* stream) {
+    HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_create(void** event) {
+    if (!event) return RQ_ERR_INVALID_INPUT;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    *event = e;
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_destroy(void* event) {
+    HIP_TRY(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)), "hipEventDestroy");
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_record(void* event, void* stream) {
+    HIP_TRY(hipEventRecord(reinterpret_cast<hipEvent_t>(event), reinterpret_cast<hipStream_t>(stream)),
+            "hipEventRecord");
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_synchronize(void* event) {
+    HIP_TRY(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_elapsed_ms(void* start, void* stop, float* ms) {
+    HIP_TRY(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop)),
+            "hipEventElapsedTime");
+    return RQ_OK;
+}
+extern "C" rq_status rq_event_query(void* event) {
+    hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(event));
+    if (e == hipSuccess) return RQ_OK;
+    if (e == hipErrorNotReady) return RQ_ERR_NOT_READY;
+    return hip_fail(e, "hipEventQuery");
+}

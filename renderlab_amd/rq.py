@@ -1,0 +1,518 @@
+"""ctypes mirror of the rq C ABI (include/rq.h).
+
+Names follow the reference's host and device interface for the software ray
+query: ``build_bvh`` / ``build_bvh_sah`` / ``refit_bvh``
+(crates/asset/src/near_field.rs:172-190, 445), ``instance_bvh.build`` /
+``refit`` / ``place`` / ``reserve`` (src/instance_bvh.rs:36-412), the packed
+buffer (src/gi_bvh.rs:151-274), and the WGSL seam ``gi_closest`` /
+``gi_visible`` / ``gi_visible_within`` (shaders/gi_rays_software.wgsl:280-291)
+as batch calls over device buffers. Errors raise ``InvalidInput`` where the
+reference returns ``Error::InvalidInput``.
+
+The compute path is librq.so only; importing this module fails loudly when the
+library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librq.so")
+
+RQ_OK = 0
+RQ_ERR_INVALID_INPUT = 1
+RQ_ERR_OUT_OF_MEMORY = 2
+RQ_ERR_DEVICE = 3
+RQ_ERR_UNSUPPORTED = 4
+RQ_ERR_BUDGET = 5
+RQ_ERR_NOT_READY = 6
+
+REF_BVH4Q8 = 0
+PLAIN_F32 = 1
+CW8_Q8 = 2
+LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8}
+
+CLOSEST = 0
+ANY = 1
+ANY_OPAQUE = 2
+
+LEAF_BIT = 0x80000000
+EMPTY = 0xFFFFFFFF
+HEADER_WORDS = 16
+INSTANCE_WORDS = 32
+
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("inst", "<u4"), ("tri", "<u4", 3), ("u", "<f4"), ("v", "<f4"),
+                      ("normal", "<f4", 3), ("flags", "<u4"), ("slot", "<u4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 48
+
+
+class RqError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+class InvalidInput(RqError):
+    pass
+
+
+class BudgetExceeded(RqError):
+    pass
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise ImportError(f"{_LIB_PATH} is not built: run `python -m renderlab_amd.build` "
+                          "(the ray-query path has no fallback)")
+    return ctypes.CDLL(_LIB_PATH)
+
+
+lib = _load()
+_P = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("rq_last_error", ctypes.c_char_p)
+_sig("rq_device_count", _i)
+_sig("rq_bvh_build", _i, _P, _u32, _P, _u32, _u32, ctypes.POINTER(_P))
+_sig("rq_bvh_refit", _i, _P, _P, _u32)
+_sig("rq_bvh_info", _i, _P, _P)
+_sig("rq_bvh_records", _i, _P, _P, _P, _P, _P)
+_sig("rq_bvh_destroy", None, _P)
+_sig("rq_instance_bvh_reserve", _u32, _u32)
+_sig("rq_instance_bvh_build", _i, _P, _u32, _u32, _P, _P)
+_sig("rq_instance_bvh_refit", _i, _P, _u32, _P, _u32)
+_sig("rq_instance_bvh_place", _i, _P, _u32, _u32, _u32, _u32, _P, _u32, _i, _P)
+_sig("rq_scene_build", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
+_sig("rq_scene_from_gi_words", _i, _P, _sz, ctypes.POINTER(_P))
+_sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
+_sig("rq_scene_stats", _i, _P, _P)
+_sig("rq_scene_upload", _i, _P, _u32)
+_sig("rq_scene_device_bytes", _i, _P, _i, ctypes.POINTER(_sz))
+_sig("rq_scene_destroy", None, _P)
+_sig("rq_trace", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
+_sig("rq_trace_count_bytes", _i, _P, _i, _P, _u32, _u32, _P, _P, ctypes.POINTER(_u64))
+_sig("rq_get_tuning", _i, _i, _u32, _P)
+_sig("rq_set_tuning", _i, _i, _u32, _P)
+_sig("rq_verify_fp64", _i, _P, _P, _P, _u32, _P, _P)
+_sig("rq_gen_primary", _i, _P, _u32, _u32, _P, _P)
+_sig("rq_gen_secondary", _i, _P, _P, _u32, _P, _u32, _u32, _P, _P, _P, _P)
+_sig("rq_malloc", _i, _sz, ctypes.POINTER(_P))
+_sig("rq_free", _i, _P)
+_sig("rq_memcpy_h2d", _i, _P, _P, _sz)
+_sig("rq_memcpy_d2h", _i, _P, _P, _sz)
+_sig("rq_memset", _i, _P, _i, _sz)
+_sig("rq_synchronize", _i)
+_sig("rq_stream_create", _i, ctypes.POINTER(_P))
+_sig("rq_stream_destroy", _i, _P)
+_sig("rq_stream_synchronize", _i, _P)
+_sig("rq_event_create", _i, ctypes.POINTER(_P))
+_sig("rq_event_destroy", _i, _P)
+_sig("rq_event_record", _i, _P, _P)
+_sig("rq_event_synchronize", _i, _P)
+_sig("rq_event_elapsed_ms", _i, _P, _P, ctypes.POINTER(ctypes.c_float))
+_sig("rq_event_query", _i, _P)
+_sig("rq_gen_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
+_sig("rq_gen_camera", _i, _u32, _u64, _u32, _u32, _P)
+
+
+def check(status):
+    if status == RQ_OK:
+        return
+    msg = lib.rq_last_error().decode(errors="replace")
+    if status == RQ_ERR_INVALID_INPUT:
+        raise InvalidInput(status, msg)
+    if status == RQ_ERR_BUDGET:
+        raise BudgetExceeded(status, msg)
+    raise RqError(status, msg)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+class RqTuning(ctypes.Structure):
+    _fields_ = [("block", _u32), ("stack", _u32), ("persistent", _u32), ("waves_per_eu", _u32),
+                ("grid_per_cu", _u32), ("reserved", _u32 * 3)]
+
+
+def device_count():
+    return lib.rq_device_count()
+
+
+# ----------------------------------------------------------------- device
+class DeviceBuffer:
+    """A device allocation owned from Python (rq_malloc / rq_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = _P()
+        check(lib.rq_malloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib.rq_memcpy_h2d(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.rq_memcpy_h2d(self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dtype)
+        if out.nbytes:
+            check(lib.rq_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib.rq_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            check(lib.rq_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.rq_free(self.ptr)
+        except Exception:
+            pass
+
+
+def synchronize():
+    check(lib.rq_synchronize())
+
+
+class Stream:
+    def __init__(self):
+        p = _P()
+        check(lib.rq_stream_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def synchronize(self):
+        check(lib.rq_stream_synchronize(self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.rq_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = _P()
+        check(lib.rq_event_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def record(self, stream=None):
+        check(lib.rq_event_record(self.ptr, stream.ptr if stream else None))
+
+    def synchronize(self):
+        check(lib.rq_event_synchronize(self.ptr))
+
+    def done(self):
+        s = lib.rq_event_query(self.ptr)
+        if s == RQ_OK:
+            return True
+        if s == RQ_ERR_NOT_READY:
+            return False
+        check(s)
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        check(lib.rq_event_elapsed_ms(self.ptr, end.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.rq_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ BLAS build
+class Bvh:
+    """A runtime BVH4 (crates/asset/src/cluster_bvh.rs Bvh, one level)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.rq_bvh_destroy(self._h)
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(6, np.uint32)
+        check(lib.rq_bvh_info(self._h, _ptr(out)))
+        return dict(nodes=int(out[0]), leaves=int(out[1]), vertices=int(out[2]), indices=int(out[3]),
+                    root=int(out[4]), depth=int(out[5]))
+
+    def records(self):
+        i = self.info()
+        nodes = np.zeros(i["nodes"] * 16, np.uint32)
+        leaves = np.zeros(i["leaves"] * 2, np.uint32)
+        positions = np.zeros(i["vertices"] * 3, np.float32)
+        indices = np.zeros(i["indices"], np.uint32)
+        check(lib.rq_bvh_records(self._h, _ptr(nodes), _ptr(leaves), _ptr(positions), _ptr(indices)))
+        return dict(nodes=nodes.reshape(-1, 16), leaves=leaves.reshape(-1, 2), positions=positions.reshape(-1, 3),
+                    indices=indices, root=i["root"], depth=i["depth"])
+
+    @property
+    def root(self):
+        return self.info()["root"]
+
+
+def _build(positions, indices, sah):
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    i = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    h = _P()
+    check(lib.rq_bvh_build(_ptr(p), p.size // 3, _ptr(i), i.size, 1 if sah else 0, ctypes.byref(h)))
+    return Bvh(h)
+
+
+def build_bvh(positions, indices):
+    """Median-split BVH4 (near_field::build_bvh)."""
+    return _build(positions, indices, False)
+
+
+def build_bvh_sah(positions, indices):
+    """Binned-SAH BVH4 with median fallback past depth 15 (near_field::build_bvh_sah)."""
+    return _build(positions, indices, True)
+
+
+def refit_bvh(bvh, positions):
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    check(lib.rq_bvh_refit(bvh._h, _ptr(p), p.size // 3))
+
+
+# ------------------------------------------------------------- top level
+class instance_bvh:  # noqa: N801 - mirrors the reference module name
+    LEAF = LEAF_BIT
+    EMPTY = EMPTY
+    WIDE_WORDS = 32
+    STACK = 48
+    MIN_INSTANCES = 40
+    REFITS = 64
+
+    @staticmethod
+    def reserve(n):
+        return int(lib.rq_instance_bvh_reserve(int(n)))
+
+    @staticmethod
+    def build(boxes, sah=True):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        out = np.zeros(max(instance_bvh.reserve(len(b)), 32), np.uint32)
+        n = _u32()
+        check(lib.rq_instance_bvh_build(_ptr(b), len(b), 1 if sah else 0, _ptr(out), ctypes.byref(n)))
+        return out[: n.value].copy()
+
+    @staticmethod
+    def refit(nodes, boxes):
+        b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+        assert nodes.dtype == np.uint32 and nodes.flags.c_contiguous
+        check(lib.rq_instance_bvh_refit(_ptr(nodes), nodes.size, _ptr(b), len(b)))
+
+    @staticmethod
+    def place(words, header, stride, slots, previous=None, force=-1):
+        w = np.ascontiguousarray(words, np.uint32)
+        out = np.zeros(w.size + instance_bvh.reserve(slots), np.uint32)
+        prev = None if previous is None else np.ascontiguousarray(previous, np.uint32)
+        check(lib.rq_instance_bvh_place(_ptr(w), w.size, header, stride, slots, _ptr(prev),
+                                        0 if prev is None else prev.size, int(force), _ptr(out)))
+        return out
+
+
+# ----------------------------------------------------------------- scene
+class _MeshDesc(ctypes.Structure):
+    _fields_ = [("positions", _P), ("vertex_count", _u32), ("indices", _P), ("index_count", _u32),
+                ("keep_primitives", _u32)]
+
+
+class _InstDesc(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_float * 16), ("mesh", _u32), ("flags", _u32), ("identity", _u32),
+                ("placement", _u32)]
+
+
+class Scene:
+    """The packed gi_geometry buffer plus its device layouts."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def build(cls, meshes, instances, top_level=-1):
+        """meshes: [(positions (V,3), indices (3T,), keep_primitives)], instances:
+        [dict(model=4x4 row-major or 16 column-major, mesh=i, flags=0, identity=0, placement=i)]."""
+        keep = []
+        md = (_MeshDesc * max(len(meshes), 1))()
+        for k, m in enumerate(meshes):
+            p = np.ascontiguousarray(m[0], np.float32).reshape(-1)
+            i = np.ascontiguousarray(m[1], np.uint32).reshape(-1)
+            keep += [p, i]
+            md[k].positions = p.ctypes.data
+            md[k].vertex_count = p.size // 3
+            md[k].indices = i.ctypes.data
+            md[k].index_count = i.size
+            md[k].keep_primitives = int(m[2]) if len(m) > 2 else 0
+        idesc = (_InstDesc * max(len(instances), 1))()
+        for k, ins in enumerate(instances):
+            model = np.asarray(ins["model"], np.float32)
+            if model.shape == (4, 4):
+                model = model.T.reshape(-1)  # row-major matrix -> column-major words
+            idesc[k].model[:] = [float(x) for x in model.reshape(-1)]
+            idesc[k].mesh = int(ins["mesh"])
+            idesc[k].flags = int(ins.get("flags", 0))
+            idesc[k].identity = int(ins.get("identity", 0))
+            idesc[k].placement = int(ins.get("placement", k))
+        h = _P()
+        check(lib.rq_scene_build(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_gi_words(cls, words):
+        w = np.ascontiguousarray(words, np.uint32)
+        h = _P()
+        check(lib.rq_scene_from_gi_words(_ptr(w), w.size, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def generate(cls, kind, seed, meshes=0, triangles=0):
+        h = _P()
+        check(lib.rq_gen_scene(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h)))
+        return cls(h)
+
+    @property
+    def words(self):
+        p = _P()
+        n = _sz()
+        check(lib.rq_scene_gi_words(self._h, ctypes.byref(p), ctypes.byref(n)))
+        buf = (ctypes.c_uint32 * n.value).from_address(p.value)
+        return np.frombuffer(buf, np.uint32).copy()
+
+    def stats(self):
+        s = np.zeros(8, np.uint64)
+        check(lib.rq_scene_stats(self._h, _ptr(s)))
+        return dict(instances=int(s[0]), meshes=int(s[1]), triangles=int(s[2]), unique_triangles=int(s[3]),
+                    blas_nodes=int(s[4]), top_root=int(s[5]), max_depth=int(s[6]))
+
+    def upload(self, layouts=("ref", "plain", "cw8")):
+        mask = 0
+        for l in layouts:
+            mask |= 1 << (LAYOUTS[l] if isinstance(l, str) else int(l))
+        check(lib.rq_scene_upload(self._h, mask))
+
+    def device_bytes(self, layout):
+        n = _sz()
+        check(lib.rq_scene_device_bytes(self._h, LAYOUTS.get(layout, layout), ctypes.byref(n)))
+        return n.value
+
+    def trace(self, layout, rays, count, mode, hits=None, vis=None, stream=None):
+        """rays/hits/vis: DeviceBuffer (or raw device pointers)."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
+        check(lib.rq_trace(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits), p(vis),
+                           stream.ptr if stream else None))
+
+    def count_bytes(self, layout, rays, count, mode, hits=None, vis=None):
+        """Global bytes one traversal launch over these rays moves (synchronous)."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
+        n = _u64()
+        own_hits = None
+        if hits is None and vis is None:
+            if int(mode) == CLOSEST:
+                own_hits = DeviceBuffer(max(int(count), 1) * HIT_DTYPE.itemsize)
+            else:
+                own_hits = DeviceBuffer(max((int(count) + 31) // 32, 1) * 4)
+        h = own_hits if (own_hits is not None and int(mode) == CLOSEST) else hits
+        v = own_hits if (own_hits is not None and int(mode) != CLOSEST) else vis
+        check(lib.rq_trace_count_bytes(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(h),
+                                       p(v), ctypes.byref(n)))
+        return n.value
+
+    def verify_fp64(self, rays, hits, count, stream=None):
+        out = np.zeros(3, np.float64)
+        check(lib.rq_verify_fp64(self._h, rays.ptr, hits.ptr, int(count), _ptr(out), stream.ptr if stream else None))
+        return dict(max_t_rel=float(out[0]), max_bary_abs=float(out[1]), checked=int(out[2]))
+
+    def destroy(self):
+        if self._h:
+            lib.rq_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def get_tuning(layout, mode):
+    t = RqTuning()
+    check(lib.rq_get_tuning(LAYOUTS.get(layout, layout), int(mode), ctypes.byref(t)))
+    return t
+
+
+def set_tuning(layout, mode, block, stack, refill, grid_per_cu=0):
+    t = RqTuning()
+    t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
+    t.reserved[0] = refill
+    check(lib.rq_set_tuning(LAYOUTS.get(layout, layout), int(mode), ctypes.byref(t)))
+
+
+def camera(kind, seed, width, height):
+    inv = np.zeros(16, np.float32)
+    check(lib.rq_gen_camera(int(kind), int(seed), int(width), int(height), _ptr(inv)))
+    return inv
+
+
+def gen_primary(inverse, width, height, rays, stream=None):
+    inv = np.ascontiguousarray(inverse, np.float32)
+    check(lib.rq_gen_primary(_ptr(inv), int(width), int(height), rays.ptr, stream.ptr if stream else None))
+
+
+def gen_secondary(primary, hits, count, sun, seed, n_gi, shadow, gi, counts, stream=None):
+    s = np.ascontiguousarray(sun, np.float32)
+    check(lib.rq_gen_secondary(primary.ptr, hits.ptr, int(count), _ptr(s), int(seed), int(n_gi), shadow.ptr,
+                               gi.ptr if gi is not None else None, counts.ptr, stream.ptr if stream else None))
+
+
+# ---- the WGSL seam, batched (shaders/gi_rays_software.wgsl:280-291)
+def gi_closest(scene, rays, count, hits, layout="plain", stream=None):
+    scene.trace(layout, rays, count, CLOSEST, hits=hits, stream=stream)
+
+
+def gi_visible(scene, rays, count, vis_bits, layout="plain", stream=None):
+    """Sky visibility (mode 1); rays carry tmin 0 and tmax 1e20 as the reference's gi_visible."""
+    scene.trace(layout, rays, count, ANY, vis=vis_bits, stream=stream)
+
+
+def gi_visible_within(scene, rays, count, vis_bits, layout="plain", stream=None):
+    """Visibility within each ray's tmax, opaque casters only (mode 2)."""
+    scene.trace(layout, rays, count, ANY_OPAQUE, vis=vis_bits, stream=stream)

@@ -1,0 +1,68 @@
+"""Shared helpers: run a workload through the C ABI on the GPU and through the
+oracle on the same rays, and compare the hit records."""
+import numpy as np
+
+import oracle
+from renderlab_amd import rq
+
+SUN = np.array([0.3, 0.8, 0.2], np.float32) / np.float32(np.linalg.norm([0.3, 0.8, 0.2]))
+
+
+def to_device(a):
+    return rq.DeviceBuffer.from_array(a)
+
+
+def gpu_trace(scene, layout, rays, mode):
+    n = len(rays)
+    dr = to_device(rays)
+    dh = rq.DeviceBuffer(max(n, 1) * rq.HIT_DTYPE.itemsize)
+    dv = rq.DeviceBuffer(max((n + 31) // 32, 1) * 4)
+    scene.trace(layout, dr, n, mode, hits=dh, vis=dv)
+    rq.synchronize()
+    return dh.download(rq.HIT_DTYPE, n), dv.download(np.uint32, (n + 31) // 32)
+
+
+def gpu_workload(scene, kind, seed, width, height, n_gi, layout="plain"):
+    """Primary rays, closest hits, then shadow + GI rays, all generated on the GPU."""
+    n = width * height
+    inv = rq.camera(kind, seed, width, height)
+    prim = rq.DeviceBuffer(n * 32)
+    rq.gen_primary(inv, width, height, prim)
+    hits = rq.DeviceBuffer(n * 48)
+    scene.trace(layout, prim, n, rq.CLOSEST, hits=hits)
+    shadow = rq.DeviceBuffer(n * 32)
+    gi = rq.DeviceBuffer(max(n_gi, 1) * n * 32)
+    counts = rq.DeviceBuffer(4 * (n_gi + 1))
+    rq.gen_secondary(prim, hits, n, SUN, seed, n_gi, shadow, gi, counts)
+    rq.synchronize()
+    c = counts.download(np.uint32)
+    p = prim.download(rq.RAY_DTYPE, n)
+    h = hits.download(rq.HIT_DTYPE, n)
+    s = shadow.download(rq.RAY_DTYPE, int(c[0]))
+    g = gi.download(rq.RAY_DTYPE, n_gi * n).reshape(max(n_gi, 1), n)
+    gis = [g[b, : int(c[1 + b])] for b in range(n_gi)]
+    return inv, p, h, s, gis
+
+
+def same_bits(a, b):
+    return a.view(np.uint8).reshape(len(a), -1) == b.view(np.uint8).reshape(len(b), -1)
+
+
+def compare_exact(g, o):
+    """Indices of rays whose 48-byte hit records differ."""
+    eq = same_bits(g, o).all(axis=1)
+    return np.nonzero(~eq)[0]
+
+
+def classify_mismatch(g, o, rel=1e-5):
+    """Rays where the hit triangle differs and the two t values are not a tie
+    within `rel`: genuine disagreements."""
+    hit_g = (g["flags"] & 1) == 1
+    hit_o = (o["flags"] & 1) == 1
+    diff = (hit_g != hit_o) | (hit_o & ((g["inst"] != o["inst"]) | (g["slot"] != o["slot"])))
+    tie = hit_g & hit_o & (np.abs(g["t"].astype(np.float64) - o["t"]) <= rel * np.abs(o["t"].astype(np.float64)))
+    return np.nonzero(diff & ~tie)[0], np.nonzero(diff & tie)[0]
+
+
+def vis_bits(vis, n):
+    return ((vis[np.arange(n) // 32] >> (np.arange(n) % 32)) & 1).astype(bool)

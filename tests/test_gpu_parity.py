@@ -1,0 +1,104 @@
+"""GPU parity: every layout's HIP traversal against the oracle on the same rays.
+
+REF and PLAIN must reproduce the oracle's 48-byte hit records bit for bit
+(same boxes, same visit order, same arithmetic). CW8 visits children in a
+different order over different (outward-quantized) boxes, so its hit
+triangle may differ only on an equal-distance tie (|dt|/t <= 1e-5), with t
+and barycentrics within 1e-5 relative and identical visibility bits.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import classify_mismatch, compare_exact, gpu_trace, gpu_workload, vis_bits
+from renderlab_amd import rq
+
+pytestmark = pytest.mark.gpu
+
+W, H = 240, 160
+
+
+@pytest.fixture(scope="module")
+def scene_c0():
+    s = rq.Scene.generate(0, 1)
+    s.upload(("ref", "plain", "cw8"))
+    return s
+
+
+@pytest.fixture(scope="module")
+def workload_c0(scene_c0):
+    print(f"workload: C0 seed 1 {W}x{H} P+S+GI", flush=True)
+    return gpu_workload(scene_c0, 0, 1, W, H, 1, layout="ref")
+
+
+def test_primary_rays_match_oracle_bits(workload_c0):
+    inv, prim, _, _, _ = workload_c0
+    want = oracle.primary_rays(inv, W, H)
+    assert (prim.view(np.uint32) == want.view(np.uint32)).all()
+
+
+@pytest.mark.parametrize("layout", ["ref", "plain"])
+def test_closest_bit_exact(scene_c0, workload_c0, layout):
+    _, prim, _, _, gis = workload_c0
+    words = scene_c0.words
+    for name, rays in (("primary", prim), ("gi", gis[0])):
+        print(f"closest {layout} {name} n={len(rays)}", flush=True)
+        g, _ = gpu_trace(scene_c0, layout, rays, rq.CLOSEST)
+        o, _, _ = oracle.traverse(words, rays, 0)
+        bad = compare_exact(g, o)
+        assert bad.size == 0, f"{layout}/{name}: {bad.size} records differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("layout", ["ref", "plain", "cw8"])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_visibility_bits(scene_c0, workload_c0, layout, mode):
+    _, _, _, shadow, _ = workload_c0
+    words = scene_c0.words
+    print(f"any-hit {layout} mode {mode} n={len(shadow)}", flush=True)
+    _, gv = gpu_trace(scene_c0, layout, shadow, mode)
+    _, ov, _ = oracle.traverse(words, shadow, mode, want_hits=False, want_vis=True)
+    assert (gv == ov).all()
+
+
+def test_cw8_closest_within_tolerance(scene_c0, workload_c0):
+    _, prim, _, _, gis = workload_c0
+    words = scene_c0.words
+    for name, rays in (("primary", prim), ("gi", gis[0])):
+        print(f"closest cw8 {name} n={len(rays)}", flush=True)
+        g, _ = gpu_trace(scene_c0, "cw8", rays, rq.CLOSEST)
+        o, _, _ = oracle.traverse(words, rays, 0)
+        bad, ties = classify_mismatch(g, o)
+        assert bad.size == 0, f"cw8/{name}: {bad.size} non-tie mismatches, first {bad[:5]}"
+        both = ((g["flags"] & 1) == 1) & ((o["flags"] & 1) == 1)
+        rel = np.abs(g["t"][both].astype(np.float64) - o["t"][both]) / np.abs(o["t"][both])
+        assert rel.max(initial=0.0) <= 1e-5
+        same = both & (g["inst"] == o["inst"]) & (g["slot"] == o["slot"])
+        assert (g["u"][same] == o["u"][same]).all() and (g["v"][same] == o["v"][same]).all()
+
+
+def test_secondary_rays_match_oracle(workload_c0):
+    inv, prim, hits, shadow, gis = workload_c0
+    masks, rays = oracle.secondary_rays(prim, hits, np.array([0.3, 0.8, 0.2], np.float32) /
+                                        np.float32(np.linalg.norm([0.3, 0.8, 0.2])), 1, 1)
+    want_s = rays[(masks & 1) == 1, 0]
+    want_g = rays[(masks & 2) == 2, 1]
+    assert len(shadow) == len(want_s) and len(gis[0]) == len(want_g)
+    assert (shadow.view(np.uint32) == want_s.view(np.uint32)).all()
+    # GI directions go through cosf/sinf: device and host libm may differ in the last bit.
+    assert np.allclose(gis[0]["direction"], want_g["direction"], rtol=0, atol=2e-6)
+    assert (gis[0]["origin"] == want_g["origin"]).all()
+
+
+def test_fp64_verification(scene_c0, workload_c0):
+    _, prim, _, _, _ = workload_c0
+    dr = rq.DeviceBuffer.from_array(prim)
+    dh = rq.DeviceBuffer(len(prim) * 48)
+    scene_c0.trace("plain", dr, len(prim), rq.CLOSEST, hits=dh)
+    v = scene_c0.verify_fp64(dr, dh, len(prim))
+    host = dh.download(rq.HIT_DTYPE, len(prim))
+    ref = oracle.reintersect_f64(scene_c0.words, prim, host)
+    hit = (host["flags"] & 1) == 1
+    rel = np.abs(host["t"][hit] - ref["t"][hit]) / np.abs(ref["t"][hit])
+    assert v["checked"] == hit.sum()
+    assert abs(v["max_t_rel"] - rel.max()) <= 1e-12 + 1e-9 * rel.max()
+    assert v["max_t_rel"] < 1e-5
