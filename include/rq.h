@@ -162,6 +162,31 @@ void rq_scene_destroy(rq_scene* scene);
 rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                    uint32_t mode, rq_hit* hits, uint32_t* vis_bits, void* stream);
 
+/* rq_trace with the ray count read on the device at launch time:
+   min(*count_dev, max_count) rays (count_dev: a device word written by an
+   earlier launch on the same stream, such as rq_gen_secondary's counts), so a
+   frame's secondary traces are enqueued without a host round trip. */
+rq_status rq_trace_indirect(rq_scene* scene, rq_layout layout, const rq_ray* rays,
+                            const uint32_t* count_dev, uint32_t max_count, uint32_t mode, rq_hit* hits,
+                            uint32_t* vis_bits, void* stream);
+
+/* One ray batch of a multi-scene launch. */
+typedef struct rq_batch {
+    uint32_t scene;  /* index into the scenes array */
+    uint32_t mode;   /* rq_mode */
+    uint32_t count;
+    uint32_t pad;
+    const rq_ray* rays;  /* device */
+    rq_hit* hits;        /* device, may be NULL */
+    uint32_t* vis_bits;  /* device, may be NULL */
+} rq_batch;
+/* Every batch of every scene in ONE persistent launch: lanes take rays from
+   the concatenated batches and traverse their batch's scene in its mode. All
+   scenes must have `layout` uploaded. No counterpart in the reference (its
+   callers dispatch per pass); results equal per-batch rq_trace calls. */
+rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
+                           const rq_batch* batches, uint32_t batch_count, void* stream);
+
 /* The same traversal with a tally of the global bytes it moves (rays,
    records, nodes, triangles, stack spills, hit records): the algorithmic
    bytes of one launch for the roofline. Synchronous; not for timing. */
@@ -179,6 +204,9 @@ typedef struct rq_tuning {
 } rq_tuning;
 rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out);
 rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* tuning);
+/* The same knobs for rq_trace_batches launches. */
+rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out);
+rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* tuning);
 
 /* ---- FP64 verification: the only FP64 on the GPU. For each hit ray,
    re-intersects its reported triangle in f64 (object-space ray computed in

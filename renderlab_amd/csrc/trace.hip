@@ -114,20 +114,37 @@ __device__ __forceinline__ bool ray_invalid(float4 a, float4 b) {
     return bad;
 }
 
-struct TraceArgs {
+struct SceneView {
     const uint32_t* words;  // REF: the packed words
     const uint32_t* inst;   // PLAIN/CW8 instance records (32 words)
     const uint32_t* top;    // top-level wide nodes
     const uint32_t* nodes;  // PLAIN 32 words, CW8 20 words a node
     const float4* tris;     // 3 float4 a triangle
     const uint32_t* tri_ids;
+    uint32_t n_inst, top_on, top_root, pad;
+};
+
+struct BatchDev {
+    SceneView sv;
+    const rq_ray* rays;
+    rq_hit* hits;
+    uint32_t* vis;
+    uint32_t mode, count;
+};
+
+struct TraceArgs {
+    SceneView sv;
     const rq_ray* rays;
     rq_hit* hits;
     uint32_t* vis;
     uint32_t* counter;
     uint32_t* spill;
     unsigned long long* bytes;  // COUNT instantiations: global bytes moved
-    uint32_t count, n_inst, top_on, top_root, refill_min, pad;
+    const uint32_t* count_ptr;  // when set, the ray count is min(*count_ptr, count)
+    const BatchDev* batches;    // BATCHED: the batch table
+    const uint32_t* prefix;     // BATCHED: first global ray of each batch
+    uint32_t n_batches;
+    uint32_t count, refill_min, pad;
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -152,7 +169,10 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // COUNT: the same traversal, also tallying the bytes every global load and
 // store moves (rays, records, nodes, triangles, spills, hit records); used
 // once, untimed, to give the roofline its algorithmic bytes per ray.
-template <int LAYOUT, int BLOCK, int S, bool ANY, bool COUNT = false>
+// AM: 0 closest hit, 1 any hit, 2 per batch (BATCHED launches mix modes).
+// BATCHED: rays come from a table of batches over several scenes; a lane
+// takes its batch's scene, mode and output buffers when it fetches a ray.
+template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
     __shared__ uint32_t lds[S * BLOCK];
@@ -171,6 +191,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t m_nodes = 0, m_leaves = 0, m_pos = 0, m_idx = 0;
     bool exhausted = false;
     unsigned long long nb = 0;
+    const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
+    SceneView v = a.sv;
+    bool any_hit = AM == 1;
+    rq_hit* out_hits = a.hits;
+    uint32_t* out_vis = a.vis;
+    uint32_t out_idx = 0;
 
     auto push = [&](uint32_t v) {
         if (sp - lo == (uint32_t)S) {
@@ -200,12 +226,30 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(a.counter, n);
             base = __shfl(base, 0);
-            if (base + n >= a.count) exhausted = true;
+            if (base + n >= count) exhausted = true;
             if (ray == NONE) {
                 uint32_t r = base + lane_rank(idle);
-                if (r < a.count) {
+                if (r < count) {
                     ray = r;
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays + r);
+                    const rq_ray* src = a.rays + r;
+                    out_idx = r;
+                    if (BATCHED) {
+                        // batch b holds global rays [prefix[b], prefix[b+1])
+                        uint32_t lo_b = 0, hi_b = a.n_batches;
+                        while (hi_b - lo_b > 1) {
+                            uint32_t mid = (lo_b + hi_b) >> 1;
+                            if (a.prefix[mid] <= r) lo_b = mid; else hi_b = mid;
+                        }
+                        const BatchDev& bd = a.batches[lo_b];
+                        v = bd.sv;
+                        any_hit = bd.mode != 0;
+                        out_hits = bd.hits;
+                        out_vis = bd.vis;
+                        out_idx = r - a.prefix[lo_b];
+                        src = bd.rays + out_idx;
+                        if (COUNT) nb += 8 + sizeof(BatchDev);
+                    }
+                    const float4* rp = reinterpret_cast<const float4*>(src);
                     float4 r0 = rp[0], r1 = rp[1];
                     if (COUNT) nb += 32;
                     wo = mk(r0.x, r0.y, r0.z);
@@ -223,7 +267,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     lo = 0;
                     in_blas = false;
                     flat_next = 0;
-                    if (!invalid && a.top_on) push(a.top_root);
+                    if (!invalid && v.top_on) push(v.top_root);
                 }
             }
         }
@@ -237,8 +281,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             if (!in_blas) {
                 uint32_t item = 0;
                 bool have;
-                if (!a.top_on) {
-                    have = flat_next < a.n_inst;
+                if (!v.top_on) {
+                    have = flat_next < v.n_inst;
                     item = LEAF_BIT | flat_next;
                     flat_next++;
                 } else {
@@ -250,12 +294,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 } else if (item & LEAF_BIT) {
                     // gi_instance prologue (:100-113)
                     const uint32_t i = item & ~LEAF_BIT;
-                    const uint32_t* r = LAYOUT == REF ? a.words + 16ull + (uint64_t)i * 32u : a.inst + (uint64_t)i * 32u;
+                    const uint32_t* r = LAYOUT == REF ? v.words + 16ull + (uint64_t)i * 32u : v.inst + (uint64_t)i * 32u;
                     uint32_t rw[24];
                     if (LAYOUT == REF) {
 #pragma unroll
                         for (int k = 0; k < 18; ++k) rw[k] = r[k];
-                        rw[20] = ANY ? r[27] : 0u;
+                        rw[20] = any_hit ? r[27] : 0u;
                     } else {
                         const uint4* r4 = reinterpret_cast<const uint4*>(r);
 #pragma unroll
@@ -264,8 +308,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                             rw[4 * k] = q.x; rw[4 * k + 1] = q.y; rw[4 * k + 2] = q.z; rw[4 * k + 3] = q.w;
                         }
                     }
-                    if (COUNT) nb += LAYOUT == REF ? (ANY ? 76 : 72) : 96;
-                    bool skip = ANY && (rw[20] & 1u);
+                    if (COUNT) nb += LAYOUT == REF ? (any_hit ? 76 : 72) : 96;
+                    bool skip = any_hit && (rw[20] & 1u);
                     float lim = limit(i);
                     if (!skip && slab_near(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
                                            mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim) {
@@ -293,7 +337,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     }
                 } else {
                     // top-level wide node (:240-261)
-                    const uint32_t* w = a.top + (uint64_t)item * 32u;
+                    const uint32_t* w = v.top + (uint64_t)item * 32u;
                     uint32_t tw[28];
                     if (LAYOUT == REF) {
 #pragma unroll
@@ -346,13 +390,13 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     // leaf (:117-169)
                     if (LAYOUT == REF) {
                         uint64_t l = m_leaves + (uint64_t)(e & ~LEAF_BIT) * 2u;
-                        uint32_t first = a.words[l], count = a.words[l + 1];
+                        uint32_t first = v.words[l], count = v.words[l + 1];
                         if (COUNT) nb += 8;
                         float lim_t = lim;
                         for (uint32_t k = 0; k < count; k += 3) {
                             uint64_t x = (uint64_t)m_idx + first + k;
-                            uint32_t ia = a.words[x], ib = a.words[x + 1], ic = a.words[x + 2];
-                            const uint32_t* P = a.words + m_pos;
+                            uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
+                            const uint32_t* P = v.words + m_pos;
                             V3 va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
                             V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
                             V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
@@ -363,14 +407,14 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                                 bt = t; bu = u; bv = v; binst = cur; bhit = true;
                                 btri = (first + k) / 3u;
                                 lim_t = limit(cur);
-                                if (ANY) { fin = true; break; }
+                                if (any_hit) { fin = true; break; }
                             }
                         }
                     } else {
                         uint32_t first = e & LEAF_MASK, n = ((e >> 27) & 0xfu) + 1u;
                         float lim_t = lim;
                         for (uint32_t j = 0; j < n; ++j) {
-                            const float4* tp = a.tris + 3ull * (first + j);
+                            const float4* tp = v.tris + 3ull * (first + j);
                             float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                             if (COUNT) nb += 48;
                             float u, v;
@@ -380,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                                 bt = t; bu = u; bv = v; binst = cur; bhit = true;
                                 btri = first + j;
                                 lim_t = limit(cur);
-                                if (ANY) { fin = true; break; }
+                                if (any_hit) { fin = true; break; }
                             }
                         }
                     }
@@ -389,7 +433,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     float t[4];
                     uint32_t c[4];
                     if (LAYOUT == REF) {
-                        const uint32_t* w = a.words + m_nodes + (uint64_t)e * 16u;
+                        const uint32_t* w = v.words + m_nodes + (uint64_t)e * 16u;
                         uint32_t nw[16];
 #pragma unroll
                         for (int k = 0; k < 16; ++k) nw[k] = w[k];
@@ -411,7 +455,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                             }
                         }
                     } else {
-                        const uint4* w4 = reinterpret_cast<const uint4*>(a.nodes + (uint64_t)e * 32u);
+                        const uint4* w4 = reinterpret_cast<const uint4*>(v.nodes + (uint64_t)e * 32u);
                         uint32_t nw[28];
                         if (COUNT) nb += 112;
 #pragma unroll
@@ -440,7 +484,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
                 } else {
                     // CW8 node: 20 words
-                    const uint4* w4 = reinterpret_cast<const uint4*>(a.nodes + (uint64_t)e * 20u);
+                    const uint4* w4 = reinterpret_cast<const uint4*>(v.nodes + (uint64_t)e * 20u);
                     uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4];
                     if (COUNT) nb += 80;
                     V3 p = mk(bf(q0.x), bf(q0.y), bf(q0.z));
@@ -503,58 +547,58 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         if (bhit) {
             V3 r0, r1, r2, va, vb, vc;
             if (LAYOUT == REF) {
-                const uint32_t* r = a.words + 16ull + (uint64_t)binst * 32u;
+                const uint32_t* r = v.words + 16ull + (uint64_t)binst * 32u;
                 r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
                 r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
                 r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
                 uint32_t pos = r[20], idx = r[21], prim_base = r[28];
                 uint64_t x = (uint64_t)idx + 3ull * btri;
-                tri0 = a.words[x];
-                tri1 = a.words[x + 1];
-                tri2 = a.words[x + 2];
-                const uint32_t* P = a.words + pos;
+                tri0 = v.words[x];
+                tri1 = v.words[x + 1];
+                tri2 = v.words[x + 2];
+                const uint32_t* P = v.words + pos;
                 va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
                 vb = mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2]));
                 vc = mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2]));
                 slot = btri;
-                if (!ANY && prim_base != 0u) {
-                    tri0 = a.words[prim_base + (uint64_t)btri];
+                if (!any_hit && prim_base != 0u) {
+                    tri0 = v.words[prim_base + (uint64_t)btri];
                     tri1 = 0;
                     tri2 = 0;
                 }
             } else {
-                const uint32_t* r = a.inst + (uint64_t)binst * 32u;
+                const uint32_t* r = v.inst + (uint64_t)binst * 32u;
                 r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
                 r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
                 r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
-                const float4* tp = a.tris + 3ull * btri;
+                const float4* tp = v.tris + 3ull * btri;
                 float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                 va = mk(t0.x, t0.y, t0.z);
                 vb = mk(t0.w, t1.x, t1.y);
                 vc = mk(t1.z, t1.w, t2.x);
                 slot = fb(t2.y);
-                const uint32_t* ids = a.tri_ids + 4ull * btri;
+                const uint32_t* ids = v.tri_ids + 4ull * btri;
                 tri0 = ids[0];
                 tri1 = ids[1];
                 tri2 = ids[2];
-                if (!ANY && (r[20] & 2u)) {
+                if (!any_hit && (r[20] & 2u)) {
                     tri0 = fb(t2.z);
                     tri1 = 0;
                     tri2 = 0;
                 }
             }
             n = hit_normal(r0, r1, r2, va, vb, vc);
-            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!ANY ? 4 : 0)) : (36 + 48 + 12);
+            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 48 + 12);
         }
-        if (a.hits) {
-            uint4* hp = reinterpret_cast<uint4*>(a.hits + ray);
+        if (out_hits) {
+            uint4* hp = reinterpret_cast<uint4*>(out_hits + out_idx);
             uint32_t flags = invalid ? 2u : (bhit ? 1u : 0u);
             hp[0] = make_uint4(fb(bt), bhit ? binst : NONE, tri0, tri1);
             hp[1] = make_uint4(tri2, fb(bhit ? bu : 0.0f), fb(bhit ? bv : 0.0f), fb(n.x));
             hp[2] = make_uint4(fb(n.y), fb(n.z), flags, slot);
         }
-        if (COUNT) nb += (a.hits ? 48 : 0) + (a.vis && !bhit ? 4 : 0);
-        if (a.vis && !bhit) atomicOr(a.vis + (ray >> 5), 1u << (ray & 31u));
+        if (COUNT) nb += (out_hits ? 48 : 0) + (out_vis && !bhit ? 4 : 0);
+        if (out_vis && !bhit) atomicOr(out_vis + (out_idx >> 5), 1u << (out_idx & 31u));
         ray = NONE;
     }
     if (COUNT) {
@@ -836,24 +880,30 @@ struct Tune {
     uint32_t block, stack, refill_min, grid_per_cu;
 };
 
+// Device memory a launch uses besides its inputs and outputs, owned per HIP
+// stream: launches on one stream run in order, so they can share it, and
+// launches on different streams never do.
+struct StreamRes {
+    hipStream_t stream = nullptr;
+    uint32_t* counter = nullptr;  // the persistent kernels' ray cursor
+    uint32_t* spill = nullptr;    // per-lane stack spill area
+    size_t spill_lanes = 0;
+    void* scratch = nullptr;      // secondary-ray block counts
+    size_t scratch_bytes = 0;
+    void* table = nullptr;        // batched launches: batch table and prefix
+    size_t table_bytes = 0;
+};
+
 struct Ctx {
     std::mutex mu;
     int device = -1;
     int cus = 0;
-    uint32_t* counters = nullptr;  // ring
-    uint32_t counter_next = 0;
-    uint32_t* spill = nullptr;
-    size_t spill_lanes = 0;
-    uint32_t spill_slots = 0;
-    uint32_t spill_next = 0;
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
+    std::vector<StreamRes> res;
     rq_tuning tuning[3][2];
+    rq_tuning batch_tuning[3];
     bool tuned = false;
 };
 Ctx g_ctx;
-constexpr uint32_t COUNTER_RING = 4096;
-constexpr uint32_t SPILL_SLOTS = 8;
 
 rq_status hip_fail(hipError_t e, const char* what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -876,6 +926,15 @@ void default_tuning(rq_tuning& t, int layout, int any) {
     (void)layout;
 }
 
+void init_tuning() {
+    if (g_ctx.tuned) return;
+    for (int l = 0; l < 3; ++l) {
+        for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
+        default_tuning(g_ctx.batch_tuning[l], l, 0);
+    }
+    g_ctx.tuned = true;
+}
+
 rq_status ensure_ctx() {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
@@ -884,31 +943,27 @@ rq_status ensure_ctx() {
     HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     g_ctx.device = dev;
     g_ctx.cus = prop.multiProcessorCount;
-    HIP_TRY(hipMalloc(&g_ctx.counters, COUNTER_RING * sizeof(uint32_t)), "hipMalloc counters");
-    HIP_TRY(hipMemset(g_ctx.counters, 0, COUNTER_RING * sizeof(uint32_t)), "hipMemset counters");
-    if (!g_ctx.tuned) {
-        for (int l = 0; l < 3; ++l)
-            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
-        g_ctx.tuned = true;
-    }
+    init_tuning();
     return RQ_OK;
 }
 
-template <int L, int B, int S, bool ANY, bool C = false>
+template <int L, int B, int S, int AM, bool C = false, bool BT = false>
 void* kernel_ptr() {
-    return reinterpret_cast<void*>(&dev::trace_kernel<L, B, S, ANY, C>);
+    return reinterpret_cast<void*>(&dev::trace_kernel<L, B, S, AM, C, BT>);
 }
 
 typedef void (*Launch)(dim3, dim3, hipStream_t, const dev::TraceArgs&);
-template <int L, int B, int S, bool ANY, bool C = false>
+template <int L, int B, int S, int AM, bool C = false, bool BT = false>
 void launch_one(dim3 g, dim3 b, hipStream_t s, const dev::TraceArgs& a) {
-    hipLaunchKernelGGL((dev::trace_kernel<L, B, S, ANY, C>), g, b, 0, s, a);
+    hipLaunchKernelGGL((dev::trace_kernel<L, B, S, AM, C, BT>), g, b, 0, s, a);
 }
 
 struct KernelEntry {
     uint32_t block, stack;
     Launch fn[3][2];
     void* ptr[3][2];
+    Launch bfn[3];  // batched, mode per batch
+    void* bptr[3];
 };
 
 template <int B, int S>
@@ -928,6 +983,12 @@ KernelEntry make_entry() {
     e.ptr[1][1] = kernel_ptr<dev::PLAIN, B, S, true>();
     e.ptr[2][0] = kernel_ptr<dev::CW8, B, S, false>();
     e.ptr[2][1] = kernel_ptr<dev::CW8, B, S, true>();
+    e.bfn[0] = &launch_one<dev::REF, B, S, 2, false, true>;
+    e.bfn[1] = &launch_one<dev::PLAIN, B, S, 2, false, true>;
+    e.bfn[2] = &launch_one<dev::CW8, B, S, 2, false, true>;
+    e.bptr[0] = kernel_ptr<dev::REF, B, S, 2, false, true>();
+    e.bptr[1] = kernel_ptr<dev::PLAIN, B, S, 2, false, true>();
+    e.bptr[2] = kernel_ptr<dev::CW8, B, S, 2, false, true>();
     return e;
 }
 
@@ -966,30 +1027,73 @@ const KernelEntry* find_kernel(uint32_t block, uint32_t stack) {
     return nullptr;
 }
 
-rq_status ensure_spill(size_t lanes) {
-    if (g_ctx.spill && g_ctx.spill_lanes >= lanes) return RQ_OK;
-    if (g_ctx.spill) {
-        HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        HIP_TRY(hipFree(g_ctx.spill), "hipFree spill");
-        g_ctx.spill = nullptr;
-    }
-    size_t bytes = lanes * dev::SPILL_WORDS * sizeof(uint32_t) * SPILL_SLOTS;
-    HIP_TRY(hipMalloc(&g_ctx.spill, bytes), "hipMalloc spill");
-    g_ctx.spill_lanes = lanes;
-    g_ctx.spill_slots = SPILL_SLOTS;
+rq_status res_for(hipStream_t s, StreamRes** out) {
+    for (auto& r : g_ctx.res)
+        if (r.stream == s) {
+            *out = &r;
+            return RQ_OK;
+        }
+    StreamRes r;
+    r.stream = s;
+    HIP_TRY(hipMalloc(&r.counter, 256), "hipMalloc counter");
+    g_ctx.res.push_back(r);
+    *out = &g_ctx.res.back();
     return RQ_OK;
 }
 
-rq_status ensure_scratch(size_t bytes) {
-    if (g_ctx.scratch && g_ctx.scratch_bytes >= bytes) return RQ_OK;
-    if (g_ctx.scratch) {
-        HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        HIP_TRY(hipFree(g_ctx.scratch), "hipFree scratch");
-        g_ctx.scratch = nullptr;
+rq_status ensure_spill(StreamRes& r, size_t lanes) {
+    if (r.spill && r.spill_lanes >= lanes) return RQ_OK;
+    if (r.spill) {
+        HIP_TRY(hipStreamSynchronize(r.stream), "hipStreamSynchronize");
+        HIP_TRY(hipFree(r.spill), "hipFree spill");
+        r.spill = nullptr;
     }
-    HIP_TRY(hipMalloc(&g_ctx.scratch, bytes), "hipMalloc scratch");
-    g_ctx.scratch_bytes = bytes;
+    HIP_TRY(hipMalloc(&r.spill, lanes * dev::SPILL_WORDS * sizeof(uint32_t)), "hipMalloc spill");
+    r.spill_lanes = lanes;
     return RQ_OK;
+}
+
+rq_status ensure_scratch(StreamRes& r, size_t bytes) {
+    if (r.scratch && r.scratch_bytes >= bytes) return RQ_OK;
+    if (r.scratch) {
+        HIP_TRY(hipStreamSynchronize(r.stream), "hipStreamSynchronize");
+        HIP_TRY(hipFree(r.scratch), "hipFree scratch");
+        r.scratch = nullptr;
+    }
+    HIP_TRY(hipMalloc(&r.scratch, bytes), "hipMalloc scratch");
+    r.scratch_bytes = bytes;
+    return RQ_OK;
+}
+
+rq_status ensure_table(StreamRes& r, size_t bytes) {
+    if (r.table && r.table_bytes >= bytes) {
+        // the previous batched launch on this stream may still read it
+        HIP_TRY(hipStreamSynchronize(r.stream), "hipStreamSynchronize");
+        return RQ_OK;
+    }
+    if (r.table) {
+        HIP_TRY(hipStreamSynchronize(r.stream), "hipStreamSynchronize");
+        HIP_TRY(hipFree(r.table), "hipFree table");
+        r.table = nullptr;
+    }
+    HIP_TRY(hipMalloc(&r.table, bytes), "hipMalloc table");
+    r.table_bytes = bytes;
+    return RQ_OK;
+}
+
+void release_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    for (size_t i = 0; i < g_ctx.res.size(); ++i)
+        if (g_ctx.res[i].stream == s) {
+            StreamRes& r = g_ctx.res[i];
+            (void)hipStreamSynchronize(s);
+            if (r.counter) (void)hipFree(r.counter);
+            if (r.spill) (void)hipFree(r.spill);
+            if (r.scratch) (void)hipFree(r.scratch);
+            if (r.table) (void)hipFree(r.table);
+            g_ctx.res.erase(g_ctx.res.begin() + i);
+            return;
+        }
 }
 
 template <typename T>
@@ -1087,11 +1191,7 @@ extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* o
         return RQ_ERR_INVALID_INPUT;
     }
     std::lock_guard<std::mutex> lock(g_ctx.mu);
-    if (!g_ctx.tuned) {
-        for (int l = 0; l < 3; ++l)
-            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
-        g_ctx.tuned = true;
-    }
+    init_tuning();
     *out = g_ctx.tuning[layout][mode ? 1 : 0];
     return RQ_OK;
 }
@@ -1103,19 +1203,41 @@ extern "C" rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tun
         return RQ_ERR_INVALID_INPUT;
     }
     std::lock_guard<std::mutex> lock(g_ctx.mu);
-    if (!g_ctx.tuned) {
-        for (int l = 0; l < 3; ++l)
-            for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
-        g_ctx.tuned = true;
-    }
+    init_tuning();
     g_ctx.tuning[layout][mode ? 1 : 0] = *t;
     return RQ_OK;
 }
 
 namespace rq {
 namespace {
+dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
+    dev::SceneView v;
+    std::memset(&v, 0, sizeof v);
+    const DeviceLayout& d = scene->dev[layout];
+    if (layout == RQ_LAYOUT_REF_BVH4Q8) {
+        v.words = reinterpret_cast<const uint32_t*>(d.base);
+        v.top = v.words + scene->words[1];
+        v.n_inst = scene->words[0];
+        v.top_on = scene->words[1] != 0;
+        v.top_root = scene->words[2];
+    } else {
+        const HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
+        char* b = reinterpret_cast<char*>(d.base);
+        v.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
+        v.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
+        v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
+        v.tris = reinterpret_cast<const float4*>(b + d.off[3]);
+        v.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
+        v.n_inst = h.instance_count;
+        v.top_on = h.top_on;
+        v.top_root = h.top_root;
+    }
+    return v;
+}
+
 rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
-                       rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes) {
+                       rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
+                       const uint32_t* count_ptr = nullptr) {
     if (!scene || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays)) {
         set_error("rq_trace: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -1155,41 +1277,21 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     uint64_t grid = (uint64_t)g_ctx.cus * per_cu;
     uint64_t need = ((uint64_t)count + k->block - 1) / k->block;
     grid = std::max<uint64_t>(1, std::min(grid, need));
-    st = ensure_spill((size_t)(grid * k->block));
-    if (st != RQ_OK) return st;
-    uint32_t slot = g_ctx.counter_next;
-    g_ctx.counter_next = (g_ctx.counter_next + 1) % COUNTER_RING;
-    uint32_t spill_slot = g_ctx.spill_next;
-    g_ctx.spill_next = (g_ctx.spill_next + 1) % g_ctx.spill_slots;
-    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, g_ctx.counters + slot, 0u, (uint64_t)1);
+    StreamRes* res = nullptr;
+    if ((st = res_for(s, &res)) != RQ_OK) return st;
+    if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
+    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
 
     dev::TraceArgs a;
     std::memset(&a, 0, sizeof a);
-    const DeviceLayout& d = scene->dev[layout];
-    if (layout == RQ_LAYOUT_REF_BVH4Q8) {
-        a.words = reinterpret_cast<const uint32_t*>(d.base);
-        a.top = a.words + scene->words[1];
-        a.n_inst = scene->words[0];
-        a.top_on = scene->words[1] != 0;
-        a.top_root = scene->words[2];
-    } else {
-        const HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
-        char* b = reinterpret_cast<char*>(d.base);
-        a.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
-        a.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
-        a.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
-        a.tris = reinterpret_cast<const float4*>(b + d.off[3]);
-        a.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
-        a.n_inst = h.instance_count;
-        a.top_on = h.top_on;
-        a.top_root = h.top_root;
-    }
+    a.sv = scene_view(scene, layout);
     a.rays = rays;
     a.hits = hits;
     a.vis = vis_bits;
     a.count = count;
-    a.counter = g_ctx.counters + slot;
-    a.spill = g_ctx.spill + (size_t)spill_slot * g_ctx.spill_lanes * dev::SPILL_WORDS;
+    a.counter = res->counter;
+    a.spill = res->spill;
+    a.count_ptr = count_ptr;
     a.refill_min = t.reserved[0] ? t.reserved[0] : 1u;
     a.bytes = bytes;
     k->fn[layout][any](dim3((unsigned)grid), dim3(k->block), s, a);
@@ -1202,6 +1304,120 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
 extern "C" rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
                               rq_hit* hits, uint32_t* vis_bits, void* stream) {
     return launch_trace(scene, layout, rays, count, mode, hits, vis_bits, stream, nullptr);
+}
+
+extern "C" rq_status rq_trace_indirect(rq_scene* scene, rq_layout layout, const rq_ray* rays,
+                                       const uint32_t* count_dev, uint32_t max_count, uint32_t mode, rq_hit* hits,
+                                       uint32_t* vis_bits, void* stream) {
+    if (!count_dev) {
+        set_error("rq_trace_indirect: null count");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    return launch_trace(scene, layout, rays, max_count, mode, hits, vis_bits, stream, nullptr, count_dev);
+}
+
+extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
+                                      const rq_batch* batches, uint32_t batch_count, void* stream) {
+    if (!scenes || !batches || (int)layout < 0 || (int)layout > 2 || batch_count == 0) {
+        set_error("rq_trace_batches: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::vector<dev::BatchDev> table(batch_count);
+    std::vector<uint32_t> prefix(batch_count + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < batch_count; ++b) {
+        const rq_batch& q = batches[b];
+        if (q.scene >= scene_count || !scenes[q.scene] || !scenes[q.scene]->dev[layout].ready || q.mode > 2 ||
+            (q.count && !q.rays)) {
+            set_error("rq_trace_batches: batch " + std::to_string(b) + " names a missing scene or layout");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        table[b].sv = scene_view(scenes[q.scene], layout);
+        table[b].rays = q.rays;
+        table[b].hits = q.hits;
+        table[b].vis = q.vis_bits;
+        table[b].mode = q.mode;
+        table[b].count = q.count;
+        prefix[b] = (uint32_t)total;
+        total += q.count;
+    }
+    if (total >= 0xffffffffull) {
+        set_error("rq_trace_batches: more than 2^32 rays");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    prefix[batch_count] = (uint32_t)total;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status st = ensure_ctx();
+    if (st != RQ_OK) return st;
+    for (uint32_t b = 0; b < batch_count; ++b) {
+        if (!batches[b].vis_bits || !batches[b].count) continue;
+        uint64_t words = (batches[b].count + 31u) / 32u;
+        hipLaunchKernelGGL(dev::fill_u32, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 4096)), dim3(256), 0,
+                           s, batches[b].vis_bits, 0u, words);
+    }
+    if (total == 0) return RQ_OK;
+    const rq_tuning& t = g_ctx.batch_tuning[layout];
+    const KernelEntry* k = find_kernel(t.block, t.stack);
+    if (!k) {
+        set_error("rq_trace_batches: no kernel for the tuning");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    int per_cu = (int)t.grid_per_cu;
+    if (per_cu == 0) {
+        int blocks = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k->bptr[layout], (int)k->block, 0),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu = std::max(blocks, 1);
+    }
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)g_ctx.cus * per_cu,
+                                                             (total + k->block - 1) / k->block));
+    StreamRes* res = nullptr;
+    if ((st = res_for(s, &res)) != RQ_OK) return st;
+    if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
+    size_t tbytes = table.size() * sizeof(dev::BatchDev), pbytes = prefix.size() * sizeof(uint32_t);
+    if ((st = ensure_table(*res, tbytes + pbytes + 256)) != RQ_OK) return st;
+    char* tb = reinterpret_cast<char*>(res->table);
+    HIP_TRY(hipMemcpyAsync(tb, table.data(), tbytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync batch table");
+    size_t poff = (tbytes + 255) & ~(size_t)255;
+    HIP_TRY(hipMemcpyAsync(tb + poff, prefix.data(), pbytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync prefix");
+    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
+    dev::TraceArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.count = (uint32_t)total;
+    a.counter = res->counter;
+    a.spill = res->spill;
+    a.batches = reinterpret_cast<const dev::BatchDev*>(tb);
+    a.prefix = reinterpret_cast<const uint32_t*>(tb + poff);
+    a.n_batches = batch_count;
+    a.refill_min = t.reserved[0] ? t.reserved[0] : 1u;
+    k->bfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
+    HIP_TRY(hipGetLastError(), "batched trace launch");
+    // The table is read by the kernel: the next call on this stream may overwrite it only after.
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
+    if (!out || (int)layout < 0 || (int)layout > 2) {
+        set_error("rq_get_batch_tuning: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    init_tuning();
+    *out = g_ctx.batch_tuning[layout];
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
+    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->reserved[0] == 0 ||
+        t->reserved[0] > 64) {
+        set_error("rq_set_batch_tuning: unsupported block/stack/refill combination");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    init_tuning();
+    g_ctx.batch_tuning[layout] = *t;
+    return RQ_OK;
 }
 
 extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
@@ -1294,7 +1510,10 @@ extern "C" rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits,
     rq_status st = ensure_ctx();
     if (st != RQ_OK) return st;
     uint32_t nblocks = (count + dev::SEC_BLOCK - 1) / dev::SEC_BLOCK;
-    st = ensure_scratch((size_t)(n_gi + 1) * nblocks * sizeof(uint32_t));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    StreamRes* res = nullptr;
+    if ((st = res_for(s, &res)) != RQ_OK) return st;
+    st = ensure_scratch(*res, (size_t)(n_gi + 1) * nblocks * sizeof(uint32_t));
     if (st != RQ_OK) return st;
     dev::SecArgs a;
     a.primary = primary;
@@ -1305,12 +1524,11 @@ extern "C" rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits,
     a.sz = sun[2];
     a.seed = seed;
     a.n_gi = n_gi;
-    a.block_counts = reinterpret_cast<uint32_t*>(g_ctx.scratch);
+    a.block_counts = reinterpret_cast<uint32_t*>(res->scratch);
     a.counts = counts;
     a.shadow = shadow;
     a.gi = gi;
     a.nblocks = nblocks;
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(dev::secondary_count, dim3(nblocks), dim3(dev::SEC_BLOCK), 0, s, a);
     hipLaunchKernelGGL(dev::secondary_scan, dim3(n_gi + 1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(dev::secondary_write, dim3(nblocks), dim3(dev::SEC_BLOCK), 0, s, a);
@@ -1352,6 +1570,7 @@ extern "C" rq_status rq_stream_create(void** stream) {
     return RQ_OK;
 }
 extern "C" rq_status rq_stream_destroy(void* stream) {
+    release_stream(reinterpret_cast<hipStream_t>(stream));
     HIP_TRY(hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)), "hipStreamDestroy");
     return RQ_OK;
 }

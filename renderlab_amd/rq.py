@@ -103,6 +103,10 @@ _sig("rq_scene_upload", _i, _P, _u32)
 _sig("rq_scene_device_bytes", _i, _P, _i, ctypes.POINTER(_sz))
 _sig("rq_scene_destroy", None, _P)
 _sig("rq_trace", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
+_sig("rq_trace_indirect", _i, _P, _i, _P, _P, _u32, _u32, _P, _P, _P)
+_sig("rq_trace_batches", _i, _P, _u32, _i, _P, _u32, _P)
+_sig("rq_get_batch_tuning", _i, _i, _P)
+_sig("rq_set_batch_tuning", _i, _i, _P)
 _sig("rq_trace_count_bytes", _i, _P, _i, _P, _u32, _u32, _P, _P, ctypes.POINTER(_u64))
 _sig("rq_get_tuning", _i, _i, _u32, _P)
 _sig("rq_set_tuning", _i, _i, _u32, _P)
@@ -349,6 +353,44 @@ class instance_bvh:  # noqa: N801 - mirrors the reference module name
 
 
 # ----------------------------------------------------------------- scene
+class Batch(ctypes.Structure):
+    """rq_batch: one ray batch of a multi-scene launch (device pointers)."""
+    _fields_ = [("scene", _u32), ("mode", _u32), ("count", _u32), ("pad", _u32), ("rays", _P), ("hits", _P),
+                ("vis_bits", _P)]
+
+
+def _dptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceBuffer):
+        return x.ptr.value
+    return int(x.value) if isinstance(x, ctypes.c_void_p) else int(x)
+
+
+def trace_batches(scenes, layout, batches, stream=None):
+    """batches: [(scene index, mode, count, rays, hits or None, vis or None)], all in one launch."""
+    arr = (ctypes.c_void_p * len(scenes))(*[sc._h.value for sc in scenes])
+    tab = (Batch * len(batches))()
+    for k, (si, mode, count, rays, hits, vis) in enumerate(batches):
+        tab[k].scene, tab[k].mode, tab[k].count = int(si), int(mode), int(count)
+        tab[k].rays, tab[k].hits, tab[k].vis_bits = _dptr(rays), _dptr(hits), _dptr(vis)
+    check(lib.rq_trace_batches(arr, len(scenes), LAYOUTS.get(layout, layout), tab, len(batches),
+                               stream.ptr if stream else None))
+
+
+def get_batch_tuning(layout):
+    t = RqTuning()
+    check(lib.rq_get_batch_tuning(LAYOUTS.get(layout, layout), ctypes.byref(t)))
+    return t
+
+
+def set_batch_tuning(layout, block, stack, refill, grid_per_cu=0):
+    t = RqTuning()
+    t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
+    t.reserved[0] = refill
+    check(lib.rq_set_batch_tuning(LAYOUTS.get(layout, layout), ctypes.byref(t)))
+
+
 class _MeshDesc(ctypes.Structure):
     _fields_ = [("positions", _P), ("vertex_count", _u32), ("indices", _P), ("index_count", _u32),
                 ("keep_primitives", _u32)]
@@ -438,6 +480,12 @@ class Scene:
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
         check(lib.rq_trace(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits), p(vis),
                            stream.ptr if stream else None))
+
+    def trace_indirect(self, layout, rays, count_dev, max_count, mode, hits=None, vis=None, stream=None):
+        """rq_trace with the ray count read from device memory (count_dev: device pointer)."""
+        check(lib.rq_trace_indirect(self._h, LAYOUTS.get(layout, layout), _dptr(rays), _dptr(count_dev),
+                                    int(max_count), int(mode), _dptr(hits), _dptr(vis),
+                                    stream.ptr if stream else None))
 
     def count_bytes(self, layout, rays, count, mode, hits=None, vis=None):
         """Global bytes one traversal launch over these rays moves (synchronous)."""
