@@ -231,6 +231,39 @@ rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t c
                            const float sun[3], uint32_t seed, uint32_t n_gi, rq_ray* shadow,
                            rq_ray* gi, uint32_t* counts, void* stream);
 
+/* ---- Serving: render-query jobs admitted by an HBM budget with continuous
+   batching. No counterpart in the reference (its renderer traces once a
+   frame on one queue); this is the request loop around the ray query. */
+typedef struct rq_job {
+    uint32_t scene;       /* index into the scenes array */
+    uint32_t scene_kind;  /* camera orbit of rq_gen_camera (0 or 1) */
+    uint32_t width, height;
+    uint32_t n_gi;        /* 0: primary + shadow; 1: primary + shadow + GI */
+    uint32_t pad;
+    uint64_t seed;        /* camera pose and GI sampling seed */
+    double arrival_s;     /* arrival time, seconds after the run starts */
+} rq_job;
+typedef struct rq_serve_config {
+    rq_layout layout;
+    uint32_t slots;       /* jobs in flight at once (one HIP stream each), 1..16 */
+    uint64_t hbm_budget;  /* bytes of job buffers in flight at once */
+    float sun[3];         /* unit direction toward the sun */
+    uint32_t pad;
+} rq_serve_config;
+typedef struct rq_job_result {
+    double arrival_s, start_s, end_s; /* host clock, seconds after the run starts */
+    double gpu_ms;                    /* first to last launch of the job, HIP events */
+    uint64_t digest;                  /* order-free hash of the job's hits and visibility bits */
+    uint64_t primary, secondary;      /* rays traced */
+    uint32_t status;                  /* RQ_OK, or RQ_ERR_BUDGET when the job alone exceeds the budget */
+    uint32_t pad;
+} rq_job_result;
+/* Runs `jobs` in real time: a job is admitted once it has arrived, a slot is
+   free and its buffers fit in the budget (first come, first served; first
+   fit in one arena of hbm_budget bytes). Returns when every job is done. */
+rq_status rq_serve(rq_scene* const* scenes, uint32_t scene_count, const rq_serve_config* config,
+                   const rq_job* jobs, uint32_t job_count, rq_job_result* results);
+
 /* ---- Device helpers (HIP runtime of this library; no torch types). */
 rq_status rq_malloc(size_t bytes, void** ptr);
 rq_status rq_free(void* ptr);
@@ -239,7 +272,11 @@ rq_status rq_memcpy_d2h(void* dst, const void* src, size_t bytes);
 rq_status rq_memset(void* dst, int value, size_t bytes);
 rq_status rq_synchronize(void);
 rq_status rq_stream_create(void** stream);
-rq_status rq_stream_destroy(void* stream);
+rq_status rq_stream_destroy(void* stream); /* also frees the stream's scratch memory */
+/* Sizes a stream's scratch memory (ray cursor, stack spill area, secondary
+   ray counts) for launches of up to max_rays rays, so later launches on it
+   allocate nothing. */
+rq_status rq_stream_reserve(void* stream, uint32_t max_rays);
 rq_status rq_stream_synchronize(void* stream);
 rq_status rq_event_create(void** event);
 rq_status rq_event_destroy(void* event);

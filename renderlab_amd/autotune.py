@@ -1,0 +1,102 @@
+"""Autotuner for the traversal launches.
+
+Sweeps the launch knobs the kernels expose -- workgroup size (the tile of
+rays a workgroup holds), LDS stack entries per lane, the refill threshold
+(how many idle lanes a wave waits for before fetching new rays) and the
+workgroups per CU (waves per CU) -- over a sample of the caller's own rays,
+timing each candidate with HIP events, and installs the fastest with
+rq_set_tuning / rq_set_batch_tuning. Results are cached in a JSON file keyed
+by (layout, mode, scene-size class, ray kind), so a service pays for the
+sweep once per workload class.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import rq
+
+CANDIDATES = [(128, 8), (128, 16), (128, 32), (256, 8), (256, 16), (256, 32), (64, 16), (64, 32)]
+REFILLS = [1, 8, 16, 32, 64]
+GRIDS = [0, 2, 4]
+
+
+def size_class(triangles):
+    return int(np.clip(np.floor(np.log10(max(triangles, 1)) * 2) / 2, 0, 9) * 10)
+
+
+def _time(fn, reps=2):
+    start, end = rq.Event(), rq.Event()
+    fn()  # warm
+    rq.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    end.synchronize()
+    return start.elapsed_ms(end) / reps
+
+
+def tune_trace(scene, layout, rays_dev, count, mode, hits=None, vis=None, quick=False):
+    """Sweep the single-scene launch for (layout, mode) on these rays; installs and returns the best."""
+    results = []
+    grids = [0] if quick else GRIDS
+    refills = [1, 16, 64] if quick else REFILLS
+    for block, stack in CANDIDATES:
+        for refill in refills:
+            for grid in grids:
+                rq.set_tuning(layout, mode, block, stack, refill, grid)
+                ms = _time(lambda: scene.trace(layout, rays_dev, count, mode, hits=hits, vis=vis))
+                results.append((ms, block, stack, refill, grid))
+    results.sort()
+    ms, block, stack, refill, grid = results[0]
+    rq.set_tuning(layout, mode, block, stack, refill, grid)
+    return dict(ms=ms, block=block, stack=stack, refill=refill, grid_per_cu=grid, candidates=len(results))
+
+
+def tune_batches(scenes, layout, batches, quick=False):
+    """Sweep the multi-scene launch on these batches; installs and returns the best."""
+    results = []
+    grids = [0] if quick else GRIDS
+    refills = [1, 16, 64] if quick else REFILLS
+    for block, stack in CANDIDATES:
+        for refill in refills:
+            for grid in grids:
+                rq.set_batch_tuning(layout, block, stack, refill, grid)
+                ms = _time(lambda: rq.trace_batches(scenes, layout, batches))
+                results.append((ms, block, stack, refill, grid))
+    results.sort()
+    ms, block, stack, refill, grid = results[0]
+    rq.set_batch_tuning(layout, block, stack, refill, grid)
+    return dict(ms=ms, block=block, stack=stack, refill=refill, grid_per_cu=grid, candidates=len(results))
+
+
+class Cache:
+    """JSON cache of tuned launches keyed by workload class."""
+
+    def __init__(self, path):
+        self.path = path
+        self.data = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                self.data = json.load(f)
+
+    @staticmethod
+    def key(layout, mode, triangles, kind):
+        return f"{layout}/{mode}/{size_class(triangles)}/{kind}"
+
+    def apply(self, layout, mode, triangles, kind):
+        e = self.data.get(self.key(layout, mode, triangles, kind))
+        if e:
+            rq.set_tuning(layout, mode, e["block"], e["stack"], e["refill"], e["grid_per_cu"])
+        return e
+
+    def store(self, layout, mode, triangles, kind, entry):
+        e = {k: entry[k] for k in ("block", "stack", "refill", "grid_per_cu", "ms")}
+        e["when"] = time.time()
+        self.data[self.key(layout, mode, triangles, kind)] = e
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)

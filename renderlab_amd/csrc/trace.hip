@@ -1397,6 +1397,19 @@ extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_co
     return RQ_OK;
 }
 
+extern "C" rq_status rq_stream_reserve(void* stream, uint32_t max_rays) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status st = ensure_ctx();
+    if (st != RQ_OK) return st;
+    StreamRes* res = nullptr;
+    if ((st = res_for(s, &res)) != RQ_OK) return st;
+    // the largest persistent grid: every wave slot of every CU
+    if ((st = ensure_spill(*res, (size_t)g_ctx.cus * 2048u)) != RQ_OK) return st;
+    uint32_t nblocks = (max_rays + dev::SEC_BLOCK - 1) / dev::SEC_BLOCK;
+    return ensure_scratch(*res, (size_t)(dev::MAX_GI + 1) * std::max(nblocks, 1u) * sizeof(uint32_t));
+}
+
 extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
     if (!out || (int)layout < 0 || (int)layout > 2) {
         set_error("rq_get_batch_tuning: bad argument");

@@ -129,6 +129,8 @@ _sig("rq_event_synchronize", _i, _P)
 _sig("rq_event_elapsed_ms", _i, _P, _P, ctypes.POINTER(ctypes.c_float))
 _sig("rq_event_query", _i, _P)
 _sig("rq_gen_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
+_sig("rq_stream_reserve", _i, _P, _u32)
+_sig("rq_serve", _i, _P, _u32, _P, _P, _u32, _P)
 _sig("rq_gen_camera", _i, _u32, _u64, _u32, _u32, _P)
 
 
@@ -215,6 +217,9 @@ class Stream:
 
     def synchronize(self):
         check(lib.rq_stream_synchronize(self.ptr))
+
+    def reserve(self, max_rays):
+        check(lib.rq_stream_reserve(self.ptr, int(max_rays)))
 
     def __del__(self):
         try:
@@ -564,3 +569,43 @@ def gi_visible(scene, rays, count, vis_bits, layout="plain", stream=None):
 def gi_visible_within(scene, rays, count, vis_bits, layout="plain", stream=None):
     """Visibility within each ray's tmax, opaque casters only (mode 2)."""
     scene.trace(layout, rays, count, ANY_OPAQUE, vis=vis_bits, stream=stream)
+
+
+# ---------------------------------------------------------------- serving
+class Job(ctypes.Structure):
+    _fields_ = [("scene", _u32), ("scene_kind", _u32), ("width", _u32), ("height", _u32), ("n_gi", _u32),
+                ("pad", _u32), ("seed", _u64), ("arrival_s", ctypes.c_double)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("layout", _i), ("slots", _u32), ("hbm_budget", _u64), ("sun", ctypes.c_float * 3), ("pad", _u32)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("start_s", ctypes.c_double), ("end_s", ctypes.c_double),
+                ("gpu_ms", ctypes.c_double), ("digest", _u64), ("primary", _u64), ("secondary", _u64),
+                ("status", _u32), ("pad", _u32)]
+
+
+def serve(scenes, jobs, layout="cw8", slots=4, hbm_budget=8 << 30, sun=(0.3, 0.8, 0.2)):
+    """jobs: [dict(scene, kind, width, height, n_gi, seed, arrival_s)] -> list of result dicts."""
+    arr = (ctypes.c_void_p * len(scenes))(*[sc._h.value for sc in scenes])
+    js = (Job * max(len(jobs), 1))()
+    for k, j in enumerate(jobs):
+        js[k].scene, js[k].scene_kind = int(j["scene"]), int(j.get("kind", 1))
+        js[k].width, js[k].height, js[k].n_gi = int(j["width"]), int(j["height"]), int(j["n_gi"])
+        js[k].seed, js[k].arrival_s = int(j["seed"]), float(j["arrival_s"])
+    cfg = ServeConfig()
+    cfg.layout, cfg.slots, cfg.hbm_budget = LAYOUTS.get(layout, layout), int(slots), int(hbm_budget)
+    sv = np.asarray(sun, np.float64)
+    sv = sv / np.linalg.norm(sv)
+    cfg.sun[:] = [float(x) for x in sv.astype(np.float32)]
+    res = (JobResult * max(len(jobs), 1))()
+    check(lib.rq_serve(arr, len(scenes), ctypes.byref(cfg), js, len(jobs), res))
+    out = []
+    for k in range(len(jobs)):
+        r = res[k]
+        out.append(dict(arrival_s=r.arrival_s, start_s=r.start_s, end_s=r.end_s, gpu_ms=r.gpu_ms, digest=r.digest,
+                        primary=r.primary, secondary=r.secondary, status=r.status,
+                        latency_ms=(r.end_s - r.arrival_s) * 1e3))
+    return out
