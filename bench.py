@@ -134,6 +134,104 @@ def time_layout(rq, frame, layout, steps, warmup):
     return total_ms / steps, wall, {k: float(np.mean(v)) for k, v in per.items()}
 
 
+C3_TRIS = [100_000, 250_000, 500_000, 1_000_000, 1_000_000, 2_000_000, 3_000_000, 4_000_000]
+
+
+def c3_scenes(rq, layout, tris=C3_TRIS):
+    scenes = []
+    for k, t in enumerate(tris):
+        meshes = int(min(5000, max(100, t // 800)))
+        sc = rq.Scene.generate(1, 100 + k, meshes, t)
+        sc.upload((layout, "plain") if layout != "plain" else ("plain",))
+        scenes.append(sc)
+    return scenes
+
+
+def c3_batches(rq, scenes, layout, n_batches=256, seed=7):
+    """256 batches, sizes lognormal in [4K, 2M], 40% primary / 30% shadow / 30% GI, drawn from
+    per-scene pools of 1920x1080 primary rays and the shadow and GI rays of their hits."""
+    g = np.random.default_rng(seed)
+    pools = []
+    for k, sc in enumerate(scenes):
+        w, h = 1920, 1080
+        n = w * h
+        inv = rq.camera(1, 100 + k, w, h)
+        prim = rq.DeviceBuffer(n * 32)
+        rq.gen_primary(inv, w, h, prim)
+        hits = rq.DeviceBuffer(n * 48)
+        sc.trace(layout, prim, n, rq.CLOSEST, hits=hits)
+        shadow = rq.DeviceBuffer(n * 32)
+        gi = rq.DeviceBuffer(n * 32)
+        counts = rq.DeviceBuffer(8)
+        rq.gen_secondary(prim, hits, n, SUN, 100 + k, 1, shadow, gi, counts)
+        rq.synchronize()
+        c = counts.download(np.uint32)
+        pools.append({0: (prim, n), 1: (shadow, int(c[0])), 2: (gi, int(c[1]))})
+    sizes = np.clip(np.exp(g.normal(np.log(64_000), 1.6, n_batches)), 4096, 2_000_000).astype(int)
+    kinds = g.choice([0, 1, 2], n_batches, p=[0.4, 0.3, 0.3])
+    batches, outs = [], []
+    for b in range(n_batches):
+        si = int(g.integers(len(scenes)))
+        buf, avail = pools[si][int(kinds[b])]
+        n = int(min(sizes[b], avail))
+        start = int(g.integers(0, max(avail - n, 0) + 1))
+        mode = rq.ANY if kinds[b] == 1 else rq.CLOSEST
+        hits = rq.DeviceBuffer(n * 48) if mode == rq.CLOSEST else None
+        vis = rq.DeviceBuffer(((n + 31) // 32) * 4) if mode != rq.CLOSEST else None
+        outs.append((hits, vis))
+        batches.append((si, mode, n, buf.ptr.value + 32 * start, hits, vis))
+    return batches, pools, outs
+
+
+def run_batching(rq, layout, quick_tune=True):
+    from renderlab_amd import autotune
+    t0 = time.perf_counter()
+    scenes = c3_scenes(rq, layout)
+    log(f"c3: 8 scenes built and uploaded in {time.perf_counter() - t0:.1f}s")
+    batches, pools, outs = c3_batches(rq, scenes, layout)
+    total = sum(b[2] for b in batches)
+    tuned = autotune.tune_batches(scenes, layout, batches, quick=quick_tune)
+    log(f"c3: {len(batches)} batches, {total} rays; tuned {tuned}")
+    start, end = rq.Event(), rq.Event()
+    reps = 3
+    start.record()
+    for _ in range(reps):
+        rq.trace_batches(scenes, layout, batches)
+    end.record()
+    end.synchronize()
+    one = start.elapsed_ms(end) / reps
+    start.record()
+    for _ in range(reps):
+        for si, mode, n, rays, hits, vis in batches:
+            scenes[si].trace(layout, rays, n, mode, hits=hits, vis=vis)
+    end.record()
+    end.synchronize()
+    sep = start.elapsed_ms(end) / reps
+    return {"rays": total, "batches": len(batches), "one_launch_ms": round(one, 3),
+            "one_launch_mrays_s": round(total / one / 1e3, 1), "per_batch_launches_ms": round(sep, 3),
+            "per_batch_mrays_s": round(total / sep / 1e3, 1),
+            "tuning": {k: tuned[k] for k in ("block", "stack", "refill", "grid_per_cu")}}
+
+
+def run_serving(rq, layout, scenes, rate, n_jobs, resolutions, mixes, seed=3, slots=4, budget=16 << 30):
+    g = np.random.default_rng(seed)
+    t = np.cumsum(g.exponential(1.0 / rate, n_jobs))
+    jobs = []
+    for k in range(n_jobs):
+        w, h = resolutions[int(g.integers(len(resolutions)))]
+        jobs.append(dict(scene=int(g.integers(len(scenes))), kind=1, width=w, height=h,
+                         n_gi=int(mixes[int(g.integers(len(mixes)))]), seed=int(g.integers(1 << 30)),
+                         arrival_s=float(t[k])))
+    out = rq.serve(scenes, jobs, layout=layout, slots=slots, hbm_budget=budget)
+    ok = [r for r in out if r["status"] == 0]
+    lat = np.array([r["latency_ms"] for r in ok])
+    rays = sum(r["primary"] + r["secondary"] for r in ok)
+    span = max(r["end_s"] for r in ok) - min(r["arrival_s"] for r in ok)
+    return {"rate_jobs_s": rate, "jobs": n_jobs, "refused": n_jobs - len(ok),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mrays_s": round(rays / span / 1e6, 1), "slots": slots, "hbm_budget_gb": budget >> 30}
+
+
 def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
     """The oracle traversal (scalar C port of gi_traverse) on this host's cores, over a bounded sample."""
     import oracle
@@ -168,6 +266,11 @@ def main():
     ap.add_argument("--layout", default="cw8", choices=["ref", "plain", "cw8"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain layout")
+    ap.add_argument("--batching", action="store_true", help="also run the 8-scene, 256-batch one-launch form")
+    ap.add_argument("--no-serving", action="store_true", help="skip the short serving run")
+    ap.add_argument("--serving-sweep", action="store_true", help="sweep Poisson rates 50..2000 jobs/s")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ray query shards by ray "
@@ -175,6 +278,14 @@ def main():
         sys.exit(2)
 
     from renderlab_amd import rq
+    if args.copy_calibration:
+        nb = (args.copy_calibration + 15) // 16 * 16
+        src, dst = rq.DeviceBuffer(nb), rq.DeviceBuffer(nb)
+        src.zero()
+        rq.device_copy(dst, src, nb)
+        rq.synchronize()
+        log(f"copy calibration: {nb} bytes read and written by copy_kernel")
+        del src, dst
     kind, meshes, tris, w, h, n_gi, desc = CONFIGS[args.config]
     t0 = time.perf_counter()
     scene = rq.Scene.generate(kind, 1, meshes, tris)
@@ -251,6 +362,19 @@ def main():
     if verify:
         line["fp64_verify"] = verify
     line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}
+    if not args.no_serving:
+        # 1080p shadow+GI load on the benchmark scene: p50/p99 job latency.
+        sv = run_serving(rq, args.layout, [scene], rate=40.0, n_jobs=60, resolutions=[(1920, 1080)], mixes=[1])
+        line["serving_1080p_sgi"] = sv
+        log(f"serving 1080p P+S+GI at 40 jobs/s: {sv}")
+    if args.serving_sweep:
+        res = [(320, 240), (960, 640), (1920, 1080), (3840, 2160)]
+        line["serving_sweep"] = [run_serving(rq, args.layout, [scene], rate=r, n_jobs=int(min(400, max(40, r))),
+                                             resolutions=res, mixes=[0, 1]) for r in (50, 200, 500, 1000, 2000)]
+        log(f"serving sweep: {line['serving_sweep']}")
+    if args.batching:
+        line["batching_c3"] = run_batching(rq, args.layout)
+        log(f"batching: {line['batching_c3']}")
     print(json.dumps(line), flush=True)
 
 

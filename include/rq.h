@@ -212,9 +212,11 @@ rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* tuning);
    re-intersects its reported triangle in f64 (object-space ray computed in
    f64 from the f32 world ray) and reduces max |t-t64|/|t64| and max
    |u-u64|, |v-v64|. out (host): [0] max rel t error, [1] max abs bary error,
-   [2] hits checked. */
+   [2] hits checked, [3] max |t-t64|*|d| / (|o| + |t64|*|d|): the distance
+   error against the size of the coordinates (object space), which bounds
+   f32 error for short hits far from the origin where [0] cannot. */
 rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,
-                         double out[3], void* stream);
+                         double out[4], void* stream);
 
 /* ---- Ray generation (device). Formulas: shaders/primary_rays.wgsl:50-66
    (primary, reversed Z), shaders/reference_path_tracer.wgsl:149-167,
@@ -270,6 +272,9 @@ rq_status rq_free(void* ptr);
 rq_status rq_memcpy_h2d(void* dst, const void* src, size_t bytes);
 rq_status rq_memcpy_d2h(void* dst, const void* src, size_t bytes);
 rq_status rq_memset(void* dst, int value, size_t bytes);
+/* A plain streaming copy kernel (bytes a multiple of 16): the known-size
+   reference the roofline's HBM byte counters are scaled by. */
+rq_status rq_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 rq_status rq_synchronize(void);
 rq_status rq_stream_create(void** stream);
 rq_status rq_stream_destroy(void* stream); /* also frees the stream's scratch memory */

@@ -607,6 +607,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     }
 }
 
+// A plain streaming copy: the known-size reference for HBM byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(uint4* dst, const uint4* src, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -799,14 +805,14 @@ struct VerifyArgs {
     const rq_ray* rays;
     const rq_hit* hits;
     uint32_t count;
-    unsigned long long* out;  // [0] max rel t err bits, [1] max bary err bits, [2] checked
+    unsigned long long* out;  // [0] max rel t err bits, [1] max bary err bits, [2] checked, [3] max scaled t err bits
 };
 
 __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
-    __shared__ double st[256], sb[256];
+    __shared__ double st[256], sb[256], ss[256];
     __shared__ uint32_t sn[256];
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    double et = 0.0, eb = 0.0;
+    double et = 0.0, eb = 0.0, es = 0.0;
     uint32_t n = 0;
     if (i < a.count) {
         const rq_hit h = a.hits[i];
@@ -848,18 +854,25 @@ __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
                 double t = (ac[0] * q[0] + ac[1] * q[1] + ac[2] * q[2]) / det;
                 et = fabs((double)h.t - t) / fmax(fabs(t), 1e-300);
                 eb = fmax(fabs((double)h.u - u), fabs((double)h.v - v));
+                // The distance error against the size of the coordinates it was computed from:
+                // f32 rounding of the hit point, not of t, bounds a short hit far from the origin.
+                double dn = sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+                double on = sqrt(ro[0] * ro[0] + ro[1] * ro[1] + ro[2] * ro[2]);
+                es = fabs((double)h.t - t) * dn / fmax(on + fabs(t) * dn, 1e-300);
                 n = 1;
             }
         }
     }
     st[threadIdx.x] = et;
     sb[threadIdx.x] = eb;
+    ss[threadIdx.x] = es;
     sn[threadIdx.x] = n;
     __syncthreads();
     for (uint32_t off = 128; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
             st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + off]);
             sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + off]);
+            ss[threadIdx.x] = fmax(ss[threadIdx.x], ss[threadIdx.x + off]);
             sn[threadIdx.x] += sn[threadIdx.x + off];
         }
         __syncthreads();
@@ -868,6 +881,7 @@ __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
         atomicMax(a.out, (unsigned long long)__double_as_longlong(st[0]));
         atomicMax(a.out + 1, (unsigned long long)__double_as_longlong(sb[0]));
         atomicAdd(a.out + 2, (unsigned long long)sn[0]);
+        atomicMax(a.out + 3, (unsigned long long)__double_as_longlong(ss[0]));
     }
 }
 
@@ -1466,8 +1480,8 @@ extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const r
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     unsigned long long* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, 3 * sizeof(unsigned long long)), "hipMalloc verify");
-    HIP_TRY(hipMemsetAsync(acc, 0, 3 * sizeof(unsigned long long), s), "hipMemset verify");
+    HIP_TRY(hipMalloc(&acc, 4 * sizeof(unsigned long long)), "hipMalloc verify");
+    HIP_TRY(hipMemsetAsync(acc, 0, 4 * sizeof(unsigned long long), s), "hipMemset verify");
     dev::VerifyArgs a;
     std::memset(&a, 0, sizeof a);
     if (scene->dev[1].ready) {
@@ -1482,17 +1496,32 @@ extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const r
     a.count = count;
     a.out = acc;
     if (count) hipLaunchKernelGGL(dev::verify_kernel, dim3((count + 255) / 256), dim3(256), 0, s, a);
-    unsigned long long host[3];
+    unsigned long long host[4];
     hipError_t e = hipMemcpyAsync(host, acc, sizeof host, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(acc);
     if (e != hipSuccess) return hip_fail(e, "verify");
-    double t, b;
+    double t, b, sc;
     std::memcpy(&t, &host[0], 8);
     std::memcpy(&b, &host[1], 8);
+    std::memcpy(&sc, &host[3], 8);
     out[0] = t;
     out[1] = b;
     out[2] = (double)host[2];
+    out[3] = sc;
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src || bytes % 16) {
+        set_error("rq_device_copy: null pointer or size not a multiple of 16");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    uint64_t n = bytes / 16;
+    hipLaunchKernelGGL(dev::copy_kernel, dim3((unsigned)std::min<uint64_t>((n + 255) / 256, 8192)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), reinterpret_cast<uint4*>(dst),
+                       reinterpret_cast<const uint4*>(src), n);
+    HIP_TRY(hipGetLastError(), "copy launch");
     return RQ_OK;
 }
 

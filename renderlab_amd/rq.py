@@ -118,6 +118,7 @@ _sig("rq_free", _i, _P)
 _sig("rq_memcpy_h2d", _i, _P, _P, _sz)
 _sig("rq_memcpy_d2h", _i, _P, _P, _sz)
 _sig("rq_memset", _i, _P, _i, _sz)
+_sig("rq_device_copy", _i, _P, _P, _sz, _P)
 _sig("rq_synchronize", _i)
 _sig("rq_stream_create", _i, ctypes.POINTER(_P))
 _sig("rq_stream_destroy", _i, _P)
@@ -207,6 +208,10 @@ class DeviceBuffer:
 
 def synchronize():
     check(lib.rq_synchronize())
+
+
+def device_copy(dst, src, nbytes, stream=None):
+    check(lib.rq_device_copy(_dptr(dst), _dptr(src), int(nbytes), stream.ptr if stream else None))
 
 
 class Stream:
@@ -510,9 +515,10 @@ class Scene:
         return n.value
 
     def verify_fp64(self, rays, hits, count, stream=None):
-        out = np.zeros(3, np.float64)
+        out = np.zeros(4, np.float64)
         check(lib.rq_verify_fp64(self._h, rays.ptr, hits.ptr, int(count), _ptr(out), stream.ptr if stream else None))
-        return dict(max_t_rel=float(out[0]), max_bary_abs=float(out[1]), checked=int(out[2]))
+        return dict(max_t_rel=float(out[0]), max_bary_abs=float(out[1]), checked=int(out[2]),
+                    max_t_scaled=float(out[3]))
 
     def destroy(self):
         if self._h:
