@@ -152,6 +152,11 @@ rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]);
 /* Uploads the requested layouts (bit 1<<rq_layout) to the current device. */
 rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
 rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes);
+/* Host image of a relayout (PLAIN or CW8), built on demand without a device:
+   part 0 instance records, 1 top-level nodes, 2 BLAS nodes, 3 triangles,
+   4 position indices, 5 {top_on, top_root, instances, 0}. For inspection. */
+rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
+                                size_t* bytes);
 void rq_scene_destroy(rq_scene* scene);
 
 /* ---- Trace: gi_closest / gi_visible / gi_visible_within over a batch.

@@ -157,3 +157,40 @@ extern "C" rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]) {
     std::memcpy(stats, scene->stats, sizeof scene->stats);
     return RQ_OK;
 }
+
+extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
+                                           size_t* bytes) {
+    if (!scene || !data || !bytes || (int)layout < 1 || (int)layout > 2 || part > 5) {
+        set_error("rq_scene_layout_image: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
+    std::string err;
+    if (h.inst.empty() && scene->words[0] > 0) {
+        if (scene->plain.inst.empty() && !build_plain(scene->words, scene->plain, err)) {
+            set_error("plain relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+        if (layout == RQ_LAYOUT_CW8_Q8 && !build_cw8(scene->words, scene->plain, scene->cw8, err)) {
+            set_error("cw8 relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    switch (part) {
+        case 0: *data = h.inst.data(); *bytes = h.inst.size() * 4; break;
+        case 1: *data = h.top.data(); *bytes = h.top.size() * 4; break;
+        case 2: *data = h.nodes.data(); *bytes = h.nodes.size() * 4; break;
+        case 3: *data = h.tris.data(); *bytes = h.tris.size() * 4; break;
+        case 4: *data = h.tri_ids.data(); *bytes = h.tri_ids.size() * 4; break;
+        default: {
+            static thread_local uint32_t meta[4];
+            meta[0] = h.top_on;
+            meta[1] = h.top_root;
+            meta[2] = h.instance_count;
+            meta[3] = 0;
+            *data = meta;
+            *bytes = sizeof meta;
+        }
+    }
+    return RQ_OK;
+}

@@ -12,10 +12,11 @@
 //   indices move to a side array read once per hit.
 // CW8_Q8: 8-wide nodes with 8-bit child boxes on a power-of-two grid (after
 //   Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
-//   Through Compressed Wide BVHs", HPG 2017), collapsed from the BVH4. A
-//   child's box encloses the PLAIN box of every subtree it holds, rounded
-//   outward, and decodes exactly as fma(2^e, q, origin) = origin + 2^e*q.
-//   Leaves and their triangle order are the BVH4's.
+//   Through Compressed Wide BVHs", HPG 2017), collapsed from the 4-wide
+//   trees, for the BLAS and the top level alike. A child's box encloses the
+//   PLAIN box of the subtree it holds, rounded outward, and decodes exactly
+//   as fma(2^e, q, origin) = origin + 2^e*q. Leaves and triangles are the
+//   PLAIN layout's, in the same order.
 #include <algorithm>
 #include <cmath>
 
@@ -27,6 +28,7 @@ namespace {
 constexpr uint32_t LEAF_COUNT_SHIFT = 27;   // leaf tag: LEAF | (n-1)<<27 | first triangle
 constexpr uint32_t MAX_LEAF_TRIS = 16;
 constexpr uint32_t MAX_FIRST_TRI = 1u << 27;
+constexpr uint32_t W8_WORDS = 24;
 
 struct MeshMap {
     uint32_t nodes_at;  // key: the mesh's node word offset
@@ -113,9 +115,11 @@ void instance_records(const std::vector<uint32_t>& words, const std::vector<Mesh
         uint32_t* o = &inst[(size_t)i * INSTANCE_WORDS];
         for (int k = 0; k < 18; ++k) o[k] = words[r + k];
         const MeshMap& m = meshes[instance_mesh[i]];
+        // 18 root tag, 19 flags (bit 0 casts no shadow, bit 1 primitive ids),
+        // 20 the mesh's first triangle, 21 mesh, 22 the reference record index.
         o[18] = roots[instance_mesh[i]];
-        o[19] = m.tri_base;
-        o[20] = (words[r + 27] & 1u) | (words[r + 28] ? 2u : 0u);
+        o[19] = (words[r + 27] & 1u) | (words[r + 28] ? 2u : 0u);
+        o[20] = m.tri_base;
         o[21] = instance_mesh[i];
         o[22] = i;
     }
@@ -207,7 +211,7 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
 namespace {
 
 struct Cand {
-    uint32_t child;  // PLAIN tag: node index or leaf tag
+    uint32_t child;  // tag in the source tree: node index or leaf tag
     Bounds box;
 };
 
@@ -217,9 +221,11 @@ float half_area(const Bounds& b) {
     return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
 }
 
-std::vector<Cand> plain_children(const HostLayout& plain, uint32_t node) {
+// Children of a 32-word 4-wide f32 node (the PLAIN BLAS node and the top-level
+// node share this format).
+std::vector<Cand> wide_children(const std::vector<uint32_t>& nodes, uint32_t node) {
     std::vector<Cand> c;
-    const uint32_t* w = &plain.nodes[(size_t)node * TOP_WORDS];
+    const uint32_t* w = &nodes[(size_t)node * TOP_WORDS];
     for (int s = 0; s < 4; ++s) {
         if (w[24 + s] == EMPTY) continue;
         Cand k;
@@ -233,112 +239,102 @@ std::vector<Cand> plain_children(const HostLayout& plain, uint32_t node) {
     return c;
 }
 
-struct Cw8Builder {
-    const HostLayout* plain;
-    std::vector<uint32_t>* nodes;  // 20 words a node
-    std::vector<float>* tris;
-    std::vector<uint32_t>* ids;
+// Collapses a 4-wide tree into 8-wide nodes of 24 words (96 B):
+//   0-2 origin (f32), 3 exponents x|y<<8|z<<16 and valid-slot mask <<24,
+//   4-15 byte planes (lo x, lo y, lo z, hi x, hi y, hi z; 8 codes, 2 words each),
+//   16-23 child tags (node index, leaf tag unchanged from the source, EMPTY).
+// A box decodes as fma(2^e, q, origin), which equals origin + 2^e*q exactly.
+struct W8Builder {
+    const std::vector<uint32_t>* src;
+    std::vector<uint32_t>* out;
     std::string* err;
 
-    // Quantizes `cands` into node `index` and recurses into inner children.
+    uint32_t alloc() {
+        uint32_t index = (uint32_t)(out->size() / W8_WORDS);
+        out->resize(out->size() + W8_WORDS, 0u);
+        return index;
+    }
+
     bool fill(uint32_t index, std::vector<Cand> cands) {
         // Open the largest inner candidate while its children fit in 8 slots.
         for (;;) {
             int pick = -1;
             for (int i = 0; i < (int)cands.size(); ++i) {
                 if (cands[i].child & LEAF_BIT) continue;
-                auto kids = plain_children(*plain, cands[i].child);
-                if (cands.size() - 1 + kids.size() > 8) continue;
+                if (cands.size() - 1 + wide_children(*src, cands[i].child).size() > 8) continue;
                 if (pick < 0 || half_area(cands[i].box) > half_area(cands[pick].box)) pick = i;
             }
             if (pick < 0) break;
-            auto kids = plain_children(*plain, cands[pick].child);
+            auto kids = wide_children(*src, cands[pick].child);
             cands.erase(cands.begin() + pick);
             cands.insert(cands.begin() + pick, kids.begin(), kids.end());
         }
         Bounds all = empty_bounds();
         for (auto& c : cands) all = merge(all, c.box);
-        uint32_t w[20] = {};
+        uint32_t w[W8_WORDS] = {};
         int exps[3];
         for (int a = 0; a < 3; ++a) {
-            float p = all.lo[a];
-            float top = all.hi[a];
+            float p = all.lo[a], top = all.hi[a];
             float ext = top - p;
             int e = -126;
             if (ext > 0.0f) {
                 int ex;
-                std::frexp(ext / 255.0f, &ex);  // ext/255 < 2^ex
+                std::frexp(ext / 255.0f, &ex);
                 e = std::max(ex - 1, -126);
             }
-            for (;;) {
-                float s = std::ldexp(1.0f, e);
-                if (p + s * 255.0f >= top) break;
-                ++e;
-            }
-            if (e > 127) {
+            while (p + std::ldexp(1.0f, e) * 255.0f < top) ++e;
+            if (e > 127 || !std::isfinite(p)) {
                 *err = "box too large to quantize";
                 return false;
             }
             exps[a] = e;
             w[a] = f_bits(p);
         }
-        uint32_t imask = 0, valid = 0, counts = 0;
-        uint32_t inner_n = 0;
-        for (auto& c : cands)
-            if (!(c.child & LEAF_BIT)) inner_n++;
-        uint32_t child_base = (uint32_t)(nodes->size() / 20);
-        nodes->resize(nodes->size() + (size_t)inner_n * 20, 0u);
-        uint32_t tri_base = (uint32_t)(tris->size() / 12);
-        std::vector<std::pair<uint32_t, std::vector<Cand>>> todo;
-        uint32_t rank = 0;
+        uint32_t valid = 0;
+        std::vector<std::pair<uint32_t, uint32_t>> todo;  // (new index, source node)
         for (size_t s = 0; s < cands.size(); ++s) {
             const Cand& c = cands[s];
             valid |= 1u << s;
             for (int a = 0; a < 3; ++a) {
                 float p = bits_f(w[a]);
                 float sc = std::ldexp(1.0f, exps[a]);
-                float fl = std::floor((c.box.lo[a] - p) / sc);
-                uint32_t ql = (uint32_t)mn(mx(fl, 0.0f), 255.0f);
+                uint32_t ql = (uint32_t)mn(mx(std::floor((c.box.lo[a] - p) / sc), 0.0f), 255.0f);
                 while (ql > 0 && p + sc * (float)ql > c.box.lo[a]) ql--;
-                float fh = std::ceil((c.box.hi[a] - p) / sc);
-                uint32_t qh = (uint32_t)mn(mx(fh, 0.0f), 255.0f);
+                uint32_t qh = (uint32_t)mn(mx(std::ceil((c.box.hi[a] - p) / sc), 0.0f), 255.0f);
                 while (qh < 255 && p + sc * (float)qh < c.box.hi[a]) qh++;
                 if (p + sc * (float)ql > c.box.lo[a] || p + sc * (float)qh < c.box.hi[a]) {
                     *err = "quantization failed to enclose";
                     return false;
                 }
-                // planes: w[8..9] lo x, [10..11] lo y, [12..13] lo z, [14..15] hi x, ...
-                w[8 + 2 * a + s / 4] |= ql << (8 * (s % 4));
-                w[14 + 2 * a + s / 4] |= qh << (8 * (s % 4));
+                w[4 + 2 * a + s / 4] |= ql << (8 * (s % 4));
+                w[10 + 2 * a + s / 4] |= qh << (8 * (s % 4));
             }
             if (c.child & LEAF_BIT) {
-                uint32_t n = ((c.child >> LEAF_COUNT_SHIFT) & 0xfu) + 1u;
-                if (n > 15) {
-                    *err = "leaf too large for an 8-wide node";
-                    return false;
-                }
-                counts |= n << (4 * s);
-                uint32_t first = c.child & (MAX_FIRST_TRI - 1);
-                for (uint32_t t = first; t < first + n; ++t) {
-                    for (int k = 0; k < 12; ++k) tris->push_back(plain->tris[12ull * t + k]);
-                    for (int k = 0; k < 4; ++k) ids->push_back(plain->tri_ids[4ull * t + k]);
-                }
+                w[16 + s] = c.child;
             } else {
-                imask |= 1u << s;
-                todo.push_back({child_base + rank, plain_children(*plain, c.child)});
-                rank++;
+                uint32_t ni = alloc();
+                w[16 + s] = ni;
+                todo.push_back({ni, c.child});
             }
         }
-        w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) |
-               ((uint32_t)(exps[2] + 127) << 16) | (imask << 24);
-        w[4] = child_base;
-        w[5] = tri_base;
-        w[6] = counts;
-        w[7] = valid;
-        std::copy(w, w + 20, nodes->begin() + (size_t)index * 20);
+        for (size_t s = cands.size(); s < 8; ++s) w[16 + s] = EMPTY;
+        w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16) |
+               (valid << 24);
+        std::copy(w, w + W8_WORDS, out->begin() + (size_t)index * W8_WORDS);
         for (auto& t : todo)
-            if (!fill(t.first, std::move(t.second))) return false;
+            if (!fill(t.first, wide_children(*src, t.second))) return false;
         return true;
+    }
+
+    // The 8-wide tree under a source root tag; returns the new root tag.
+    bool root(uint32_t src_root, uint32_t& out_root) {
+        if (src_root & LEAF_BIT) {
+            out_root = src_root;
+            return true;
+        }
+        uint32_t index = alloc();
+        out_root = index;
+        return fill(index, wide_children(*src, src_root));
     }
 };
 
@@ -346,38 +342,35 @@ struct Cw8Builder {
 
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
     out = HostLayout();
-    std::vector<MeshMap> meshes;
-    std::vector<uint32_t> instance_mesh;
-    if (!collect_meshes(words, meshes, instance_mesh, err)) return false;
-    std::vector<uint32_t> roots(meshes.size());
-    out.tris.reserve(plain.tris.size());
-    out.tri_ids.reserve(plain.tri_ids.size());
-    // PLAIN roots per mesh, from any instance of it.
-    std::vector<uint32_t> plain_root(meshes.size(), EMPTY);
-    for (uint32_t i = 0; i < words[0]; ++i) plain_root[instance_mesh[i]] = plain.inst[(size_t)i * INSTANCE_WORDS + 18];
-    Cw8Builder b{&plain, &out.nodes, &out.tris, &out.tri_ids, &err};
-    for (size_t mi = 0; mi < meshes.size(); ++mi) {
-        uint32_t pr = plain_root[mi];
-        meshes[mi].tri_base = (uint32_t)(out.tris.size() / 12);
-        if (pr & LEAF_BIT) {
-            uint32_t n = ((pr >> LEAF_COUNT_SHIFT) & 0xfu) + 1u, first = pr & (MAX_FIRST_TRI - 1);
-            uint32_t nf = (uint32_t)(out.tris.size() / 12);
-            for (uint32_t t = first; t < first + n; ++t) {
-                for (int k = 0; k < 12; ++k) out.tris.push_back(plain.tris[12ull * t + k]);
-                for (int k = 0; k < 4; ++k) out.tri_ids.push_back(plain.tri_ids[4ull * t + k]);
-            }
-            roots[mi] = leaf_tag(nf, n, err);
-            continue;
+    out.inst = plain.inst;
+    out.tris = plain.tris;
+    out.tri_ids = plain.tri_ids;
+    out.instance_count = plain.instance_count;
+    // BLAS: every mesh root reachable from an instance record, collapsed once.
+    W8Builder b{&plain.nodes, &out.nodes, &err};
+    std::vector<std::pair<uint32_t, uint32_t>> done;  // plain root -> cw8 root
+    for (uint32_t i = 0; i < plain.instance_count; ++i) {
+        uint32_t pr = plain.inst[(size_t)i * INSTANCE_WORDS + 18];
+        uint32_t nr = EMPTY;
+        for (auto& d : done)
+            if (d.first == pr) nr = d.second;
+        if (nr == EMPTY) {
+            if (!b.root(pr, nr)) return false;
+            done.push_back({pr, nr});
         }
-        uint32_t index = (uint32_t)(out.nodes.size() / 20);
-        out.nodes.resize(out.nodes.size() + 20, 0u);
-        roots[mi] = index;
-        if (!b.fill(index, plain_children(plain, pr))) return false;
+        out.inst[(size_t)i * INSTANCE_WORDS + 18] = nr;
     }
-    // The hit's BVH-order slot is recovered from the side array, so tri_base
-    // in the instance record is the mesh's first triangle in this layout.
-    instance_records(words, meshes, instance_mesh, roots, out.inst);
-    copy_top(words, out);
+    // Top level: the reference's 4-wide float top level collapsed the same way
+    // (instance leaves keep their LEAF | instance tags).
+    out.top_on = plain.top_on;
+    out.top_root = plain.top_root;
+    if (plain.top_on) {
+        W8Builder t{&plain.top, &out.top, &err};
+        if (!t.root(plain.top_root, out.top_root)) return false;
+    }
+    if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
+    if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);
+    (void)words;
     return true;
 }
 

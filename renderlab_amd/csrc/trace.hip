@@ -31,11 +31,11 @@ namespace dev {
 
 constexpr uint32_t NONE = 0xffffffffu;
 constexpr float FMAXV = 3.4028235e38f;
-// Per-lane stack words: 48 top-level entries (the reference's bound) plus
-// the BLAS bound: 48 for the 4-wide layouts (depth <= 15 pushes at most
-// 1 + 3*15), 112 for CW8 (at most 1 + 7*15; a CW8 tree is never deeper
-// than the BVH4 it is collapsed from).
-constexpr uint32_t SPILL_WORDS = 160;
+// Per-lane stack words: a top-level and a BLAS bound of 48 each for the
+// 4-wide layouts (the reference's; depth <= 15 pushes at most 1 + 3*15), of
+// 112 each for CW8 (at most 1 + 7*15; an 8-wide tree is never deeper than
+// the 4-wide tree it is collapsed from).
+constexpr uint32_t SPILL_WORDS = 224;
 constexpr uint32_t STACK_CAP = 48;
 constexpr uint32_t CW8_CAP = 112;
 constexpr uint32_t LEAF_MASK = (1u << 27) - 1u;
@@ -172,6 +172,12 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // AM: 0 closest hit, 1 any hit, 2 per batch (BATCHED launches mix modes).
 // BATCHED: rays come from a table of batches over several scenes; a lane
 // takes its batch's scene, mode and output buffers when it fetches a ray.
+//
+// Loop structure ("while-while"): in the node phase every lane pops and
+// visits nodes and instance entries until it holds a triangle leaf or its ray
+// is done; then all lanes holding a leaf test its triangles together. A lane
+// never pops past an untested leaf, so each ray visits exactly the sequence a
+// one-item-at-a-time traversal visits.
 template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
@@ -180,6 +186,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     const uint32_t lane = __lane_id();
     uint32_t* st = lds + tid;
     uint32_t* spill = a.spill + ((uint64_t)blockIdx.x * BLOCK + tid) * SPILL_WORDS;
+    constexpr uint32_t TOP_CAP = LAYOUT == CW8 ? CW8_CAP : STACK_CAP;
+    constexpr uint32_t BLAS_CAP = LAYOUT == CW8 ? CW8_CAP : STACK_CAP;
 
     // Lane state.
     uint32_t ray = NONE;
@@ -198,13 +206,13 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t* out_vis = a.vis;
     uint32_t out_idx = 0;
 
-    auto push = [&](uint32_t v) {
+    auto push = [&](uint32_t x) {
         if (sp - lo == (uint32_t)S) {
             spill[lo] = st[(lo & (S - 1)) * BLOCK];
             lo++;
             if (COUNT) nb += 4;
         }
-        st[(sp & (S - 1)) * BLOCK] = v;
+        st[(sp & (S - 1)) * BLOCK] = x;
         sp++;
     };
     auto pop = [&]() -> uint32_t {
@@ -272,84 +280,88 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
         }
         if (exhausted && __ballot(ray != NONE) == 0ull) break;
-        if (ray == NONE) continue;
 
-        // ---- one traversal step
-        bool fin = invalid;
-        if (!fin) {
+        // ---- node phase
+        uint32_t leaf = NONE;
+        bool fin = ray != NONE && invalid;
+        for (;;) {
+            const bool want = ray != NONE && !fin && leaf == NONE;
+            if (__ballot(want) == 0ull) break;
+            if (!want) continue;
             if (in_blas && sp == floor_) in_blas = false;
-            if (!in_blas) {
-                uint32_t item = 0;
-                bool have;
-                if (!v.top_on) {
-                    have = flat_next < v.n_inst;
-                    item = LEAF_BIT | flat_next;
-                    flat_next++;
-                } else {
-                    have = sp > 0;
-                    if (have) item = pop();
+            uint32_t item = 0;
+            bool have;
+            if (!in_blas && !v.top_on) {
+                have = flat_next < v.n_inst;
+                item = LEAF_BIT | flat_next;
+                flat_next++;
+            } else {
+                have = sp > 0;
+                if (have) item = pop();
+            }
+            if (!have) {
+                fin = true;
+                continue;
+            }
+            if (item & LEAF_BIT) {
+                if (in_blas) {
+                    leaf = item;
+                    continue;
                 }
-                if (!have) {
-                    fin = true;
-                } else if (item & LEAF_BIT) {
-                    // gi_instance prologue (:100-113)
-                    const uint32_t i = item & ~LEAF_BIT;
-                    const uint32_t* r = LAYOUT == REF ? v.words + 16ull + (uint64_t)i * 32u : v.inst + (uint64_t)i * 32u;
-                    uint32_t rw[24];
-                    if (LAYOUT == REF) {
+                // gi_instance prologue (:100-113)
+                const uint32_t i = item & ~LEAF_BIT;
+                uint32_t rw[20];
+                if (LAYOUT == REF) {
+                    const uint32_t* r = v.words + 16ull + (uint64_t)i * 32u;
 #pragma unroll
-                        for (int k = 0; k < 18; ++k) rw[k] = r[k];
-                        rw[20] = any_hit ? r[27] : 0u;
-                    } else {
-                        const uint4* r4 = reinterpret_cast<const uint4*>(r);
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            uint4 q = r4[k];
-                            rw[4 * k] = q.x; rw[4 * k + 1] = q.y; rw[4 * k + 2] = q.z; rw[4 * k + 3] = q.w;
-                        }
-                    }
-                    if (COUNT) nb += LAYOUT == REF ? (any_hit ? 76 : 72) : 96;
-                    bool skip = any_hit && (rw[20] & 1u);
-                    float lim = limit(i);
-                    if (!skip && slab_near(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
-                                           mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim) {
-                        V3 r0 = mk(bf(rw[0]), bf(rw[1]), bf(rw[2])), r1 = mk(bf(rw[4]), bf(rw[5]), bf(rw[6])),
-                           r2 = mk(bf(rw[8]), bf(rw[9]), bf(rw[10]));
-                        oo = mk(dot3(r0, wo) + bf(rw[3]) * 1.0f, dot3(r1, wo) + bf(rw[7]) * 1.0f,
-                                dot3(r2, wo) + bf(rw[11]) * 1.0f);
-                        od = mk(dot3(r0, wd), dot3(r1, wd), dot3(r2, wd));
-                        oinv = safe_inverse(od);
-                        cur = i;
-                        uint32_t root;
-                        if (LAYOUT == REF) {
-                            m_nodes = r[18];
-                            m_leaves = r[19];
-                            m_pos = r[20];
-                            m_idx = r[21];
-                            root = r[22];
-                            if (COUNT) nb += 20;
-                        } else {
-                            root = rw[18];
-                        }
-                        floor_ = sp;
-                        in_blas = true;
-                        push(root);
-                    }
+                    for (int k = 0; k < 18; ++k) rw[k] = r[k];
+                    rw[19] = any_hit ? r[27] : 0u;
+                    if (COUNT) nb += any_hit ? 76 : 72;
                 } else {
-                    // top-level wide node (:240-261)
+                    const uint4* r4 = reinterpret_cast<const uint4*>(v.inst + (uint64_t)i * 32u);
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        uint4 q = r4[k];
+                        rw[4 * k] = q.x; rw[4 * k + 1] = q.y; rw[4 * k + 2] = q.z; rw[4 * k + 3] = q.w;
+                    }
+                    if (COUNT) nb += 80;
+                }
+                const bool skip = any_hit && (rw[19] & 1u);
+                const float lim = limit(i);
+                if (!skip && slab_near(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
+                                       mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim) {
+                    V3 r0 = mk(bf(rw[0]), bf(rw[1]), bf(rw[2])), r1 = mk(bf(rw[4]), bf(rw[5]), bf(rw[6])),
+                       r2 = mk(bf(rw[8]), bf(rw[9]), bf(rw[10]));
+                    oo = mk(dot3(r0, wo) + bf(rw[3]) * 1.0f, dot3(r1, wo) + bf(rw[7]) * 1.0f,
+                            dot3(r2, wo) + bf(rw[11]) * 1.0f);
+                    od = mk(dot3(r0, wd), dot3(r1, wd), dot3(r2, wd));
+                    oinv = safe_inverse(od);
+                    cur = i;
+                    uint32_t root;
+                    if (LAYOUT == REF) {
+                        const uint32_t* r = v.words + 16ull + (uint64_t)i * 32u;
+                        m_nodes = r[18];
+                        m_leaves = r[19];
+                        m_pos = r[20];
+                        m_idx = r[21];
+                        root = r[22];
+                        if (COUNT) nb += 20;
+                    } else {
+                        root = rw[18];
+                    }
+                    floor_ = sp;
+                    in_blas = true;
+                    push(root);
+                }
+                continue;
+            }
+            if (LAYOUT == REF) {
+                if (!in_blas) {
+                    // top-level wide node (:240-261), the reference's own sort network
                     const uint32_t* w = v.top + (uint64_t)item * 32u;
                     uint32_t tw[28];
-                    if (LAYOUT == REF) {
 #pragma unroll
-                        for (int k = 0; k < 28; ++k) tw[k] = w[k];
-                    } else {
-                        const uint4* w4 = reinterpret_cast<const uint4*>(w);
-#pragma unroll
-                        for (int k = 0; k < 7; ++k) {
-                            uint4 q = w4[k];
-                            tw[4 * k] = q.x; tw[4 * k + 1] = q.y; tw[4 * k + 2] = q.z; tw[4 * k + 3] = q.w;
-                        }
-                    }
+                    for (int k = 0; k < 28; ++k) tw[k] = w[k];
                     if (COUNT) nb += 112;
                     float bound = bhit ? bf(fb(bt) + 1u) : bt;
                     float dist[4];
@@ -382,164 +394,171 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 #pragma unroll
                     for (int l = 3; l >= 0; --l)
                         if (dist[l] <= bound && sp < STACK_CAP) push(ids[l]);
+                    continue;
+                }
+                // BLAS node (:171-204), decoded from the quantized record
+                const float lim = limit(cur);
+                const uint32_t* w = v.words + m_nodes + (uint64_t)item * 16u;
+                uint32_t nw[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) nw[k] = w[k];
+                if (COUNT) nb += 64;
+                V3 oq = mk(bf(nw[0]), bf(nw[1]), bf(nw[2])), stp = mk(bf(nw[3]), bf(nw[4]), bf(nw[5]));
+                float t[4];
+                uint32_t c[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    c[s] = nw[6 + s];
+                    t[s] = FMAXV;
+                    if (c[s] != EMPTY) {
+                        uint32_t sh = 8u * s;
+                        V3 lq = mk((float)((nw[10] >> sh) & 0xffu), (float)((nw[11] >> sh) & 0xffu),
+                                   (float)((nw[12] >> sh) & 0xffu));
+                        V3 hq = mk((float)((nw[13] >> sh) & 0xffu), (float)((nw[14] >> sh) & 0xffu),
+                                   (float)((nw[15] >> sh) & 0xffu));
+                        V3 lo3 = mk(oq.x + stp.x * lq.x, oq.y + stp.y * lq.y, oq.z + stp.z * lq.z);
+                        V3 hi3 = mk(oq.x + stp.x * hq.x, oq.y + stp.y * hq.y, oq.z + stp.z * hq.z);
+                        t[s] = slab_near(oo, oinv, lo3, hi3, tmin, lim);
+                    }
+                }
+                uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
+                cas_desc(k0, c[0], k1, c[1]);
+                cas_desc(k2, c[2], k3, c[3]);
+                cas_desc(k0, c[0], k2, c[2]);
+                cas_desc(k1, c[1], k3, c[3]);
+                cas_desc(k1, c[1], k2, c[2]);
+                if (bf(k0) <= lim && sp - floor_ < STACK_CAP) push(c[0]);
+                if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
+                if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
+                if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
+                continue;
+            }
+            // PLAIN / CW8: one node format for the top level and the BLAS; the
+            // level only picks the ray, the bound and the stack budget.
+            const bool blas = in_blas;
+            const V3 ro = blas ? oo : wo, ri = blas ? oinv : winv;
+            const float lim = blas ? limit(cur) : (bhit ? bf(fb(bt) + 1u) : bt);
+            const uint32_t cap = blas ? floor_ + BLAS_CAP : TOP_CAP;
+            if (LAYOUT == PLAIN) {
+                const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 32u);
+                uint32_t nw[28];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    uint4 q = w4[k];
+                    nw[4 * k] = q.x; nw[4 * k + 1] = q.y; nw[4 * k + 2] = q.z; nw[4 * k + 3] = q.w;
+                }
+                if (COUNT) nb += 112;
+                float t[4];
+                uint32_t c[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    c[s] = nw[24 + s];
+                    t[s] = FMAXV;
+                    if (c[s] != EMPTY)
+                        t[s] = slab_near(ro, ri, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
+                                         mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
+                }
+                uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
+                cas_desc(k0, c[0], k1, c[1]);
+                cas_desc(k2, c[2], k3, c[3]);
+                cas_desc(k0, c[0], k2, c[2]);
+                cas_desc(k1, c[1], k3, c[3]);
+                cas_desc(k1, c[1], k2, c[2]);
+                if (bf(k0) <= lim && sp < cap) push(c[0]);
+                if (bf(k1) <= lim && sp < cap) push(c[1]);
+                if (bf(k2) <= lim && sp < cap) push(c[2]);
+                if (bf(k3) <= lim && sp < cap) push(c[3]);
+            } else {
+                // CW8 node: 24 words (origin, exponents + valid mask, byte planes, tags)
+                const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
+                uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4], q5 = w4[5];
+                if (COUNT) nb += 96;
+                const V3 p = mk(bf(q0.x), bf(q0.y), bf(q0.z));
+                const uint32_t valid = q0.w >> 24;
+                const V3 sc = mk(bf((q0.w & 0xffu) << 23), bf(((q0.w >> 8) & 0xffu) << 23),
+                                 bf(((q0.w >> 16) & 0xffu) << 23));
+                const uint32_t plane[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+                const uint32_t tag[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
+                uint32_t kk[8], cc[8];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    kk[s] = fb(FMAXV);
+                    cc[s] = tag[s];
+                    if ((valid >> s) & 1u) {
+                        const int wi = s >> 2, sh = 8 * (s & 3);
+                        V3 lq = mk((float)((plane[0 + wi] >> sh) & 0xffu), (float)((plane[2 + wi] >> sh) & 0xffu),
+                                   (float)((plane[4 + wi] >> sh) & 0xffu));
+                        V3 hq = mk((float)((plane[6 + wi] >> sh) & 0xffu), (float)((plane[8 + wi] >> sh) & 0xffu),
+                                   (float)((plane[10 + wi] >> sh) & 0xffu));
+                        V3 lo3 = mk(__builtin_fmaf(sc.x, lq.x, p.x), __builtin_fmaf(sc.y, lq.y, p.y),
+                                    __builtin_fmaf(sc.z, lq.z, p.z));
+                        V3 hi3 = mk(__builtin_fmaf(sc.x, hq.x, p.x), __builtin_fmaf(sc.y, hq.y, p.y),
+                                    __builtin_fmaf(sc.z, hq.z, p.z));
+                        kk[s] = fb(slab_near(ro, ri, lo3, hi3, tmin, lim));
+                    }
+                }
+                // Batcher odd-even merge network for 8, descending.
+                cas_desc(kk[0], cc[0], kk[1], cc[1]); cas_desc(kk[2], cc[2], kk[3], cc[3]);
+                cas_desc(kk[4], cc[4], kk[5], cc[5]); cas_desc(kk[6], cc[6], kk[7], cc[7]);
+                cas_desc(kk[0], cc[0], kk[2], cc[2]); cas_desc(kk[1], cc[1], kk[3], cc[3]);
+                cas_desc(kk[4], cc[4], kk[6], cc[6]); cas_desc(kk[5], cc[5], kk[7], cc[7]);
+                cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[5], cc[5], kk[6], cc[6]);
+                cas_desc(kk[0], cc[0], kk[4], cc[4]); cas_desc(kk[1], cc[1], kk[5], cc[5]);
+                cas_desc(kk[2], cc[2], kk[6], cc[6]); cas_desc(kk[3], cc[3], kk[7], cc[7]);
+                cas_desc(kk[2], cc[2], kk[4], cc[4]); cas_desc(kk[3], cc[3], kk[5], cc[5]);
+                cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[3], cc[3], kk[4], cc[4]);
+                cas_desc(kk[5], cc[5], kk[6], cc[6]);
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    if (bf(kk[s]) <= lim && sp < cap) push(cc[s]);
+            }
+        }
+
+        // ---- leaf phase (:117-169): triangles in leaf order against the running bound
+        if (leaf != NONE) {
+            const float lim0 = limit(cur);
+            if (LAYOUT == REF) {
+                uint64_t l = m_leaves + (uint64_t)(leaf & ~LEAF_BIT) * 2u;
+                uint32_t first = v.words[l], cnt = v.words[l + 1];
+                if (COUNT) nb += 8;
+                float lim_t = lim0;
+                for (uint32_t k = 0; k < cnt; k += 3) {
+                    uint64_t x = (uint64_t)m_idx + first + k;
+                    uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
+                    const uint32_t* P = v.words + m_pos;
+                    V3 va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
+                    V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
+                    V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
+                    if (COUNT) nb += 48;
+                    float u, vv;
+                    float t = tri_test(oo, od, va, vb, vc, tmin, lim_t, u, vv);
+                    if (t >= 0.0f) {
+                        bt = t; bu = u; bv = vv; binst = cur; bhit = true;
+                        btri = (first + k) / 3u;
+                        lim_t = limit(cur);
+                        if (any_hit) { fin = true; break; }
+                    }
                 }
             } else {
-                const uint32_t e = pop();
-                const float lim = limit(cur);
-                if (e & LEAF_BIT) {
-                    // leaf (:117-169)
-                    if (LAYOUT == REF) {
-                        uint64_t l = m_leaves + (uint64_t)(e & ~LEAF_BIT) * 2u;
-                        uint32_t first = v.words[l], count = v.words[l + 1];
-                        if (COUNT) nb += 8;
-                        float lim_t = lim;
-                        for (uint32_t k = 0; k < count; k += 3) {
-                            uint64_t x = (uint64_t)m_idx + first + k;
-                            uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
-                            const uint32_t* P = v.words + m_pos;
-                            V3 va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
-                            V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
-                            V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
-                            if (COUNT) nb += 48;
-                            float u, v;
-                            float t = tri_test(oo, od, va, vb, vc, tmin, lim_t, u, v);
-                            if (t >= 0.0f) {
-                                bt = t; bu = u; bv = v; binst = cur; bhit = true;
-                                btri = (first + k) / 3u;
-                                lim_t = limit(cur);
-                                if (any_hit) { fin = true; break; }
-                            }
-                        }
-                    } else {
-                        uint32_t first = e & LEAF_MASK, n = ((e >> 27) & 0xfu) + 1u;
-                        float lim_t = lim;
-                        for (uint32_t j = 0; j < n; ++j) {
-                            const float4* tp = v.tris + 3ull * (first + j);
-                            float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                            if (COUNT) nb += 48;
-                            float u, v;
-                            float t = tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t0.w, t1.x, t1.y),
-                                               mk(t1.z, t1.w, t2.x), tmin, lim_t, u, v);
-                            if (t >= 0.0f) {
-                                bt = t; bu = u; bv = v; binst = cur; bhit = true;
-                                btri = first + j;
-                                lim_t = limit(cur);
-                                if (any_hit) { fin = true; break; }
-                            }
-                        }
+                uint32_t first = leaf & LEAF_MASK, n = ((leaf >> 27) & 0xfu) + 1u;
+                float lim_t = lim0;
+                for (uint32_t j = 0; j < n; ++j) {
+                    const float4* tp = v.tris + 3ull * (first + j);
+                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    if (COUNT) nb += 48;
+                    float u, vv;
+                    float t = tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t0.w, t1.x, t1.y), mk(t1.z, t1.w, t2.x), tmin,
+                                       lim_t, u, vv);
+                    if (t >= 0.0f) {
+                        bt = t; bu = u; bv = vv; binst = cur; bhit = true;
+                        btri = first + j;
+                        lim_t = limit(cur);
+                        if (any_hit) { fin = true; break; }
                     }
-                } else if (LAYOUT == REF || LAYOUT == PLAIN) {
-                    // BLAS node (:171-204)
-                    float t[4];
-                    uint32_t c[4];
-                    if (LAYOUT == REF) {
-                        const uint32_t* w = v.words + m_nodes + (uint64_t)e * 16u;
-                        uint32_t nw[16];
-#pragma unroll
-                        for (int k = 0; k < 16; ++k) nw[k] = w[k];
-                        if (COUNT) nb += 64;
-                        V3 oq = mk(bf(nw[0]), bf(nw[1]), bf(nw[2])), stp = mk(bf(nw[3]), bf(nw[4]), bf(nw[5]));
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) {
-                            c[s] = nw[6 + s];
-                            t[s] = FMAXV;
-                            if (c[s] != EMPTY) {
-                                uint32_t sh = 8u * s;
-                                V3 lq = mk((float)((nw[10] >> sh) & 0xffu), (float)((nw[11] >> sh) & 0xffu),
-                                           (float)((nw[12] >> sh) & 0xffu));
-                                V3 hq = mk((float)((nw[13] >> sh) & 0xffu), (float)((nw[14] >> sh) & 0xffu),
-                                           (float)((nw[15] >> sh) & 0xffu));
-                                V3 lo3 = mk(oq.x + stp.x * lq.x, oq.y + stp.y * lq.y, oq.z + stp.z * lq.z);
-                                V3 hi3 = mk(oq.x + stp.x * hq.x, oq.y + stp.y * hq.y, oq.z + stp.z * hq.z);
-                                t[s] = slab_near(oo, oinv, lo3, hi3, tmin, lim);
-                            }
-                        }
-                    } else {
-                        const uint4* w4 = reinterpret_cast<const uint4*>(v.nodes + (uint64_t)e * 32u);
-                        uint32_t nw[28];
-                        if (COUNT) nb += 112;
-#pragma unroll
-                        for (int k = 0; k < 7; ++k) {
-                            uint4 q = w4[k];
-                            nw[4 * k] = q.x; nw[4 * k + 1] = q.y; nw[4 * k + 2] = q.z; nw[4 * k + 3] = q.w;
-                        }
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) {
-                            c[s] = nw[24 + s];
-                            t[s] = FMAXV;
-                            if (c[s] != EMPTY)
-                                t[s] = slab_near(oo, oinv, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
-                                                 mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
-                        }
-                    }
-                    uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
-                    cas_desc(k0, c[0], k1, c[1]);
-                    cas_desc(k2, c[2], k3, c[3]);
-                    cas_desc(k0, c[0], k2, c[2]);
-                    cas_desc(k1, c[1], k3, c[3]);
-                    cas_desc(k1, c[1], k2, c[2]);
-                    if (bf(k0) <= lim && sp - floor_ < STACK_CAP) push(c[0]);
-                    if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
-                    if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
-                    if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
-                } else {
-                    // CW8 node: 20 words
-                    const uint4* w4 = reinterpret_cast<const uint4*>(v.nodes + (uint64_t)e * 20u);
-                    uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4];
-                    if (COUNT) nb += 80;
-                    V3 p = mk(bf(q0.x), bf(q0.y), bf(q0.z));
-                    uint32_t ex = q0.w & 0xffu, ey = (q0.w >> 8) & 0xffu, ez = (q0.w >> 16) & 0xffu;
-                    uint32_t imask = q0.w >> 24;
-                    uint32_t child_base = q1.x, tri_base = q1.y, counts = q1.z, valid = q1.w & 0xffu;
-                    V3 sc = mk(bf(ex << 23), bf(ey << 23), bf(ez << 23));
-                    uint32_t plane[12] = {q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w};
-                    uint32_t kk[8], cc[8];
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        kk[s] = fb(FMAXV);
-                        cc[s] = EMPTY;
-                        if ((valid >> s) & 1u) {
-                            const int wi = s >> 2, sh = 8 * (s & 3);
-                            V3 lq = mk((float)((plane[0 + wi] >> sh) & 0xffu), (float)((plane[2 + wi] >> sh) & 0xffu),
-                                       (float)((plane[4 + wi] >> sh) & 0xffu));
-                            V3 hq = mk((float)((plane[6 + wi] >> sh) & 0xffu), (float)((plane[8 + wi] >> sh) & 0xffu),
-                                       (float)((plane[10 + wi] >> sh) & 0xffu));
-                            V3 lo3 = mk(__builtin_fmaf(sc.x, lq.x, p.x), __builtin_fmaf(sc.y, lq.y, p.y),
-                                        __builtin_fmaf(sc.z, lq.z, p.z));
-                            V3 hi3 = mk(__builtin_fmaf(sc.x, hq.x, p.x), __builtin_fmaf(sc.y, hq.y, p.y),
-                                        __builtin_fmaf(sc.z, hq.z, p.z));
-                            kk[s] = fb(slab_near(oo, oinv, lo3, hi3, tmin, lim));
-                            const uint32_t below = (1u << s) - 1u;
-                            if ((imask >> s) & 1u) {
-                                cc[s] = child_base + (uint32_t)__popc(imask & below);
-                            } else {
-                                uint32_t x = counts & ((1u << (4 * s)) - 1u);
-                                if (s == 0) x = 0u;
-                                uint32_t y = (x & 0x0f0f0f0fu) + ((x >> 4) & 0x0f0f0f0fu);
-                                uint32_t off = (y * 0x01010101u) >> 24;
-                                uint32_t nt = (counts >> (4 * s)) & 0xfu;
-                                cc[s] = LEAF_BIT | ((nt - 1u) << 27) | (tri_base + off);
-                            }
-                        }
-                    }
-                    // Batcher odd-even merge network for 8, descending.
-                    cas_desc(kk[0], cc[0], kk[1], cc[1]); cas_desc(kk[2], cc[2], kk[3], cc[3]);
-                    cas_desc(kk[4], cc[4], kk[5], cc[5]); cas_desc(kk[6], cc[6], kk[7], cc[7]);
-                    cas_desc(kk[0], cc[0], kk[2], cc[2]); cas_desc(kk[1], cc[1], kk[3], cc[3]);
-                    cas_desc(kk[4], cc[4], kk[6], cc[6]); cas_desc(kk[5], cc[5], kk[7], cc[7]);
-                    cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[5], cc[5], kk[6], cc[6]);
-                    cas_desc(kk[0], cc[0], kk[4], cc[4]); cas_desc(kk[1], cc[1], kk[5], cc[5]);
-                    cas_desc(kk[2], cc[2], kk[6], cc[6]); cas_desc(kk[3], cc[3], kk[7], cc[7]);
-                    cas_desc(kk[2], cc[2], kk[4], cc[4]); cas_desc(kk[3], cc[3], kk[5], cc[5]);
-                    cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[3], cc[3], kk[4], cc[4]);
-                    cas_desc(kk[5], cc[5], kk[6], cc[6]);
-#pragma unroll
-                    for (int s = 0; s < 8; ++s)
-                        if (bf(kk[s]) <= lim && sp - floor_ < CW8_CAP) push(cc[s]);
                 }
             }
         }
-        if (!fin) continue;
+        if (ray == NONE || !fin) continue;
 
         // ---- hit writeback
         uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
@@ -581,7 +600,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 tri0 = ids[0];
                 tri1 = ids[1];
                 tri2 = ids[2];
-                if (!any_hit && (r[20] & 2u)) {
+                if (!any_hit && (r[19] & 2u)) {
                     tri0 = fb(t2.z);
                     tri1 = 0;
                     tri2 = 0;
@@ -828,7 +847,7 @@ __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
                         (double)bf(w[2]) * r.direction[2];
             }
             if (a.inst) {
-                const float4* tp = a.tris + 3ull * (rec[19] + h.slot);
+                const float4* tp = a.tris + 3ull * (rec[20] + h.slot);
                 float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                 va[0] = t0.x; va[1] = t0.y; va[2] = t0.z;
                 vb[0] = t0.w; vb[1] = t1.x; vb[2] = t1.y;

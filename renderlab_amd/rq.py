@@ -102,6 +102,7 @@ _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
 _sig("rq_scene_device_bytes", _i, _P, _i, ctypes.POINTER(_sz))
 _sig("rq_scene_destroy", None, _P)
+_sig("rq_scene_layout_image", _i, _P, _i, _u32, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_trace", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_trace_indirect", _i, _P, _i, _P, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_trace_batches", _i, _P, _u32, _i, _P, _u32, _P)
@@ -478,6 +479,17 @@ class Scene:
         for l in layouts:
             mask |= 1 << (LAYOUTS[l] if isinstance(l, str) else int(l))
         check(lib.rq_scene_upload(self._h, mask))
+
+    def layout_image(self, layout, part):
+        """Host image of a relayout part (0 inst, 1 top, 2 nodes, 3 tris, 4 ids, 5 meta) as u32 words."""
+        p = _P()
+        n = _sz()
+        check(lib.rq_scene_layout_image(self._h, LAYOUTS.get(layout, layout), int(part), ctypes.byref(p),
+                                        ctypes.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, np.uint32)
+        buf = (ctypes.c_uint32 * (n.value // 4)).from_address(p.value)
+        return np.frombuffer(buf, np.uint32).copy()
 
     def device_bytes(self, layout):
         n = _sz()

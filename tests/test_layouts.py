@@ -1,0 +1,152 @@
+"""Relayouts, checked on the host: PLAIN boxes are the reference decode bit
+for bit; every CW8 child box (decoded exactly as the kernel decodes it)
+encloses the PLAIN box of the subtree it holds, at every level; both keep
+every triangle, in leaf order."""
+import numpy as np
+import pytest
+
+from renderlab_amd import rq
+
+LEAF, EMPTY = rq.LEAF_BIT, rq.EMPTY
+
+
+def plain_children(nodes, i):
+    w = nodes[i * 32:(i + 1) * 32]
+    f = w.view(np.float32)
+    out = []
+    for s in range(4):
+        if w[24 + s] == EMPTY:
+            continue
+        lo = np.array([f[a * 4 + s] for a in range(3)], np.float32)
+        hi = np.array([f[12 + a * 4 + s] for a in range(3)], np.float32)
+        out.append((int(w[24 + s]), lo, hi))
+    return out
+
+
+def w8_children(nodes, i):
+    w = nodes[i * 24:(i + 1) * 24]
+    p = w[:3].view(np.float32)
+    sc = [np.float32(2.0) ** (int((w[3] >> (8 * a)) & 0xFF) - 127) for a in range(3)]
+    valid = int(w[3] >> 24)
+    out = []
+    for s in range(8):
+        if not (valid >> s) & 1:
+            assert w[16 + s] == EMPTY
+            continue
+        def q(plane):
+            return (int(w[4 + plane * 2 + s // 4]) >> (8 * (s % 4))) & 0xFF
+        lo = np.array([p[a] + sc[a] * np.float32(q(a)) for a in range(3)], np.float32)
+        hi = np.array([p[a] + sc[a] * np.float32(q(3 + a)) for a in range(3)], np.float32)
+        out.append((int(w[16 + s]), lo, hi))
+    return out
+
+
+def leaf_ranges(tag):
+    return (tag & ((1 << 27) - 1), ((tag >> 27) & 0xF) + 1)
+
+
+@pytest.mark.parametrize("kind,seed,meshes,tris", [(0, 1, 0, 0), (1, 5, 60, 20000)])
+def test_plain_boxes_are_the_reference_decode(kind, seed, meshes, tris):
+    sc = rq.Scene.generate(kind, seed, meshes, tris)
+    w = sc.words
+    inst = sc.layout_image("plain", 0).reshape(-1, 32)
+    nodes = sc.layout_image("plain", 2)
+    r = 16
+    nodes_at = int(w[r + 18])
+    node_count = (int(w[r + 19]) - nodes_at) // 16
+    root = int(inst[0, 18])
+    if root & LEAF:
+        return
+    base = root - int(w[r + 22])  # the mesh's first node in the PLAIN node array
+    for n in range(node_count):
+        ref = w[nodes_at + 16 * n: nodes_at + 16 * (n + 1)]
+        o, s_ = ref[:3].view(np.float32), ref[3:6].view(np.float32)
+        kids = plain_children(nodes, base + n)
+        for k, (tag, lo, hi) in enumerate(kids):
+            ql = np.array([(int(ref[10 + a]) >> (8 * k)) & 0xFF for a in range(3)], np.float32)
+            qh = np.array([(int(ref[13 + a]) >> (8 * k)) & 0xFF for a in range(3)], np.float32)
+            assert np.array_equal(lo.view(np.uint32), (o + s_ * ql).astype(np.float32).view(np.uint32))
+            assert np.array_equal(hi.view(np.uint32), (o + s_ * qh).astype(np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("kind,seed,meshes,tris", [(0, 1, 0, 0), (1, 5, 60, 20000)])
+def test_cw8_boxes_enclose_the_plain_subtrees(kind, seed, meshes, tris):
+    sc = rq.Scene.generate(kind, seed, meshes, tris)
+    pn, cn = sc.layout_image("plain", 2), sc.layout_image("cw8", 2)
+    pi, ci = sc.layout_image("plain", 0).reshape(-1, 32), sc.layout_image("cw8", 0).reshape(-1, 32)
+    ptris, ctris = sc.layout_image("plain", 3), sc.layout_image("cw8", 3)
+    assert np.array_equal(ptris, ctris)
+
+    def plain_leaves(tag):
+        if tag & LEAF:
+            return [tag]
+        out = []
+        for t, _, _ in plain_children(pn, tag):
+            out += plain_leaves(t)
+        return out
+
+    def plain_box_of_leaf(root):
+        """The box each leaf tag has in its PLAIN parent."""
+        boxes = {}
+        stack = [root]
+        while stack:
+            t = stack.pop()
+            if t & LEAF:
+                continue
+            for c, lo, hi in plain_children(pn, t):
+                if c & LEAF:
+                    boxes[c] = (lo, hi)
+                else:
+                    stack.append(c)
+        return boxes
+
+    checked = 0
+    for i in range(len(pi)):
+        proot, croot = int(pi[i, 18]), int(ci[i, 18])
+        if proot & LEAF:
+            assert croot == proot
+            continue
+        leaf_box = plain_box_of_leaf(proot)
+        want = sorted(plain_leaves(proot))
+        got = []
+        stack = [(croot, None)]
+        while stack:
+            t, box = stack.pop()
+            if t & LEAF:
+                got.append(t)
+                lo, hi = leaf_box[t]
+                assert (box[0] <= lo).all() and (box[1] >= hi).all()
+                checked += 1
+                continue
+            for c, lo, hi in w8_children(cn, t):
+                if box is not None:
+                    pass
+                stack.append((c, (lo, hi)))
+        assert sorted(got) == want
+        if i > 40:
+            break
+    assert checked > 0
+
+
+def test_cw8_top_level_reaches_every_instance_once():
+    sc = rq.Scene.generate(0, 2)
+    meta = sc.layout_image("cw8", 5)
+    top = sc.layout_image("cw8", 1)
+    assert meta[0] == 1
+    seen = np.zeros(int(meta[2]), int)
+    stack = [int(meta[1])]
+    w = sc.words
+    while stack:
+        t = stack.pop()
+        if t & LEAF:
+            i = t & ~LEAF & 0xFFFFFFFF
+            seen[i] += 1
+            continue
+        for c, lo, hi in w8_children(top, t):
+            if c & LEAF:
+                i = c & ~LEAF & 0xFFFFFFFF
+                r = 16 + 32 * i
+                blo, bhi = w[r + 12:r + 15].view(np.float32), w[r + 15:r + 18].view(np.float32)
+                assert (lo <= blo).all() and (hi >= bhi).all()
+            stack.append(c)
+    assert (seen == 1).all()
