@@ -173,11 +173,12 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // BATCHED: rays come from a table of batches over several scenes; a lane
 // takes its batch's scene, mode and output buffers when it fetches a ray.
 //
-// Loop structure ("while-while"): in the node phase every lane pops and
-// visits nodes and instance entries until it holds a triangle leaf or its ray
-// is done; then all lanes holding a leaf test its triangles together. A lane
-// never pops past an untested leaf, so each ray visits exactly the sequence a
-// one-item-at-a-time traversal visits.
+// Loop structure: every iteration each lane takes exactly one small step --
+// visit one node, enter one instance, or test the next two triangles of its
+// current leaf (the reference's pair step) -- so a wave never executes a
+// whole leaf loop for one lane while the others wait on nodes, and the lanes
+// whose rays finished are refilled at the top of every iteration. Each ray
+// still visits exactly the sequence a one-item-at-a-time traversal visits.
 template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     bool bhit = false, invalid = false, in_blas = false;
     uint32_t sp = 0, lo = 0, floor_ = 0, flat_next = 0;
     uint32_t m_nodes = 0, m_leaves = 0, m_pos = 0, m_idx = 0;
+    uint32_t tri_next = 0, tri_left = 0;  // the current leaf's untested triangles
     bool exhausted = false;
     unsigned long long nb = 0;
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
@@ -275,19 +277,51 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     lo = 0;
                     in_blas = false;
                     flat_next = 0;
+                    tri_left = 0;
                     if (!invalid && v.top_on) push(v.top_root);
                 }
             }
         }
         if (exhausted && __ballot(ray != NONE) == 0ull) break;
+        if (ray == NONE) continue;
 
-        // ---- node phase
-        uint32_t leaf = NONE;
-        bool fin = ray != NONE && invalid;
-        for (;;) {
-            const bool want = ray != NONE && !fin && leaf == NONE;
-            if (__ballot(want) == 0ull) break;
-            if (!want) continue;
+        bool fin = invalid;
+        if (!fin && tri_left > 0) {
+            // ---- triangle step (:126-168): the next two triangles in leaf order
+            const uint32_t k = tri_left >= 2 ? 2u : 1u;
+#pragma unroll
+            for (uint32_t j = 0; j < 2; ++j) {
+                if (j >= k || fin) break;
+                V3 va, vb, vc;
+                uint32_t slot_id;
+                if (LAYOUT == REF) {
+                    uint64_t x = (uint64_t)m_idx + 3ull * (tri_next + j);
+                    uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
+                    const uint32_t* P = v.words + m_pos;
+                    va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
+                    vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
+                    vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
+                    slot_id = tri_next + j;
+                } else {
+                    const float4* tp = v.tris + 3ull * (tri_next + j);
+                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    va = mk(t0.x, t0.y, t0.z);
+                    vb = mk(t0.w, t1.x, t1.y);
+                    vc = mk(t1.z, t1.w, t2.x);
+                    slot_id = tri_next + j;
+                }
+                if (COUNT) nb += 48;
+                float u, vv;
+                float t = tri_test(oo, od, va, vb, vc, tmin, limit(cur), u, vv);
+                if (t >= 0.0f) {
+                    bt = t; bu = u; bv = vv; binst = cur; bhit = true;
+                    btri = slot_id;
+                    if (any_hit) fin = true;
+                }
+            }
+            tri_next += k;
+            tri_left -= k;
+        } else if (!fin) {
             if (in_blas && sp == floor_) in_blas = false;
             uint32_t item = 0;
             bool have;
@@ -301,13 +335,19 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
             if (!have) {
                 fin = true;
-                continue;
-            }
-            if (item & LEAF_BIT) {
+            } else if (item & LEAF_BIT) {
                 if (in_blas) {
-                    leaf = item;
-                    continue;
-                }
+                    // a leaf: its triangles are tested by the next steps
+                    if (LAYOUT == REF) {
+                        uint64_t l = m_leaves + (uint64_t)(item & ~LEAF_BIT) * 2u;
+                        tri_next = v.words[l] / 3u;
+                        tri_left = v.words[l + 1] / 3u;
+                        if (COUNT) nb += 8;
+                    } else {
+                        tri_next = item & LEAF_MASK;
+                        tri_left = ((item >> 27) & 0xfu) + 1u;
+                    }
+                } else {
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
@@ -353,9 +393,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     in_blas = true;
                     push(root);
                 }
-                continue;
-            }
-            if (LAYOUT == REF) {
+                }
+            } else if (LAYOUT == REF) {
                 if (!in_blas) {
                     // top-level wide node (:240-261), the reference's own sort network
                     const uint32_t* w = v.top + (uint64_t)item * 32u;
@@ -394,8 +433,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 #pragma unroll
                     for (int l = 3; l >= 0; --l)
                         if (dist[l] <= bound && sp < STACK_CAP) push(ids[l]);
-                    continue;
-                }
+                } else {
                 // BLAS node (:171-204), decoded from the quantized record
                 const float lim = limit(cur);
                 const uint32_t* w = v.words + m_nodes + (uint64_t)item * 16u;
@@ -431,8 +469,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
                 if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
                 if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
-                continue;
-            }
+                }
+            } else {
             // PLAIN / CW8: one node format for the top level and the BLAS; the
             // level only picks the ray, the bound and the stack budget.
             const bool blas = in_blas;
@@ -512,53 +550,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 for (int s = 0; s < 8; ++s)
                     if (bf(kk[s]) <= lim && sp < cap) push(cc[s]);
             }
-        }
-
-        // ---- leaf phase (:117-169): triangles in leaf order against the running bound
-        if (leaf != NONE) {
-            const float lim0 = limit(cur);
-            if (LAYOUT == REF) {
-                uint64_t l = m_leaves + (uint64_t)(leaf & ~LEAF_BIT) * 2u;
-                uint32_t first = v.words[l], cnt = v.words[l + 1];
-                if (COUNT) nb += 8;
-                float lim_t = lim0;
-                for (uint32_t k = 0; k < cnt; k += 3) {
-                    uint64_t x = (uint64_t)m_idx + first + k;
-                    uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
-                    const uint32_t* P = v.words + m_pos;
-                    V3 va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
-                    V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
-                    V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
-                    if (COUNT) nb += 48;
-                    float u, vv;
-                    float t = tri_test(oo, od, va, vb, vc, tmin, lim_t, u, vv);
-                    if (t >= 0.0f) {
-                        bt = t; bu = u; bv = vv; binst = cur; bhit = true;
-                        btri = (first + k) / 3u;
-                        lim_t = limit(cur);
-                        if (any_hit) { fin = true; break; }
-                    }
-                }
-            } else {
-                uint32_t first = leaf & LEAF_MASK, n = ((leaf >> 27) & 0xfu) + 1u;
-                float lim_t = lim0;
-                for (uint32_t j = 0; j < n; ++j) {
-                    const float4* tp = v.tris + 3ull * (first + j);
-                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                    if (COUNT) nb += 48;
-                    float u, vv;
-                    float t = tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t0.w, t1.x, t1.y), mk(t1.z, t1.w, t2.x), tmin,
-                                       lim_t, u, vv);
-                    if (t >= 0.0f) {
-                        bt = t; bu = u; bv = vv; binst = cur; bhit = true;
-                        btri = first + j;
-                        lim_t = limit(cur);
-                        if (any_hit) { fin = true; break; }
-                    }
-                }
             }
         }
-        if (ray == NONE || !fin) continue;
+        if (!fin) continue;
 
         // ---- hit writeback
         uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
