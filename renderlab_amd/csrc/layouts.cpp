@@ -19,6 +19,7 @@
 //   PLAIN layout's, in the same order.
 #include <algorithm>
 #include <cmath>
+#include <tuple>
 
 #include "internal.h"
 
@@ -248,6 +249,8 @@ struct W8Builder {
     const std::vector<uint32_t>* src;
     std::vector<uint32_t>* out;
     std::string* err;
+    std::vector<Cand> cands_by_slot_;
+    std::vector<char> slots_used_;
 
     uint32_t alloc() {
         uint32_t index = (uint32_t)(out->size() / W8_WORDS);
@@ -271,6 +274,44 @@ struct W8Builder {
         }
         Bounds all = empty_bounds();
         for (auto& c : cands) all = merge(all, c.box);
+        // Slot assignment by ray octant (Ylitie et al. 2017, sec. 3.2): slot s
+        // takes the child that comes first for rays whose direction signs are
+        // the bits of s, so visiting slots k ^ octant for k = 0..7 runs near to
+        // far without sorting. Greedy over (cost, child, slot) ascending; cost
+        // is the child centre along the octant's diagonal.
+        {
+            std::vector<Cand> slots(8);
+            std::vector<char> used(8, 0), placed(cands.size(), 0);
+            std::vector<std::tuple<float, int, int>> costs;
+            for (size_t c = 0; c < cands.size(); ++c)
+                for (int sl = 0; sl < 8; ++sl) {
+                    float cost = 0.0f;
+                    for (int a = 0; a < 3; ++a) {
+                        float ctr = (cands[c].box.lo[a] + cands[c].box.hi[a]) * 0.5f -
+                                    (all.lo[a] + all.hi[a]) * 0.5f;
+                        cost += ((sl >> a) & 1) ? -ctr : ctr;
+                    }
+                    costs.emplace_back(cost, (int)c, sl);
+                }
+            std::stable_sort(costs.begin(), costs.end(),
+                             [](const auto& x, const auto& y) { return std::get<0>(x) < std::get<0>(y); });
+            std::vector<int> slot_of(cands.size(), -1);
+            for (auto& [cost, c, sl] : costs) {
+                (void)cost;
+                if (placed[c] || used[sl]) continue;
+                placed[c] = 1;
+                used[sl] = 1;
+                slot_of[c] = sl;
+            }
+            std::vector<Cand> ordered(8);
+            std::vector<char> occupied(8, 0);
+            for (size_t c = 0; c < cands.size(); ++c) {
+                ordered[slot_of[c]] = cands[c];
+                occupied[slot_of[c]] = 1;
+            }
+            slots_used_.assign(occupied.begin(), occupied.end());
+            cands_by_slot_ = ordered;
+        }
         uint32_t w[W8_WORDS] = {};
         int exps[3];
         for (int a = 0; a < 3; ++a) {
@@ -292,8 +333,11 @@ struct W8Builder {
         }
         uint32_t valid = 0;
         std::vector<std::pair<uint32_t, uint32_t>> todo;  // (new index, source node)
-        for (size_t s = 0; s < cands.size(); ++s) {
-            const Cand& c = cands[s];
+        const std::vector<Cand> by_slot = cands_by_slot_;
+        const std::vector<char> occupied = slots_used_;
+        for (size_t s = 0; s < 8; ++s) {
+            if (!occupied[s]) continue;
+            const Cand& c = by_slot[s];
             valid |= 1u << s;
             for (int a = 0; a < 3; ++a) {
                 float p = bits_f(w[a]);
@@ -317,7 +361,8 @@ struct W8Builder {
                 todo.push_back({ni, c.child});
             }
         }
-        for (size_t s = cands.size(); s < 8; ++s) w[16 + s] = EMPTY;
+        for (size_t s = 0; s < 8; ++s)
+            if (!occupied[s]) w[16 + s] = EMPTY;
         w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16) |
                (valid << 24);
         std::copy(w, w + W8_WORDS, out->begin() + (size_t)index * W8_WORDS);

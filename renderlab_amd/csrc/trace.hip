@@ -486,15 +486,21 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     nw[4 * k] = q.x; nw[4 * k + 1] = q.y; nw[4 * k + 2] = q.z; nw[4 * k + 3] = q.w;
                 }
                 if (COUNT) nb += 112;
+                // The near plane of each axis is lo when the inverse direction is
+                // nonnegative, else hi: the same floats min/max would pick
+                // (rounding is monotone, the inverse is finite and nonzero).
+                const uint32_t nx = ri.x >= 0.0f ? 0u : 12u, ny = ri.y >= 0.0f ? 4u : 16u, nz = ri.z >= 0.0f ? 8u : 20u;
                 float t[4];
                 uint32_t c[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     c[s] = nw[24 + s];
-                    t[s] = FMAXV;
-                    if (c[s] != EMPTY)
-                        t[s] = slab_near(ro, ri, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
-                                         mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
+                    const float ax = (bf(nw[nx + s]) - ro.x) * ri.x, bx = (bf(nw[(nx ^ 12u) + s]) - ro.x) * ri.x;
+                    const float ay = (bf(nw[ny + s]) - ro.y) * ri.y, by = (bf(nw[(ny ^ 20u) + s]) - ro.y) * ri.y;
+                    const float az = (bf(nw[nz + s]) - ro.z) * ri.z, bz = (bf(nw[(nz ^ 28u) + s]) - ro.z) * ri.z;
+                    const float near = fmx(fmx(fmx(ax, ay), az), tmin);
+                    const float far = fmn(fmn(fmn(bx, by), bz), lim);
+                    t[s] = (near <= far && c[s] != EMPTY) ? near : FMAXV;
                 }
                 uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
                 cas_desc(k0, c[0], k1, c[1]);
@@ -507,48 +513,48 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 if (bf(k2) <= lim && sp < cap) push(c[2]);
                 if (bf(k3) <= lim && sp < cap) push(c[3]);
             } else {
-                // CW8 node: 24 words (origin, exponents + valid mask, byte planes, tags)
+                // CW8 node: 24 words (origin, exponents + valid mask, byte planes, tags).
+                // Each bound is fma(q, 2^e / d, (origin - o) / d): one fma a plane
+                // after a per-node setup, widened by an error slack that covers the
+                // rounding of either evaluation order, so culling stays conservative.
                 const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
                 uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4], q5 = w4[5];
                 if (COUNT) nb += 96;
-                const V3 p = mk(bf(q0.x), bf(q0.y), bf(q0.z));
                 const uint32_t valid = q0.w >> 24;
-                const V3 sc = mk(bf((q0.w & 0xffu) << 23), bf(((q0.w >> 8) & 0xffu) << 23),
-                                 bf(((q0.w >> 16) & 0xffu) << 23));
-                const uint32_t plane[12] = {q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+                const V3 scl = mk(bf((q0.w & 0xffu) << 23) * ri.x, bf(((q0.w >> 8) & 0xffu) << 23) * ri.y,
+                                  bf(((q0.w >> 16) & 0xffu) << 23) * ri.z);
+                const V3 base = mk((bf(q0.x) - ro.x) * ri.x, (bf(q0.y) - ro.y) * ri.y, (bf(q0.z) - ro.z) * ri.z);
+                const float err = fmx(fmx(fabsf(base.x) + 255.0f * fabsf(scl.x), fabsf(base.y) + 255.0f * fabsf(scl.y)),
+                                      fabsf(base.z) + 255.0f * fabsf(scl.z)) * 4.76837158e-7f;  // 2^-21
+                // planes: q1 = lo x (2 words), lo y; q2 = lo z, hi x; q3 = hi y, hi z
+                const bool px = ri.x >= 0.0f, py = ri.y >= 0.0f, pz = ri.z >= 0.0f;
+                const uint32_t nx0 = px ? q1.x : q2.z, nx1 = px ? q1.y : q2.w, fx0 = px ? q2.z : q1.x, fx1 = px ? q2.w : q1.y;
+                const uint32_t ny0 = py ? q1.z : q3.x, ny1 = py ? q1.w : q3.y, fy0 = py ? q3.x : q1.z, fy1 = py ? q3.y : q1.w;
+                const uint32_t nz0 = pz ? q2.x : q3.z, nz1 = pz ? q2.y : q3.w, fz0 = pz ? q3.z : q2.x, fz1 = pz ? q3.w : q2.y;
                 const uint32_t tag[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
-                uint32_t kk[8], cc[8];
+                uint32_t hit = 0;
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
-                    kk[s] = fb(FMAXV);
-                    cc[s] = tag[s];
-                    if ((valid >> s) & 1u) {
-                        const int wi = s >> 2, sh = 8 * (s & 3);
-                        V3 lq = mk((float)((plane[0 + wi] >> sh) & 0xffu), (float)((plane[2 + wi] >> sh) & 0xffu),
-                                   (float)((plane[4 + wi] >> sh) & 0xffu));
-                        V3 hq = mk((float)((plane[6 + wi] >> sh) & 0xffu), (float)((plane[8 + wi] >> sh) & 0xffu),
-                                   (float)((plane[10 + wi] >> sh) & 0xffu));
-                        V3 lo3 = mk(__builtin_fmaf(sc.x, lq.x, p.x), __builtin_fmaf(sc.y, lq.y, p.y),
-                                    __builtin_fmaf(sc.z, lq.z, p.z));
-                        V3 hi3 = mk(__builtin_fmaf(sc.x, hq.x, p.x), __builtin_fmaf(sc.y, hq.y, p.y),
-                                    __builtin_fmaf(sc.z, hq.z, p.z));
-                        kk[s] = fb(slab_near(ro, ri, lo3, hi3, tmin, lim));
-                    }
+                    const int sh = 8 * (s & 3);
+                    const uint32_t wnx = s < 4 ? nx0 : nx1, wny = s < 4 ? ny0 : ny1, wnz = s < 4 ? nz0 : nz1;
+                    const uint32_t wfx = s < 4 ? fx0 : fx1, wfy = s < 4 ? fy0 : fy1, wfz = s < 4 ? fz0 : fz1;
+                    const float tnx = __builtin_fmaf((float)((wnx >> sh) & 0xffu), scl.x, base.x);
+                    const float tny = __builtin_fmaf((float)((wny >> sh) & 0xffu), scl.y, base.y);
+                    const float tnz = __builtin_fmaf((float)((wnz >> sh) & 0xffu), scl.z, base.z);
+                    const float tfx = __builtin_fmaf((float)((wfx >> sh) & 0xffu), scl.x, base.x);
+                    const float tfy = __builtin_fmaf((float)((wfy >> sh) & 0xffu), scl.y, base.y);
+                    const float tfz = __builtin_fmaf((float)((wfz >> sh) & 0xffu), scl.z, base.z);
+                    const float near = fmx(fmx(fmx(tnx, tny), tnz) - err, tmin);
+                    const float far = fmn(fmn(fmn(tfx, tfy), tfz) + err, lim);
+                    if (near <= far && ((valid >> s) & 1u)) hit |= 1u << s;
                 }
-                // Batcher odd-even merge network for 8, descending.
-                cas_desc(kk[0], cc[0], kk[1], cc[1]); cas_desc(kk[2], cc[2], kk[3], cc[3]);
-                cas_desc(kk[4], cc[4], kk[5], cc[5]); cas_desc(kk[6], cc[6], kk[7], cc[7]);
-                cas_desc(kk[0], cc[0], kk[2], cc[2]); cas_desc(kk[1], cc[1], kk[3], cc[3]);
-                cas_desc(kk[4], cc[4], kk[6], cc[6]); cas_desc(kk[5], cc[5], kk[7], cc[7]);
-                cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[5], cc[5], kk[6], cc[6]);
-                cas_desc(kk[0], cc[0], kk[4], cc[4]); cas_desc(kk[1], cc[1], kk[5], cc[5]);
-                cas_desc(kk[2], cc[2], kk[6], cc[6]); cas_desc(kk[3], cc[3], kk[7], cc[7]);
-                cas_desc(kk[2], cc[2], kk[4], cc[4]); cas_desc(kk[3], cc[3], kk[5], cc[5]);
-                cas_desc(kk[1], cc[1], kk[2], cc[2]); cas_desc(kk[3], cc[3], kk[4], cc[4]);
-                cas_desc(kk[5], cc[5], kk[6], cc[6]);
+                // Push far to near in octant order: slot k ^ oct is the k-th nearest guess.
+                const uint32_t oct = (ri.x < 0.0f ? 1u : 0u) | (ri.y < 0.0f ? 2u : 0u) | (ri.z < 0.0f ? 4u : 0u);
 #pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    if (bf(kk[s]) <= lim && sp < cap) push(cc[s]);
+                for (int k = 7; k >= 0; --k) {
+                    const uint32_t s = (uint32_t)k ^ oct;
+                    if (((hit >> s) & 1u) && sp < cap) push(tag[s]);
+                }
             }
             }
         }
