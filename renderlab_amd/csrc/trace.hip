@@ -61,6 +61,13 @@ __device__ __forceinline__ float safe_inv1(float d) {
     return 1.0f / (fabsf(d) < 1e-30f ? tiny : d);
 }
 __device__ __forceinline__ V3 safe_inverse(V3 d) { return mk(safe_inv1(d.x), safe_inv1(d.y), safe_inv1(d.z)); }
+// The same clamp with the hardware reciprocal (1 ulp): for box tests that
+// carry their own error slack (the CW8 layout), never for hit arithmetic.
+__device__ __forceinline__ float fast_inv1(float d) {
+    float tiny = d >= 0.0f ? 1e-30f : -1e-30f;
+    return __builtin_amdgcn_rcpf(fabsf(d) < 1e-30f ? tiny : d);
+}
+__device__ __forceinline__ V3 fast_inverse(V3 d) { return mk(fast_inv1(d.x), fast_inv1(d.y), fast_inv1(d.z)); }
 
 // gi_slab_near (:35-40)
 __device__ __forceinline__ float slab_near(V3 o, V3 inv, V3 lo, V3 hi, float tmin, float tmax) {
@@ -227,11 +234,80 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         return st[(sp & (S - 1)) * BLOCK];
     };
     auto limit = [&](uint32_t i) -> float { return (bhit && i < binst) ? bf(fb(bt) + 1u) : bt; };
+    // ---- hit writeback (the record of a finished ray)
+    auto writeback = [&]() {
+        uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
+        V3 n = mk(0.0f, 0.0f, 0.0f);
+        if (bhit) {
+            V3 r0, r1, r2, va, vb, vc;
+            if (LAYOUT == REF) {
+                const uint32_t* r = v.words + 16ull + (uint64_t)binst * 32u;
+                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
+                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
+                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
+                uint32_t pos = r[20], idx = r[21], prim_base = r[28];
+                uint64_t x = (uint64_t)idx + 3ull * btri;
+                tri0 = v.words[x];
+                tri1 = v.words[x + 1];
+                tri2 = v.words[x + 2];
+                const uint32_t* P = v.words + pos;
+                va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
+                vb = mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2]));
+                vc = mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2]));
+                slot = btri;
+                if (!any_hit && prim_base != 0u) {
+                    tri0 = v.words[prim_base + (uint64_t)btri];
+                    tri1 = 0;
+                    tri2 = 0;
+                }
+            } else {
+                const uint32_t* r = v.inst + (uint64_t)binst * 32u;
+                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
+                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
+                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
+                const float4* tp = v.tris + 3ull * btri;
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                va = mk(t0.x, t0.y, t0.z);
+                vb = mk(t0.w, t1.x, t1.y);
+                vc = mk(t1.z, t1.w, t2.x);
+                slot = fb(t2.y);
+                const uint32_t* ids = v.tri_ids + 4ull * btri;
+                tri0 = ids[0];
+                tri1 = ids[1];
+                tri2 = ids[2];
+                if (!any_hit && (r[19] & 2u)) {
+                    tri0 = fb(t2.z);
+                    tri1 = 0;
+                    tri2 = 0;
+                }
+            }
+            n = hit_normal(r0, r1, r2, va, vb, vc);
+            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 48 + 12);
+        }
+        if (out_hits) {
+            uint4* hp = reinterpret_cast<uint4*>(out_hits + out_idx);
+            uint32_t flags = invalid ? 2u : (bhit ? 1u : 0u);
+            hp[0] = make_uint4(fb(bt), bhit ? binst : NONE, tri0, tri1);
+            hp[1] = make_uint4(tri2, fb(bhit ? bu : 0.0f), fb(bhit ? bv : 0.0f), fb(n.x));
+            hp[2] = make_uint4(fb(n.y), fb(n.z), flags, slot);
+        }
+        if (COUNT) nb += (out_hits ? 48 : 0) + (out_vis && !bhit ? 4 : 0);
+        if (out_vis && !bhit) atomicOr(out_vis + (out_idx >> 5), 1u << (out_idx & 31u));
+    };
 
+    bool done = false;  // the ray is finished; its record is written at the next refill
     for (;;) {
-        // ---- refill idle lanes from the global cursor (wave-uniform decision)
-        const uint64_t idle = __ballot(ray == NONE);
-        if (!exhausted && idle != 0ull && ((uint32_t)__popcll(idle) >= a.refill_min || idle == ~0ull)) {
+        // ---- write back finished rays and refill idle lanes (wave-uniform decision):
+        // only once enough lanes are free, so this code runs once per many rays.
+        const uint64_t idle = __ballot(ray == NONE || done);
+        const bool refill_now = idle != 0ull &&
+                                ((uint32_t)__popcll(idle) >= a.refill_min || idle == ~0ull || exhausted);
+        if (refill_now && done) {
+            writeback();
+            ray = NONE;
+            done = false;
+        }
+        if (!exhausted && refill_now) {
             uint32_t n = (uint32_t)__popcll(idle);
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(a.counter, n);
@@ -267,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     wd = mk(r1.x, r1.y, r1.z);
                     bt = r1.w;
                     invalid = ray_invalid(r0, r1);
-                    winv = safe_inverse(wd);
+                    winv = LAYOUT == CW8 ? fast_inverse(wd) : safe_inverse(wd);
                     binst = NONE;
                     bhit = false;
                     btri = 0;
@@ -283,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
         }
         if (exhausted && __ballot(ray != NONE) == 0ull) break;
-        if (ray == NONE) continue;
+        if (ray == NONE || done) continue;
 
         bool fin = invalid;
         if (!fin && tri_left > 0) {
@@ -375,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     oo = mk(dot3(r0, wo) + bf(rw[3]) * 1.0f, dot3(r1, wo) + bf(rw[7]) * 1.0f,
                             dot3(r2, wo) + bf(rw[11]) * 1.0f);
                     od = mk(dot3(r0, wd), dot3(r1, wd), dot3(r2, wd));
-                    oinv = safe_inverse(od);
+                    oinv = LAYOUT == CW8 ? fast_inverse(od) : safe_inverse(od);
                     cur = i;
                     uint32_t root;
                     if (LAYOUT == REF) {
@@ -486,21 +562,15 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     nw[4 * k] = q.x; nw[4 * k + 1] = q.y; nw[4 * k + 2] = q.z; nw[4 * k + 3] = q.w;
                 }
                 if (COUNT) nb += 112;
-                // The near plane of each axis is lo when the inverse direction is
-                // nonnegative, else hi: the same floats min/max would pick
-                // (rounding is monotone, the inverse is finite and nonzero).
-                const uint32_t nx = ri.x >= 0.0f ? 0u : 12u, ny = ri.y >= 0.0f ? 4u : 16u, nz = ri.z >= 0.0f ? 8u : 20u;
                 float t[4];
                 uint32_t c[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     c[s] = nw[24 + s];
-                    const float ax = (bf(nw[nx + s]) - ro.x) * ri.x, bx = (bf(nw[(nx ^ 12u) + s]) - ro.x) * ri.x;
-                    const float ay = (bf(nw[ny + s]) - ro.y) * ri.y, by = (bf(nw[(ny ^ 20u) + s]) - ro.y) * ri.y;
-                    const float az = (bf(nw[nz + s]) - ro.z) * ri.z, bz = (bf(nw[(nz ^ 28u) + s]) - ro.z) * ri.z;
-                    const float near = fmx(fmx(fmx(ax, ay), az), tmin);
-                    const float far = fmn(fmn(fmn(bx, by), bz), lim);
-                    t[s] = (near <= far && c[s] != EMPTY) ? near : FMAXV;
+                    t[s] = FMAXV;
+                    if (c[s] != EMPTY)
+                        t[s] = slab_near(ro, ri, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
+                                         mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
                 }
                 uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
                 cas_desc(k0, c[0], k1, c[1]);
@@ -548,77 +618,35 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     const float far = fmn(fmn(fmn(tfx, tfy), tfz) + err, lim);
                     if (near <= far && ((valid >> s) & 1u)) hit |= 1u << s;
                 }
-                // Push far to near in octant order: slot k ^ oct is the k-th nearest guess.
-                const uint32_t oct = (ri.x < 0.0f ? 1u : 0u) | (ri.y < 0.0f ? 2u : 0u) | (ri.z < 0.0f ? 4u : 0u);
+                // Push far to near in octant order: slot k ^ oct is the k-th nearest
+                // guess. The XOR permutation is three conditional swap stages on
+                // registers (a lane-varying index into a register array would not be).
+                uint32_t tg[8] = {tag[0], tag[1], tag[2], tag[3], tag[4], tag[5], tag[6], tag[7]};
+                if (ri.x < 0.0f) {
 #pragma unroll
-                for (int k = 7; k >= 0; --k) {
-                    const uint32_t s = (uint32_t)k ^ oct;
-                    if (((hit >> s) & 1u) && sp < cap) push(tag[s]);
+                    for (int k = 0; k < 8; k += 2) { uint32_t x = tg[k]; tg[k] = tg[k + 1]; tg[k + 1] = x; }
+                    hit = ((hit & 0x55u) << 1) | ((hit >> 1) & 0x55u);
                 }
+                if (ri.y < 0.0f) {
+#pragma unroll
+                    for (int k = 0; k < 8; k += 4) {
+                        uint32_t x = tg[k]; tg[k] = tg[k + 2]; tg[k + 2] = x;
+                        x = tg[k + 1]; tg[k + 1] = tg[k + 3]; tg[k + 3] = x;
+                    }
+                    hit = ((hit & 0x33u) << 2) | ((hit >> 2) & 0x33u);
+                }
+                if (ri.z < 0.0f) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) { uint32_t x = tg[k]; tg[k] = tg[k + 4]; tg[k + 4] = x; }
+                    hit = ((hit & 0x0fu) << 4) | ((hit >> 4) & 0x0fu);
+                }
+#pragma unroll
+                for (int k = 7; k >= 0; --k)
+                    if (((hit >> k) & 1u) && sp < cap) push(tg[k]);
             }
             }
         }
-        if (!fin) continue;
-
-        // ---- hit writeback
-        uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
-        V3 n = mk(0.0f, 0.0f, 0.0f);
-        if (bhit) {
-            V3 r0, r1, r2, va, vb, vc;
-            if (LAYOUT == REF) {
-                const uint32_t* r = v.words + 16ull + (uint64_t)binst * 32u;
-                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
-                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
-                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
-                uint32_t pos = r[20], idx = r[21], prim_base = r[28];
-                uint64_t x = (uint64_t)idx + 3ull * btri;
-                tri0 = v.words[x];
-                tri1 = v.words[x + 1];
-                tri2 = v.words[x + 2];
-                const uint32_t* P = v.words + pos;
-                va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
-                vb = mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2]));
-                vc = mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2]));
-                slot = btri;
-                if (!any_hit && prim_base != 0u) {
-                    tri0 = v.words[prim_base + (uint64_t)btri];
-                    tri1 = 0;
-                    tri2 = 0;
-                }
-            } else {
-                const uint32_t* r = v.inst + (uint64_t)binst * 32u;
-                r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
-                r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
-                r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
-                const float4* tp = v.tris + 3ull * btri;
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                va = mk(t0.x, t0.y, t0.z);
-                vb = mk(t0.w, t1.x, t1.y);
-                vc = mk(t1.z, t1.w, t2.x);
-                slot = fb(t2.y);
-                const uint32_t* ids = v.tri_ids + 4ull * btri;
-                tri0 = ids[0];
-                tri1 = ids[1];
-                tri2 = ids[2];
-                if (!any_hit && (r[19] & 2u)) {
-                    tri0 = fb(t2.z);
-                    tri1 = 0;
-                    tri2 = 0;
-                }
-            }
-            n = hit_normal(r0, r1, r2, va, vb, vc);
-            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 48 + 12);
-        }
-        if (out_hits) {
-            uint4* hp = reinterpret_cast<uint4*>(out_hits + out_idx);
-            uint32_t flags = invalid ? 2u : (bhit ? 1u : 0u);
-            hp[0] = make_uint4(fb(bt), bhit ? binst : NONE, tri0, tri1);
-            hp[1] = make_uint4(tri2, fb(bhit ? bu : 0.0f), fb(bhit ? bv : 0.0f), fb(n.x));
-            hp[2] = make_uint4(fb(n.y), fb(n.z), flags, slot);
-        }
-        if (COUNT) nb += (out_hits ? 48 : 0) + (out_vis && !bhit ? 4 : 0);
-        if (out_vis && !bhit) atomicOr(out_vis + (out_idx >> 5), 1u << (out_idx & 31u));
-        ray = NONE;
+        if (fin) done = true;
     }
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off);
