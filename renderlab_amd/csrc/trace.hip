@@ -206,6 +206,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t sp = 0, lo = 0, floor_ = 0, flat_next = 0;
     uint32_t m_nodes = 0, m_leaves = 0, m_pos = 0, m_idx = 0;
     uint32_t tri_next = 0, tri_left = 0;  // the current leaf's untested triangles
+    uint32_t nxt = NONE;                  // the next item to visit (the top of the stack, in a register)
     bool exhausted = false;
     unsigned long long nb = 0;
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     in_blas = false;
                     flat_next = 0;
                     tri_left = 0;
-                    if (!invalid && v.top_on) push(v.top_root);
+                    nxt = (!invalid && v.top_on) ? v.top_root : NONE;
                 }
             }
         }
@@ -362,7 +363,40 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         if (ray == NONE || done) continue;
 
         bool fin = invalid;
-        if (!fin && tri_left > 0) {
+        // ---- the next item of a lane that has none: pop it, or leave the BLAS
+        if (!fin && tri_left == 0) {
+            if (nxt == NONE) {
+                if (in_blas && sp == floor_) in_blas = false;
+                if (!in_blas && !v.top_on) {
+                    if (flat_next < v.n_inst) nxt = LEAF_BIT | flat_next++;
+                } else if (sp > 0) {
+                    nxt = pop();
+                }
+                if (nxt == NONE) fin = true;
+            }
+            if (!fin && in_blas && (nxt & LEAF_BIT)) {
+                // a leaf: its triangles are tested by the next steps
+                if (LAYOUT == REF) {
+                    uint64_t l = m_leaves + (uint64_t)(nxt & ~LEAF_BIT) * 2u;
+                    tri_next = v.words[l] / 3u;
+                    tri_left = v.words[l + 1] / 3u;
+                    if (COUNT) nb += 8;
+                } else {
+                    tri_next = nxt & LEAF_MASK;
+                    tri_left = ((nxt >> 27) & 0xfu) + 1u;
+                }
+                nxt = NONE;
+            }
+        }
+        // ---- the wave runs one kind of step at a time: whichever most lanes want
+        // (1 triangle pair, 2 instance entry, 3 node). The others wait a step; each
+        // lane's own sequence of steps is unchanged.
+        const int kind = fin ? 0 : (tri_left > 0 ? 1 : ((nxt & LEAF_BIT) ? 2 : 3));
+        const uint32_t n_tri = (uint32_t)__popcll(__ballot(kind == 1));
+        const uint32_t n_inst = (uint32_t)__popcll(__ballot(kind == 2));
+        const uint32_t n_node = (uint32_t)__popcll(__ballot(kind == 3));
+        const int pick = (n_node >= n_tri && n_node >= n_inst) ? 3 : (n_tri >= n_inst ? 1 : 2);
+        if (kind == 1 && pick == 1) {
             // ---- triangle step (:126-168): the next two triangles in leaf order
             const uint32_t k = tri_left >= 2 ? 2u : 1u;
 #pragma unroll
@@ -397,33 +431,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
             tri_next += k;
             tri_left -= k;
-        } else if (!fin) {
-            if (in_blas && sp == floor_) in_blas = false;
-            uint32_t item = 0;
-            bool have;
-            if (!in_blas && !v.top_on) {
-                have = flat_next < v.n_inst;
-                item = LEAF_BIT | flat_next;
-                flat_next++;
-            } else {
-                have = sp > 0;
-                if (have) item = pop();
-            }
-            if (!have) {
-                fin = true;
-            } else if (item & LEAF_BIT) {
-                if (in_blas) {
-                    // a leaf: its triangles are tested by the next steps
-                    if (LAYOUT == REF) {
-                        uint64_t l = m_leaves + (uint64_t)(item & ~LEAF_BIT) * 2u;
-                        tri_next = v.words[l] / 3u;
-                        tri_left = v.words[l + 1] / 3u;
-                        if (COUNT) nb += 8;
-                    } else {
-                        tri_next = item & LEAF_MASK;
-                        tri_left = ((item >> 27) & 0xfu) + 1u;
-                    }
-                } else {
+        } else if (kind >= 2 && kind == pick) {
+            const uint32_t item = nxt;
+            nxt = NONE;
+            if (kind == 2) {
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
@@ -467,8 +478,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     }
                     floor_ = sp;
                     in_blas = true;
-                    push(root);
-                }
+                    nxt = root;  // pushed and popped at once
                 }
             } else if (LAYOUT == REF) {
                 if (!in_blas) {
@@ -507,8 +517,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     if (dist[2] < dist[1]) RQ_SW(1, 2);
 #undef RQ_SW
 #pragma unroll
-                    for (int l = 3; l >= 0; --l)
+                    for (int l = 3; l >= 1; --l)
                         if (dist[l] <= bound && sp < STACK_CAP) push(ids[l]);
+                    if (dist[0] <= bound && sp < STACK_CAP) nxt = ids[0];  // the nearest, popped next
                 } else {
                 // BLAS node (:171-204), decoded from the quantized record
                 const float lim = limit(cur);
@@ -544,7 +555,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 if (bf(k0) <= lim && sp - floor_ < STACK_CAP) push(c[0]);
                 if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
                 if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
-                if (bf(k3) <= lim && sp - floor_ < STACK_CAP) push(c[3]);
+                if (bf(k3) <= lim && sp - floor_ < STACK_CAP) nxt = c[3];  // the nearest, popped next
                 }
             } else {
             // PLAIN / CW8: one node format for the top level and the BLAS; the
@@ -581,7 +592,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 if (bf(k0) <= lim && sp < cap) push(c[0]);
                 if (bf(k1) <= lim && sp < cap) push(c[1]);
                 if (bf(k2) <= lim && sp < cap) push(c[2]);
-                if (bf(k3) <= lim && sp < cap) push(c[3]);
+                if (bf(k3) <= lim && sp < cap) nxt = c[3];  // the nearest, popped next
             } else {
                 // CW8 node: 24 words (origin, exponents + valid mask, byte planes, tags).
                 // Each bound is fma(q, 2^e / d, (origin - o) / d): one fma a plane
@@ -641,8 +652,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     hit = ((hit & 0x0fu) << 4) | ((hit >> 4) & 0x0fu);
                 }
 #pragma unroll
-                for (int k = 7; k >= 0; --k)
+                for (int k = 7; k >= 1; --k)
                     if (((hit >> k) & 1u) && sp < cap) push(tg[k]);
+                if ((hit & 1u) && sp < cap) nxt = tg[0];  // the nearest guess, popped next
             }
             }
         }
