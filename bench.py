@@ -318,11 +318,19 @@ def main():
     dom_mode = rq.ANY if dom == "shadow" else rq.CLOSEST
     dom_ptr = {"primary": frame.prim.ptr.value, "shadow": frame.shadow.ptr.value}.get(
         dom, frame.gi.ptr.value + (int(dom[2:]) if dom.startswith("gi") else 0) * frame.n * 32)
-    nbytes = scene.count_bytes(args.layout, dom_ptr, dom_rays, dom_mode)
+    stats = scene.count_stats(args.layout, dom_ptr, dom_rays, dom_mode)
+    nbytes = stats["bytes"]
+    lanes = 64 * stats["iterations"]
+    log(f"dominant launch {dom}: {dom_rays} rays, steps/ray node {stats['node_steps'] / dom_rays:.2f} "
+        f"tri-pair {stats['tri_steps'] / dom_rays:.2f} inst {stats['inst_steps'] / dom_rays:.2f}; lane use "
+        f"{(stats['node_steps'] + stats['tri_steps'] + stats['inst_steps']) / max(lanes, 1):.3f}; "
+        f"iterations/wave-ray {stats['iterations'] * 64 / dom_rays:.1f}")
     achieved = nbytes / (per[dom] * 1e-3)
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK, 4), "traffic": None, "kernel": f"trace {dom}",
-                "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4)}
+                "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4),
+                "lane_use": round((stats["node_steps"] + stats["tri_steps"] + stats["inst_steps"]) /
+                                  max(64 * stats["iterations"], 1), 3)}
 
     # FP64 verification of the GI hits (outside the timed region).
     verify = None

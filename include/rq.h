@@ -197,6 +197,10 @@ rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_lay
    bytes of one launch for the roofline. Synchronous; not for timing. */
 rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes);
+/* The same launch with its step statistics: [0] bytes, [1] wave iterations,
+   [2] node steps, [3] triangle-pair steps, [4] instance entries, [5] refills. */
+rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]);
 
 /* Launch parameters of the traversal kernels (the autotuner's knobs). */
 typedef struct rq_tuning {

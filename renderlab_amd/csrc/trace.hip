@@ -209,6 +209,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t nxt = NONE;                  // the next item to visit (the top of the stack, in a register)
     bool exhausted = false;
     unsigned long long nb = 0;
+    // COUNT: [1] wave iterations, [2] node, [3] triangle-pair, [4] instance
+    // steps (lane sums), [5] refill events (per wave)
+    unsigned long long st_iter = 0, st_node = 0, st_tri = 0, st_inst = 0, st_refill = 0;
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
     SceneView v = a.sv;
     bool any_hit = AM == 1;
@@ -308,6 +311,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             ray = NONE;
             done = false;
         }
+        if (COUNT && refill_now && !exhausted) st_refill++;
         if (!exhausted && refill_now) {
             uint32_t n = (uint32_t)__popcll(idle);
             uint32_t base = 0;
@@ -360,6 +364,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
         }
         if (exhausted && __ballot(ray != NONE) == 0ull) break;
+        if (COUNT) st_iter++;
         if (ray == NONE || done) continue;
 
         bool fin = invalid;
@@ -396,6 +401,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         const uint32_t n_inst = (uint32_t)__popcll(__ballot(kind == 2));
         const uint32_t n_node = (uint32_t)__popcll(__ballot(kind == 3));
         const int pick = (n_node >= n_tri && n_node >= n_inst) ? 3 : (n_tri >= n_inst ? 1 : 2);
+        if (COUNT) {
+            st_node += kind == 3 && pick == 3;
+            st_tri += kind == 1 && pick == 1;
+            st_inst += kind == 2 && pick == 2;
+        }
         if (kind == 1 && pick == 1) {
             // ---- triangle step (:126-168): the next two triangles in leaf order
             const uint32_t k = tri_left >= 2 ? 2u : 1u;
@@ -661,8 +671,20 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         if (fin) done = true;
     }
     if (COUNT) {
-        for (int off = 32; off > 0; off >>= 1) nb += __shfl_xor(nb, off);
-        if (lane == 0) atomicAdd(a.bytes, nb);
+        for (int off = 32; off > 0; off >>= 1) {
+            nb += __shfl_xor(nb, off);
+            st_node += __shfl_xor(st_node, off);
+            st_tri += __shfl_xor(st_tri, off);
+            st_inst += __shfl_xor(st_inst, off);
+        }
+        if (lane == 0) {
+            atomicAdd(a.bytes, nb);
+            atomicAdd(a.bytes + 1, st_iter);
+            atomicAdd(a.bytes + 2, st_node);
+            atomicAdd(a.bytes + 3, st_tri);
+            atomicAdd(a.bytes + 4, st_inst);
+            atomicAdd(a.bytes + 5, st_refill);
+        }
     }
 }
 
@@ -1506,25 +1528,33 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
     return RQ_OK;
 }
 
-extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
-                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes) {
+extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]) {
     if (!scene || !bytes || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
     unsigned long long* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, sizeof(unsigned long long)), "hipMalloc count");
-    HIP_TRY(hipMemset(acc, 0, sizeof(unsigned long long)), "hipMemset count");
+    HIP_TRY(hipMalloc(&acc, 6 * sizeof(unsigned long long)), "hipMalloc count");
+    HIP_TRY(hipMemset(acc, 0, 6 * sizeof(unsigned long long)), "hipMemset count");
     rq_status st = launch_trace(scene, layout, rays, count, mode, hits, vis_bits, nullptr, acc);
-    unsigned long long host = 0;
+    unsigned long long host[6] = {0, 0, 0, 0, 0, 0};
     hipError_t e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(&host, acc, sizeof host, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(host, acc, sizeof host, hipMemcpyDeviceToHost);
     (void)hipFree(acc);
     if (st != RQ_OK) return st;
     if (e != hipSuccess) return hip_fail(e, "count");
-    *bytes = host;
+    for (int k = 0; k < 6; ++k) stats[k] = host[k];
     return RQ_OK;
+}
+
+extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
+                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes) {
+    uint64_t stats[6];
+    rq_status st = rq_trace_count_stats(scene, layout, rays, count, mode, hits, vis_bits, stats);
+    if (st == RQ_OK) *bytes = stats[0];
+    return st;
 }
 
 extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,

@@ -109,6 +109,7 @@ _sig("rq_trace_batches", _i, _P, _u32, _i, _P, _u32, _P)
 _sig("rq_get_batch_tuning", _i, _i, _P)
 _sig("rq_set_batch_tuning", _i, _i, _P)
 _sig("rq_trace_count_bytes", _i, _P, _i, _P, _u32, _u32, _P, _P, ctypes.POINTER(_u64))
+_sig("rq_trace_count_stats", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_get_tuning", _i, _i, _u32, _P)
 _sig("rq_set_tuning", _i, _i, _u32, _P)
 _sig("rq_verify_fp64", _i, _P, _P, _P, _u32, _P, _P)
@@ -525,6 +526,23 @@ class Scene:
         check(lib.rq_trace_count_bytes(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(h),
                                        p(v), ctypes.byref(n)))
         return n.value
+
+    def count_stats(self, layout, rays, count, mode, hits=None, vis=None):
+        """Step statistics of one launch: bytes, wave iterations, node / triangle-pair / instance steps, refills."""
+        def p(x):
+            return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
+        out = np.zeros(6, np.uint64)
+        own = None
+        if hits is None and vis is None:
+            own = DeviceBuffer(max(int(count), 1) * HIT_DTYPE.itemsize)
+            if int(mode) == CLOSEST:
+                hits = own
+            else:
+                vis = own
+        check(lib.rq_trace_count_stats(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits),
+                                       p(vis), _ptr(out)))
+        return dict(zip(["bytes", "iterations", "node_steps", "tri_steps", "inst_steps", "refills"],
+                        (int(x) for x in out)))
 
     def verify_fp64(self, rays, hits, count, stream=None):
         out = np.zeros(4, np.float64)
