@@ -1530,7 +1530,7 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
 
 extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                           uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]) {
-    if (!scene || !bytes || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays) ||
+    if (!scene || !stats || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
