@@ -42,8 +42,10 @@ def log(msg):
 class Frame:
     """Device buffers of one frame and the launches of one step."""
 
-    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream):
+    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream, reorder=False):
         self.rq, self.scene, self.w, self.h, self.n_gi, self.seed = rq, scene, w, h, n_gi, seed
+        # secondary launches sorted by (octant, origin Morton code) before tracing
+        self.sec = rq.REORDER if reorder else 0
         n = w * h
         self.n = n
         self.inv = rq.camera(kind, seed, w, h)
@@ -68,12 +70,12 @@ class Frame:
                          self.gi if self.n_gi else None, self.counts, stream=s)
         rec("secondary_gen")
         c = self.counts_host
-        self.scene.trace(layout, self.shadow, int(c[0]), rq.ANY, vis=self.vis, stream=s)
+        self.scene.trace(layout, self.shadow, int(c[0]), rq.ANY | self.sec, vis=self.vis, stream=s)
         rec("shadow")
         for b in range(self.n_gi):
             ptr = self.gi.ptr.value + b * self.n * 32
             hp = self.gi_hits.ptr.value + b * self.n * 48
-            self.scene.trace(layout, ptr, int(c[1 + b]), rq.CLOSEST, hits=hp, stream=s)
+            self.scene.trace(layout, ptr, int(c[1 + b]), rq.CLOSEST | self.sec, hits=hp, stream=s)
             rec(f"gi{b}")
 
     def prime(self, layout):
@@ -269,6 +271,8 @@ def main():
     ap.add_argument("--batching", action="store_true", help="also run the 8-scene, 256-batch one-launch form")
     ap.add_argument("--no-serving", action="store_true", help="skip the short serving run")
     ap.add_argument("--serving-sweep", action="store_true", help="sweep Poisson rates 50..2000 jobs/s")
+    ap.add_argument("--reorder", default="none", choices=["none", "secondary"],
+                    help="sort shadow and GI launches by direction octant and origin Morton code")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -297,7 +301,7 @@ def main():
     log(f"uploaded {layouts} in {time.perf_counter() - t0:.1f}s; bytes " +
         ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
     stream = rq.Stream()
-    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream)
+    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary")
     rays = frame.prime(args.layout)
     log(f"rays per step: {rays} (primary {frame.n}, secondary {list(frame.counts_host)})")
 
@@ -318,7 +322,7 @@ def main():
     dom_mode = rq.ANY if dom == "shadow" else rq.CLOSEST
     dom_ptr = {"primary": frame.prim.ptr.value, "shadow": frame.shadow.ptr.value}.get(
         dom, frame.gi.ptr.value + (int(dom[2:]) if dom.startswith("gi") else 0) * frame.n * 32)
-    stats = scene.count_stats(args.layout, dom_ptr, dom_rays, dom_mode)
+    stats = scene.count_stats(args.layout, dom_ptr, dom_rays, dom_mode | (frame.sec if dom != "primary" else 0))
     nbytes = stats["bytes"]
     lanes = 64 * stats["iterations"]
     log(f"dominant launch {dom}: {dom_rays} rays, steps/ray node {stats['node_steps'] / dom_rays:.2f} "
@@ -360,7 +364,7 @@ def main():
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded scene and camera)",
         "config": {"workload": desc, "layout": args.layout, "rays_per_step": rays, "triangles": st["triangles"],
-                   "instances": st["instances"], "parallelism": "rays across one GPU"},
+                   "instances": st["instances"], "reorder": args.reorder, "parallelism": "rays across one GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if "plain" in results and args.layout != "plain":

@@ -50,6 +50,10 @@ typedef enum rq_mode {
     RQ_MODE_ANY = 1,        /* gi_visible: any hit, sky visibility */
     RQ_MODE_ANY_OPAQUE = 2  /* gi_visible_within: opaque casters only */
 } rq_mode;
+/* Or'ed into the mode of rq_trace / rq_trace_indirect: trace a launch of at
+   least 65536 rays in (direction octant, origin Morton code) order, for
+   incoherent secondary rays. Results keep each ray's index and bits. */
+#define RQ_REORDER 0x100u
 
 /* One ray, 32 bytes: origin, tmin, direction (need not be unit), tmax. */
 typedef struct rq_ray {
@@ -209,7 +213,8 @@ typedef struct rq_tuning {
     uint32_t persistent;   /* 0: one lane per ray; 1: persistent, per-lane refill */
     uint32_t waves_per_eu; /* occupancy hint 0 (none), 1 .. 8 */
     uint32_t grid_per_cu;  /* persistent workgroups per CU (0: auto) */
-    uint32_t reserved[3];
+    uint32_t refill;       /* idle lanes a wave gathers before it takes new rays: 1 .. 64 */
+    uint32_t reserved[2];
 } rq_tuning;
 rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out);
 rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* tuning);

@@ -99,6 +99,7 @@ void top_place(std::vector<uint32_t>& words, uint32_t header, uint32_t stride, u
 bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* source, uint32_t n,
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
+void scene_bounds(rq_scene* scene);
 
 // Per-mesh facts recovered from an instance record of packed words.
 struct MeshView {
@@ -143,6 +144,7 @@ struct rq_bvh {
 
 struct rq_scene {
     std::vector<uint32_t> words;  // the reference gi_geometry words
+    float bounds[6] = {0, 0, 0, 1, 1, 1};  // union of the instance world boxes (ray reordering keys)
     uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     rq::HostLayout plain, cw8;
     rq::DeviceLayout dev[3];

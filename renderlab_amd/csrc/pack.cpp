@@ -142,6 +142,29 @@ bool mesh_view(const std::vector<uint32_t>& words, uint32_t record, MeshView& m,
     return true;
 }
 
+void scene_bounds(rq_scene* scene) {
+    const auto& w = scene->words;
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    uint32_t n = w.size() > 0 ? w[0] : 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        if (r + 18 > w.size()) break;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = mn(lo[a], bits_f(w[r + 12 + a]));
+            hi[a] = mx(hi[a], bits_f(w[r + 15 + a]));
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!(lo[a] <= hi[a]) || !std::isfinite(lo[a]) || !std::isfinite(hi[a])) {
+            lo[a] = 0.0f;
+            hi[a] = 1.0f;
+        }
+        scene->bounds[a] = lo[a];
+        scene->bounds[3 + a] = hi[a];
+    }
+}
+
 namespace {
 
 struct Blas {
@@ -310,6 +333,7 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
     scene->stats[4] = nodes;
     scene->stats[5] = words[2];
     scene->stats[6] = depth;
+    scene_bounds(scene);
     *out = scene;
     return RQ_OK;
 }
@@ -349,6 +373,7 @@ extern "C" rq_status rq_scene_from_gi_words(const uint32_t* w, size_t n, rq_scen
     scene->stats[2] = tris;
     scene->stats[4] = nodes;
     scene->stats[5] = w[2];
+    scene_bounds(scene);
     *out = scene;
     return RQ_OK;
 }

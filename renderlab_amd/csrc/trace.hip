@@ -150,6 +150,7 @@ struct TraceArgs {
     const uint32_t* count_ptr;  // when set, the ray count is min(*count_ptr, count)
     const BatchDev* batches;    // BATCHED: the batch table
     const uint32_t* prefix;     // BATCHED: first global ray of each batch
+    const uint32_t* order;      // when set, the i-th ray taken is ray order[i]
     uint32_t n_batches;
     uint32_t count, refill_min, pad;
 };
@@ -321,6 +322,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             if (ray == NONE) {
                 uint32_t r = base + lane_rank(idle);
                 if (r < count) {
+                    if (!BATCHED && a.order) {
+                        r = a.order[r];
+                        if (COUNT) nb += 4;
+                    }
                     ray = r;
                     const rq_ray* src = a.rays + r;
                     out_idx = r;
@@ -969,7 +974,14 @@ __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
 }  // namespace dev
 
 // ------------------------------------------------------------- host side
+hipError_t ray_order(const rq_ray* rays, uint32_t n, const uint32_t* count_ptr, const float bounds[6], uint32_t* keys_in,
+                     uint32_t* keys_out, uint32_t* vals_in, uint32_t* order, void* temp, size_t& temp_bytes,
+                     hipStream_t s);
+
 namespace {
+
+// Launches shorter than this are traced in submission order.
+constexpr uint32_t REORDER_MIN = 65536;
 
 struct Tune {
     uint32_t block, stack, refill_min, grid_per_cu;
@@ -987,6 +999,8 @@ struct StreamRes {
     size_t scratch_bytes = 0;
     void* table = nullptr;        // batched launches: batch table and prefix
     size_t table_bytes = 0;
+    void* order = nullptr;        // ray reordering: keys, indices and sort space
+    size_t order_bytes = 0;
 };
 
 struct Ctx {
@@ -1017,7 +1031,7 @@ void default_tuning(rq_tuning& t, int layout, int any) {
     t.persistent = 1;
     t.waves_per_eu = 0;
     t.grid_per_cu = 0;
-    t.reserved[0] = any ? 16u : 16u;  // refill threshold (lanes)
+    t.refill = any ? 16u : 16u;  // refill threshold (lanes)
     (void)layout;
 }
 
@@ -1176,6 +1190,18 @@ rq_status ensure_table(StreamRes& r, size_t bytes) {
     return RQ_OK;
 }
 
+rq_status ensure_order(StreamRes& r, size_t bytes) {
+    if (r.order && r.order_bytes >= bytes) return RQ_OK;
+    if (r.order) {
+        HIP_TRY(hipStreamSynchronize(r.stream), "hipStreamSynchronize");
+        HIP_TRY(hipFree(r.order), "hipFree order");
+        r.order = nullptr;
+    }
+    HIP_TRY(hipMalloc(&r.order, bytes), "hipMalloc order");
+    r.order_bytes = bytes;
+    return RQ_OK;
+}
+
 void release_stream(hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_ctx.mu);
     for (size_t i = 0; i < g_ctx.res.size(); ++i)
@@ -1186,6 +1212,7 @@ void release_stream(hipStream_t s) {
             if (r.spill) (void)hipFree(r.spill);
             if (r.scratch) (void)hipFree(r.scratch);
             if (r.table) (void)hipFree(r.table);
+            if (r.order) (void)hipFree(r.order);
             g_ctx.res.erase(g_ctx.res.begin() + i);
             return;
         }
@@ -1292,8 +1319,8 @@ extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* o
 }
 
 extern "C" rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->reserved[0] == 0 ||
-        t->reserved[0] > 64) {
+    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+        t->refill > 64) {
         set_error("rq_set_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1333,6 +1360,8 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
 rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
                        rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
                        const uint32_t* count_ptr = nullptr) {
+    const bool reorder = (mode & RQ_REORDER) != 0;
+    mode &= ~RQ_REORDER;
     if (!scene || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays)) {
         set_error("rq_trace: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -1387,8 +1416,24 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     a.counter = res->counter;
     a.spill = res->spill;
     a.count_ptr = count_ptr;
-    a.refill_min = t.reserved[0] ? t.reserved[0] : 1u;
+    a.refill_min = t.refill ? t.refill : 1u;
     a.bytes = bytes;
+    if (reorder && count >= REORDER_MIN) {
+        // keys, indices, sorted keys, order, then the sort's own space
+        const size_t arr = ((size_t)count * 4 + 255) & ~(size_t)255;
+        size_t temp = 0;
+        HIP_TRY(ray_order(rays, count, count_ptr, scene->bounds, nullptr, nullptr, nullptr, nullptr, nullptr, temp, s),
+                "ray order (size)");
+        if ((st = ensure_order(*res, 4 * arr + temp)) != RQ_OK) return st;
+        char* b = reinterpret_cast<char*>(res->order);
+        uint32_t* keys = reinterpret_cast<uint32_t*>(b);
+        uint32_t* vals = reinterpret_cast<uint32_t*>(b + arr);
+        uint32_t* sorted = reinterpret_cast<uint32_t*>(b + 2 * arr);
+        uint32_t* order = reinterpret_cast<uint32_t*>(b + 3 * arr);
+        HIP_TRY(ray_order(rays, count, count_ptr, scene->bounds, keys, sorted, vals, order, b + 4 * arr, temp, s),
+                "ray order");
+        a.order = order;
+    }
     k->fn[layout][any](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "trace launch");
     return RQ_OK;
@@ -1485,7 +1530,7 @@ extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_co
     a.batches = reinterpret_cast<const dev::BatchDev*>(tb);
     a.prefix = reinterpret_cast<const uint32_t*>(tb + poff);
     a.n_batches = batch_count;
-    a.refill_min = t.reserved[0] ? t.reserved[0] : 1u;
+    a.refill_min = t.refill ? t.refill : 1u;
     k->bfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "batched trace launch");
     // The table is read by the kernel: the next call on this stream may overwrite it only after.
@@ -1517,8 +1562,8 @@ extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
 }
 
 extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->reserved[0] == 0 ||
-        t->reserved[0] > 64) {
+    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+        t->refill > 64) {
         set_error("rq_set_batch_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1530,7 +1575,7 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
 
 extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                           uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]) {
-    if (!scene || !stats || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays) ||
+    if (!scene || !stats || (int)layout < 0 || (int)layout > 2 || (mode & ~RQ_REORDER) > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;

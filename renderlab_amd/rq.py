@@ -36,6 +36,7 @@ LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8}
 CLOSEST = 0
 ANY = 1
 ANY_OPAQUE = 2
+REORDER = 0x100  # or'ed into a trace mode: sort the launch's rays before tracing
 
 LEAF_BIT = 0x80000000
 EMPTY = 0xFFFFFFFF
@@ -154,7 +155,7 @@ def _ptr(a):
 
 class RqTuning(ctypes.Structure):
     _fields_ = [("block", _u32), ("stack", _u32), ("persistent", _u32), ("waves_per_eu", _u32),
-                ("grid_per_cu", _u32), ("reserved", _u32 * 3)]
+                ("grid_per_cu", _u32), ("refill", _u32), ("reserved", _u32 * 2)]
 
 
 def device_count():
@@ -399,7 +400,7 @@ def get_batch_tuning(layout):
 def set_batch_tuning(layout, block, stack, refill, grid_per_cu=0):
     t = RqTuning()
     t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
-    t.reserved[0] = refill
+    t.refill = refill
     check(lib.rq_set_batch_tuning(LAYOUTS.get(layout, layout), ctypes.byref(t)))
 
 
@@ -535,7 +536,7 @@ class Scene:
         own = None
         if hits is None and vis is None:
             own = DeviceBuffer(max(int(count), 1) * HIT_DTYPE.itemsize)
-            if int(mode) == CLOSEST:
+            if int(mode) & 3 == CLOSEST:
                 hits = own
             else:
                 vis = own
@@ -571,7 +572,7 @@ def get_tuning(layout, mode):
 def set_tuning(layout, mode, block, stack, refill, grid_per_cu=0):
     t = RqTuning()
     t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
-    t.reserved[0] = refill
+    t.refill = refill
     check(lib.rq_set_tuning(LAYOUTS.get(layout, layout), int(mode), ctypes.byref(t)))
 
 
