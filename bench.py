@@ -297,7 +297,7 @@ def main():
     log(f"scene {args.config}: {st} built in {time.perf_counter() - t0:.1f}s")
     layouts = [args.layout] + ([] if args.no_compare or args.layout == "plain" else ["plain"])
     t0 = time.perf_counter()
-    scene.upload(tuple(set(layouts) | {"plain"}))
+    scene.upload(tuple(set(layouts) | {"plain", "ref"}))  # REF: original positions for the FP64 check
     log(f"uploaded {layouts} in {time.perf_counter() - t0:.1f}s; bytes " +
         ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
     stream = rq.Stream()
@@ -328,13 +328,22 @@ def main():
     log(f"dominant launch {dom}: {dom_rays} rays, steps/ray node {stats['node_steps'] / dom_rays:.2f} "
         f"tri-pair {stats['tri_steps'] / dom_rays:.2f} inst {stats['inst_steps'] / dom_rays:.2f}; lane use "
         f"{(stats['node_steps'] + stats['tri_steps'] + stats['inst_steps']) / max(lanes, 1):.3f}; "
-        f"iterations/wave-ray {stats['iterations'] * 64 / dom_rays:.1f}")
+        f"iterations/wave-ray {stats['iterations'] * 64 / dom_rays:.1f}; per iteration: live lanes "
+        f"{stats['live_lanes'] / max(stats['iterations'], 1):.1f}, wanting node {stats['want_node'] / max(stats['iterations'], 1):.1f} "
+        f"tri {stats['want_tri'] / max(stats['iterations'], 1):.1f} inst {stats['want_inst'] / max(stats['iterations'], 1):.1f}")
+    plain_stats = None
+    if "plain" in results and args.layout != "plain":
+        plain_stats = scene.count_stats("plain", dom_ptr, dom_rays, dom_mode)
+        log(f"plain on the same launch: steps/ray node {plain_stats['node_steps'] / dom_rays:.2f} tri-pair "
+            f"{plain_stats['tri_steps'] / dom_rays:.2f} inst {plain_stats['inst_steps'] / dom_rays:.2f}; bytes/ray "
+            f"{plain_stats['bytes'] / dom_rays:.0f}; iterations/wave-ray {plain_stats['iterations'] * 64 / dom_rays:.1f}")
     achieved = nbytes / (per[dom] * 1e-3)
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK, 4), "traffic": None, "kernel": f"trace {dom}",
                 "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4),
                 "lane_use": round((stats["node_steps"] + stats["tri_steps"] + stats["inst_steps"]) /
-                                  max(64 * stats["iterations"], 1), 3)}
+                                  max(64 * stats["iterations"], 1), 3),
+                "live_lanes": round(stats["live_lanes"] / max(64 * stats["iterations"], 1), 3)}
 
     # FP64 verification of the GI hits (outside the timed region).
     verify = None
@@ -370,7 +379,13 @@ def main():
     if "plain" in results and args.layout != "plain":
         pm = results["plain"][0]
         line["plain_f32"] = {"value": round(rays / (pm * 1e-3) / 1e6, 2), "ms_per_step": round(pm, 4),
-                             "quantized_speedup": round(pm / ms, 3)}
+                             "quantized_speedup": round(pm / ms, 3),
+                             "per_launch_ms": {k: round(v, 4) for k, v in results["plain"][2].items()}}
+        if plain_stats:
+            line["plain_f32"]["dominant_steps_per_ray"] = {
+                k: round(plain_stats[f"{k}_steps"] / max(dom_rays, 1), 2) for k in ("node", "tri", "inst")}
+        line["roofline"]["steps_per_ray"] = {k: round(stats[f"{k}_steps"] / max(dom_rays, 1), 2)
+                                             for k in ("node", "tri", "inst")}
     if verify:
         line["fp64_verify"] = verify
     line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}

@@ -202,9 +202,11 @@ rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_lay
 rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes);
 /* The same launch with its step statistics: [0] bytes, [1] wave iterations,
-   [2] node steps, [3] triangle-pair steps, [4] instance entries, [5] refills. */
+   [2] node steps, [3] triangle-pair steps, [4] instance entries, [5] refills;
+   summed over wave iterations: [6] lanes holding a live ray, [7] [8] [9]
+   lanes wanting a node / triangle-pair / instance step. */
 rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
-                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]);
+                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[10]);
 
 /* Launch parameters of the traversal kernels (the autotuner's knobs). */
 typedef struct rq_tuning {
@@ -224,13 +226,21 @@ rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* tuning);
 
 /* ---- FP64 verification: the only FP64 on the GPU. For each hit ray,
    re-intersects its reported triangle in f64 (object-space ray computed in
-   f64 from the f32 world ray) and reduces max |t-t64|/|t64| and max
-   |u-u64|, |v-v64|. out (host): [0] max rel t error, [1] max abs bary error,
-   [2] hits checked, [3] max |t-t64|*|d| / (|o| + |t64|*|d|): the distance
-   error against the size of the coordinates (object space), which bounds
-   f32 error for short hits far from the origin where [0] cannot. */
+   f64 from the f32 world ray) against the original positions (needs the REF
+   layout uploaded). With S the largest magnitude the f32 computation rounds
+   (the world-to-object terms |r_k . o| + |t_k|, the vertex coordinates) plus
+   |t64|*|d|, out (host):
+   [0] max |t-t64|/|t64|, [1] max |u-u64|, |v-v64|, [2] hits checked,
+   [3] max |t-t64|*|d| / S: the distance error against the size of the
+       numbers it came from, which bounds f32 error for short hits far from
+       the origin where [0] cannot,
+   [4] hits with [0]'s ratio above 1e-5,
+   [5] over hits away from grazing (cosine to the normal > 0.01) and slivers
+       (edge-angle sine > 1e-3): max of [3]'s ratio * cosine / 2^-24, the
+       error in f32 roundings of S,
+   [6] hits counted in [5], [7] zero. */
 rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,
-                         double out[4], void* stream);
+                         double out[8], void* stream);
 
 /* ---- Ray generation (device). Formulas: shaders/primary_rays.wgsl:50-66
    (primary, reversed Z), shaders/reference_path_tracer.wgsl:149-167,

@@ -15,7 +15,7 @@
 //   Through Compressed Wide BVHs", HPG 2017), collapsed from the 4-wide
 //   trees, for the BLAS and the top level alike. A child's box encloses the
 //   PLAIN box of the subtree it holds, rounded outward, and decodes exactly
-//   as fma(2^e, q, origin) = origin + 2^e*q. Leaves and triangles are the
+//   as origin + 2^e*q with no rounding. Leaves and triangles are the
 //   PLAIN layout's, in the same order.
 #include <algorithm>
 #include <cmath>
@@ -27,6 +27,14 @@ namespace rq {
 namespace {
 
 constexpr uint32_t LEAF_COUNT_SHIFT = 27;   // leaf tag: LEAF | (n-1)<<27 | first triangle
+
+// Exponent of the spacing of f32 values at |x| (ulp), -149 at zero.
+int ulp_exp(float x) {
+    if (x == 0.0f) return -149;
+    int ex;
+    std::frexp(std::fabs(x), &ex);
+    return std::max(ex - 24, -149);
+}
 constexpr uint32_t MAX_LEAF_TRIS = 16;
 constexpr uint32_t MAX_FIRST_TRI = 1u << 27;
 constexpr uint32_t W8_WORDS = 24;
@@ -93,11 +101,17 @@ void decode_unfused(const uint32_t* w, int slot, float lo[3], float hi[3]) {
 void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint32_t slot_first,
                       uint32_t n, std::vector<float>& tris, std::vector<uint32_t>& ids) {
     for (uint32_t k = slot_first; k < slot_first + n; ++k) {
+        // Record words 0-8: vertex a, then the edges b - a and c - a rounded to
+        // f32 -- the same roundings the intersection test would make, done once.
+        float p[3][3];
         for (int v = 0; v < 3; ++v) {
             uint32_t vi = words[m.view.indices + 3ull * k + v];
-            for (int a = 0; a < 3; ++a) tris.push_back(bits_f(words[m.view.positions + 3ull * vi + a]));
+            for (int a = 0; a < 3; ++a) p[v][a] = bits_f(words[m.view.positions + 3ull * vi + a]);
             ids.push_back(vi);
         }
+        for (int a = 0; a < 3; ++a) tris.push_back(p[0][a]);
+        for (int e = 1; e < 3; ++e)
+            for (int a = 0; a < 3; ++a) tris.push_back(p[e][a] - p[0][a]);
         // Record words 9-11: BVH-order slot, primitive id, zero (read with c.z).
         tris.push_back(bits_f(k));
         tris.push_back(bits_f(m.prims ? words[m.prim_base + k] : 0u));
@@ -244,7 +258,8 @@ std::vector<Cand> wide_children(const std::vector<uint32_t>& nodes, uint32_t nod
 //   0-2 origin (f32), 3 exponents x|y<<8|z<<16 and valid-slot mask <<24,
 //   4-15 byte planes (lo x, lo y, lo z, hi x, hi y, hi z; 8 codes, 2 words each),
 //   16-23 child tags (node index, leaf tag unchanged from the source, EMPTY).
-// A box decodes as fma(2^e, q, origin), which equals origin + 2^e*q exactly.
+// A box decodes as origin + 2^e*q, which is a representable f32 for every
+// code (the origin is aligned and the step chosen for that).
 struct W8Builder {
     const std::vector<uint32_t>* src;
     std::vector<uint32_t>* out;
@@ -315,16 +330,36 @@ struct W8Builder {
         uint32_t w[W8_WORDS] = {};
         int exps[3];
         for (int a = 0; a < 3; ++a) {
-            float p = all.lo[a], top = all.hi[a];
-            float ext = top - p;
+            const float lo = all.lo[a], hi = all.hi[a];
+            if (!std::isfinite(lo) || !std::isfinite(hi)) {
+                *err = "box too large to quantize";
+                return false;
+            }
             int e = -126;
-            if (ext > 0.0f) {
+            if (hi - lo > 0.0f) {
                 int ex;
-                std::frexp(ext / 255.0f, &ex);
+                std::frexp((hi - lo) / 255.0f, &ex);
                 e = std::max(ex - 1, -126);
             }
-            while (p + std::ldexp(1.0f, e) * 255.0f < top) ++e;
-            if (e > 127 || !std::isfinite(p)) {
+            // Every code must decode exactly: the origin sits on a grid of 2^ue,
+            // where 2^ue is the largest ulp over [origin, origin + 255 * 2^e],
+            // and the step 2^e is at least 2^ue. Then origin + q * 2^e is a
+            // representable multiple of 2^ue for every q.
+            int ue = -149;
+            float p = lo;
+            bool ok = false;
+            for (int guard = 0; guard < 1024 && e <= 127; ++guard) {
+                p = ue <= ulp_exp(lo) ? lo : std::floor(lo / std::ldexp(1.0f, ue)) * std::ldexp(1.0f, ue);
+                const float top = p + std::ldexp(1.0f, e) * 255.0f;
+                const int need = std::max(ulp_exp(p), ulp_exp(top));
+                if (!std::isfinite(top)) break;
+                if (need > ue) { ue = need; continue; }
+                if (e < ue) { e = ue; continue; }
+                if (top < hi) { ++e; continue; }
+                ok = true;
+                break;
+            }
+            if (!ok) {
                 *err = "box too large to quantize";
                 return false;
             }

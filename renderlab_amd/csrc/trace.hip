@@ -79,10 +79,10 @@ __device__ __forceinline__ float slab_near(V3 o, V3 inv, V3 lo, V3 hi, float tmi
     return near <= far ? near : FMAXV;
 }
 
-// gi_triangle (:42-58); returns t < 0 on a miss.
-__device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 b, V3 c, float tmin, float tmax, float& u_out,
+// gi_triangle (:42-58) on vertex a and the edges e1 = b - a, e2 = c - a;
+// returns t < 0 on a miss.
+__device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 e1, V3 e2, float tmin, float tmax, float& u_out,
                                           float& v_out) {
-    V3 e1 = sub3(b, a), e2 = sub3(c, a);
     V3 p = cross3(d, e2);
     float det = dot3(e1, p);
     if (fabsf(det) < 1e-30f) return -1.0f;
@@ -102,9 +102,9 @@ __device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 b, V3 c, float tm
     return -1.0f;
 }
 
-// shaders/scene.wgsl:86-90 applied to mat3x3(r0,r1,r2) * cross(b-a, c-a)
-__device__ __forceinline__ V3 hit_normal(V3 r0, V3 r1, V3 r2, V3 a, V3 b, V3 c) {
-    V3 x = cross3(sub3(b, a), sub3(c, a));
+// shaders/scene.wgsl:86-90 applied to mat3x3(r0,r1,r2) * cross(e1, e2)
+__device__ __forceinline__ V3 hit_normal(V3 r0, V3 r1, V3 r2, V3 e1, V3 e2) {
+    V3 x = cross3(e1, e2);
     V3 n = mk(r0.x * x.x + r1.x * x.y + r2.x * x.z, r0.y * x.x + r1.y * x.y + r2.y * x.z,
               r0.z * x.x + r1.z * x.y + r2.z * x.z);
     float scale = fmx(fmx(fabsf(n.x), fabsf(n.y)), fabsf(n.z));
@@ -213,6 +213,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     // COUNT: [1] wave iterations, [2] node, [3] triangle-pair, [4] instance
     // steps (lane sums), [5] refill events (per wave)
     unsigned long long st_iter = 0, st_node = 0, st_tri = 0, st_inst = 0, st_refill = 0;
+    // [6] live lanes, [7] [8] [9] lanes wanting a node / triangle-pair / instance
+    // step, summed over wave iterations (wave-uniform)
+    unsigned long long st_live = 0, st_want_node = 0, st_want_tri = 0, st_want_inst = 0;
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
     SceneView v = a.sv;
     bool any_hit = AM == 1;
@@ -220,14 +223,20 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t* out_vis = a.vis;
     uint32_t out_idx = 0;
 
-    auto push = [&](uint32_t x) {
-        if (sp - lo == (uint32_t)S) {
+    // A node step pushes at most n entries: spill the oldest ring entries
+    // first so that n more fit, then push without per-entry checks.
+    auto room = [&](uint32_t n) {
+        while (sp - lo + n > (uint32_t)S) {
             spill[lo] = st[(lo & (S - 1)) * BLOCK];
             lo++;
             if (COUNT) nb += 4;
         }
+    };
+    // Branch-free push after room(): the slot above the top is written either
+    // way and kept only when `c` holds.
+    auto push_if = [&](uint32_t x, bool c) {
         st[(sp & (S - 1)) * BLOCK] = x;
-        sp++;
+        sp += c ? 1u : 0u;
     };
     auto pop = [&]() -> uint32_t {
         sp--;
@@ -244,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
         V3 n = mk(0.0f, 0.0f, 0.0f);
         if (bhit) {
-            V3 r0, r1, r2, va, vb, vc;
+            V3 r0, r1, r2, e1, e2;
             if (LAYOUT == REF) {
                 const uint32_t* r = v.words + 16ull + (uint64_t)binst * 32u;
                 r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
@@ -256,9 +265,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 tri1 = v.words[x + 1];
                 tri2 = v.words[x + 2];
                 const uint32_t* P = v.words + pos;
-                va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
-                vb = mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2]));
-                vc = mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2]));
+                const V3 va = mk(bf(P[3ull * tri0]), bf(P[3ull * tri0 + 1]), bf(P[3ull * tri0 + 2]));
+                e1 = sub3(mk(bf(P[3ull * tri1]), bf(P[3ull * tri1 + 1]), bf(P[3ull * tri1 + 2])), va);
+                e2 = sub3(mk(bf(P[3ull * tri2]), bf(P[3ull * tri2 + 1]), bf(P[3ull * tri2 + 2])), va);
                 slot = btri;
                 if (!any_hit && prim_base != 0u) {
                     tri0 = v.words[prim_base + (uint64_t)btri];
@@ -272,9 +281,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
                 const float4* tp = v.tris + 3ull * btri;
                 float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                va = mk(t0.x, t0.y, t0.z);
-                vb = mk(t0.w, t1.x, t1.y);
-                vc = mk(t1.z, t1.w, t2.x);
+                e1 = mk(t0.w, t1.x, t1.y);
+                e2 = mk(t1.z, t1.w, t2.x);
                 slot = fb(t2.y);
                 const uint32_t* ids = v.tri_ids + 4ull * btri;
                 tri0 = ids[0];
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     tri2 = 0;
                 }
             }
-            n = hit_normal(r0, r1, r2, va, vb, vc);
+            n = hit_normal(r0, r1, r2, e1, e2);
             if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 48 + 12);
         }
         if (out_hits) {
@@ -369,7 +377,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
         }
         if (exhausted && __ballot(ray != NONE) == 0ull) break;
-        if (COUNT) st_iter++;
+        if (COUNT) {
+            st_iter++;
+            st_live += (unsigned long long)__popcll(__ballot(ray != NONE && !done));
+        }
         if (ray == NONE || done) continue;
 
         bool fin = invalid;
@@ -407,6 +418,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         const uint32_t n_node = (uint32_t)__popcll(__ballot(kind == 3));
         const int pick = (n_node >= n_tri && n_node >= n_inst) ? 3 : (n_tri >= n_inst ? 1 : 2);
         if (COUNT) {
+            st_want_node += n_node;
+            st_want_tri += n_tri;
+            st_want_inst += n_inst;
             st_node += kind == 3 && pick == 3;
             st_tri += kind == 1 && pick == 1;
             st_inst += kind == 2 && pick == 2;
@@ -417,27 +431,27 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 #pragma unroll
             for (uint32_t j = 0; j < 2; ++j) {
                 if (j >= k || fin) break;
-                V3 va, vb, vc;
+                V3 va, e1, e2;
                 uint32_t slot_id;
                 if (LAYOUT == REF) {
                     uint64_t x = (uint64_t)m_idx + 3ull * (tri_next + j);
                     uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
                     const uint32_t* P = v.words + m_pos;
                     va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
-                    vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
-                    vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
+                    e1 = sub3(mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2])), va);
+                    e2 = sub3(mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2])), va);
                     slot_id = tri_next + j;
                 } else {
                     const float4* tp = v.tris + 3ull * (tri_next + j);
                     float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                     va = mk(t0.x, t0.y, t0.z);
-                    vb = mk(t0.w, t1.x, t1.y);
-                    vc = mk(t1.z, t1.w, t2.x);
+                    e1 = mk(t0.w, t1.x, t1.y);
+                    e2 = mk(t1.z, t1.w, t2.x);
                     slot_id = tri_next + j;
                 }
                 if (COUNT) nb += 48;
                 float u, vv;
-                float t = tri_test(oo, od, va, vb, vc, tmin, limit(cur), u, vv);
+                float t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
                 if (t >= 0.0f) {
                     bt = t; bu = u; bv = vv; binst = cur; bhit = true;
                     btri = slot_id;
@@ -531,9 +545,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     if (dist[3] < dist[1]) RQ_SW(1, 3);
                     if (dist[2] < dist[1]) RQ_SW(1, 2);
 #undef RQ_SW
+                    room(3);
 #pragma unroll
-                    for (int l = 3; l >= 1; --l)
-                        if (dist[l] <= bound && sp < STACK_CAP) push(ids[l]);
+                    for (int l = 3; l >= 1; --l) push_if(ids[l], dist[l] <= bound && sp < STACK_CAP);
                     if (dist[0] <= bound && sp < STACK_CAP) nxt = ids[0];  // the nearest, popped next
                 } else {
                 // BLAS node (:171-204), decoded from the quantized record
@@ -567,9 +581,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 cas_desc(k0, c[0], k2, c[2]);
                 cas_desc(k1, c[1], k3, c[3]);
                 cas_desc(k1, c[1], k2, c[2]);
-                if (bf(k0) <= lim && sp - floor_ < STACK_CAP) push(c[0]);
-                if (bf(k1) <= lim && sp - floor_ < STACK_CAP) push(c[1]);
-                if (bf(k2) <= lim && sp - floor_ < STACK_CAP) push(c[2]);
+                room(3);
+                push_if(c[0], bf(k0) <= lim && sp - floor_ < STACK_CAP);
+                push_if(c[1], bf(k1) <= lim && sp - floor_ < STACK_CAP);
+                push_if(c[2], bf(k2) <= lim && sp - floor_ < STACK_CAP);
                 if (bf(k3) <= lim && sp - floor_ < STACK_CAP) nxt = c[3];  // the nearest, popped next
                 }
             } else {
@@ -604,15 +619,19 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 cas_desc(k0, c[0], k2, c[2]);
                 cas_desc(k1, c[1], k3, c[3]);
                 cas_desc(k1, c[1], k2, c[2]);
-                if (bf(k0) <= lim && sp < cap) push(c[0]);
-                if (bf(k1) <= lim && sp < cap) push(c[1]);
-                if (bf(k2) <= lim && sp < cap) push(c[2]);
+                room(3);
+                push_if(c[0], bf(k0) <= lim && sp < cap);
+                push_if(c[1], bf(k1) <= lim && sp < cap);
+                push_if(c[2], bf(k2) <= lim && sp < cap);
                 if (bf(k3) <= lim && sp < cap) nxt = c[3];  // the nearest, popped next
             } else {
                 // CW8 node: 24 words (origin, exponents + valid mask, byte planes, tags).
-                // Each bound is fma(q, 2^e / d, (origin - o) / d): one fma a plane
-                // after a per-node setup, widened by an error slack that covers the
-                // rounding of either evaluation order, so culling stays conservative.
+                // Planes decode exactly (origin + 2^e * q is an f32 by construction).
+                // Each bound is fma(q, 2^e / d, (origin - o) / d - err): one fma a
+                // plane after a per-node setup. The roundings (origin - o, the
+                // reciprocal's 1 ulp, two products, the slack subtraction and the fma)
+                // stay below 7 * 2^-24 * M with M = |base| + 255 |scale|, so a slack
+                // of 2^-20 * M keeps culling conservative.
                 const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
                 uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4], q5 = w4[5];
                 if (COUNT) nb += 96;
@@ -621,7 +640,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                                   bf(((q0.w >> 16) & 0xffu) << 23) * ri.z);
                 const V3 base = mk((bf(q0.x) - ro.x) * ri.x, (bf(q0.y) - ro.y) * ri.y, (bf(q0.z) - ro.z) * ri.z);
                 const float err = fmx(fmx(fabsf(base.x) + 255.0f * fabsf(scl.x), fabsf(base.y) + 255.0f * fabsf(scl.y)),
-                                      fabsf(base.z) + 255.0f * fabsf(scl.z)) * 4.76837158e-7f;  // 2^-21
+                                      fabsf(base.z) + 255.0f * fabsf(scl.z)) * 9.53674316e-7f;  // 2^-20
+                const V3 bn = mk(base.x - err, base.y - err, base.z - err);  // near planes
+                const V3 bfr = mk(base.x + err, base.y + err, base.z + err); // far planes
                 // planes: q1 = lo x (2 words), lo y; q2 = lo z, hi x; q3 = hi y, hi z
                 const bool px = ri.x >= 0.0f, py = ri.y >= 0.0f, pz = ri.z >= 0.0f;
                 const uint32_t nx0 = px ? q1.x : q2.z, nx1 = px ? q1.y : q2.w, fx0 = px ? q2.z : q1.x, fx1 = px ? q2.w : q1.y;
@@ -629,31 +650,49 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 const uint32_t nz0 = pz ? q2.x : q3.z, nz1 = pz ? q2.y : q3.w, fz0 = pz ? q3.z : q2.x, fz1 = pz ? q3.w : q2.y;
                 const uint32_t tag[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
                 uint32_t hit = 0;
+                // Closest-hit rays visit the nearest hit child next: the minimum of
+                // (entry distance bits with the slot in the low 3 bits); entry
+                // distances are >= +-0, so clearing the sign orders them as integers.
+                const bool order = AM == 0 || (AM == 2 && !any_hit);
+                uint32_t kmin = 0xffffffffu;
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
                     const int sh = 8 * (s & 3);
                     const uint32_t wnx = s < 4 ? nx0 : nx1, wny = s < 4 ? ny0 : ny1, wnz = s < 4 ? nz0 : nz1;
                     const uint32_t wfx = s < 4 ? fx0 : fx1, wfy = s < 4 ? fy0 : fy1, wfz = s < 4 ? fz0 : fz1;
-                    const float tnx = __builtin_fmaf((float)((wnx >> sh) & 0xffu), scl.x, base.x);
-                    const float tny = __builtin_fmaf((float)((wny >> sh) & 0xffu), scl.y, base.y);
-                    const float tnz = __builtin_fmaf((float)((wnz >> sh) & 0xffu), scl.z, base.z);
-                    const float tfx = __builtin_fmaf((float)((wfx >> sh) & 0xffu), scl.x, base.x);
-                    const float tfy = __builtin_fmaf((float)((wfy >> sh) & 0xffu), scl.y, base.y);
-                    const float tfz = __builtin_fmaf((float)((wfz >> sh) & 0xffu), scl.z, base.z);
-                    const float near = fmx(fmx(fmx(tnx, tny), tnz) - err, tmin);
-                    const float far = fmn(fmn(fmn(tfx, tfy), tfz) + err, lim);
-                    if (near <= far && ((valid >> s) & 1u)) hit |= 1u << s;
+                    const float tnx = __builtin_fmaf((float)((wnx >> sh) & 0xffu), scl.x, bn.x);
+                    const float tny = __builtin_fmaf((float)((wny >> sh) & 0xffu), scl.y, bn.y);
+                    const float tnz = __builtin_fmaf((float)((wnz >> sh) & 0xffu), scl.z, bn.z);
+                    const float tfx = __builtin_fmaf((float)((wfx >> sh) & 0xffu), scl.x, bfr.x);
+                    const float tfy = __builtin_fmaf((float)((wfy >> sh) & 0xffu), scl.y, bfr.y);
+                    const float tfz = __builtin_fmaf((float)((wfz >> sh) & 0xffu), scl.z, bfr.z);
+                    const float near = fmx(fmx(fmx(tnx, tny), tnz), tmin);
+                    const float far = fmn(fmn(fmn(tfx, tfy), tfz), lim);
+                    const bool h = near <= far && ((valid >> s) & 1u);
+                    if (h) hit |= 1u << s;
+                    if (order) kmin = min(kmin, h ? ((fb(near) & 0x7ffffff8u) | (uint32_t)s) : 0xffffffffu);
+                }
+                uint32_t first = NONE;
+                if (order && hit) {
+                    const uint32_t m = kmin & 7u;
+                    const uint32_t a0 = (m & 1u) ? tag[1] : tag[0], a1 = (m & 1u) ? tag[3] : tag[2];
+                    const uint32_t a2 = (m & 1u) ? tag[5] : tag[4], a3 = (m & 1u) ? tag[7] : tag[6];
+                    const uint32_t b0 = (m & 2u) ? a1 : a0, b1 = (m & 2u) ? a3 : a2;
+                    first = (m & 4u) ? b1 : b0;
+                    hit &= ~(1u << m);
                 }
                 // Push far to near in octant order: slot k ^ oct is the k-th nearest
                 // guess. The XOR permutation is three conditional swap stages on
                 // registers (a lane-varying index into a register array would not be).
                 uint32_t tg[8] = {tag[0], tag[1], tag[2], tag[3], tag[4], tag[5], tag[6], tag[7]};
-                if (ri.x < 0.0f) {
+                // Any-hit rays end at their first hit and their visibility bit does
+                // not depend on the visit order: they skip the ordering.
+                if (order && ri.x < 0.0f) {
 #pragma unroll
                     for (int k = 0; k < 8; k += 2) { uint32_t x = tg[k]; tg[k] = tg[k + 1]; tg[k + 1] = x; }
                     hit = ((hit & 0x55u) << 1) | ((hit >> 1) & 0x55u);
                 }
-                if (ri.y < 0.0f) {
+                if (order && ri.y < 0.0f) {
 #pragma unroll
                     for (int k = 0; k < 8; k += 4) {
                         uint32_t x = tg[k]; tg[k] = tg[k + 2]; tg[k + 2] = x;
@@ -661,15 +700,23 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     }
                     hit = ((hit & 0x33u) << 2) | ((hit >> 2) & 0x33u);
                 }
-                if (ri.z < 0.0f) {
+                if (order && ri.z < 0.0f) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k) { uint32_t x = tg[k]; tg[k] = tg[k + 4]; tg[k + 4] = x; }
                     hit = ((hit & 0x0fu) << 4) | ((hit >> 4) & 0x0fu);
                 }
+                if (order) {
+                    // the rest far to near in octant order, then the nearest
+                    room(8);
 #pragma unroll
-                for (int k = 7; k >= 1; --k)
-                    if (((hit >> k) & 1u) && sp < cap) push(tg[k]);
-                if ((hit & 1u) && sp < cap) nxt = tg[0];  // the nearest guess, popped next
+                    for (int k = 7; k >= 0; --k) push_if(tg[k], ((hit >> k) & 1u) && sp < cap);
+                    if (first != NONE && sp < cap) nxt = first;
+                } else {
+                    room(7);
+#pragma unroll
+                    for (int k = 7; k >= 1; --k) push_if(tg[k], ((hit >> k) & 1u) && sp < cap);
+                    if ((hit & 1u) && sp < cap) nxt = tg[0];
+                }
             }
             }
         }
@@ -689,6 +736,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             atomicAdd(a.bytes + 3, st_tri);
             atomicAdd(a.bytes + 4, st_inst);
             atomicAdd(a.bytes + 5, st_refill);
+            atomicAdd(a.bytes + 6, st_live);
+            atomicAdd(a.bytes + 7, st_want_node);
+            atomicAdd(a.bytes + 8, st_want_tri);
+            atomicAdd(a.bytes + 9, st_want_inst);
         }
     }
 }
@@ -885,9 +936,7 @@ __global__ __launch_bounds__(SEC_BLOCK) void secondary_write(SecArgs a) {
 
 // ------------------------------------------------------------ FP64 verify
 struct VerifyArgs {
-    const uint32_t* words;  // REF words, or null
-    const uint32_t* inst;   // PLAIN instance records, or null
-    const float4* tris;     // PLAIN triangles
+    const uint32_t* words;  // REF words: instance records, positions, indices
     const rq_ray* rays;
     const rq_hit* hits;
     uint32_t count;
@@ -895,31 +944,28 @@ struct VerifyArgs {
 };
 
 __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
-    __shared__ double st[256], sb[256], ss[256];
-    __shared__ uint32_t sn[256];
+    __shared__ double st[256], sb[256], ss[256], sw[256];
+    __shared__ uint32_t sn[256], sx[256], sc[256];
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    double et = 0.0, eb = 0.0, es = 0.0;
-    uint32_t n = 0;
+    double et = 0.0, eb = 0.0, es = 0.0, ew = 0.0;
+    uint32_t n = 0, nx = 0, nw = 0;
     if (i < a.count) {
         const rq_hit h = a.hits[i];
         if (h.flags & 1u) {
             const rq_ray r = a.rays[i];
             double ro[3], rd[3], va[3], vb[3], vc[3];
-            const uint32_t* rec = a.inst ? a.inst + (uint64_t)h.inst * 32u : a.words + 16ull + (uint64_t)h.inst * 32u;
+            double size = 0.0;  // the largest magnitude the f32 computation rounds
+            const uint32_t* rec = a.words + 16ull + (uint64_t)h.inst * 32u;
             for (int k = 0; k < 3; ++k) {
                 const uint32_t* w = rec + 4 * k;
                 ro[k] = (double)bf(w[0]) * r.origin[0] + (double)bf(w[1]) * r.origin[1] +
                         (double)bf(w[2]) * r.origin[2] + (double)bf(w[3]);
+                size = fmax(size, fabs((double)bf(w[0]) * r.origin[0]) + fabs((double)bf(w[1]) * r.origin[1]) +
+                                      fabs((double)bf(w[2]) * r.origin[2]) + fabs((double)bf(w[3])));
                 rd[k] = (double)bf(w[0]) * r.direction[0] + (double)bf(w[1]) * r.direction[1] +
                         (double)bf(w[2]) * r.direction[2];
             }
-            if (a.inst) {
-                const float4* tp = a.tris + 3ull * (rec[20] + h.slot);
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                va[0] = t0.x; va[1] = t0.y; va[2] = t0.z;
-                vb[0] = t0.w; vb[1] = t1.x; vb[2] = t1.y;
-                vc[0] = t1.z; vc[1] = t1.w; vc[2] = t2.x;
-            } else {
+            {
                 uint32_t pos = rec[20], idx = rec[21];
                 uint64_t x = (uint64_t)idx + 3ull * h.slot;
                 for (int k = 0; k < 3; ++k) {
@@ -940,34 +986,59 @@ __global__ __launch_bounds__(256) void verify_kernel(VerifyArgs a) {
                 double t = (ac[0] * q[0] + ac[1] * q[1] + ac[2] * q[2]) / det;
                 et = fabs((double)h.t - t) / fmax(fabs(t), 1e-300);
                 eb = fmax(fabs((double)h.u - u), fabs((double)h.v - v));
-                // The distance error against the size of the coordinates it was computed from:
-                // f32 rounding of the hit point, not of t, bounds a short hit far from the origin.
+                // The distance error against the size of the numbers it was computed
+                // from (transform terms, vertices, hit distance): f32 rounding of those,
+                // not of t, bounds a short hit far from the origin.
+                for (int k = 0; k < 3; ++k) size = fmax(size, fmax(fabs(va[k]), fmax(fabs(vb[k]), fabs(vc[k]))));
                 double dn = sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
-                double on = sqrt(ro[0] * ro[0] + ro[1] * ro[1] + ro[2] * ro[2]);
-                es = fabs((double)h.t - t) * dn / fmax(on + fabs(t) * dn, 1e-300);
+                double scale = size + fabs(t) * dn;
+                es = fabs((double)h.t - t) * dn / fmax(scale, 1e-300);
                 n = 1;
+                nx = et > 1e-5 ? 1u : 0u;
+                // Away from grazing (cosine to the normal > 0.01) and slivers (edge angle
+                // sine > 1e-3), the same error times the cosine, in units of 2^-24 of
+                // that size: a count of f32 roundings, small for a sound f32 test.
+                double nn[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+                double nl = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                double l1 = sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+                double l2 = sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+                double cosine = fabs(det) / fmax(dn * nl, 1e-300);
+                if (cosine >= 1e-2 && nl >= 1e-3 * l1 * l2) {
+                    nw = 1;
+                    ew = es * cosine * 16777216.0;
+                }
             }
         }
     }
     st[threadIdx.x] = et;
     sb[threadIdx.x] = eb;
     ss[threadIdx.x] = es;
+    sw[threadIdx.x] = ew;
     sn[threadIdx.x] = n;
+    sx[threadIdx.x] = nx;
+    sc[threadIdx.x] = nw;
     __syncthreads();
     for (uint32_t off = 128; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
             st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + off]);
             sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + off]);
             ss[threadIdx.x] = fmax(ss[threadIdx.x], ss[threadIdx.x + off]);
+            sw[threadIdx.x] = fmax(sw[threadIdx.x], sw[threadIdx.x + off]);
             sn[threadIdx.x] += sn[threadIdx.x + off];
+            sx[threadIdx.x] += sx[threadIdx.x + off];
+            sc[threadIdx.x] += sc[threadIdx.x + off];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        // non-negative doubles order like their bit patterns
         atomicMax(a.out, (unsigned long long)__double_as_longlong(st[0]));
         atomicMax(a.out + 1, (unsigned long long)__double_as_longlong(sb[0]));
         atomicAdd(a.out + 2, (unsigned long long)sn[0]);
         atomicMax(a.out + 3, (unsigned long long)__double_as_longlong(ss[0]));
+        atomicAdd(a.out + 4, (unsigned long long)sx[0]);
+        atomicMax(a.out + 5, (unsigned long long)__double_as_longlong(sw[0]));
+        atomicAdd(a.out + 6, (unsigned long long)sc[0]);
     }
 }
 
@@ -1574,75 +1645,74 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
 }
 
 extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
-                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[6]) {
+                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[10]) {
     if (!scene || !stats || (int)layout < 0 || (int)layout > 2 || (mode & ~RQ_REORDER) > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
     unsigned long long* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, 6 * sizeof(unsigned long long)), "hipMalloc count");
-    HIP_TRY(hipMemset(acc, 0, 6 * sizeof(unsigned long long)), "hipMemset count");
+    HIP_TRY(hipMalloc(&acc, 10 * sizeof(unsigned long long)), "hipMalloc count");
+    HIP_TRY(hipMemset(acc, 0, 10 * sizeof(unsigned long long)), "hipMemset count");
     rq_status st = launch_trace(scene, layout, rays, count, mode, hits, vis_bits, nullptr, acc);
-    unsigned long long host[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long host[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(host, acc, sizeof host, hipMemcpyDeviceToHost);
     (void)hipFree(acc);
     if (st != RQ_OK) return st;
     if (e != hipSuccess) return hip_fail(e, "count");
-    for (int k = 0; k < 6; ++k) stats[k] = host[k];
+    for (int k = 0; k < 10; ++k) stats[k] = host[k];
     return RQ_OK;
 }
 
 extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                           uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes) {
-    uint64_t stats[6];
+    uint64_t stats[10];
     rq_status st = rq_trace_count_stats(scene, layout, rays, count, mode, hits, vis_bits, stats);
     if (st == RQ_OK) *bytes = stats[0];
     return st;
 }
 
 extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const rq_hit* hits, uint32_t count,
-                                    double out[3], void* stream) {
+                                    double out[8], void* stream) {
     if (!scene || !out || (count && (!rays || !hits))) {
         set_error("rq_verify_fp64: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
-    if (!scene->dev[1].ready && !scene->dev[0].ready) {
-        set_error("rq_verify_fp64: needs the REF or PLAIN layout uploaded");
+    if (!scene->dev[0].ready) {
+        set_error("rq_verify_fp64: needs the REF layout uploaded (the original positions)");
         return RQ_ERR_INVALID_INPUT;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     unsigned long long* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, 4 * sizeof(unsigned long long)), "hipMalloc verify");
-    HIP_TRY(hipMemsetAsync(acc, 0, 4 * sizeof(unsigned long long), s), "hipMemset verify");
+    HIP_TRY(hipMalloc(&acc, 8 * sizeof(unsigned long long)), "hipMalloc verify");
+    HIP_TRY(hipMemsetAsync(acc, 0, 8 * sizeof(unsigned long long), s), "hipMemset verify");
     dev::VerifyArgs a;
     std::memset(&a, 0, sizeof a);
-    if (scene->dev[1].ready) {
-        char* b = reinterpret_cast<char*>(scene->dev[1].base);
-        a.inst = reinterpret_cast<const uint32_t*>(b + scene->dev[1].off[0]);
-        a.tris = reinterpret_cast<const float4*>(b + scene->dev[1].off[3]);
-    } else {
-        a.words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
-    }
+    a.words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
     a.rays = rays;
     a.hits = hits;
     a.count = count;
     a.out = acc;
     if (count) hipLaunchKernelGGL(dev::verify_kernel, dim3((count + 255) / 256), dim3(256), 0, s, a);
-    unsigned long long host[4];
+    unsigned long long host[8];
     hipError_t e = hipMemcpyAsync(host, acc, sizeof host, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(acc);
     if (e != hipSuccess) return hip_fail(e, "verify");
-    double t, b, sc;
-    std::memcpy(&t, &host[0], 8);
-    std::memcpy(&b, &host[1], 8);
-    std::memcpy(&sc, &host[3], 8);
-    out[0] = t;
-    out[1] = b;
+    auto as_double = [](unsigned long long bits) {
+        double d;
+        std::memcpy(&d, &bits, 8);
+        return d;
+    };
+    out[0] = as_double(host[0]);
+    out[1] = as_double(host[1]);
     out[2] = (double)host[2];
-    out[3] = sc;
+    out[3] = as_double(host[3]);
+    out[4] = (double)host[4];
+    out[5] = as_double(host[5]);
+    out[6] = (double)host[6];
+    out[7] = 0.0;
     return RQ_OK;
 }
 

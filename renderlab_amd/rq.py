@@ -532,7 +532,7 @@ class Scene:
         """Step statistics of one launch: bytes, wave iterations, node / triangle-pair / instance steps, refills."""
         def p(x):
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
-        out = np.zeros(6, np.uint64)
+        out = np.zeros(10, np.uint64)
         own = None
         if hits is None and vis is None:
             own = DeviceBuffer(max(int(count), 1) * HIT_DTYPE.itemsize)
@@ -542,14 +542,16 @@ class Scene:
                 vis = own
         check(lib.rq_trace_count_stats(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits),
                                        p(vis), _ptr(out)))
-        return dict(zip(["bytes", "iterations", "node_steps", "tri_steps", "inst_steps", "refills"],
+        return dict(zip(["bytes", "iterations", "node_steps", "tri_steps", "inst_steps", "refills", "live_lanes",
+                         "want_node", "want_tri", "want_inst"],
                         (int(x) for x in out)))
 
     def verify_fp64(self, rays, hits, count, stream=None):
-        out = np.zeros(4, np.float64)
+        out = np.zeros(8, np.float64)
         check(lib.rq_verify_fp64(self._h, rays.ptr, hits.ptr, int(count), _ptr(out), stream.ptr if stream else None))
         return dict(max_t_rel=float(out[0]), max_bary_abs=float(out[1]), checked=int(out[2]),
-                    max_t_scaled=float(out[3]))
+                    max_t_scaled=float(out[3]), over_1e5=int(out[4]), max_ulps_conditioned=float(out[5]),
+                    conditioned=int(out[6]))
 
     def destroy(self):
         if self._h:

@@ -102,3 +102,22 @@ def test_fp64_verification(scene_c0, workload_c0):
     assert v["checked"] == hit.sum()
     assert abs(v["max_t_rel"] - rel.max()) <= 1e-12 + 1e-9 * rel.max()
     assert v["max_t_rel"] < 1e-5
+    assert v["over_1e5"] == int((rel > 1e-5).sum())
+    assert 0 < v["conditioned"] <= v["checked"]
+    assert v["max_ulps_conditioned"] < 16
+
+
+def test_fp64_verification_gi_conditioned(scene_c0, workload_c0):
+    """Incoherent GI hits: away from grazing and slivers, the f32 hit distance
+    is within a few f32 roundings of the numbers it is computed from
+    (the f32 world-to-object transform of a ray far from a small instance
+    cancels digits, so a relative bound on t alone does not hold)."""
+    _, _, _, _, gis = workload_c0
+    g = gis[0]
+    dr = rq.DeviceBuffer.from_array(g)
+    dh = rq.DeviceBuffer(len(g) * 48)
+    scene_c0.trace("cw8", dr, len(g), rq.CLOSEST, hits=dh)
+    v = scene_c0.verify_fp64(dr, dh, len(g))
+    print(f"fp64 gi: {v}", flush=True)
+    assert v["conditioned"] > 0.5 * v["checked"]
+    assert v["max_ulps_conditioned"] < 16

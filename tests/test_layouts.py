@@ -150,3 +150,68 @@ def test_cw8_top_level_reaches_every_instance_once():
                 assert (lo <= blo).all() and (hi >= bhi).all()
             stack.append(c)
     assert (seen == 1).all()
+
+
+def far_tiny_scene():
+    """Millimetre triangles 5 km from the origin (object space), and
+    instances of them placed across binade boundaries in world space."""
+    g = np.random.default_rng(11)
+    centres = 5000.0 + g.uniform(-0.5, 0.5, (400, 1, 3))
+    pts = (centres + g.normal(scale=1e-3, size=(400, 3, 3))).astype(np.float32).reshape(-1, 3)
+    meshes = [(pts, np.arange(len(pts), dtype=np.uint32), 0)]
+    inst = []
+    for k in range(48):
+        m = np.eye(4)
+        m[:3, 3] = [-5000.0 + 2.0 ** (k % 12) - 1.0, -5000.0 - 0.3 * k, -5000.0 + 1e-3 * k]
+        inst.append(dict(model=m, mesh=0))
+    return rq.Scene.build(meshes, inst)
+
+
+@pytest.mark.parametrize("which", ["lognormal", "far_tiny"])
+def test_cw8_codes_decode_exactly(which):
+    """Every code of every CW8 node (BLAS and top level) decodes to an f32
+    with no rounding, so the kernel's ray-relative evaluation of origin +
+    2^e * q and the builder's enclosure check see the same plane."""
+    sc = rq.Scene.generate(1, 5, 60, 20000) if which == "lognormal" else far_tiny_scene()
+    checked = 0
+    for part in (2, 1):
+        nodes = sc.layout_image("cw8", part)
+        w = nodes.reshape(-1, 24)
+        if len(w) == 0:
+            continue
+        origin = w[:, :3].view(np.float32).astype(np.float64)
+        q = np.arange(256, dtype=np.float64)
+        for a in range(3):
+            e = ((w[:, 3] >> (8 * a)) & 0xFF).astype(np.int64) - 127
+            assert (e >= -126).all()
+            dec = origin[:, a:a + 1] + np.ldexp(1.0, e)[:, None] * q[None, :]  # exact in f64
+            assert np.array_equal(dec.astype(np.float32).astype(np.float64), dec)
+            checked += dec.size
+    assert checked > 0
+
+
+def test_plain_triangle_records_hold_vertex_and_edges():
+    """Triangle record words 0-8 are vertex a and the f32 edges b - a, c - a of
+    the positions the record's index side array names."""
+    sc = rq.Scene.generate(0, 1)
+    words = sc.words
+    tris = sc.layout_image("plain", 3).view(np.float32).reshape(-1, 12)
+    ids = sc.layout_image("plain", 4).reshape(-1, 4)
+    inst = sc.layout_image("plain", 0).reshape(-1, 32)
+    bases = sorted(set(int(x) for x in inst[:, 20]) | {len(tris)})
+    checked = 0
+    for i in range(0, len(inst), 97):
+        rec = words[16 + 32 * i:16 + 32 * (i + 1)]
+        pos = words[int(rec[20]):].view(np.float32)
+        base = int(inst[i, 20])
+        end = bases[bases.index(base) + 1]  # the mesh's triangles end where the next mesh's begin
+        for k in range(base, min(base + 64, end)):
+            ia, ib, ic = (int(x) for x in ids[k, :3])
+            a = pos[3 * ia:3 * ia + 3]
+            b = pos[3 * ib:3 * ib + 3]
+            c = pos[3 * ic:3 * ic + 3]
+            assert np.array_equal(tris[k, 0:3].view(np.uint32), a.view(np.uint32))
+            assert np.array_equal(tris[k, 3:6].view(np.uint32), (b - a).view(np.uint32))
+            assert np.array_equal(tris[k, 6:9].view(np.uint32), (c - a).view(np.uint32))
+            checked += 1
+    assert checked > 0
