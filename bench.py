@@ -39,18 +39,46 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+PROFILES = {"c1": "trace_kernel_c1_1080p_psgi.md", "c2": "trace_kernel_c2_2160p_ps2gi.md"}
+
+
+def profiled_traffic(config, layout, launch):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration) for this workload, layout and launch, from the committed profile summary; None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", PROFILES.get(config, ""))
+    if not os.path.isfile(path):
+        return None
+    pat = re.compile(r"^traffic (\w+) (\w+): read (\d+) B, written (\d+) B per launch")
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line.strip().lstrip("- "))
+            if m and m.group(1) == layout and m.group(2) == launch:
+                return int(m.group(3)) + int(m.group(4)), os.path.relpath(path, ROOT)
+    return None
+
+
 class Frame:
     """Device buffers of one frame and the launches of one step."""
 
-    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream, reorder=False):
+    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream, reorder=False, schedule="fused"):
         self.rq, self.scene, self.w, self.h, self.n_gi, self.seed = rq, scene, w, h, n_gi, seed
         # secondary launches sorted by (octant, origin Morton code) before tracing
         self.sec = rq.REORDER if reorder else 0
+        # fused: shadow and GI rays as ranges of one persistent launch (they do not depend on each
+        # other). pipelined: one launch per step carries the next frame's primary rays and this
+        # frame's shadow and GI rays (software pipelining over a double-buffered primary stage).
+        self.schedule = "separate" if reorder else schedule
+        self.fused = self.schedule == "fused"
+        self.parity = 0
         n = w * h
         self.n = n
         self.inv = rq.camera(kind, seed, w, h)
         self.prim = rq.DeviceBuffer(n * 32)
         self.hits = rq.DeviceBuffer(n * 48)
+        # the second primary stage of the pipelined schedule
+        self.prim2 = rq.DeviceBuffer(n * 32) if self.schedule == "pipelined" else None
+        self.hits2 = rq.DeviceBuffer(n * 48) if self.schedule == "pipelined" else None
         self.shadow = rq.DeviceBuffer(n * 32)
         self.gi = rq.DeviceBuffer(max(n_gi, 1) * n * 32)
         self.gi_hits = rq.DeviceBuffer(max(n_gi, 1) * n * 48)
@@ -59,9 +87,26 @@ class Frame:
         self.stream = stream
         self.counts_host = None
 
+    def stage(self, parity):
+        return (self.prim, self.hits) if parity == 0 else (self.prim2, self.hits2)
+
     def step(self, layout, events=None):
         rq, s = self.rq, self.stream
         rec = (lambda name: events.mark(name)) if events else (lambda name: None)
+        if self.schedule == "pipelined":
+            # this frame's primary hits are in stage `parity` (traced by the previous step)
+            cur_p, cur_h = self.stage(self.parity)
+            nxt_p, nxt_h = self.stage(1 - self.parity)
+            rq.gen_primary(self.inv, self.w, self.h, nxt_p, stream=s)
+            rec("raygen")
+            rq.gen_secondary(cur_p, cur_h, self.n, SUN, self.seed, self.n_gi, self.shadow,
+                             self.gi if self.n_gi else None, self.counts, stream=s)
+            rec("secondary_gen")
+            ranges = [(nxt_p, self.n, rq.CLOSEST, nxt_h, None)] + self.secondary_ranges()
+            self.scene.trace_ranges(layout, ranges, stream=s)
+            rec("frame")
+            self.parity = 1 - self.parity
+            return
         rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=s)
         rec("raygen")
         self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=s)
@@ -70,6 +115,10 @@ class Frame:
                          self.gi if self.n_gi else None, self.counts, stream=s)
         rec("secondary_gen")
         c = self.counts_host
+        if self.fused:
+            self.scene.trace_ranges(layout, self.secondary_ranges(), stream=s)
+            rec("secondary")
+            return
         self.scene.trace(layout, self.shadow, int(c[0]), rq.ANY | self.sec, vis=self.vis, stream=s)
         rec("shadow")
         for b in range(self.n_gi):
@@ -78,9 +127,43 @@ class Frame:
             self.scene.trace(layout, ptr, int(c[1 + b]), rq.CLOSEST | self.sec, hits=hp, stream=s)
             rec(f"gi{b}")
 
-    def prime(self, layout):
-        """One untimed pass that fixes the secondary ray counts (deterministic)."""
+    def secondary_ranges(self):
+        rq, c = self.rq, self.counts_host
+        out = [(self.shadow, int(c[0]), rq.ANY, None, self.vis)]
+        for b in range(self.n_gi):
+            out.append((self.gi.ptr.value + b * self.n * 32, int(c[1 + b]), rq.CLOSEST,
+                        self.gi_hits.ptr.value + b * self.n * 48, None))
+        return out
+
+    def tuning_ranges(self):
+        """The ranges of this schedule's largest launch (for the autotuner), or None."""
         rq = self.rq
+        if self.schedule == "pipelined":
+            return [(self.prim, self.n, rq.CLOSEST, self.hits2, None)] + self.secondary_ranges()
+        if self.schedule == "fused":
+            return self.secondary_ranges()
+        return None
+
+    def launch_parts(self, name):
+        """(rays pointer, count, mode) of each pass a timed launch traces."""
+        rq, c = self.rq, self.counts_host
+        shadow = [(self.shadow.ptr.value, int(c[0]), rq.ANY | self.sec)]
+        gi = [(self.gi.ptr.value + b * self.n * 32, int(c[1 + b]), rq.CLOSEST | self.sec) for b in range(self.n_gi)]
+        if name == "primary":
+            return [(self.prim.ptr.value, self.n, rq.CLOSEST)]
+        if name == "shadow":
+            return shadow
+        if name == "secondary":
+            return shadow + gi
+        if name == "frame":
+            return [(self.prim.ptr.value, self.n, rq.CLOSEST)] + shadow + gi
+        return [gi[int(name[2:])]]
+
+    def prime(self, layout):
+        """One untimed pass that fixes the secondary ray counts (deterministic) and, for the pipelined
+        schedule, leaves the first frame's primary hits in stage 0."""
+        rq = self.rq
+        self.parity = 0
         rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=self.stream)
         self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=self.stream)
         rq.gen_secondary(self.prim, self.hits, self.n, SUN, self.seed, self.n_gi, self.shadow,
@@ -273,6 +356,10 @@ def main():
     ap.add_argument("--serving-sweep", action="store_true", help="sweep Poisson rates 50..2000 jobs/s")
     ap.add_argument("--reorder", default="none", choices=["none", "secondary"],
                     help="sort shadow and GI launches by direction octant and origin Morton code")
+    ap.add_argument("--secondary", default="pipelined", choices=["fused", "separate", "pipelined"],
+                    help="shadow and GI rays as ranges of one launch (fused), one launch per pass (separate), or "
+                         "one launch per step with the next frame's primary rays (pipelined)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -301,45 +388,65 @@ def main():
     log(f"uploaded {layouts} in {time.perf_counter() - t0:.1f}s; bytes " +
         ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
     stream = rq.Stream()
-    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary")
+    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary",
+                  schedule=args.secondary)
     rays = frame.prime(args.layout)
     log(f"rays per step: {rays} (primary {frame.n}, secondary {list(frame.counts_host)})")
 
-    results = {}
+    results, tuned = {}, {}
     for layout in layouts:
         frame.prime(layout)
+        tr = frame.tuning_ranges()
+        if tr is not None and not args.no_tune:
+            from renderlab_amd import autotune
+            t = autotune.tune_ranges(scene, layout, tr, stream=stream)
+            tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu")}
+            log(f"{layout}: autotuned {t}")
+            frame.prime(layout)
         ms, wall, per = time_layout(rq, frame, layout, args.steps, args.warmup)
         results[layout] = (ms, wall, per)
         log(f"{layout}: {ms:.3f} ms/step, {rays / ms / 1e3:.1f} Mrays/s, per launch {per}")
     ms, wall, per = results[args.layout]
     value = rays / (ms * 1e-3) / 1e6
 
-    # Dominant trace launch and its algorithmic bytes (counted on the same rays).
-    trace_names = [k for k in per if k in ("primary", "shadow") or k.startswith("gi")]
+    # Dominant trace launch and its algorithmic bytes (counted on the same rays, pass by pass: a ray's
+    # traversal and the bytes it moves do not depend on which launch carries it).
+    trace_names = [k for k in per if k in ("primary", "shadow", "secondary", "frame") or k.startswith("gi")]
     dom = max(trace_names, key=lambda k: per[k])
+    parts = frame.launch_parts(dom)
+    dom_rays = sum(n for _, n, _ in parts)
+
+    def counted(layout):
+        tot = None
+        for ptr, n, mode in parts:
+            st1 = scene.count_stats(layout, ptr, n, mode)
+            tot = st1 if tot is None else {k: tot[k] + st1[k] for k in tot}
+        return tot
+
     counts = frame.counts_host
-    dom_rays = frame.n if dom == "primary" else int(counts[0]) if dom == "shadow" else int(counts[1 + int(dom[2:])])
-    dom_mode = rq.ANY if dom == "shadow" else rq.CLOSEST
-    dom_ptr = {"primary": frame.prim.ptr.value, "shadow": frame.shadow.ptr.value}.get(
-        dom, frame.gi.ptr.value + (int(dom[2:]) if dom.startswith("gi") else 0) * frame.n * 32)
-    stats = scene.count_stats(args.layout, dom_ptr, dom_rays, dom_mode | (frame.sec if dom != "primary" else 0))
+    stats = counted(args.layout)
     nbytes = stats["bytes"]
     lanes = 64 * stats["iterations"]
     log(f"dominant launch {dom}: {dom_rays} rays, steps/ray node {stats['node_steps'] / dom_rays:.2f} "
         f"tri-pair {stats['tri_steps'] / dom_rays:.2f} inst {stats['inst_steps'] / dom_rays:.2f}; lane use "
-        f"{(stats['node_steps'] + stats['tri_steps'] + stats['inst_steps']) / max(lanes, 1):.3f}; "
+        f"{(stats['node_steps'] + stats['tri_steps'] + stats['inst_steps']) / max(lanes, 1):.3f} (pass by pass); "
         f"iterations/wave-ray {stats['iterations'] * 64 / dom_rays:.1f}; per iteration: live lanes "
         f"{stats['live_lanes'] / max(stats['iterations'], 1):.1f}, wanting node {stats['want_node'] / max(stats['iterations'], 1):.1f} "
         f"tri {stats['want_tri'] / max(stats['iterations'], 1):.1f} inst {stats['want_inst'] / max(stats['iterations'], 1):.1f}")
     plain_stats = None
     if "plain" in results and args.layout != "plain":
-        plain_stats = scene.count_stats("plain", dom_ptr, dom_rays, dom_mode)
+        plain_stats = counted("plain")
         log(f"plain on the same launch: steps/ray node {plain_stats['node_steps'] / dom_rays:.2f} tri-pair "
             f"{plain_stats['tri_steps'] / dom_rays:.2f} inst {plain_stats['inst_steps'] / dom_rays:.2f}; bytes/ray "
             f"{plain_stats['bytes'] / dom_rays:.0f}; iterations/wave-ray {plain_stats['iterations'] * 64 / dom_rays:.1f}")
     achieved = nbytes / (per[dom] * 1e-3)
+    kind_of = "gi" if dom.startswith("gi") else dom  # profile labels: primary, shadow, gi, secondary
+    measured = profiled_traffic(args.config, args.layout, kind_of)
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK, 4), "traffic": None, "kernel": f"trace {dom}",
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": measured[0] if measured else None,
+                "traffic_source": measured[1] if measured else None,
+                "kernel": {"secondary": "trace shadow+GI (ranges of one launch)",
+                           "frame": "trace next primary+shadow+GI (ranges of one launch)"}.get(dom, f"trace {dom}"),
                 "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4),
                 "lane_use": round((stats["node_steps"] + stats["tri_steps"] + stats["inst_steps"]) /
                                   max(64 * stats["iterations"], 1), 3),
@@ -373,7 +480,9 @@ def main():
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded scene and camera)",
         "config": {"workload": desc, "layout": args.layout, "rays_per_step": rays, "triangles": st["triangles"],
-                   "instances": st["instances"], "reorder": args.reorder, "parallelism": "rays across one GPU"},
+                   "instances": st["instances"], "reorder": args.reorder, "secondary": args.secondary,
+                   "tuning": tuned.get(args.layout),
+                   "parallelism": "rays across one GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if "plain" in results and args.layout != "plain":

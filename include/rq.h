@@ -196,6 +196,23 @@ typedef struct rq_batch {
 rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
                            const rq_batch* batches, uint32_t batch_count, void* stream);
 
+/* Up to RQ_MAX_RANGES ray ranges of ONE scene, each with its own mode and
+   outputs, in ONE persistent launch: the passes of a frame that do not
+   depend on each other (sun shadow and GI bounces) share one pool of rays,
+   so one launch's drain overlaps the other's work. Uses the closest-hit
+   tuning of `layout`. No counterpart in the reference (its passes dispatch
+   separately); results equal one rq_trace call per range. */
+#define RQ_MAX_RANGES 4u
+typedef struct rq_range {
+    const rq_ray* rays;  /* device */
+    uint32_t count;
+    uint32_t mode;       /* rq_mode (RQ_REORDER is not accepted here) */
+    rq_hit* hits;        /* device, may be NULL */
+    uint32_t* vis_bits;  /* device, may be NULL; cleared first */
+} rq_range;
+rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq_range* ranges, uint32_t range_count,
+                          void* stream);
+
 /* The same traversal with a tally of the global bytes it moves (rays,
    records, nodes, triangles, stack spills, hit records): the algorithmic
    bytes of one launch for the roofline. Synchronous; not for timing. */

@@ -26,14 +26,15 @@ def size_class(triangles):
     return int(np.clip(np.floor(np.log10(max(triangles, 1)) * 2) / 2, 0, 9) * 10)
 
 
-def _time(fn, reps=2):
+def _time(fn, reps=2, stream=None):
+    """Mean time of fn() in ms; fn must launch on `stream` (None: the default stream), where the events go."""
     start, end = rq.Event(), rq.Event()
     fn()  # warm
     rq.synchronize()
-    start.record()
+    start.record(stream)
     for _ in range(reps):
         fn()
-    end.record()
+    end.record(stream)
     end.synchronize()
     return start.elapsed_ms(end) / reps
 
@@ -52,6 +53,24 @@ def tune_trace(scene, layout, rays_dev, count, mode, hits=None, vis=None, quick=
     results.sort()
     ms, block, stack, refill, grid = results[0]
     rq.set_tuning(layout, mode, block, stack, refill, grid)
+    return dict(ms=ms, block=block, stack=stack, refill=refill, grid_per_cu=grid, candidates=len(results))
+
+
+def tune_ranges(scene, layout, ranges, quick=False, stream=None):
+    """Sweep the one-scene, several-range launch (it runs with the closest-hit tuning of `layout`) on
+    these ranges; installs and returns the best."""
+    results = []
+    grids = [0] if quick else GRIDS
+    refills = [1, 16, 64] if quick else REFILLS
+    for block, stack in CANDIDATES:
+        for refill in refills:
+            for grid in grids:
+                rq.set_tuning(layout, 0, block, stack, refill, grid)
+                ms = _time(lambda: scene.trace_ranges(layout, ranges, stream=stream), stream=stream)
+                results.append((ms, block, stack, refill, grid))
+    results.sort()
+    ms, block, stack, refill, grid = results[0]
+    rq.set_tuning(layout, 0, block, stack, refill, grid)
     return dict(ms=ms, block=block, stack=stack, refill=refill, grid_per_cu=grid, candidates=len(results))
 
 

@@ -151,6 +151,13 @@ struct TraceArgs {
     const BatchDev* batches;    // BATCHED: the batch table
     const uint32_t* prefix;     // BATCHED: first global ray of each batch
     const uint32_t* order;      // when set, the i-th ray taken is ray order[i]
+    // AM 2 without BATCHED: up to 4 ranges of one scene; range j holds the
+    // global rays [rstart[j], rstart[j + 1]) (unused entries: the total).
+    const rq_ray* rrays[4];
+    rq_hit* rhits[4];
+    uint32_t* rvis[4];
+    uint32_t rstart[5];
+    uint32_t rany;              // bit j: range j is an any-hit range
     uint32_t n_batches;
     uint32_t count, refill_min, pad;
 };
@@ -177,9 +184,11 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // COUNT: the same traversal, also tallying the bytes every global load and
 // store moves (rays, records, nodes, triangles, spills, hit records); used
 // once, untimed, to give the roofline its algorithmic bytes per ray.
-// AM: 0 closest hit, 1 any hit, 2 per batch (BATCHED launches mix modes).
+// AM: 0 closest hit, 1 any hit, 2 per batch or range (mixed modes).
 // BATCHED: rays come from a table of batches over several scenes; a lane
 // takes its batch's scene, mode and output buffers when it fetches a ray.
+// AM 2 without BATCHED: up to 4 ranges of one scene described in the
+// arguments; a lane takes its range's mode and outputs the same way.
 //
 // Loop structure: every iteration each lane takes exactly one small step --
 // visit one node, enter one instance, or test the next two triangles of its
@@ -337,6 +346,16 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     ray = r;
                     const rq_ray* src = a.rays + r;
                     out_idx = r;
+                    if (AM == 2 && !BATCHED) {
+                        const uint32_t j = (r >= a.rstart[1] ? 1u : 0u) + (r >= a.rstart[2] ? 1u : 0u) +
+                                           (r >= a.rstart[3] ? 1u : 0u);
+                        const rq_ray* rb = j == 0 ? a.rrays[0] : j == 1 ? a.rrays[1] : j == 2 ? a.rrays[2] : a.rrays[3];
+                        out_hits = j == 0 ? a.rhits[0] : j == 1 ? a.rhits[1] : j == 2 ? a.rhits[2] : a.rhits[3];
+                        out_vis = j == 0 ? a.rvis[0] : j == 1 ? a.rvis[1] : j == 2 ? a.rvis[2] : a.rvis[3];
+                        out_idx = r - (j == 0 ? a.rstart[0] : j == 1 ? a.rstart[1] : j == 2 ? a.rstart[2] : a.rstart[3]);
+                        any_hit = ((a.rany >> j) & 1u) != 0u;
+                        src = rb + out_idx;
+                    }
                     if (BATCHED) {
                         // batch b holds global rays [prefix[b], prefix[b+1])
                         uint32_t lo_b = 0, hi_b = a.n_batches;
@@ -1102,7 +1121,7 @@ void default_tuning(rq_tuning& t, int layout, int any) {
     t.persistent = 1;
     t.waves_per_eu = 0;
     t.grid_per_cu = 0;
-    t.refill = any ? 16u : 16u;  // refill threshold (lanes)
+    t.refill = 32u;  // refill threshold (lanes): the autotuner's pick on every measured workload
     (void)layout;
 }
 
@@ -1144,6 +1163,8 @@ struct KernelEntry {
     void* ptr[3][2];
     Launch bfn[3];  // batched, mode per batch
     void* bptr[3];
+    Launch rfn[3];  // ranges of one scene, mode per range
+    void* rptr[3];
 };
 
 template <int B, int S>
@@ -1169,6 +1190,12 @@ KernelEntry make_entry() {
     e.bptr[0] = kernel_ptr<dev::REF, B, S, 2, false, true>();
     e.bptr[1] = kernel_ptr<dev::PLAIN, B, S, 2, false, true>();
     e.bptr[2] = kernel_ptr<dev::CW8, B, S, 2, false, true>();
+    e.rfn[0] = &launch_one<dev::REF, B, S, 2>;
+    e.rfn[1] = &launch_one<dev::PLAIN, B, S, 2>;
+    e.rfn[2] = &launch_one<dev::CW8, B, S, 2>;
+    e.rptr[0] = kernel_ptr<dev::REF, B, S, 2>();
+    e.rptr[1] = kernel_ptr<dev::PLAIN, B, S, 2>();
+    e.rptr[2] = kernel_ptr<dev::CW8, B, S, 2>();
     return e;
 }
 
@@ -1525,6 +1552,82 @@ extern "C" rq_status rq_trace_indirect(rq_scene* scene, rq_layout layout, const 
         return RQ_ERR_INVALID_INPUT;
     }
     return launch_trace(scene, layout, rays, max_count, mode, hits, vis_bits, stream, nullptr, count_dev);
+}
+
+extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq_range* ranges,
+                                     uint32_t range_count, void* stream) {
+    if (!scene || !ranges || (int)layout < 0 || (int)layout > 2 || range_count == 0 ||
+        range_count > RQ_MAX_RANGES) {
+        set_error("rq_trace_ranges: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (!scene->dev[layout].ready) {
+        set_error("rq_trace_ranges: layout not uploaded (rq_scene_upload)");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    dev::TraceArgs a;
+    std::memset(&a, 0, sizeof a);
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < range_count; ++j) {
+        const rq_range& q = ranges[j];
+        if (q.mode > 2 || (q.count && !q.rays)) {
+            set_error("rq_trace_ranges: range " + std::to_string(j) + " is malformed");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        a.rrays[j] = q.rays;
+        a.rhits[j] = q.hits;
+        a.rvis[j] = q.vis_bits;
+        a.rstart[j] = (uint32_t)total;
+        if (q.mode != 0) a.rany |= 1u << j;
+        total += q.count;
+    }
+    if (total >= 0xffffffffull) {
+        set_error("rq_trace_ranges: more than 2^32 rays");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    for (uint32_t j = range_count; j < 5; ++j) a.rstart[j] = (uint32_t)total;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(g_ctx.mu);
+    rq_status st = ensure_ctx();
+    if (st != RQ_OK) return st;
+    if (scene->device != g_ctx.device) {
+        set_error("rq_trace_ranges: scene lives on another device");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    for (uint32_t j = 0; j < range_count; ++j) {
+        if (!ranges[j].vis_bits || !ranges[j].count) continue;
+        uint64_t words = (ranges[j].count + 31u) / 32u;
+        hipLaunchKernelGGL(dev::fill_u32, dim3((unsigned)std::min<uint64_t>((words + 255) / 256, 4096)), dim3(256), 0,
+                           s, ranges[j].vis_bits, 0u, words);
+    }
+    if (total == 0) return RQ_OK;
+    const rq_tuning& t = g_ctx.tuning[layout][0];
+    const KernelEntry* k = find_kernel(t.block, t.stack);
+    if (!k) {
+        set_error("rq_trace_ranges: no kernel for the tuning");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    int per_cu = (int)t.grid_per_cu;
+    if (per_cu == 0) {
+        int blocks = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k->rptr[layout], (int)k->block, 0),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        per_cu = std::max(blocks, 1);
+    }
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)g_ctx.cus * per_cu,
+                                                             (total + k->block - 1) / k->block));
+    StreamRes* res = nullptr;
+    if ((st = res_for(s, &res)) != RQ_OK) return st;
+    if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
+    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
+    a.sv = scene_view(scene, layout);
+    a.count = (uint32_t)total;
+    a.counter = res->counter;
+    a.spill = res->spill;
+    a.refill_min = t.refill ? t.refill : 1u;
+    k->rfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
+    HIP_TRY(hipGetLastError(), "trace ranges launch");
+    return RQ_OK;
 }
 
 extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,

@@ -107,6 +107,7 @@ _sig("rq_scene_layout_image", _i, _P, _i, _u32, ctypes.POINTER(_P), ctypes.POINT
 _sig("rq_trace", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_trace_indirect", _i, _P, _i, _P, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_trace_batches", _i, _P, _u32, _i, _P, _u32, _P)
+_sig("rq_trace_ranges", _i, _P, _i, _P, _u32, _P)
 _sig("rq_get_batch_tuning", _i, _i, _P)
 _sig("rq_set_batch_tuning", _i, _i, _P)
 _sig("rq_trace_count_bytes", _i, _P, _i, _P, _u32, _u32, _P, _P, ctypes.POINTER(_u64))
@@ -372,6 +373,14 @@ class Batch(ctypes.Structure):
                 ("vis_bits", _P)]
 
 
+class Range(ctypes.Structure):
+    """rq_range: one ray range of a one-scene, several-pass launch (device pointers)."""
+    _fields_ = [("rays", _P), ("count", _u32), ("mode", _u32), ("hits", _P), ("vis_bits", _P)]
+
+
+MAX_RANGES = 4
+
+
 def _dptr(x):
     if x is None:
         return None
@@ -510,6 +519,16 @@ class Scene:
         check(lib.rq_trace_indirect(self._h, LAYOUTS.get(layout, layout), _dptr(rays), _dptr(count_dev),
                                     int(max_count), int(mode), _dptr(hits), _dptr(vis),
                                     stream.ptr if stream else None))
+
+    def trace_ranges(self, layout, ranges, stream=None):
+        """Several passes over this scene in one persistent launch.
+        ranges: [(rays, count, mode, hits, vis)], at most MAX_RANGES; buffers or device pointers."""
+        tab = (Range * len(ranges))()
+        for k, (rays, count, mode, hits, vis) in enumerate(ranges):
+            tab[k].rays, tab[k].count, tab[k].mode = _dptr(rays), int(count), int(mode)
+            tab[k].hits, tab[k].vis_bits = _dptr(hits), _dptr(vis)
+        check(lib.rq_trace_ranges(self._h, LAYOUTS.get(layout, layout), tab, len(ranges),
+                                  stream.ptr if stream else None))
 
     def count_bytes(self, layout, rays, count, mode, hits=None, vis=None):
         """Global bytes one traversal launch over these rays moves (synchronous)."""
