@@ -128,11 +128,13 @@ class Frame:
             rec(f"gi{b}")
 
     def secondary_ranges(self):
+        """GI ranges first, shadow last: the launch's pool then drains on the cheapest (any-hit) rays."""
         rq, c = self.rq, self.counts_host
-        out = [(self.shadow, int(c[0]), rq.ANY, None, self.vis)]
+        out = []
         for b in range(self.n_gi):
             out.append((self.gi.ptr.value + b * self.n * 32, int(c[1 + b]), rq.CLOSEST,
                         self.gi_hits.ptr.value + b * self.n * 48, None))
+        out.append((self.shadow, int(c[0]), rq.ANY, None, self.vis))
         return out
 
     def tuning_ranges(self):
@@ -440,7 +442,8 @@ def main():
             f"{plain_stats['tri_steps'] / dom_rays:.2f} inst {plain_stats['inst_steps'] / dom_rays:.2f}; bytes/ray "
             f"{plain_stats['bytes'] / dom_rays:.0f}; iterations/wave-ray {plain_stats['iterations'] * 64 / dom_rays:.1f}")
     achieved = nbytes / (per[dom] * 1e-3)
-    kind_of = "gi" if dom.startswith("gi") else dom  # profile labels: primary, shadow, gi, secondary
+    # profile labels: primary, shadow, gi, ranges (the multi-range launch of the fused and pipelined schedules)
+    kind_of = "gi" if dom.startswith("gi") else ("ranges" if dom in ("secondary", "frame") else dom)
     measured = profiled_traffic(args.config, args.layout, kind_of)
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK, 4), "traffic": measured[0] if measured else None,
@@ -470,9 +473,11 @@ def main():
         if n_gi:
             sets.append((0, frame.gi.download(rq.RAY_DTYPE, int(counts[1]))))
         mrays, threads, sampled = cpu_baseline(scene.words, sets)
+        whole = sampled == sum(len(r) for _, r in sets)
+        what = "every ray of one frame" if whole else "the first rays of each pass of one frame"
         cpu = {"value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"first rays of each set of the same frame, {sampled} rays, oracle gi_traverse on "
-                         f"{threads} threads"}
+               "sample": f"{what} (primary, shadow, GI: {sampled} rays), the oracle's gi_traverse (C restatement "
+                         f"of the reference traversal) on {threads} threads"}
 
     line = {
         "metric": "Mrays/s (primary+shadow+GI) on 1 MI355X, quantized vs plain; p50/p99 job latency",
