@@ -23,6 +23,10 @@ constexpr uint32_t TOP_BIN_WORDS = 16;      // src/instance_bvh.rs:28 (binary no
 constexpr uint32_t TOP_STACK = 48;          // src/instance_bvh.rs:32
 constexpr uint32_t MIN_INSTANCES = 40;      // src/instance_bvh.rs:72
 constexpr uint32_t REFITS = 64;             // src/instance_bvh.rs:94
+// PLAIN/CW8 triangle records: vertex a and the f32 edges b - a, c - a (36 B);
+// the side record read once per hit: position indices, BVH slot, primitive id.
+constexpr uint32_t TRI_FLOATS = 9;
+constexpr uint32_t TRI_ID_WORDS = 6;
 
 inline float bits_f(uint32_t w) { float f; std::memcpy(&f, &w, 4); return f; }
 inline uint32_t f_bits(float f) { uint32_t w; std::memcpy(&w, &f, 4); return w; }
@@ -123,8 +127,8 @@ struct HostLayout {
     std::vector<uint32_t> inst;    // 32 words per instance
     std::vector<uint32_t> top;     // top-level wide nodes (32 words)
     std::vector<uint32_t> nodes;   // BLAS nodes (plain: 32 words, cw8: 20 words)
-    std::vector<float> tris;       // 12 floats per triangle (a, b, c, pad)
-    std::vector<uint32_t> tri_ids; // 4 words per triangle: ia, ib, ic, primitive
+    std::vector<float> tris;       // TRI_FLOATS per triangle: a, b - a, c - a
+    std::vector<uint32_t> tri_ids; // TRI_ID_WORDS per triangle: ia, ib, ic, slot, primitive, 0
     uint32_t top_root = 0;
     uint32_t top_on = 0;
     uint32_t instance_count = 0;

@@ -7,9 +7,10 @@
 //   each box is the unfused decode origin + step*q of the reference record
 //   (shaders/gi_rays_software.wgsl:184-186), so every slab test sees the same
 //   floats as the reference traversal. Triangles are gathered from the
-//   index/position arrays into 48-byte records (a, b, c, BVH-order slot,
-//   primitive id) so a leaf costs no dependent index load; the position
-//   indices move to a side array read once per hit.
+//   index/position arrays into 36-byte records (vertex a and the f32 edges
+//   b - a, c - a) so a leaf costs no dependent index load; the position
+//   indices, BVH-order slot and primitive id move to a side array read once
+//   per hit.
 // CW8_Q8: 8-wide nodes with 8-bit child boxes on a power-of-two grid (after
 //   Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
 //   Through Compressed Wide BVHs", HPG 2017), collapsed from the 4-wide
@@ -101,7 +102,7 @@ void decode_unfused(const uint32_t* w, int slot, float lo[3], float hi[3]) {
 void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint32_t slot_first,
                       uint32_t n, std::vector<float>& tris, std::vector<uint32_t>& ids) {
     for (uint32_t k = slot_first; k < slot_first + n; ++k) {
-        // Record words 0-8: vertex a, then the edges b - a and c - a rounded to
+        // Record (TRI_FLOATS): vertex a, then the edges b - a and c - a rounded to
         // f32 -- the same roundings the intersection test would make, done once.
         float p[3][3];
         for (int v = 0; v < 3; ++v) {
@@ -112,10 +113,10 @@ void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint
         for (int a = 0; a < 3; ++a) tris.push_back(p[0][a]);
         for (int e = 1; e < 3; ++e)
             for (int a = 0; a < 3; ++a) tris.push_back(p[e][a] - p[0][a]);
-        // Record words 9-11: BVH-order slot, primitive id, zero (read with c.z).
-        tris.push_back(bits_f(k));
-        tris.push_back(bits_f(m.prims ? words[m.prim_base + k] : 0u));
-        tris.push_back(0.0f);
+        // Side record (TRI_ID_WORDS), read once per hit: the three position
+        // indices above, the BVH-order slot, the primitive id, zero.
+        ids.push_back(k);
+        ids.push_back(m.prims ? words[m.prim_base + k] : 0u);
         ids.push_back(0u);
     }
 }
@@ -173,8 +174,8 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
         return false;
     }
     out.nodes.assign((size_t)node_total * TOP_WORDS, 0u);
-    out.tris.reserve((size_t)tri_total * 12);
-    out.tri_ids.reserve((size_t)tri_total * 4);
+    out.tris.reserve((size_t)tri_total * TRI_FLOATS);
+    out.tri_ids.reserve((size_t)tri_total * TRI_ID_WORDS);
     std::vector<uint32_t> roots(meshes.size());
     for (size_t mi = 0; mi < meshes.size(); ++mi) {
         const MeshMap& m = meshes[mi];

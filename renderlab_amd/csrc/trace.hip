@@ -125,9 +125,9 @@ struct SceneView {
     const uint32_t* words;  // REF: the packed words
     const uint32_t* inst;   // PLAIN/CW8 instance records (32 words)
     const uint32_t* top;    // top-level wide nodes
-    const uint32_t* nodes;  // PLAIN 32 words, CW8 20 words a node
-    const float4* tris;     // 3 float4 a triangle
-    const uint32_t* tri_ids;
+    const uint32_t* nodes;  // PLAIN 32 words, CW8 24 words a node
+    const float* tris;      // TRI_FLOATS a triangle: a, b - a, c - a
+    const uint32_t* tri_ids;  // TRI_ID_WORDS a triangle: ia, ib, ic, slot, primitive, 0
     uint32_t n_inst, top_on, top_root, pad;
 };
 
@@ -288,23 +288,24 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
                 r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
                 r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
-                const float4* tp = v.tris + 3ull * btri;
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                e1 = mk(t0.w, t1.x, t1.y);
-                e2 = mk(t1.z, t1.w, t2.x);
-                slot = fb(t2.y);
-                const uint32_t* ids = v.tri_ids + 4ull * btri;
-                tri0 = ids[0];
-                tri1 = ids[1];
-                tri2 = ids[2];
+                const float3* tp = reinterpret_cast<const float3*>(v.tris + (uint64_t)TRI_FLOATS * btri);
+                const float3 t1 = tp[1], t2 = tp[2];
+                e1 = mk(t1.x, t1.y, t1.z);
+                e2 = mk(t2.x, t2.y, t2.z);
+                const uint2* ids = reinterpret_cast<const uint2*>(v.tri_ids + (uint64_t)TRI_ID_WORDS * btri);
+                const uint2 i0 = ids[0], i1 = ids[1], i2 = ids[2];
+                tri0 = i0.x;
+                tri1 = i0.y;
+                tri2 = i1.x;
+                slot = i1.y;
                 if (!any_hit && (r[19] & 2u)) {
-                    tri0 = fb(t2.z);
+                    tri0 = i2.x;
                     tri1 = 0;
                     tri2 = 0;
                 }
             }
             n = hit_normal(r0, r1, r2, e1, e2);
-            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 48 + 12);
+            if (COUNT) nb += LAYOUT == REF ? (36 + 24 + 12 + 36 + (!any_hit ? 4 : 0)) : (36 + 24 + 24);
         }
         if (out_hits) {
             uint4* hp = reinterpret_cast<uint4*>(out_hits + out_idx);
@@ -461,14 +462,14 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     e2 = sub3(mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2])), va);
                     slot_id = tri_next + j;
                 } else {
-                    const float4* tp = v.tris + 3ull * (tri_next + j);
-                    float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    const float3* tp = reinterpret_cast<const float3*>(v.tris + (uint64_t)TRI_FLOATS * (tri_next + j));
+                    const float3 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                     va = mk(t0.x, t0.y, t0.z);
-                    e1 = mk(t0.w, t1.x, t1.y);
-                    e2 = mk(t1.z, t1.w, t2.x);
+                    e1 = mk(t1.x, t1.y, t1.z);
+                    e2 = mk(t2.x, t2.y, t2.z);
                     slot_id = tri_next + j;
                 }
-                if (COUNT) nb += 48;
+                if (COUNT) nb += LAYOUT == REF ? 48 : 4 * TRI_FLOATS;
                 float u, vv;
                 float t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
                 if (t >= 0.0f) {
@@ -669,11 +670,14 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 const uint32_t nz0 = pz ? q2.x : q3.z, nz1 = pz ? q2.y : q3.w, fz0 = pz ? q3.z : q2.x, fz1 = pz ? q3.w : q2.y;
                 const uint32_t tag[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
                 uint32_t hit = 0;
-                // Closest-hit rays visit the nearest hit child next: the minimum of
-                // (entry distance bits with the slot in the low 3 bits); entry
-                // distances are >= +-0, so clearing the sign orders them as integers.
+                // Closest-hit rays visit the two nearest hit children first, in order:
+                // the two smallest of (entry distance bits with the slot in the low 3
+                // bits) -- entry distances are >= +-0, so clearing the sign orders them
+                // as integers. The other hit children go below them in slot order.
+                // Any-hit rays end at their first hit and their visibility bit does
+                // not depend on the order: they push in slot order.
                 const bool order = AM == 0 || (AM == 2 && !any_hit);
-                uint32_t kmin = 0xffffffffu;
+                uint32_t kmin = 0xffffffffu, kmin2 = 0xffffffffu;
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
                     const int sh = 8 * (s & 3);
@@ -689,52 +693,39 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     const float far = fmn(fmn(fmn(tfx, tfy), tfz), lim);
                     const bool h = near <= far && ((valid >> s) & 1u);
                     if (h) hit |= 1u << s;
-                    if (order) kmin = min(kmin, h ? ((fb(near) & 0x7ffffff8u) | (uint32_t)s) : 0xffffffffu);
+                    if (order) {
+                        const uint32_t key = h ? ((fb(near) & 0x7ffffff8u) | (uint32_t)s) : 0xffffffffu;
+                        kmin2 = min(kmin2, max(kmin, key));
+                        kmin = min(kmin, key);
+                    }
                 }
-                uint32_t first = NONE;
-                if (order && hit) {
-                    const uint32_t m = kmin & 7u;
+                // tag[m] for a lane-varying slot m: a select tree on its bits
+                auto tag_of = [&](uint32_t m) {
                     const uint32_t a0 = (m & 1u) ? tag[1] : tag[0], a1 = (m & 1u) ? tag[3] : tag[2];
                     const uint32_t a2 = (m & 1u) ? tag[5] : tag[4], a3 = (m & 1u) ? tag[7] : tag[6];
                     const uint32_t b0 = (m & 2u) ? a1 : a0, b1 = (m & 2u) ? a3 : a2;
-                    first = (m & 4u) ? b1 : b0;
-                    hit &= ~(1u << m);
-                }
-                // Push far to near in octant order: slot k ^ oct is the k-th nearest
-                // guess. The XOR permutation is three conditional swap stages on
-                // registers (a lane-varying index into a register array would not be).
-                uint32_t tg[8] = {tag[0], tag[1], tag[2], tag[3], tag[4], tag[5], tag[6], tag[7]};
-                // Any-hit rays end at their first hit and their visibility bit does
-                // not depend on the visit order: they skip the ordering.
-                if (order && ri.x < 0.0f) {
-#pragma unroll
-                    for (int k = 0; k < 8; k += 2) { uint32_t x = tg[k]; tg[k] = tg[k + 1]; tg[k + 1] = x; }
-                    hit = ((hit & 0x55u) << 1) | ((hit >> 1) & 0x55u);
-                }
-                if (order && ri.y < 0.0f) {
-#pragma unroll
-                    for (int k = 0; k < 8; k += 4) {
-                        uint32_t x = tg[k]; tg[k] = tg[k + 2]; tg[k + 2] = x;
-                        x = tg[k + 1]; tg[k + 1] = tg[k + 3]; tg[k + 3] = x;
-                    }
-                    hit = ((hit & 0x33u) << 2) | ((hit >> 2) & 0x33u);
-                }
-                if (order && ri.z < 0.0f) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) { uint32_t x = tg[k]; tg[k] = tg[k + 4]; tg[k + 4] = x; }
-                    hit = ((hit & 0x0fu) << 4) | ((hit >> 4) & 0x0fu);
-                }
+                    return (m & 4u) ? b1 : b0;
+                };
                 if (order) {
-                    // the rest far to near in octant order, then the nearest
+                    uint32_t first = NONE, second = NONE;
+                    if (hit) {
+                        first = tag_of(kmin & 7u);
+                        hit &= ~(1u << (kmin & 7u));
+                    }
+                    if (hit) {
+                        second = tag_of(kmin2 & 7u);
+                        hit &= ~(1u << (kmin2 & 7u));
+                    }
                     room(8);
 #pragma unroll
-                    for (int k = 7; k >= 0; --k) push_if(tg[k], ((hit >> k) & 1u) && sp < cap);
+                    for (int k = 7; k >= 0; --k) push_if(tag[k], ((hit >> k) & 1u) && sp < cap);
+                    push_if(second, second != NONE && sp < cap);
                     if (first != NONE && sp < cap) nxt = first;
                 } else {
                     room(7);
 #pragma unroll
-                    for (int k = 7; k >= 1; --k) push_if(tg[k], ((hit >> k) & 1u) && sp < cap);
-                    if ((hit & 1u) && sp < cap) nxt = tg[0];
+                    for (int k = 7; k >= 1; --k) push_if(tag[k], ((hit >> k) & 1u) && sp < cap);
+                    if ((hit & 1u) && sp < cap) nxt = tag[0];
                 }
             }
             }
@@ -1372,8 +1363,8 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             HostLayout& h = l == 1 ? scene->plain : scene->cw8;
             if (h.inst.empty()) h.inst.assign(32, 0u);
             if (h.nodes.empty()) h.nodes.assign(32, 0u);
-            if (h.tris.empty()) h.tris.assign(12, 0.0f);
-            if (h.tri_ids.empty()) h.tri_ids.assign(4, 0u);
+            if (h.tris.empty()) h.tris.assign(TRI_FLOATS, 0.0f);
+            if (h.tri_ids.empty()) h.tri_ids.assign(TRI_ID_WORDS, 0u);
             d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.tris) + padded(h.tri_ids);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
@@ -1446,7 +1437,7 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
         v.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
         v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
-        v.tris = reinterpret_cast<const float4*>(b + d.off[3]);
+        v.tris = reinterpret_cast<const float*>(b + d.off[3]);
         v.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
         v.n_inst = h.instance_count;
         v.top_on = h.top_on;

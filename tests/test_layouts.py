@@ -191,12 +191,13 @@ def test_cw8_codes_decode_exactly(which):
 
 
 def test_plain_triangle_records_hold_vertex_and_edges():
-    """Triangle record words 0-8 are vertex a and the f32 edges b - a, c - a of
-    the positions the record's index side array names."""
+    """Triangle records are vertex a and the f32 edges b - a, c - a (9 floats) of the
+    positions the side record (ia, ib, ic, slot, primitive, 0) names; the slot is the
+    triangle's index in the mesh's BVH order."""
     sc = rq.Scene.generate(0, 1)
     words = sc.words
-    tris = sc.layout_image("plain", 3).view(np.float32).reshape(-1, 12)
-    ids = sc.layout_image("plain", 4).reshape(-1, 4)
+    tris = sc.layout_image("plain", 3).view(np.float32).reshape(-1, 9)
+    ids = sc.layout_image("plain", 4).reshape(-1, 6)
     inst = sc.layout_image("plain", 0).reshape(-1, 32)
     bases = sorted(set(int(x) for x in inst[:, 20]) | {len(tris)})
     checked = 0
@@ -213,5 +214,8 @@ def test_plain_triangle_records_hold_vertex_and_edges():
             assert np.array_equal(tris[k, 0:3].view(np.uint32), a.view(np.uint32))
             assert np.array_equal(tris[k, 3:6].view(np.uint32), (b - a).view(np.uint32))
             assert np.array_equal(tris[k, 6:9].view(np.uint32), (c - a).view(np.uint32))
+            assert ids[k, 3] == k - base and ids[k, 5] == 0
+            idx = int(rec[21])
+            assert [int(x) for x in words[idx + 3 * (k - base):idx + 3 * (k - base) + 3]] == [ia, ib, ic]
             checked += 1
     assert checked > 0
