@@ -319,6 +319,36 @@ def run_serving(rq, layout, scenes, rate, n_jobs, resolutions, mixes, seed=3, sl
             "mrays_s": round(rays / span / 1e6, 1), "slots": slots, "hbm_budget_gb": budget >> 30}
 
 
+def run_reflection(rq, kind, meshes, tris, frame, reps=3):
+    """The reflection rays' traversal (rs_trace over rl_geometry) on the same seeded scene: the frame's
+    primary and GI rays as closest-hit queries (rs_closest), the shadow rays as any-hit (rs_blocked)."""
+    t0 = time.perf_counter()
+    sc = rq.Scene.generate(kind, 1, meshes, tris, reflection=True)
+    sc.upload(("rs",))
+    build_s = time.perf_counter() - t0
+    c = frame.counts_host
+    n_gi = int(c[1]) if frame.n_gi else 0
+    hits = rq.DeviceBuffer(max(frame.n, n_gi, 1) * 48)
+    vis = rq.DeviceBuffer(((int(c[0]) + 31) // 32 + 1) * 4)
+    ranges = [(frame.prim, frame.n, rq.CLOSEST, hits, None), (frame.shadow, int(c[0]), rq.ANY, None, vis)]
+    if n_gi:
+        ranges.insert(1, (frame.gi, n_gi, rq.CLOSEST, frame.gi_hits, None))
+    rays = sum(r[1] for r in ranges)
+    start, end = rq.Event(), rq.Event()
+    sc.trace_ranges("rs", ranges)
+    rq.synchronize()
+    start.record()
+    for _ in range(reps):
+        sc.trace_ranges("rs", ranges)
+    end.record()
+    end.synchronize()
+    ms = start.elapsed_ms(end) / reps
+    nbytes = sum(sc.count_stats("rs", r[0], r[1], r[2])["bytes"] for r in ranges)
+    return {"rays": int(rays), "ms": round(ms, 4), "mrays_s": round(rays / ms / 1e3, 1),
+            "bytes_per_ray": round(nbytes / max(rays, 1), 1), "achieved_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words))}
+
+
 def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
     """The oracle traversal (scalar C port of gi_traverse) on this host's cores, over a bounded sample."""
     import oracle
@@ -362,6 +392,7 @@ def main():
                     help="shadow and GI rays as ranges of one launch (fused), one launch per pass (separate), or "
                          "one launch per step with the next frame's primary rays (pipelined)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
+    ap.add_argument("--no-reflection", action="store_true", help="skip the reflection-ray (rs_trace) line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -503,6 +534,9 @@ def main():
     if verify:
         line["fp64_verify"] = verify
     line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}
+    if not args.no_reflection:
+        line["reflection_rs"] = run_reflection(rq, kind, meshes, tris, frame)
+        log(f"reflection rays (rs_trace, one launch of primary + GI + shadow): {line['reflection_rs']}")
     if not args.no_serving:
         # 1080p shadow+GI load on the benchmark scene: p50/p99 job latency.
         sv = run_serving(rq, args.layout, [scene], rate=40.0, n_jobs=60, resolutions=[(1920, 1080)], mixes=[1])

@@ -41,7 +41,8 @@ typedef enum rq_status {
 typedef enum rq_layout {
     RQ_LAYOUT_REF_BVH4Q8 = 0, /* the reference gi_geometry words, verbatim */
     RQ_LAYOUT_PLAIN_F32 = 1,  /* BVH4, child boxes pre-decoded to f32, triangles pre-gathered */
-    RQ_LAYOUT_CW8_Q8 = 2      /* 8-wide nodes, 8-bit outward-quantized child boxes, f32 triangles */
+    RQ_LAYOUT_CW8_Q8 = 2,     /* 8-wide nodes, 8-bit outward-quantized child boxes, f32 triangles */
+    RQ_LAYOUT_RS_BVH4Q8 = 3   /* the reflection rays' rl_geometry words, verbatim (rs_trace) */
 } rq_layout;
 
 /* gi_traverse modes (shaders/gi_rays_software.wgsl:214-216). */
@@ -148,7 +149,24 @@ rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_in
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
    would upload, src/gi_cache.rs:4361). */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
-/* The packed reference words (host memory owned by the scene). */
+
+/* ---- Reflection geometry: the rl_geometry buffer the non-RT reflection rays
+   traverse (shaders/reflections_software_bvh4.wgsl rs_trace, binding 36).
+   rq_scene_build_reflection replaces Blas::build + pack + instance_words +
+   instance_bvh::place + finish_scene_bounds (src/reflection_software.rs:92-388):
+   the same SAH BVH4 per mesh with its own leaf tags, 12-word triangles with
+   source primitive ids, 24-word instance records (instances with a singular
+   model are skipped), the top level and the scene bounds. Instance flags bit 0
+   = casts no shadow; `placement` is the record's slot. Such a scene traces only
+   with RQ_LAYOUT_RS_BVH4Q8, modes RQ_MODE_CLOSEST (rs_closest) and
+   RQ_MODE_ANY (rs_blocked). Its hit records: inst = record index, tri =
+   (primitive, 0, 0), u, v, t, slot = placement slot, normal zero. */
+rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                                    uint32_t instance_count, int top_level, rq_scene** out);
+/* Adopts reference-packed rl_geometry words verbatim (src/gi_cache.rs binding 36 upload). */
+rq_status rq_scene_from_rl_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* The packed reference words, gi_geometry or (reflection scene) rl_geometry
+   (host memory owned by the scene). */
 rq_status rq_scene_gi_words(const rq_scene* scene, const uint32_t** words, size_t* word_count);
 /* [0] instances, [1] meshes, [2] triangles (over instances), [3] unique triangles,
    [4] BLAS nodes, [5] top-level root tag, [6] max BLAS depth */
@@ -337,6 +355,9 @@ rq_status rq_event_query(void* event); /* RQ_OK when complete, RQ_ERR_NOT_READY 
            `meshes` meshes and about `triangles` triangles in all. */
 rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
                        rq_scene** out);
+/* The same meshes and placements packed as reflection geometry (rl_geometry). */
+rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                                  rq_scene** out);
 /* Camera on a seeded orbit around the scene: column-major inverse of the
    reversed-Z view-projection (60 degree vertical field of view). */
 rq_status rq_gen_camera(uint32_t kind, uint64_t seed, uint32_t width, uint32_t height,

@@ -37,6 +37,8 @@ def _sig(name, res, *args):
 _sig("orc_gi_traverse_batch", ctypes.c_int, _P, _sz, _P, _u32, _u32, _P, _P, _P, _u32)
 _sig("orc_flat_f32", ctypes.c_int, _P, _sz, _P, _u32, _u32, _P)
 _sig("orc_brute_f64", ctypes.c_int, _P, _sz, _P, _u32, _u32, _P, _u32)
+_sig("orc_rs_trace_batch", ctypes.c_int, _P, _sz, _P, _u32, _u32, _P, _P, _P, _u32)
+_sig("orc_rs_brute_f64", ctypes.c_int, _P, _sz, _P, _u32, _u32, ctypes.c_double, _P, _u32)
 _sig("orc_reintersect_f64", ctypes.c_int, _P, _sz, _P, _P, _u32, _P)
 _sig("orc_validate_blas", ctypes.c_int, _P, _sz, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _P)
 _sig("orc_primary_ray", None, _P, _u32, _u32, _u32, _u32, _P)
@@ -60,6 +62,29 @@ def traverse(words, rays, mode, threads=0, want_hits=True, want_vis=False):
     if fails:
         raise RuntimeError(f"oracle traversal read outside the buffer on {fails} rays")
     return hits, vis, dict(zip(COUNTS_FIELDS, (int(x) for x in counts)))
+
+
+def rs_traverse(words, rays, mode, threads=0, want_hits=True, want_vis=False):
+    """rs_trace (the reflection rays) over rl_geometry words; returns (hits, vis_bits, counts dict)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    r = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(len(r), HIT_DTYPE) if want_hits else None
+    vis = np.zeros((len(r) + 31) // 32, np.uint32) if want_vis else None
+    counts = np.zeros(8, np.uint64)
+    fails = lib.orc_rs_trace_batch(_p(w), w.size, _p(r), len(r), int(mode), _p(hits), _p(vis), _p(counts),
+                                   int(threads))
+    if fails:
+        raise RuntimeError(f"oracle rs_trace read outside the buffer on {fails} rays")
+    return hits, vis, dict(zip(COUNTS_FIELDS, (int(x) for x in counts)))
+
+
+def rs_brute_f64(words, rays, mode, slack=4e-6, threads=0):
+    """f64 brute force over rl_geometry words (hit64 slot = primitive)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    r = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(len(r), HIT64_DTYPE)
+    lib.orc_rs_brute_f64(_p(w), w.size, _p(r), len(r), int(mode), float(slack), _p(hits), int(threads))
+    return hits
 
 
 def flat_f32(words, rays, mode):

@@ -321,10 +321,218 @@ int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mo
     return x.bad ? -1 : 0;
 }
 
+/* ------------------------------------------- reflection rays (rs_trace) */
+/* shaders/reflections_software_bvh4.wgsl, over the rl_geometry words packed by
+   src/reflection_software.rs: 8-word header, 24-word instance records, the GI
+   BVH4 node records with leaf tags LEAF | first << 4 | count, 12-word
+   triangles (a, e1, e2, primitive). */
+#define RS_HEADER 8u
+#define RS_REC 24u
+#define RS_TRI 12u
+static const float RS_EDGE = 4e-6f; /* :19 */
+
+typedef struct rs_best {
+    int hit;
+    float t;
+    uint32_t prim, slot;
+    float u, v;
+} rs_best;
+
+/* :45-47 rs_limit */
+static inline float rs_limit(const rs_best* b, uint32_t best_i, uint32_t i) {
+    return (b->hit && i <= best_i) ? bf(fb(b->t) + 1u) : b->t;
+}
+
+/* :50-141 rs_instance; 1 when an any-hit ray found a blocker. */
+static int rs_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, float tmin, int any_hit,
+                       rs_best* best, uint32_t* best_i, uint32_t* stack) {
+    uint64_t r = RS_HEADER + (uint64_t)i * RS_REC;
+    x->c.inst_tests++;
+    if (any_hit && W(x, r + 22) != 0u) return 0;
+    float bound = rs_limit(best, *best_i, i);
+    if (slab_near(origin, world_inv, GV(x, r + 12), GV(x, r + 15), tmin, bound) > bound) return 0;
+    x->c.inst_entries++;
+    v3 r0 = GV(x, r), r1 = GV(x, r + 4), r2 = GV(x, r + 8);
+    /* dot(row, vec4(origin, 1)): x, y, z, then w * 1 */
+    v3 o = mk(dot3(r0, origin) + F(x, r + 3) * 1.0f, dot3(r1, origin) + F(x, r + 7) * 1.0f,
+              dot3(r2, origin) + F(x, r + 11) * 1.0f);
+    v3 d = mk(dot3(r0, dir), dot3(r1, dir), dot3(r2, dir));
+    v3 inv = safe_inverse(d);
+    uint64_t nodes = W(x, r + 18), tris = W(x, r + 19);
+    uint32_t slot = W(x, r + 21);
+    stack[0] = W(x, r + 20);
+    uint32_t sp = 1;
+    while (sp > 0 && !x->bad) {
+        sp -= 1;
+        uint32_t child = stack[sp];
+        for (;;) {
+            if (child & LEAF) {
+                x->c.leaves++;
+                uint32_t start = (child & ~LEAF) >> 4, cnt = child & 15u;
+                for (uint32_t k = 0; k < cnt && !x->bad; ++k) {
+                    uint64_t t = tris + (uint64_t)(start + k) * RS_TRI;
+                    x->c.tri_tests++;
+                    v3 a = GV(x, t), e1 = GV(x, t + 3), e2 = GV(x, t + 6);
+                    uint32_t prim = W(x, t + 9);
+                    v3 p = cross3(d, e2);
+                    float det = dot3(e1, p);
+                    if (fabsf(det) < 1e-30f) continue;
+                    float inv_det = 1.0f / det;
+                    v3 s = sub3(o, a);
+                    float u = dot3(s, p) * inv_det;
+                    if (u < -RS_EDGE || u > 1.0f + RS_EDGE) continue;
+                    v3 q = cross3(s, e1);
+                    float v = dot3(d, q) * inv_det;
+                    if (v < -RS_EDGE || u + v > 1.0f + RS_EDGE) continue;
+                    float tt = dot3(e2, q) * inv_det;
+                    int tied = best->hit && tt == best->t && (i < *best_i || (i == *best_i && prim < best->prim));
+                    if (tt > tmin && (tt < best->t || tied)) {
+                        best->hit = 1; best->t = tt; best->prim = prim; best->slot = slot;
+                        best->u = u; best->v = v;
+                        *best_i = i;
+                        if (any_hit) return 1;
+                    }
+                }
+                break;
+            }
+            x->c.blas_nodes++;
+            uint64_t nb = nodes + (uint64_t)child * 16u;
+            v3 base = GV(x, nb), step = GV(x, nb + 3);
+            uint32_t ids[4] = {W(x, nb + 6), W(x, nb + 7), W(x, nb + 8), W(x, nb + 9)};
+            uint32_t lower[3] = {W(x, nb + 10), W(x, nb + 11), W(x, nb + 12)};
+            uint32_t upper[3] = {W(x, nb + 13), W(x, nb + 14), W(x, nb + 15)};
+            float lim = rs_limit(best, *best_i, i);
+            float dist[4];
+            for (int c = 0; c < 4; ++c) {
+                if (ids[c] == EMPTY) { dist[c] = FMAXV; continue; }
+                uint32_t sh = 8u * (uint32_t)c;
+                /* :36-42 rs_child: f32(byte) * step + base */
+                v3 lo = mk((float)((lower[0] >> sh) & 255u) * step.x + base.x,
+                           (float)((lower[1] >> sh) & 255u) * step.y + base.y,
+                           (float)((lower[2] >> sh) & 255u) * step.z + base.z);
+                v3 hi = mk((float)((upper[0] >> sh) & 255u) * step.x + base.x,
+                           (float)((upper[1] >> sh) & 255u) * step.y + base.y,
+                           (float)((upper[2] >> sh) & 255u) * step.z + base.z);
+                dist[c] = slab_near(o, inv, lo, hi, tmin, lim);
+            }
+#define SW(A, B) do { float td = dist[A]; dist[A] = dist[B]; dist[B] = td; \
+                      uint32_t ti = ids[A]; ids[A] = ids[B]; ids[B] = ti; } while (0)
+            if (dist[1] < dist[0]) SW(0, 1);
+            if (dist[3] < dist[2]) SW(2, 3);
+            if (dist[2] < dist[0]) SW(0, 2);
+            if (dist[3] < dist[1]) SW(1, 3);
+            if (dist[2] < dist[1]) SW(1, 2);
+#undef SW
+            if (dist[3] <= lim && sp < STACK) stack[sp++] = ids[3];
+            if (dist[2] <= lim && sp < STACK) stack[sp++] = ids[2];
+            if (dist[1] <= lim && sp < STACK) stack[sp++] = ids[1];
+            if (dist[0] <= lim) { child = ids[0]; continue; }
+            break;
+        }
+    }
+    return 0;
+}
+
+static void rs_write(const rs_best* b, uint32_t best_i, orc_hit* h) {
+    memset(h, 0, sizeof *h);
+    h->t = b->t;
+    h->inst = b->hit ? best_i : 0xffffffffu;
+    h->tri[0] = b->hit ? b->prim : 0u;
+    h->u = b->hit ? b->u : 0.0f;
+    h->v = b->hit ? b->v : 0.0f;
+    h->flags = b->hit ? 1u : 0u;
+    h->slot = b->hit ? b->slot : 0u;
+}
+
+/* :163-203 rs_trace (RS_TOP_WIDE); mode 0 closest, else any hit. */
+int orc_rs_trace(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mode, orc_hit* hit,
+                 orc_counts* counts) {
+    ctx x;
+    memset(&x, 0, sizeof x);
+    x.g = g; x.n = n;
+    rs_best best;
+    memset(&best, 0, sizeof best);
+    best.t = ray->tmax;
+    uint32_t best_i = 0xffffffffu;
+    if (ray_invalid(ray)) {
+        rs_write(&best, best_i, hit);
+        hit->flags = 2u;
+        if (counts) *counts = x.c;
+        return 0;
+    }
+    int any_hit = mode != 0u;
+    uint32_t stack[STACK];
+    v3 origin = mk(ray->o[0], ray->o[1], ray->o[2]);
+    v3 dir = mk(ray->d[0], ray->d[1], ray->d[2]);
+    float tmin = ray->tmin;
+    v3 world_inv = safe_inverse(dir);
+    uint32_t instances = W(&x, 0), top = W(&x, 1), root = W(&x, 2);
+    if (instances != 0u) {
+        /* the CPU-padded union of every instance (:175-187) */
+        uint64_t b = W(&x, 4);
+        v3 lo = GV(&x, b), hi = GV(&x, b + 4);
+        v3 a = mul3(sub3(lo, origin), world_inv), c = mul3(sub3(hi, origin), world_inv);
+        float near = mx(mx(mx(mn(a.x, c.x), mn(a.y, c.y)), mn(a.z, c.z)), tmin);
+        float far = mn(mn(mn(mx(a.x, c.x), mx(a.y, c.y)), mx(a.z, c.z)), ray->tmax);
+        if (near <= far) {
+            best.t = mn(ray->tmax, bf(fb(far) + 2u));
+            if (top == 0u) {
+                for (uint32_t i = 0; i < instances && !x.bad; ++i)
+                    if (rs_instance(&x, i, origin, dir, world_inv, tmin, any_hit, &best, &best_i, stack)) break;
+            } else {
+                uint32_t nodes[STACK];
+                nodes[0] = root;
+                uint32_t sp = 1;
+                while (sp > 0 && !x.bad) {
+                    sp -= 1;
+                    uint32_t item = nodes[sp];
+                    if (item & LEAF) {
+                        if (rs_instance(&x, item & ~LEAF, origin, dir, world_inv, tmin, any_hit, &best, &best_i,
+                                        stack))
+                            break;
+                        continue;
+                    }
+                    x.c.top_nodes++;
+                    uint64_t nb = top + (uint64_t)item * 32u;
+                    float bound = best.hit ? bf(fb(best.t) + 1u) : best.t;
+                    float dist[4];
+                    uint32_t ids[4];
+                    for (int l = 0; l < 4; ++l) {
+                        float ax = (F(&x, nb + l) - origin.x) * world_inv.x;
+                        float bx = (F(&x, nb + 12 + l) - origin.x) * world_inv.x;
+                        float ay = (F(&x, nb + 4 + l) - origin.y) * world_inv.y;
+                        float by = (F(&x, nb + 16 + l) - origin.y) * world_inv.y;
+                        float az = (F(&x, nb + 8 + l) - origin.z) * world_inv.z;
+                        float bz = (F(&x, nb + 20 + l) - origin.z) * world_inv.z;
+                        float nr = mx(mx(mx(mn(ax, bx), mn(ay, by)), mn(az, bz)), tmin);
+                        float fr = mn(mn(mn(mx(ax, bx), mx(ay, by)), mx(az, bz)), bound);
+                        ids[l] = W(&x, nb + 24 + l);
+                        dist[l] = (nr <= fr && ids[l] != EMPTY) ? nr : FMAXV;
+                    }
+#define SW(A, B) do { float td = dist[A]; dist[A] = dist[B]; dist[B] = td; \
+                      uint32_t ti = ids[A]; ids[A] = ids[B]; ids[B] = ti; } while (0)
+                    if (dist[1] < dist[0]) SW(0, 1);
+                    if (dist[3] < dist[2]) SW(2, 3);
+                    if (dist[2] < dist[0]) SW(0, 2);
+                    if (dist[3] < dist[1]) SW(1, 3);
+                    if (dist[2] < dist[1]) SW(1, 2);
+#undef SW
+                    for (int l = 3; l >= 0; --l)
+                        if (dist[l] <= bound && sp < STACK) nodes[sp++] = ids[l];
+                }
+            }
+        }
+    }
+    rs_write(&best, best_i, hit);
+    if (x.bad) hit->flags |= 4u;
+    if (counts) *counts = x.c;
+    return x.bad ? -1 : 0;
+}
+
 /* ------------------------------------------------------------- batching */
 typedef struct job {
     const uint32_t* g; size_t n; const orc_ray* rays; uint32_t count; uint32_t mode;
-    orc_hit* hits; uint32_t* vis; orc_counts sum; int fails;
+    orc_hit* hits; uint32_t* vis; orc_counts sum; int fails; int rs;
     uint32_t next; /* shared cursor */
     pthread_mutex_t lock;
 } job;
@@ -340,7 +548,9 @@ static void* run_worker(void* arg) {
         for (uint32_t i = start; i < end; ++i) {
             orc_hit h;
             orc_counts c;
-            if (orc_gi_traverse(j->g, j->n, &j->rays[i], j->mode, &h, &c)) w->fails++;
+            int rc = j->rs ? orc_rs_trace(j->g, j->n, &j->rays[i], j->mode, &h, &c)
+                           : orc_gi_traverse(j->g, j->n, &j->rays[i], j->mode, &h, &c);
+            if (rc) w->fails++;
             if (j->hits) j->hits[i] = h;
             if (j->vis && !(h.flags & 1u)) __atomic_fetch_or(&j->vis[i / 32u], 1u << (i % 32u), __ATOMIC_RELAXED);
             w->c.words += c.words; w->c.top_nodes += c.top_nodes; w->c.inst_tests += c.inst_tests;
@@ -351,8 +561,8 @@ static void* run_worker(void* arg) {
     return NULL;
 }
 
-int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
-                          orc_hit* hits, uint32_t* vis_bits, orc_counts* counts, uint32_t threads) {
+static int traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                          orc_hit* hits, uint32_t* vis_bits, orc_counts* counts, uint32_t threads, int rs) {
     if (threads == 0) {
         long c = sysconf(_SC_NPROCESSORS_ONLN);
         threads = c > 0 ? (uint32_t)c : 1u;
@@ -360,7 +570,7 @@ int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint
     if (threads > 256) threads = 256;
     job j;
     memset(&j, 0, sizeof j);
-    j.g = g; j.n = n; j.rays = rays; j.count = count; j.mode = mode; j.hits = hits; j.vis = vis_bits;
+    j.g = g; j.n = n; j.rays = rays; j.count = count; j.mode = mode; j.hits = hits; j.vis = vis_bits; j.rs = rs;
     if (vis_bits) memset(vis_bits, 0, ((size_t)count + 31u) / 32u * 4u);
     worker ws[256];
     pthread_t th[256];
@@ -381,6 +591,16 @@ int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint
     }
     if (counts) *counts = sum;
     return fails;
+}
+
+int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                          orc_hit* hits, uint32_t* vis_bits, orc_counts* counts, uint32_t threads) {
+    return traverse_batch(g, n, rays, count, mode, hits, vis_bits, counts, threads, 0);
+}
+
+int orc_rs_trace_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                       orc_hit* hits, uint32_t* vis_bits, orc_counts* counts, uint32_t threads) {
+    return traverse_batch(g, n, rays, count, mode, hits, vis_bits, counts, threads, 1);
 }
 
 /* -------------------------------------------------- flat f32 loop (no tree) */
@@ -550,6 +770,89 @@ int orc_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t cou
     bjob j = {g, n, rays, count, mode, hits, 0};
     pthread_t th[256];
     for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, brute_worker, &j);
+    for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* f64 brute force over rl_geometry words: every triangle (a, e1, e2 as
+   stored) of every instance, the same edge slack as rs_trace, lexicographic
+   (t, instance, primitive) tie rule; hit64.slot holds the primitive. A mesh's
+   triangles run from its triangle base to the next base word packed after it. */
+typedef struct rjob {
+    const uint32_t* g; size_t n; const orc_ray* rays; uint32_t count; uint32_t mode; double slack;
+    orc_hit64* hits; uint32_t next;
+} rjob;
+
+static uint64_t rs_tri_end(const uint32_t* g, size_t n, uint64_t tris) {
+    uint64_t end = n;
+    for (uint32_t i = 0; i < g[0]; ++i) {
+        uint64_t r = RS_HEADER + (uint64_t)i * RS_REC;
+        if (r + RS_REC > n) break;
+        if (g[r + 18] > tris && g[r + 18] < end) end = g[r + 18];
+        if (g[r + 19] > tris && g[r + 19] < end) end = g[r + 19];
+    }
+    return end;
+}
+
+static void rs_brute_one(const rjob* j, uint32_t q) {
+    const uint32_t* g = j->g;
+    const orc_ray* ray = &j->rays[q];
+    orc_hit64 best;
+    memset(&best, 0, sizeof best);
+    best.t = -1.0;
+    best.inst = 0xffffffffu;
+    if (ray_invalid(ray)) { j->hits[q] = best; return; }
+    int any_hit = j->mode != 0u;
+    for (uint32_t i = 0; i < g[0]; ++i) {
+        uint64_t r = RS_HEADER + (uint64_t)i * RS_REC;
+        if (r + RS_REC > j->n) break;
+        if (any_hit && g[r + 22] != 0u) continue;
+        d3 o, d;
+        obj_ray64(g, r, ray, &o, &d);
+        uint64_t tris = g[r + 19], end = rs_tri_end(g, j->n, tris);
+        for (uint64_t t = tris; t + RS_TRI <= end; t += RS_TRI) {
+            d3 a = dv(g, t), e1 = dv(g, t + 3), e2 = dv(g, t + 6);
+            uint32_t prim = g[t + 9];
+            d3 p = dcross(d, e2);
+            double det = ddot(e1, p);
+            if (det == 0.0) continue;
+            d3 s = dsub(o, a);
+            double u = ddot(s, p) / det;
+            if (u < -j->slack || u > 1.0 + j->slack) continue;
+            d3 qv = dcross(s, e1);
+            double v = ddot(d, qv) / det;
+            if (v < -j->slack || u + v > 1.0 + j->slack) continue;
+            double tt = ddot(e2, qv) / det;
+            if (!(tt > (double)ray->tmin && tt < (double)ray->tmax)) continue;
+            if (!best.hit || tt < best.t || (tt == best.t && (i < best.inst || (i == best.inst && prim < best.slot)))) {
+                best.hit = 1; best.t = tt; best.u = u; best.v = v; best.inst = i; best.slot = prim;
+            }
+            if (any_hit) { j->hits[q] = best; return; }
+        }
+    }
+    j->hits[q] = best;
+}
+
+static void* rs_brute_worker(void* arg) {
+    rjob* j = (rjob*)arg;
+    for (;;) {
+        uint32_t q = __atomic_fetch_add(&j->next, 1u, __ATOMIC_RELAXED);
+        if (q >= j->count) break;
+        rs_brute_one(j, q);
+    }
+    return NULL;
+}
+
+int orc_rs_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                     double slack, orc_hit64* hits, uint32_t threads) {
+    if (threads == 0) {
+        long c = sysconf(_SC_NPROCESSORS_ONLN);
+        threads = c > 0 ? (uint32_t)c : 1u;
+    }
+    if (threads > 256) threads = 256;
+    rjob j = {g, n, rays, count, mode, slack, hits, 0};
+    pthread_t th[256];
+    for (uint32_t t = 0; t < threads; ++t) pthread_create(&th[t], NULL, rs_brute_worker, &j);
     for (uint32_t t = 0; t < threads; ++t) pthread_join(th[t], NULL);
     return 0;
 }

@@ -15,6 +15,10 @@
  *                        184-194 (f64 Moller-Trumbore, exact det==0 rejection,
  *                        lexicographic tie rule) over every instance.
  *   orc_reintersect_f64  the same f64 test applied to one reported hit.
+ *   orc_rs_trace         shaders/reflections_software_bvh4.wgsl:163-203 (rs_trace),
+ *                        :50-141 (rs_instance), :19-47 (helpers): the reflection
+ *                        rays over rl_geometry (src/reflection_software.rs).
+ *   orc_rs_brute_f64     the same geometry and slack in f64 over every triangle.
  *   orc_validate_blas    crates/asset/src/cluster_bvh.rs:170-333 (Bvh::validate)
  *                        on one mesh's runtime records.
  *   orc_primary_ray      shaders/primary_rays.wgsl:50-66 (primary_ray).
@@ -76,6 +80,13 @@ int orc_gi_traverse_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint
                           orc_counts* counts, uint32_t threads);
 int orc_flat_f32(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count,
                  uint32_t mode, orc_hit* hits);
+/* The reflection rays' rs_trace over rl_geometry words (record: inst = record
+   index, tri[0] = primitive, u, v, t, slot = placement slot, normal zero). */
+int orc_rs_trace(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mode, orc_hit* hit,
+                 orc_counts* counts);
+int orc_rs_trace_batch(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count,
+                       uint32_t mode, orc_hit* hits, uint32_t* vis_bits, orc_counts* counts,
+                       uint32_t threads);
 
 typedef struct orc_hit64 {
     double t, u, v;
@@ -89,6 +100,9 @@ int orc_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t cou
 /* f64 re-intersection of each reported hit (inst, slot); misses get t=-1. */
 int orc_reintersect_f64(const uint32_t* g, size_t n, const orc_ray* rays, const orc_hit* hits,
                         uint32_t count, orc_hit64* out);
+/* f64 brute force over rl_geometry words with edge slack `slack`; slot = primitive. */
+int orc_rs_brute_f64(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t count, uint32_t mode,
+                     double slack, orc_hit64* hits, uint32_t threads);
 
 /* Bvh::validate on one mesh of the packed buffer (node/leaf/position/index word
    offsets, root tag, counts). Returns 0 when valid, else a negative code. */

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librq.so")
 BUILD = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["api.cpp", "builder.cpp", "toplevel.cpp", "pack.cpp", "layouts.cpp", "scenegen.cpp"]
+HOST_SOURCES = ["api.cpp", "builder.cpp", "toplevel.cpp", "pack.cpp", "layouts.cpp", "scenegen.cpp", "reflect.cpp"]
 DEVICE_SOURCES = ["trace.hip", "serve.hip", "sort.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

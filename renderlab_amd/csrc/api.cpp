@@ -160,7 +160,7 @@ extern "C" rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]) {
 
 extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
                                            size_t* bytes) {
-    if (!scene || !data || !bytes || (int)layout < 1 || (int)layout > 2 || part > 5) {
+    if (!scene || !data || !bytes || (int)layout < 1 || (int)layout > 2 || part > 5 || scene->kind != RQ_SCENE_GI) {
         set_error("rq_scene_layout_image: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }

@@ -27,6 +27,15 @@ constexpr uint32_t REFITS = 64;             // src/instance_bvh.rs:94
 // the side record read once per hit: position indices, BVH slot, primitive id.
 constexpr uint32_t TRI_FLOATS = 9;
 constexpr uint32_t TRI_ID_WORDS = 6;
+// Reflection geometry (src/reflection_software.rs:26-44).
+constexpr uint32_t RS_HEADER_WORDS = 8;
+constexpr uint32_t RS_RECORD_WORDS = 24;
+constexpr uint32_t RS_NODE_WORDS = 16;
+constexpr uint32_t RS_TRI_WORDS = 12;
+constexpr uint32_t RS_TOP_NODE_WORDS = 32;
+// Scene kinds: the GI rays' gi_geometry words, or the reflection rays' rl_geometry words.
+constexpr int RQ_SCENE_GI = 0;
+constexpr int RQ_SCENE_REFLECTION = 1;
 
 inline float bits_f(uint32_t w) { float f; std::memcpy(&f, &w, 4); return f; }
 inline uint32_t f_bits(float f) { uint32_t w; std::memcpy(&w, &f, 4); return w; }
@@ -104,6 +113,8 @@ bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* so
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
+// reflect.cpp: words of the reflection header, records, top level and scene bounds
+uint32_t rs_records_words(uint32_t instances);
 
 // Per-mesh facts recovered from an instance record of packed words.
 struct MeshView {
@@ -147,10 +158,11 @@ struct rq_bvh {
 };
 
 struct rq_scene {
-    std::vector<uint32_t> words;  // the reference gi_geometry words
+    int kind = rq::RQ_SCENE_GI;
+    std::vector<uint32_t> words;  // the reference gi_geometry (or rl_geometry) words
     float bounds[6] = {0, 0, 0, 1, 1, 1};  // union of the instance world boxes (ray reordering keys)
     uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     rq::HostLayout plain, cw8;
-    rq::DeviceLayout dev[3];
+    rq::DeviceLayout dev[4];  // by rq_layout
     int device = -1;
 };

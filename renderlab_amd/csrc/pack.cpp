@@ -147,8 +147,9 @@ void scene_bounds(rq_scene* scene) {
     float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
     float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
     uint32_t n = w.size() > 0 ? w[0] : 0;
+    const bool rs = scene->kind == RQ_SCENE_REFLECTION;
     for (uint32_t i = 0; i < n; ++i) {
-        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        size_t r = rs ? RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS : HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
         if (r + 18 > w.size()) break;
         for (int a = 0; a < 3; ++a) {
             lo[a] = mn(lo[a], bits_f(w[r + 12 + a]));

@@ -168,7 +168,8 @@ MeshData cluster(Rng& g, uint32_t n, double median, double sigma) {
 
 using namespace rq;
 
-extern "C" rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, rq_scene** out) {
+static rq_status gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, bool reflection,
+                           rq_scene** out) {
     if (!out || kind > 1) {
         set_error("rq_gen_scene: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -224,7 +225,18 @@ extern "C" rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes,
         ms[m].index_count = (uint32_t)data[m].i.size();
         ms[m].keep_primitives = 1;
     }
-    return rq_scene_build(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out);
+    return reflection
+               ? rq_scene_build_reflection(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out)
+               : rq_scene_build(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out);
+}
+
+extern "C" rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, rq_scene** out) {
+    return gen_scene(kind, seed, meshes, triangles, false, out);
+}
+
+extern "C" rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                                             rq_scene** out) {
+    return gen_scene(kind, seed, meshes, triangles, true, out);
 }
 
 extern "C" rq_status rq_gen_camera(uint32_t kind, uint64_t seed, uint32_t width, uint32_t height, float inverse[16]) {

@@ -120,12 +120,13 @@ static double now_s(std::chrono::steady_clock::time_point t0) {
 extern "C" rq_status rq_serve(rq_scene* const* scenes, uint32_t scene_count, const rq_serve_config* cfg,
                               const rq_job* jobs, uint32_t job_count, rq_job_result* results) {
     if (!scenes || !cfg || (!jobs && job_count) || (!results && job_count) || cfg->slots == 0 || cfg->slots > 16 ||
-        (int)cfg->layout < 0 || (int)cfg->layout > 2) {
+        (int)cfg->layout < 0 || (int)cfg->layout > 2 || scene_count == 0) {
         set_error("rq_serve: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
     for (uint32_t i = 0; i < job_count; ++i)
-        if (jobs[i].scene >= scene_count || !scenes[jobs[i].scene] || jobs[i].n_gi > 1 || jobs[i].width == 0 ||
+        if (jobs[i].scene >= scene_count || !scenes[jobs[i].scene] || scenes[jobs[i].scene]->kind != RQ_SCENE_GI ||
+            jobs[i].n_gi > 1 || jobs[i].width == 0 ||
             jobs[i].height == 0 || !scenes[jobs[i].scene]->dev[cfg->layout].ready) {
             set_error("rq_serve: job " + std::to_string(i) + " is malformed or names a scene without the layout");
             return RQ_ERR_INVALID_INPUT;

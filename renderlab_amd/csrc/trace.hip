@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -40,7 +41,7 @@ constexpr uint32_t STACK_CAP = 48;
 constexpr uint32_t CW8_CAP = 112;
 constexpr uint32_t LEAF_MASK = (1u << 27) - 1u;
 
-enum : int { REF = 0, PLAIN = 1, CW8 = 2 };
+enum : int { REF = 0, PLAIN = 1, CW8 = 2, RS = 3 };
 
 __device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 __device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
@@ -100,6 +101,29 @@ __device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 e1, V3 e2, float 
         return t;
     }
     return -1.0f;
+}
+
+// rs_trace's triangle test (shaders/reflections_software_bvh4.wgsl:124-135): the
+// same Moller-Trumbore arithmetic with RS_EDGE slack on the barycentric tests
+// and no interval test (the caller applies tmin, the best t and the tie rule).
+// False on a rejection.
+constexpr float RS_EDGE = 4e-6f;
+__device__ __forceinline__ bool rs_tri_test(V3 o, V3 d, V3 a, V3 e1, V3 e2, float& t_out, float& u_out,
+                                            float& v_out) {
+    V3 p = cross3(d, e2);
+    float det = dot3(e1, p);
+    if (fabsf(det) < 1e-30f) return false;
+    float inv = 1.0f / det;
+    V3 s = sub3(o, a);
+    float u = dot3(s, p) * inv;
+    if (u < -RS_EDGE || u > 1.0f + RS_EDGE) return false;
+    V3 q = cross3(s, e1);
+    float v = dot3(d, q) * inv;
+    if (v < -RS_EDGE || u + v > 1.0f + RS_EDGE) return false;
+    t_out = dot3(e2, q) * inv;
+    u_out = u;
+    v_out = v;
+    return true;
 }
 
 // shaders/scene.wgsl:86-90 applied to mat3x3(r0,r1,r2) * cross(e1, e2)
@@ -256,12 +280,21 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         }
         return st[(sp & (S - 1)) * BLOCK];
     };
-    auto limit = [&](uint32_t i) -> float { return (bhit && i < binst) ? bf(fb(bt) + 1u) : bt; };
+    // gi_limit (:64-70); rs_limit keeps equal-distance candidates of the same
+    // instance eligible too (lower primitive wins), hence i <= best there.
+    auto limit = [&](uint32_t i) -> float {
+        return (bhit && (LAYOUT == RS ? i <= binst : i < binst)) ? bf(fb(bt) + 1u) : bt;
+    };
     // ---- hit writeback (the record of a finished ray)
     auto writeback = [&]() {
         uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
         V3 n = mk(0.0f, 0.0f, 0.0f);
-        if (bhit) {
+        if (LAYOUT == RS && bhit) {
+            // rs_trace's record (slot + 1, primitive, u, v): the slot, the primitive
+            tri0 = btri;
+            slot = v.words[RS_HEADER_WORDS + (uint64_t)binst * RS_RECORD_WORDS + 21u];
+            if (COUNT) nb += 4;
+        } else if (bhit) {
             V3 r0, r1, r2, e1, e2;
             if (LAYOUT == REF) {
                 const uint32_t* r = v.words + 16ull + (uint64_t)binst * 32u;
@@ -393,6 +426,28 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     flat_next = 0;
                     tri_left = 0;
                     nxt = (!invalid && v.top_on) ? v.top_root : NONE;
+                    if (LAYOUT == RS && !invalid) {
+                        // rs_trace's scene-bounds pre-cull (:175-187): a ray that misses
+                        // the padded union of the instances is a miss; otherwise the best
+                        // t starts 2 ulps past the exit so boundary triangles stay eligible.
+                        bool out = v.n_inst == 0u;
+                        if (!out) {
+                            const uint4* bp = reinterpret_cast<const uint4*>(v.words + v.words[4]);
+                            const uint4 q0 = bp[0], q1 = bp[1];
+                            if (COUNT) nb += 32;
+                            const float ax = (bf(q0.x) - wo.x) * winv.x, bx = (bf(q1.x) - wo.x) * winv.x;
+                            const float ay = (bf(q0.y) - wo.y) * winv.y, by = (bf(q1.y) - wo.y) * winv.y;
+                            const float az = (bf(q0.z) - wo.z) * winv.z, bz = (bf(q1.z) - wo.z) * winv.z;
+                            const float near = fmx(fmx(fmx(fmn(ax, bx), fmn(ay, by)), fmn(az, bz)), tmin);
+                            const float far = fmn(fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz)), bt);
+                            if (near > far) out = true;
+                            else bt = fmn(bt, bf(fb(far) + 2u));
+                        }
+                        if (out) {
+                            nxt = NONE;
+                            flat_next = v.n_inst;
+                        }
+                    }
                 }
             }
         }
@@ -422,6 +477,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     tri_next = v.words[l] / 3u;
                     tri_left = v.words[l + 1] / 3u;
                     if (COUNT) nb += 8;
+                } else if (LAYOUT == RS) {
+                    tri_next = (nxt & ~LEAF_BIT) >> 4;  // LEAF | first triangle << 4 | count
+                    tri_left = nxt & 15u;
                 } else {
                     tri_next = nxt & LEAF_MASK;
                     tri_left = ((nxt >> 27) & 0xfu) + 1u;
@@ -451,6 +509,23 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 #pragma unroll
             for (uint32_t j = 0; j < 2; ++j) {
                 if (j >= k || fin) break;
+                if (LAYOUT == RS) {
+                    // rs_trace leaf (:118-141): words 0-2 corner a, 3-5 e1, 6-8 e2, 9 primitive
+                    const uint4* tp = reinterpret_cast<const uint4*>(v.words + m_idx + (uint64_t)RS_TRI_WORDS * (tri_next + j));
+                    const uint4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+                    if (COUNT) nb += 48;
+                    float t, u, vv;
+                    if (rs_tri_test(oo, od, mk(bf(w0.x), bf(w0.y), bf(w0.z)), mk(bf(w0.w), bf(w1.x), bf(w1.y)),
+                                    mk(bf(w1.z), bf(w1.w), bf(w2.x)), t, u, vv)) {
+                        const bool tied = bhit && t == bt && (cur < binst || (cur == binst && w2.y < btri));
+                        if (t > tmin && (t < bt || tied)) {
+                            bt = t; bu = u; bv = vv; binst = cur; bhit = true;
+                            btri = w2.y;  // the source primitive: the tie key and the record
+                            if (any_hit) fin = true;
+                        }
+                    }
+                    continue;
+                }
                 V3 va, e1, e2;
                 uint32_t slot_id;
                 if (LAYOUT == REF) {
@@ -487,7 +562,22 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
-                if (LAYOUT == REF) {
+                if (LAYOUT == RS) {
+                    // rs_instance (:79-99): record words 0-11 rows, 12-17 world box,
+                    // 18 node base, 19 triangle base, 20 root, 21 slot, 22 casts no shadow
+                    const uint4* r4 = reinterpret_cast<const uint4*>(v.words + RS_HEADER_WORDS + (uint64_t)i * RS_RECORD_WORDS);
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        uint4 q = r4[k];
+                        rw[4 * k] = q.x; rw[4 * k + 1] = q.y; rw[4 * k + 2] = q.z; rw[4 * k + 3] = q.w;
+                    }
+                    const uint4 tail = r4[5];
+                    m_nodes = rw[18];
+                    m_idx = rw[19];
+                    rw[18] = tail.x;  // root
+                    rw[19] = tail.z != 0u ? 1u : 0u;  // casts no shadow
+                    if (COUNT) nb += 96;
+                } else if (LAYOUT == REF) {
                     const uint32_t* r = v.words + 16ull + (uint64_t)i * 32u;
 #pragma unroll
                     for (int k = 0; k < 18; ++k) rw[k] = r[k];
@@ -514,7 +604,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     oinv = LAYOUT == CW8 ? fast_inverse(od) : safe_inverse(od);
                     cur = i;
                     uint32_t root;
-                    if (LAYOUT == REF) {
+                    if (LAYOUT == RS) {
+                        root = rw[18];
+                    } else if (LAYOUT == REF) {
                         const uint32_t* r = v.words + 16ull + (uint64_t)i * 32u;
                         m_nodes = r[18];
                         m_leaves = r[19];
@@ -529,7 +621,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     in_blas = true;
                     nxt = root;  // pushed and popped at once
                 }
-            } else if (LAYOUT == REF) {
+            } else if (LAYOUT == REF || LAYOUT == RS) {
+                // RS: rs_trace's top level and BLAS nodes are the same records; its
+                // closest hit does not depend on the visit order (rs_limit + the
+                // primitive tie rule), and any-hit rays report visibility bits
                 if (!in_blas) {
                     // top-level wide node (:240-261), the reference's own sort network
                     const uint32_t* w = v.top + (uint64_t)item * 32u;
@@ -1089,11 +1184,18 @@ struct Ctx {
     int device = -1;
     int cus = 0;
     std::vector<StreamRes> res;
-    rq_tuning tuning[3][2];
-    rq_tuning batch_tuning[3];
+    rq_tuning tuning[4][2];
+    rq_tuning batch_tuning[4];
     bool tuned = false;
 };
 Ctx g_ctx;
+
+// A layout traces a scene of its kind: REF, PLAIN and CW8 the GI words, RS the
+// reflection words.
+bool layout_ok(const rq_scene* s, int layout) {
+    if (!s || layout < 0 || layout > 3) return false;
+    return s->kind == RQ_SCENE_REFLECTION ? layout == RQ_LAYOUT_RS_BVH4Q8 : layout != RQ_LAYOUT_RS_BVH4Q8;
+}
 
 rq_status hip_fail(hipError_t e, const char* what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -1118,7 +1220,7 @@ void default_tuning(rq_tuning& t, int layout, int any) {
 
 void init_tuning() {
     if (g_ctx.tuned) return;
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 4; ++l) {
         for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
         default_tuning(g_ctx.batch_tuning[l], l, 0);
     }
@@ -1150,43 +1252,35 @@ void launch_one(dim3 g, dim3 b, hipStream_t s, const dev::TraceArgs& a) {
 
 struct KernelEntry {
     uint32_t block, stack;
-    Launch fn[3][2];
-    void* ptr[3][2];
-    Launch bfn[3];  // batched, mode per batch
-    void* bptr[3];
-    Launch rfn[3];  // ranges of one scene, mode per range
-    void* rptr[3];
+    Launch fn[4][2];    // by layout, any-hit
+    void* ptr[4][2];
+    Launch bfn[4];      // batched, mode per batch
+    void* bptr[4];
+    Launch rfn[4];      // ranges of one scene, mode per range
+    void* rptr[4];
 };
+
+template <int L, int B, int S>
+void fill_layout(KernelEntry& e) {
+    e.fn[L][0] = &launch_one<L, B, S, 0>;
+    e.fn[L][1] = &launch_one<L, B, S, 1>;
+    e.ptr[L][0] = kernel_ptr<L, B, S, 0>();
+    e.ptr[L][1] = kernel_ptr<L, B, S, 1>();
+    e.bfn[L] = &launch_one<L, B, S, 2, false, true>;
+    e.bptr[L] = kernel_ptr<L, B, S, 2, false, true>();
+    e.rfn[L] = &launch_one<L, B, S, 2>;
+    e.rptr[L] = kernel_ptr<L, B, S, 2>();
+}
 
 template <int B, int S>
 KernelEntry make_entry() {
     KernelEntry e;
     e.block = B;
     e.stack = S;
-    e.fn[0][0] = &launch_one<dev::REF, B, S, false>;
-    e.fn[0][1] = &launch_one<dev::REF, B, S, true>;
-    e.fn[1][0] = &launch_one<dev::PLAIN, B, S, false>;
-    e.fn[1][1] = &launch_one<dev::PLAIN, B, S, true>;
-    e.fn[2][0] = &launch_one<dev::CW8, B, S, false>;
-    e.fn[2][1] = &launch_one<dev::CW8, B, S, true>;
-    e.ptr[0][0] = kernel_ptr<dev::REF, B, S, false>();
-    e.ptr[0][1] = kernel_ptr<dev::REF, B, S, true>();
-    e.ptr[1][0] = kernel_ptr<dev::PLAIN, B, S, false>();
-    e.ptr[1][1] = kernel_ptr<dev::PLAIN, B, S, true>();
-    e.ptr[2][0] = kernel_ptr<dev::CW8, B, S, false>();
-    e.ptr[2][1] = kernel_ptr<dev::CW8, B, S, true>();
-    e.bfn[0] = &launch_one<dev::REF, B, S, 2, false, true>;
-    e.bfn[1] = &launch_one<dev::PLAIN, B, S, 2, false, true>;
-    e.bfn[2] = &launch_one<dev::CW8, B, S, 2, false, true>;
-    e.bptr[0] = kernel_ptr<dev::REF, B, S, 2, false, true>();
-    e.bptr[1] = kernel_ptr<dev::PLAIN, B, S, 2, false, true>();
-    e.bptr[2] = kernel_ptr<dev::CW8, B, S, 2, false, true>();
-    e.rfn[0] = &launch_one<dev::REF, B, S, 2>;
-    e.rfn[1] = &launch_one<dev::PLAIN, B, S, 2>;
-    e.rfn[2] = &launch_one<dev::CW8, B, S, 2>;
-    e.rptr[0] = kernel_ptr<dev::REF, B, S, 2>();
-    e.rptr[1] = kernel_ptr<dev::PLAIN, B, S, 2>();
-    e.rptr[2] = kernel_ptr<dev::CW8, B, S, 2>();
+    fill_layout<dev::REF, B, S>(e);
+    fill_layout<dev::PLAIN, B, S>(e);
+    fill_layout<dev::CW8, B, S>(e);
+    fill_layout<dev::RS, B, S>(e);
     return e;
 }
 
@@ -1200,20 +1294,20 @@ const std::vector<KernelEntry>& kernels() {
 const KernelEntry& count_kernel() {
     static KernelEntry e = [] {
         KernelEntry k;
+        std::memset(&k, 0, sizeof k);
         k.block = 256;
         k.stack = 16;
-        k.fn[0][0] = &launch_one<dev::REF, 256, 16, false, true>;
-        k.fn[0][1] = &launch_one<dev::REF, 256, 16, true, true>;
-        k.fn[1][0] = &launch_one<dev::PLAIN, 256, 16, false, true>;
-        k.fn[1][1] = &launch_one<dev::PLAIN, 256, 16, true, true>;
-        k.fn[2][0] = &launch_one<dev::CW8, 256, 16, false, true>;
-        k.fn[2][1] = &launch_one<dev::CW8, 256, 16, true, true>;
-        k.ptr[0][0] = kernel_ptr<dev::REF, 256, 16, false, true>();
-        k.ptr[0][1] = kernel_ptr<dev::REF, 256, 16, true, true>();
-        k.ptr[1][0] = kernel_ptr<dev::PLAIN, 256, 16, false, true>();
-        k.ptr[1][1] = kernel_ptr<dev::PLAIN, 256, 16, true, true>();
-        k.ptr[2][0] = kernel_ptr<dev::CW8, 256, 16, false, true>();
-        k.ptr[2][1] = kernel_ptr<dev::CW8, 256, 16, true, true>();
+        auto put = [&](auto lc) {
+            constexpr int L = decltype(lc)::value;
+            k.fn[L][0] = &launch_one<L, 256, 16, 0, true>;
+            k.fn[L][1] = &launch_one<L, 256, 16, 1, true>;
+            k.ptr[L][0] = kernel_ptr<L, 256, 16, 0, true>();
+            k.ptr[L][1] = kernel_ptr<L, 256, 16, 1, true>();
+        };
+        put(std::integral_constant<int, dev::REF>());
+        put(std::integral_constant<int, dev::PLAIN>());
+        put(std::integral_constant<int, dev::CW8>());
+        put(std::integral_constant<int, dev::RS>());
         return k;
     }();
     return e;
@@ -1334,6 +1428,16 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
         set_error("rq_scene_upload: null scene");
         return RQ_ERR_INVALID_INPUT;
     }
+    for (int l = 0; l < 4; ++l)
+        if ((layout_mask & (1u << l)) && !layout_ok(scene, l)) {
+            set_error(scene->kind == RQ_SCENE_REFLECTION ? "rq_scene_upload: a reflection scene has only the RS layout"
+                                                        : "rq_scene_upload: the RS layout needs a reflection scene");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    if (layout_mask & ~15u) {
+        set_error("rq_scene_upload: unknown layout bits");
+        return RQ_ERR_INVALID_INPUT;
+    }
     std::lock_guard<std::mutex> lock(g_ctx.mu);
     rq_status s = ensure_ctx();
     if (s != RQ_OK) return s;
@@ -1350,10 +1454,10 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             return RQ_ERR_INVALID_INPUT;
         }
     }
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 4; ++l) {
         if (!(layout_mask & (1u << l)) || scene->dev[l].ready) continue;
         DeviceLayout& d = scene->dev[l];
-        if (l == 0) {
+        if (l == 0 || l == 3) {  // the reference words verbatim
             d.bytes = padded(scene->words);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
@@ -1381,7 +1485,7 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
 }
 
 extern "C" rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes) {
-    if (!scene || !bytes || (int)layout < 0 || (int)layout > 2) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout > 3) {
         set_error("rq_scene_device_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1397,7 +1501,7 @@ extern "C" void rq_scene_destroy(rq_scene* scene) {
 }
 
 extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out) {
-    if (!out || (int)layout < 0 || (int)layout > 2) {
+    if (!out || (int)layout < 0 || (int)layout > 3) {
         set_error("rq_get_tuning: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1408,7 +1512,7 @@ extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* o
 }
 
 extern "C" rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+    if (!t || (int)layout < 0 || (int)layout > 3 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
         t->refill > 64) {
         set_error("rq_set_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;
@@ -1425,7 +1529,7 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
     dev::SceneView v;
     std::memset(&v, 0, sizeof v);
     const DeviceLayout& d = scene->dev[layout];
-    if (layout == RQ_LAYOUT_REF_BVH4Q8) {
+    if (layout == RQ_LAYOUT_REF_BVH4Q8 || layout == RQ_LAYOUT_RS_BVH4Q8) {
         v.words = reinterpret_cast<const uint32_t*>(d.base);
         v.top = v.words + scene->words[1];
         v.n_inst = scene->words[0];
@@ -1451,8 +1555,8 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
                        const uint32_t* count_ptr = nullptr) {
     const bool reorder = (mode & RQ_REORDER) != 0;
     mode &= ~RQ_REORDER;
-    if (!scene || (int)layout < 0 || (int)layout > 2 || mode > 2 || (count && !rays)) {
-        set_error("rq_trace: bad argument");
+    if (!scene || !layout_ok(scene, layout) || mode > 2 || (count && !rays)) {
+        set_error("rq_trace: bad argument (or a layout of the other scene kind)");
         return RQ_ERR_INVALID_INPUT;
     }
     if (!scene->dev[layout].ready) {
@@ -1547,7 +1651,7 @@ extern "C" rq_status rq_trace_indirect(rq_scene* scene, rq_layout layout, const 
 
 extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq_range* ranges,
                                      uint32_t range_count, void* stream) {
-    if (!scene || !ranges || (int)layout < 0 || (int)layout > 2 || range_count == 0 ||
+    if (!scene || !ranges || !layout_ok(scene, layout) || range_count == 0 ||
         range_count > RQ_MAX_RANGES) {
         set_error("rq_trace_ranges: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -1623,7 +1727,7 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
 
 extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
                                       const rq_batch* batches, uint32_t batch_count, void* stream) {
-    if (!scenes || !batches || (int)layout < 0 || (int)layout > 2 || batch_count == 0) {
+    if (!scenes || !batches || (int)layout < 0 || (int)layout > 3 || batch_count == 0) {
         set_error("rq_trace_batches: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1632,7 +1736,8 @@ extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_co
     uint64_t total = 0;
     for (uint32_t b = 0; b < batch_count; ++b) {
         const rq_batch& q = batches[b];
-        if (q.scene >= scene_count || !scenes[q.scene] || !scenes[q.scene]->dev[layout].ready || q.mode > 2 ||
+        if (q.scene >= scene_count || !layout_ok(scenes[q.scene], layout) || !scenes[q.scene]->dev[layout].ready ||
+            q.mode > 2 ||
             (q.count && !q.rays)) {
             set_error("rq_trace_batches: batch " + std::to_string(b) + " names a missing scene or layout");
             return RQ_ERR_INVALID_INPUT;
@@ -1716,7 +1821,7 @@ extern "C" rq_status rq_stream_reserve(void* stream, uint32_t max_rays) {
 }
 
 extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
-    if (!out || (int)layout < 0 || (int)layout > 2) {
+    if (!out || (int)layout < 0 || (int)layout > 3) {
         set_error("rq_get_batch_tuning: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1727,7 +1832,7 @@ extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
 }
 
 extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 2 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+    if (!t || (int)layout < 0 || (int)layout > 3 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
         t->refill > 64) {
         set_error("rq_set_batch_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;
@@ -1740,7 +1845,7 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
 
 extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                           uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[10]) {
-    if (!scene || !stats || (int)layout < 0 || (int)layout > 2 || (mode & ~RQ_REORDER) > 2 || (count && !rays) ||
+    if (!scene || !stats || !layout_ok(scene, layout) || (mode & ~RQ_REORDER) > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -1773,8 +1878,8 @@ extern "C" rq_status rq_verify_fp64(rq_scene* scene, const rq_ray* rays, const r
         set_error("rq_verify_fp64: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
-    if (!scene->dev[0].ready) {
-        set_error("rq_verify_fp64: needs the REF layout uploaded (the original positions)");
+    if (scene->kind != RQ_SCENE_GI || !scene->dev[0].ready) {
+        set_error("rq_verify_fp64: needs a GI scene with the REF layout uploaded (the original positions)");
         return RQ_ERR_INVALID_INPUT;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -31,7 +31,8 @@ RQ_ERR_NOT_READY = 6
 REF_BVH4Q8 = 0
 PLAIN_F32 = 1
 CW8_Q8 = 2
-LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8}
+RS_BVH4Q8 = 3
+LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8, "rs": RS_BVH4Q8}
 
 CLOSEST = 0
 ANY = 1
@@ -98,6 +99,9 @@ _sig("rq_instance_bvh_refit", _i, _P, _u32, _P, _u32)
 _sig("rq_instance_bvh_place", _i, _P, _u32, _u32, _u32, _u32, _P, _u32, _i, _P)
 _sig("rq_scene_build", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_gi_words", _i, _P, _sz, ctypes.POINTER(_P))
+_sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
+_sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
+_sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
 _sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
@@ -430,9 +434,10 @@ class Scene:
         self._h = handle
 
     @classmethod
-    def build(cls, meshes, instances, top_level=-1):
+    def build(cls, meshes, instances, top_level=-1, reflection=False):
         """meshes: [(positions (V,3), indices (3T,), keep_primitives)], instances:
-        [dict(model=4x4 row-major or 16 column-major, mesh=i, flags=0, identity=0, placement=i)]."""
+        [dict(model=4x4 row-major or 16 column-major, mesh=i, flags=0, identity=0, placement=i)].
+        reflection=True packs the reflection rays' rl_geometry instead (traced with layout "rs")."""
         keep = []
         md = (_MeshDesc * max(len(meshes), 1))()
         for k, m in enumerate(meshes):
@@ -455,7 +460,16 @@ class Scene:
             idesc[k].identity = int(ins.get("identity", 0))
             idesc[k].placement = int(ins.get("placement", k))
         h = _P()
-        check(lib.rq_scene_build(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
+        fn = lib.rq_scene_build_reflection if reflection else lib.rq_scene_build
+        check(fn(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_rl_words(cls, words):
+        """Reflection geometry words packed by the reference (binding 36)."""
+        w = np.ascontiguousarray(words, np.uint32)
+        h = _P()
+        check(lib.rq_scene_from_rl_words(_ptr(w), w.size, ctypes.byref(h)))
         return cls(h)
 
     @classmethod
@@ -466,9 +480,10 @@ class Scene:
         return cls(h)
 
     @classmethod
-    def generate(cls, kind, seed, meshes=0, triangles=0):
+    def generate(cls, kind, seed, meshes=0, triangles=0, reflection=False):
         h = _P()
-        check(lib.rq_gen_scene(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h)))
+        fn = lib.rq_gen_reflection_scene if reflection else lib.rq_gen_scene
+        check(fn(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h)))
         return cls(h)
 
     @property
