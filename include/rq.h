@@ -149,6 +149,21 @@ rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_in
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
    would upload, src/gi_cache.rs:4361). */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* GPU refit of one mesh of a built scene (rq_scene_build) whose REF layout is
+   uploaded, for a re-skinned pose: replaces the skin_refit path of
+   src/gi_cache.rs:2843-2863 -- near_field::refit_bvh (crates/asset/src/
+   near_field.rs:445-479) on the mesh, its instances' world boxes recomputed
+   (src/gi_bvh.rs instance_words) and instance_bvh::place over the previous
+   top level (src/instance_bvh.rs:162-220: refit while only boxes moved,
+   rebuild every 64 refits). `positions`: device memory, vertex_count * 3
+   floats in the mesh's vertex order. The BVH is refit on the device (one
+   launch per node height); the scene's host words follow, so the packed words
+   equal what the CPU refit and repack give. PLAIN and CW8 copies are dropped
+   (upload them again to trace with them). Same vertex count and finite
+   positions, else RQ_ERR_INVALID_INPUT with nothing changed. Synchronizes
+   `stream`. A mesh no instance draws is left alone. */
+rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const float* positions, uint32_t vertex_count,
+                              void* stream);
 
 /* ---- Reflection geometry: the rl_geometry buffer the non-RT reflection rays
    traverse (shaders/reflections_software_bvh4.wgsl rs_trace, binding 36).

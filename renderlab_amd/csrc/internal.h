@@ -113,6 +113,9 @@ bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* so
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
+// Instance record words 12-17: the padded world box of a mesh box under a model
+// (src/gi_bvh.rs instance_words).
+void instance_box(const float model[16], const float lo[3], const float hi[3], uint32_t out[6]);
 // reflect.cpp: words of the reflection header, records, top level and scene bounds
 uint32_t rs_records_words(uint32_t instances);
 
@@ -131,6 +134,13 @@ struct DeviceLayout {
     // word/element offsets into `base`, layout specific
     uint64_t off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// One mesh's GPU refit plan: its node ids grouped by height (device), cached.
+struct RefitPlan {
+    uint32_t nodes = 0;                  // the mesh's node word offset (its key)
+    void* order = nullptr;               // device: node ids, lowest height first
+    std::vector<uint32_t> level_start;   // slice of `order` per height
 };
 
 struct HostLayout {
@@ -164,5 +174,12 @@ struct rq_scene {
     uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     rq::HostLayout plain, cw8;
     rq::DeviceLayout dev[4];  // by rq_layout
+    std::vector<rq::RefitPlan> refit_plans;
+    // Kept by rq_scene_build for refits: each instance's model (16 floats,
+    // column-major), each mesh's node word offset, the top-level switch.
+    std::vector<float> models;
+    std::vector<uint32_t> mesh_nodes;
+    int top_force = -1;
+    void* refit_flag = nullptr;  // device word: non-finite position seen
     int device = -1;
 };

@@ -166,6 +166,29 @@ void scene_bounds(rq_scene* scene) {
     }
 }
 
+void instance_box(const float model[16], const float blo[3], const float bhi[3], uint32_t out[6]) {
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    const float* M = model;
+    for (int c = 0; c < 8; ++c) {
+        float p[3];
+        for (int a = 0; a < 3; ++a) p[a] = (c & (1 << a)) ? bhi[a] : blo[a];
+        for (int a = 0; a < 3; ++a) {
+            // glam transform_point3: ((x_axis*x + y_axis*y) + z_axis*z) + w_axis
+            float wv = M[a] * p[0] + M[4 + a] * p[1];
+            wv = wv + M[8 + a] * p[2];
+            wv = wv + M[12 + a];
+            lo[a] = mn(lo[a], wv);
+            hi[a] = mx(hi[a], wv);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        float pad = 1e-5f * (1.0f + mx(std::fabs(lo[a]), std::fabs(hi[a])));
+        out[a] = f_bits(lo[a] - pad);
+        out[3 + a] = f_bits(hi[a] + pad);
+    }
+}
+
 namespace {
 
 struct Blas {
@@ -285,26 +308,7 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
         size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
         for (int row = 0; row < 3; ++row)
             for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
-        float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-        float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-        const float* M = ins.model;
-        for (int c = 0; c < 8; ++c) {
-            float p[3];
-            for (int a = 0; a < 3; ++a) p[a] = (c & (1 << a)) ? b.hi[a] : b.lo[a];
-            for (int a = 0; a < 3; ++a) {
-                // glam transform_point3: ((x_axis*x + y_axis*y) + z_axis*z) + w_axis
-                float wv = M[a] * p[0] + M[4 + a] * p[1];
-                wv = wv + M[8 + a] * p[2];
-                wv = wv + M[12 + a];
-                lo[a] = mn(lo[a], wv);
-                hi[a] = mx(hi[a], wv);
-            }
-        }
-        for (int a = 0; a < 3; ++a) {
-            float pad = 1e-5f * (1.0f + mx(std::fabs(lo[a]), std::fabs(hi[a])));
-            head[r + 12 + a] = f_bits(lo[a] - pad);
-            head[r + 15 + a] = f_bits(hi[a] + pad);
-        }
+        instance_box(ins.model, b.lo, b.hi, &head[r + 12]);
         for (int k = 0; k < 4; ++k) head[r + 18 + k] = offsets[4 * ins.mesh + k];
         head[r + 22] = b.bvh.root;
         head[r + 23] = 0;  // no near-field masks
@@ -334,6 +338,11 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
     scene->stats[4] = nodes;
     scene->stats[5] = words[2];
     scene->stats[6] = depth;
+    scene->models.resize(16ull * instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i) std::copy(instances[i].model, instances[i].model + 16, &scene->models[16ull * i]);
+    scene->mesh_nodes.resize(mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
+    scene->top_force = top_level;
     scene_bounds(scene);
     *out = scene;
     return RQ_OK;

@@ -1,0 +1,319 @@
+// Refit of one mesh's BVH4 on the GPU, in place in a resident scene's REF
+// words: near_field::refit_bvh (crates/asset/src/near_field.rs:445-479) --
+// same topology, leaves and indices; every node's child boxes recomputed from
+// the new positions (a leaf: the bounds of its triangles' corners; an inner
+// child: the union of both decodes of its own, already refit, children) and
+// quantized as quantized_node (:481-525) quantizes them, fused and unfused
+// decodes both enclosing. Nodes are in post-order (children first), so the
+// nodes of one height only read finished ones: one launch per height.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace rq {
+namespace refitk {
+
+// The host builder's min/max (keep the first argument unless the second is
+// strictly smaller / larger): the refit must round and order like the build.
+__device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return b > a ? b : a; }
+__device__ __forceinline__ float bf(uint32_t w) { return __uint_as_float(w); }
+__device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+
+struct Args {
+    uint32_t* words;
+    uint32_t nodes, leaves, positions, indices;  // word offsets of the mesh's arrays
+    const uint32_t* order;                       // node ids grouped by height
+    uint32_t begin, end;                         // this launch's slice of `order`
+};
+
+__global__ __launch_bounds__(256) void refit_level(Args a) {
+    const uint32_t k = a.begin + blockIdx.x * 256u + threadIdx.x;
+    if (k >= a.end) return;
+    const uint32_t n = a.order[k];
+    uint32_t* node = a.words + a.nodes + 16ull * n;
+    uint32_t children[4];
+    float blo[4][3], bhi[4][3];
+    uint32_t count = 0;
+    for (int c = 0; c < 4; ++c) {
+        const uint32_t child = node[6 + c];
+        if (child == EMPTY) continue;
+        float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+        float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        if (child & LEAF_BIT) {
+            const uint32_t l = child & ~LEAF_BIT;
+            const uint32_t off = a.words[a.leaves + 2ull * l], cnt = a.words[a.leaves + 2ull * l + 1];
+            for (uint32_t i = off; i < off + cnt; ++i) {
+                const uint32_t* p = a.words + a.positions + 3ull * a.words[a.indices + i];
+                for (int x = 0; x < 3; ++x) {
+                    lo[x] = mn(lo[x], bf(p[x]));
+                    hi[x] = mx(hi[x], bf(p[x]));
+                }
+            }
+        } else {
+            const uint32_t* cn = a.words + a.nodes + 16ull * child;
+            for (int slot = 0; slot < 4; ++slot) {
+                if (cn[6 + slot] == EMPTY) continue;
+                for (int fused = 0; fused < 2; ++fused)
+                    for (int x = 0; x < 3; ++x) {
+                        const float o = bf(cn[x]), s = bf(cn[3 + x]);
+                        const float ql = (float)((cn[10 + x] >> (8 * slot)) & 0xffu);
+                        const float qh = (float)((cn[13 + x] >> (8 * slot)) & 0xffu);
+                        const float l = fused ? __builtin_fmaf(s, ql, o) : o + s * ql;
+                        const float h = fused ? __builtin_fmaf(s, qh, o) : o + s * qh;
+                        lo[x] = mn(lo[x], l);
+                        hi[x] = mx(hi[x], h);
+                    }
+            }
+        }
+        children[count] = child;
+        for (int x = 0; x < 3; ++x) {
+            blo[count][x] = lo[x];
+            bhi[count][x] = hi[x];
+        }
+        ++count;
+    }
+    // quantized_node
+    float all_lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float all_hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t c = 0; c < count; ++c)
+        for (int x = 0; x < 3; ++x) {
+            all_lo[x] = mn(all_lo[x], blo[c][x]);
+            all_hi[x] = mx(all_hi[x], bhi[c][x]);
+        }
+    uint32_t out[16];
+    float step[3];
+    const float grow = 1.0f + 1.0f / 1048576.0f;
+    for (int x = 0; x < 3; ++x) {
+        const float extent = all_hi[x] - all_lo[x];
+        float s = mx(extent / 255.0f, mx(1.17549435e-38f, fabsf(all_lo[x]) * 1e-7f));
+        // grow until code 255 covers the top in both decodes (bounded: s grows
+        // by 2^-20 relative per pass and starts within a few ulps of the answer)
+        for (int guard = 0; guard < (1 << 22) && (all_lo[x] + s * 255.0f < all_hi[x] ||
+                                                   __builtin_fmaf(s, 255.0f, all_lo[x]) < all_hi[x]);
+             ++guard)
+            s *= grow;
+        step[x] = s;
+        out[x] = fb(all_lo[x]);
+        out[3 + x] = fb(s);
+    }
+    for (int c = 0; c < 4; ++c) out[6 + c] = c < (int)count ? children[c] : EMPTY;
+    for (int x = 0; x < 3; ++x) {
+        uint32_t lw = 0, hw = 0;
+        const float o = all_lo[x], s = step[x];
+        for (uint32_t c = 0; c < count; ++c) {
+            const float lv = blo[c][x], hv = bhi[c][x];
+            float fl = floorf((lv - o) / s);
+            fl = mn(mx(fl, 0.0f), 255.0f);
+            uint32_t low = (uint32_t)fl;
+            while (low > 0 && (o + s * (float)low > lv || __builtin_fmaf(s, (float)low, o) > lv)) low -= 1;
+            float fh = ceilf((hv - o) / s);
+            fh = mn(mx(fh, 0.0f), 255.0f);
+            uint32_t high = (uint32_t)fh;
+            while (high < 255 && (o + s * (float)high < hv || __builtin_fmaf(s, (float)high, o) < hv)) high += 1;
+            lw |= low << (8 * c);
+            hw |= high << (8 * c);
+        }
+        out[10 + x] = lw;
+        out[13 + x] = hw;
+    }
+    for (int w = 0; w < 16; ++w) node[w] = out[w];
+}
+
+// New positions (3 floats a vertex) into the mesh's position words; any
+// non-finite value raises *bad and nothing is written by the later launches.
+__global__ __launch_bounds__(256) void check_positions(const float* p, uint32_t n, uint32_t* bad) {
+    uint32_t any = 0;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) any |= !isfinite(p[i]);
+    if (any) atomicOr(bad, 1u);
+}
+
+__global__ __launch_bounds__(256) void put_positions(uint32_t* words, uint32_t at, const float* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) words[at + i] = fb(p[i]);
+}
+
+}  // namespace refitk
+}  // namespace rq
+
+using namespace rq;
+
+namespace {
+rq_status fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RQ_ERR_OUT_OF_MEMORY : RQ_ERR_DEVICE;
+}
+#define RF_TRY(x, what)                       \
+    do {                                      \
+        hipError_t e_ = (x);                  \
+        if (e_ != hipSuccess) return fail(e_, what); \
+    } while (0)
+
+// Node ids of one mesh grouped by height (leaf-only nodes first), uploaded once.
+rq_status refit_plan(rq_scene* scene, const MeshView& m, RefitPlan*& out) {
+    for (auto& p : scene->refit_plans)
+        if (p.nodes == m.nodes) {
+            out = &p;
+            return RQ_OK;
+        }
+    const uint32_t* w = scene->words.data();
+    std::vector<uint32_t> h(m.node_count, 0);
+    uint32_t top = 0;
+    for (uint32_t n = 0; n < m.node_count; ++n) {  // post-order: children first
+        uint32_t hh = 0;
+        for (int c = 0; c < 4; ++c) {
+            uint32_t ch = w[m.nodes + 16ull * n + 6 + c];
+            if (ch == EMPTY) continue;
+            if (ch & LEAF_BIT) {
+                if ((ch & ~LEAF_BIT) >= m.leaf_count) ch = n;  // out of range: refused below
+                else continue;
+            }
+            if (ch >= n) {
+                set_error("rq_scene_refit_mesh: mesh nodes are not in post-order");
+                return RQ_ERR_INVALID_INPUT;
+            }
+            hh = std::max(hh, h[ch] + 1);
+        }
+        h[n] = hh;
+        top = std::max(top, hh);
+    }
+    // Every leaf span and vertex index stays inside the mesh (the kernel reads them unchecked).
+    for (uint32_t l = 0; l < m.leaf_count; ++l) {
+        const uint64_t off = w[m.leaves + 2ull * l], cnt = w[m.leaves + 2ull * l + 1];
+        if (off + cnt > m.index_count) {
+            set_error("rq_scene_refit_mesh: leaf past the mesh's indices");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    for (uint32_t i = 0; i < m.index_count; ++i)
+        if (w[m.indices + i] >= m.vertex_count) {
+            set_error("rq_scene_refit_mesh: index past the mesh's vertices");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    RefitPlan plan;
+    plan.nodes = m.nodes;
+    plan.level_start.assign(top + 2, 0);
+    for (uint32_t n = 0; n < m.node_count; ++n) plan.level_start[h[n] + 1]++;
+    for (uint32_t l = 0; l <= top; ++l) plan.level_start[l + 1] += plan.level_start[l];
+    std::vector<uint32_t> order(m.node_count), fill(plan.level_start.begin(), plan.level_start.end() - 1);
+    for (uint32_t n = 0; n < m.node_count; ++n) order[fill[h[n]]++] = n;
+    if (!order.empty()) {
+        RF_TRY(hipMalloc(&plan.order, order.size() * 4), "hipMalloc refit plan");
+        RF_TRY(hipMemcpy(plan.order, order.data(), order.size() * 4, hipMemcpyHostToDevice), "hipMemcpy refit plan");
+    }
+    scene->refit_plans.push_back(plan);
+    out = &scene->refit_plans.back();
+    return RQ_OK;
+}
+}  // namespace
+
+extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const float* positions,
+                                         uint32_t vertex_count, void* stream) {
+    if (!scene || !positions || scene->kind != RQ_SCENE_GI) {
+        set_error("rq_scene_refit_mesh: needs a GI scene and device positions");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (mesh >= scene->mesh_nodes.size()) {
+        set_error("rq_scene_refit_mesh: no such mesh (refits need a scene made by rq_scene_build)");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (!scene->dev[0].ready) {
+        set_error("rq_scene_refit_mesh: the REF layout is not uploaded");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    int dev = 0;
+    RF_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != scene->device) {
+        set_error("rq_scene_refit_mesh: scene lives on another device");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::vector<uint32_t>& w = scene->words;
+    const uint32_t count = w[0];
+    uint32_t record = count;
+    for (uint32_t i = 0; i < count && record == count; ++i)
+        if (w[HEADER_WORDS + (size_t)i * INSTANCE_WORDS + 18] == scene->mesh_nodes[mesh]) record = i;
+    if (record == count) return RQ_OK;  // no instance draws this mesh: nothing packed to refit
+    MeshView m;
+    std::string err;
+    if (!mesh_view(w, record, m, err)) {
+        set_error("rq_scene_refit_mesh: " + err);
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (vertex_count != m.vertex_count) {
+        set_error("BVH refit needs finite positions of the same vertices");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    RefitPlan* plan = nullptr;
+    rq_status st = refit_plan(scene, m, plan);
+    if (st != RQ_OK) return st;
+    if (!scene->refit_flag) RF_TRY(hipMalloc(&scene->refit_flag, 4), "hipMalloc refit flag");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint32_t* words = reinterpret_cast<uint32_t*>(scene->dev[0].base);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(scene->refit_flag);
+    const uint32_t nf = 3u * vertex_count;
+    const dim3 grid(std::max<uint32_t>(1, std::min<uint32_t>((nf + 255) / 256, 2048)));
+    uint32_t bad = 0;
+    RF_TRY(hipMemsetAsync(flag, 0, 4, s), "refit flag");
+    hipLaunchKernelGGL(refitk::check_positions, grid, dim3(256), 0, s, positions, nf, flag);
+    RF_TRY(hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, s), "refit flag");
+    RF_TRY(hipStreamSynchronize(s), "refit check");
+    if (bad) {
+        set_error("BVH refit needs finite positions of the same vertices");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    hipLaunchKernelGGL(refitk::put_positions, grid, dim3(256), 0, s, words, m.positions, positions, nf);
+    refitk::Args a;
+    a.words = words;
+    a.nodes = m.nodes;
+    a.leaves = m.leaves;
+    a.positions = m.positions;
+    a.indices = m.indices;
+    a.order = reinterpret_cast<const uint32_t*>(plan->order);
+    for (size_t l = 0; l + 1 < plan->level_start.size(); ++l) {
+        a.begin = plan->level_start[l];
+        a.end = plan->level_start[l + 1];
+        if (a.end > a.begin)
+            hipLaunchKernelGGL(refitk::refit_level, dim3((a.end - a.begin + 255) / 256), dim3(256), 0, s, a);
+    }
+    RF_TRY(hipGetLastError(), "refit launch");
+    // The host words (the source of the other layouts and of the records) follow.
+    RF_TRY(hipMemcpyAsync(&w[m.positions], words + m.positions, (size_t)nf * 4, hipMemcpyDeviceToHost, s),
+           "refit positions");
+    RF_TRY(hipMemcpyAsync(&w[m.nodes], words + m.nodes, (size_t)m.node_count * NODE_WORDS * 4,
+                          hipMemcpyDeviceToHost, s),
+           "refit nodes");
+    RF_TRY(hipStreamSynchronize(s), "refit");
+    // Instance records over the new mesh bounds, then the top level placed over
+    // the previous one (refit while only boxes moved), as a repack would.
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t v = 0; v < vertex_count; ++v)
+        for (int x = 0; x < 3; ++x) {
+            const float p = bits_f(w[m.positions + 3ull * v + x]);
+            lo[x] = rq::mn(lo[x], p);
+            hi[x] = rq::mx(hi[x], p);
+        }
+    const size_t base = HEADER_WORDS + (size_t)count * INSTANCE_WORDS, reserve = top_reserve(count);
+    std::vector<uint32_t> head(w.begin(), w.begin() + base);
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        if (head[r + 18] == m.nodes) instance_box(&scene->models[16ull * i], lo, hi, &head[r + 12]);
+    }
+    top_place(head, HEADER_WORDS, INSTANCE_WORDS, count, w.data(), base + reserve, scene->top_force);
+    std::copy(head.begin(), head.end(), w.begin());
+    RF_TRY(hipMemcpyAsync(words, w.data(), head.size() * 4, hipMemcpyHostToDevice, s), "refit records");
+    RF_TRY(hipStreamSynchronize(s), "refit records");
+    scene_bounds(scene);
+    // The PLAIN and CW8 copies hold the old boxes and triangles: drop them; the
+    // next rq_scene_upload derives them again from the refit words.
+    for (int l = 1; l <= 2; ++l) {
+        if (scene->dev[l].base) (void)hipFree(scene->dev[l].base);
+        scene->dev[l] = DeviceLayout();
+    }
+    scene->plain = HostLayout();
+    scene->cw8 = HostLayout();
+    return RQ_OK;
+}

@@ -1497,6 +1497,9 @@ extern "C" void rq_scene_destroy(rq_scene* scene) {
     if (!scene) return;
     for (auto& d : scene->dev)
         if (d.base) (void)hipFree(d.base);
+    for (auto& p : scene->refit_plans)
+        if (p.order) (void)hipFree(p.order);
+    if (scene->refit_flag) (void)hipFree(scene->refit_flag);
     delete scene;
 }
 

@@ -99,6 +99,7 @@ _sig("rq_instance_bvh_refit", _i, _P, _u32, _P, _u32)
 _sig("rq_instance_bvh_place", _i, _P, _u32, _u32, _u32, _u32, _P, _u32, _i, _P)
 _sig("rq_scene_build", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_gi_words", _i, _P, _sz, ctypes.POINTER(_P))
+_sig("rq_scene_refit_mesh", _i, _P, _u32, _P, _u32, _P)
 _sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
@@ -499,6 +500,14 @@ class Scene:
         check(lib.rq_scene_stats(self._h, _ptr(s)))
         return dict(instances=int(s[0]), meshes=int(s[1]), triangles=int(s[2]), unique_triangles=int(s[3]),
                     blas_nodes=int(s[4]), top_root=int(s[5]), max_depth=int(s[6]))
+
+    def refit_mesh(self, mesh, positions, vertex_count=None, stream=None):
+        """GPU refit of mesh `mesh` to new positions (DeviceBuffer or device pointer of
+        vertex_count*3 floats): the BVH, its instances' boxes and the top level; PLAIN
+        and CW8 must be uploaded again afterwards. Synchronous."""
+        n = vertex_count if vertex_count is not None else positions.nbytes // 12
+        check(lib.rq_scene_refit_mesh(self._h, int(mesh), _dptr(positions), int(n),
+                                      stream.ptr if stream else None))
 
     def upload(self, layouts=("ref", "plain", "cw8")):
         mask = 0
