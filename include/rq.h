@@ -189,6 +189,13 @@ rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]);
 /* Uploads the requested layouts (bit 1<<rq_layout) to the current device. */
 rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
 rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes);
+/* Copies one part of an uploaded layout back to the host (no reference
+   counterpart: a diagnostic for checking device images). PLAIN / CW8 parts as
+   rq_scene_layout_image numbers them (0-4, 256-byte padded); REF / RS part 0
+   is the whole word buffer. *bytes = the part's size; nothing is copied when
+   `capacity` is smaller. */
+rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
+                                 size_t capacity, size_t* bytes);
 /* Host image of a relayout (PLAIN or CW8), built on demand without a device:
    part 0 instance records, 1 top-level nodes, 2 BLAS nodes, 3 triangles,
    4 position indices, 5 {top_on, top_root, instances, 0}. For inspection. */

@@ -144,6 +144,8 @@ extern "C" rq_status rq_scene_gi_words(const rq_scene* scene, const uint32_t** w
         set_error("rq_scene_gi_words: null argument");
         return RQ_ERR_INVALID_INPUT;
     }
+    rq_status st = sync_host_words(scene);
+    if (st != RQ_OK) return st;
     *words = scene->words.data();
     *word_count = scene->words.size();
     return RQ_OK;
@@ -166,6 +168,8 @@ extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, ui
     }
     HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
     std::string err;
+    rq_status st = sync_host_words(scene);
+    if (st != RQ_OK) return st;
     if (h.inst.empty() && scene->words[0] > 0) {
         if (scene->plain.inst.empty() && !build_plain(scene->words, scene->plain, err)) {
             set_error("plain relayout: " + err);

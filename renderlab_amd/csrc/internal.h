@@ -113,6 +113,8 @@ bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* so
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
+// Fetches the stale ranges of the host words from the device REF copy.
+rq_status sync_host_words(const rq_scene* scene);
 // Instance record words 12-17: the padded world box of a mesh box under a model
 // (src/gi_bvh.rs instance_words).
 void instance_box(const float model[16], const float lo[3], const float hi[3], uint32_t out[6]);
@@ -134,6 +136,9 @@ struct DeviceLayout {
     // word/element offsets into `base`, layout specific
     uint64_t off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // PLAIN and CW8: per distinct mesh (node word offset in the reference
+    // words, first node, first triangle), for in-place refreshes after a refit.
+    std::vector<uint32_t> mesh_base;
 };
 
 // One mesh's GPU refit plan: its node ids grouped by height (device), cached.
@@ -148,6 +153,7 @@ struct HostLayout {
     std::vector<uint32_t> inst;    // 32 words per instance
     std::vector<uint32_t> top;     // top-level wide nodes (32 words)
     std::vector<uint32_t> nodes;   // BLAS nodes (plain: 32 words, cw8: 20 words)
+    std::vector<uint32_t> mesh_base;  // (node word offset, first node, first triangle) per mesh
     std::vector<float> tris;       // TRI_FLOATS per triangle: a, b - a, c - a
     std::vector<uint32_t> tri_ids; // TRI_ID_WORDS per triangle: ia, ib, ic, slot, primitive, 0
     uint32_t top_root = 0;
@@ -180,6 +186,9 @@ struct rq_scene {
     std::vector<float> models;
     std::vector<uint32_t> mesh_nodes;
     int top_force = -1;
-    void* refit_flag = nullptr;  // device word: non-finite position seen
+    void* refit_scratch = nullptr;  // device: per-block bounds partials of a refit
+    // Word ranges (offset, count) where the device REF words are newer than
+    // `words` (a refit wrote them on the device); sync_host_words fetches them.
+    mutable std::vector<std::pair<uint64_t, uint64_t>> stale;
     int device = -1;
 };

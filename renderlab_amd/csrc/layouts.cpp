@@ -221,6 +221,11 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
     }
     instance_records(words, meshes, instance_mesh, roots, out.inst);
     copy_top(words, out);
+    for (const MeshMap& m : meshes) {
+        out.mesh_base.push_back(m.nodes_at);
+        out.mesh_base.push_back(m.node_base);
+        out.mesh_base.push_back(m.tri_base);
+    }
     return true;
 }
 

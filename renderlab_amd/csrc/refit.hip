@@ -124,16 +124,107 @@ __global__ __launch_bounds__(256) void refit_level(Args a) {
     for (int w = 0; w < 16; ++w) node[w] = out[w];
 }
 
-// New positions (3 floats a vertex) into the mesh's position words; any
-// non-finite value raises *bad and nothing is written by the later launches.
-__global__ __launch_bounds__(256) void check_positions(const float* p, uint32_t n, uint32_t* bad) {
-    uint32_t any = 0;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) any |= !isfinite(p[i]);
-    if (any) atomicOr(bad, 1u);
+constexpr uint32_t BOUNDS_BLOCKS = 512;
+
+// Bounds of the new positions (per block: lo xyz, hi xyz, non-finite flag);
+// bounds_finish folds the partials into scratch[BOUNDS_BLOCKS * 8 ...].
+// Min and max of finite floats do not depend on the order they are taken in
+// but for the sign of a zero, which no instance box word depends on.
+__global__ __launch_bounds__(256) void bounds_partial(const float* p, uint32_t vertices, float* part) {
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    uint32_t bad = 0;
+    for (uint32_t v = blockIdx.x * 256u + threadIdx.x; v < vertices; v += gridDim.x * 256u)
+        for (int x = 0; x < 3; ++x) {
+            const float f = p[3ull * v + x];
+            bad |= !isfinite(f);
+            lo[x] = fminf(lo[x], f);
+            hi[x] = fmaxf(hi[x], f);
+        }
+    __shared__ float slo[3][256], shi[3][256];
+    __shared__ uint32_t sbad[256];
+    for (int x = 0; x < 3; ++x) {
+        slo[x][threadIdx.x] = lo[x];
+        shi[x][threadIdx.x] = hi[x];
+    }
+    sbad[threadIdx.x] = bad;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int x = 0; x < 3; ++x) {
+                slo[x][threadIdx.x] = fminf(slo[x][threadIdx.x], slo[x][threadIdx.x + w]);
+                shi[x][threadIdx.x] = fmaxf(shi[x][threadIdx.x], shi[x][threadIdx.x + w]);
+                sbad[threadIdx.x] |= sbad[threadIdx.x + w];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) {
+        float* o = part + 8ull * blockIdx.x;
+        const uint32_t t = threadIdx.x;
+        o[t] = t < 3 ? slo[t][0] : t < 6 ? shi[t - 3][0] : __uint_as_float(t == 6 ? sbad[0] : 0u);
+    }
+}
+
+__global__ __launch_bounds__(64) void bounds_finish(float* part, uint32_t blocks) {
+    const uint32_t t = threadIdx.x;
+    if (t >= 7) return;
+    float acc = t < 3 ? __builtin_inff() : -__builtin_inff();
+    uint32_t bad = 0;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const float f = part[8ull * b + t];
+        if (t < 3) acc = fminf(acc, f);
+        else if (t < 6) acc = fmaxf(acc, f);
+        else bad |= __float_as_uint(f);
+    }
+    part[8ull * BOUNDS_BLOCKS + t] = t < 6 ? acc : __uint_as_float(bad);
 }
 
 __global__ __launch_bounds__(256) void put_positions(uint32_t* words, uint32_t at, const float* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) words[at + i] = fb(p[i]);
+}
+
+// PLAIN refresh of one refit mesh: each 32-word node's child boxes are the
+// unfused decodes (origin + step * q) of its reference node, tags untouched;
+// each 36-byte triangle record is vertex a and the edges b - a, c - a.
+__global__ __launch_bounds__(256) void plain_nodes(const uint32_t* words, uint32_t nodes, uint32_t count,
+                                                   uint32_t* out) {
+    const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+    if (n >= count) return;
+    const uint32_t* w = words + nodes + 16ull * n;
+    uint32_t* o = out + 32ull * n;
+    for (int slot = 0; slot < 4; ++slot) {
+        if (w[6 + slot] == EMPTY) continue;
+        for (int x = 0; x < 3; ++x) {
+            const float org = bf(w[x]), s = bf(w[3 + x]);
+            const float ql = (float)((w[10 + x] >> (8 * slot)) & 0xffu);
+            const float qh = (float)((w[13 + x] >> (8 * slot)) & 0xffu);
+            o[x * 4 + slot] = fb(org + s * ql);
+            o[12 + x * 4 + slot] = fb(org + s * qh);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void plain_tris(const uint32_t* words, uint32_t indices, uint32_t positions,
+                                                  uint32_t count, float* out) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= count) return;
+    float p[3][3];
+    for (int v = 0; v < 3; ++v) {
+        const uint32_t vi = words[indices + 3ull * k + v];
+        for (int x = 0; x < 3; ++x) p[v][x] = bf(words[positions + 3ull * vi + x]);
+    }
+    float* o = out + (size_t)TRI_FLOATS * k;
+    for (int x = 0; x < 3; ++x) o[x] = p[0][x];
+    for (int e = 1; e < 3; ++e)
+        for (int x = 0; x < 3; ++x) o[3 * e + x] = p[e][x] - p[0][x];
+}
+
+// Instance record words 0-17 (rows and world box) into the PLAIN records.
+__global__ __launch_bounds__(256) void plain_records(const uint32_t* words, uint32_t count, uint32_t* inst) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= count * 18u) return;
+    const uint32_t i = t / 18u, k = t % 18u;
+    inst[(size_t)i * INSTANCE_WORDS + k] = words[HEADER_WORDS + (size_t)i * INSTANCE_WORDS + k];
 }
 
 }  // namespace refitk
@@ -210,6 +301,26 @@ rq_status refit_plan(rq_scene* scene, const MeshView& m, RefitPlan*& out) {
 }
 }  // namespace
 
+namespace rq {
+rq_status sync_host_words(const rq_scene* scene) {
+    if (!scene || scene->stale.empty()) return RQ_OK;
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
+    uint32_t* host = const_cast<uint32_t*>(scene->words.data());
+    for (const auto& r : scene->stale)
+        RF_TRY(hipMemcpy(host + r.first, words + r.first, r.second * 4, hipMemcpyDeviceToHost), "fetch refit words");
+    scene->stale.clear();
+    return RQ_OK;
+}
+}  // namespace rq
+
+namespace {
+void mark_stale(rq_scene* scene, uint64_t at, uint64_t count) {
+    for (const auto& r : scene->stale)
+        if (r.first == at && r.second == count) return;
+    scene->stale.emplace_back(at, count);
+}
+}  // namespace
+
 extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const float* positions,
                                          uint32_t vertex_count, void* stream) {
     if (!scene || !positions || scene->kind != RQ_SCENE_GI) {
@@ -230,7 +341,7 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
         set_error("rq_scene_refit_mesh: scene lives on another device");
         return RQ_ERR_INVALID_INPUT;
     }
-    std::vector<uint32_t>& w = scene->words;
+    std::vector<uint32_t>& w = scene->words;  // header, records, tags and indices are current on the host
     const uint32_t count = w[0];
     uint32_t record = count;
     for (uint32_t i = 0; i < count && record == count; ++i)
@@ -242,29 +353,45 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
         set_error("rq_scene_refit_mesh: " + err);
         return RQ_ERR_INVALID_INPUT;
     }
-    if (vertex_count != m.vertex_count) {
+    if (vertex_count != m.vertex_count || vertex_count == 0) {
         set_error("BVH refit needs finite positions of the same vertices");
         return RQ_ERR_INVALID_INPUT;
+    }
+    DeviceLayout& d = scene->dev[1];
+    size_t k = 0;
+    if (d.ready) {
+        while (k + 3 <= d.mesh_base.size() && d.mesh_base[k] != m.nodes) k += 3;
+        if (k + 3 > d.mesh_base.size()) {
+            set_error("rq_scene_refit_mesh: the PLAIN layout does not hold this mesh");
+            return RQ_ERR_INVALID_INPUT;
+        }
     }
     RefitPlan* plan = nullptr;
     rq_status st = refit_plan(scene, m, plan);
     if (st != RQ_OK) return st;
-    if (!scene->refit_flag) RF_TRY(hipMalloc(&scene->refit_flag, 4), "hipMalloc refit flag");
+    if (!scene->refit_scratch)
+        RF_TRY(hipMalloc(&scene->refit_scratch, (refitk::BOUNDS_BLOCKS + 1) * 8 * sizeof(float)), "hipMalloc refit");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     uint32_t* words = reinterpret_cast<uint32_t*>(scene->dev[0].base);
-    uint32_t* flag = reinterpret_cast<uint32_t*>(scene->refit_flag);
-    const uint32_t nf = 3u * vertex_count;
-    const dim3 grid(std::max<uint32_t>(1, std::min<uint32_t>((nf + 255) / 256, 2048)));
-    uint32_t bad = 0;
-    RF_TRY(hipMemsetAsync(flag, 0, 4, s), "refit flag");
-    hipLaunchKernelGGL(refitk::check_positions, grid, dim3(256), 0, s, positions, nf, flag);
-    RF_TRY(hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, s), "refit flag");
-    RF_TRY(hipStreamSynchronize(s), "refit check");
+    float* part = reinterpret_cast<float*>(scene->refit_scratch);
+    // 1. bounds of the new positions, and whether they are all finite
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((vertex_count + 255) / 256, refitk::BOUNDS_BLOCKS));
+    hipLaunchKernelGGL(refitk::bounds_partial, dim3(blocks), dim3(256), 0, s, positions, vertex_count, part);
+    hipLaunchKernelGGL(refitk::bounds_finish, dim3(1), dim3(64), 0, s, part, blocks);
+    float b[8];
+    RF_TRY(hipMemcpyAsync(b, part + 8ull * refitk::BOUNDS_BLOCKS, 7 * sizeof(float), hipMemcpyDeviceToHost, s),
+           "refit bounds");
+    RF_TRY(hipStreamSynchronize(s), "refit bounds");
+    uint32_t bad;
+    std::memcpy(&bad, &b[6], 4);
     if (bad) {
         set_error("BVH refit needs finite positions of the same vertices");
         return RQ_ERR_INVALID_INPUT;
     }
-    hipLaunchKernelGGL(refitk::put_positions, grid, dim3(256), 0, s, words, m.positions, positions, nf);
+    // 2. positions into the words, then the nodes bottom-up, one launch per height
+    const uint32_t nf = 3u * vertex_count;
+    hipLaunchKernelGGL(refitk::put_positions, dim3(std::min<uint32_t>((nf + 255) / 256, 2048)), dim3(256), 0, s,
+                       words, m.positions, positions, nf);
     refitk::Args a;
     a.words = words;
     a.nodes = m.nodes;
@@ -279,41 +406,43 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
             hipLaunchKernelGGL(refitk::refit_level, dim3((a.end - a.begin + 255) / 256), dim3(256), 0, s, a);
     }
     RF_TRY(hipGetLastError(), "refit launch");
-    // The host words (the source of the other layouts and of the records) follow.
-    RF_TRY(hipMemcpyAsync(&w[m.positions], words + m.positions, (size_t)nf * 4, hipMemcpyDeviceToHost, s),
-           "refit positions");
-    RF_TRY(hipMemcpyAsync(&w[m.nodes], words + m.nodes, (size_t)m.node_count * NODE_WORDS * 4,
-                          hipMemcpyDeviceToHost, s),
-           "refit nodes");
-    RF_TRY(hipStreamSynchronize(s), "refit");
-    // Instance records over the new mesh bounds, then the top level placed over
-    // the previous one (refit while only boxes moved), as a repack would.
-    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-    for (uint32_t v = 0; v < vertex_count; ++v)
-        for (int x = 0; x < 3; ++x) {
-            const float p = bits_f(w[m.positions + 3ull * v + x]);
-            lo[x] = rq::mn(lo[x], p);
-            hi[x] = rq::mx(hi[x], p);
-        }
+    mark_stale(scene, m.positions, nf);
+    mark_stale(scene, m.nodes, (uint64_t)m.node_count * NODE_WORDS);
+    // 3. instance records over the new mesh bounds, then the top level placed
+    //    over the previous one (refit while only boxes moved), as a repack would
     const size_t base = HEADER_WORDS + (size_t)count * INSTANCE_WORDS, reserve = top_reserve(count);
     std::vector<uint32_t> head(w.begin(), w.begin() + base);
     for (uint32_t i = 0; i < count; ++i) {
         const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
-        if (head[r + 18] == m.nodes) instance_box(&scene->models[16ull * i], lo, hi, &head[r + 12]);
+        if (head[r + 18] == m.nodes) instance_box(&scene->models[16ull * i], b, b + 3, &head[r + 12]);
     }
     top_place(head, HEADER_WORDS, INSTANCE_WORDS, count, w.data(), base + reserve, scene->top_force);
     std::copy(head.begin(), head.end(), w.begin());
     RF_TRY(hipMemcpyAsync(words, w.data(), head.size() * 4, hipMemcpyHostToDevice, s), "refit records");
-    RF_TRY(hipStreamSynchronize(s), "refit records");
     scene_bounds(scene);
-    // The PLAIN and CW8 copies hold the old boxes and triangles: drop them; the
-    // next rq_scene_upload derives them again from the refit words.
-    for (int l = 1; l <= 2; ++l) {
-        if (scene->dev[l].base) (void)hipFree(scene->dev[l].base);
-        scene->dev[l] = DeviceLayout();
+    // 4. PLAIN follows in place (same nodes, same triangle order); CW8, whose
+    //    collapse depends on the boxes, is dropped and derived again on upload
+    if (d.ready) {
+        char* pb = reinterpret_cast<char*>(d.base);
+        uint32_t* pnodes = reinterpret_cast<uint32_t*>(pb + d.off[2]) + 32ull * d.mesh_base[k + 1];
+        float* ptris = reinterpret_cast<float*>(pb + d.off[3]) + (size_t)TRI_FLOATS * d.mesh_base[k + 2];
+        const uint32_t tris = m.index_count / 3;
+        hipLaunchKernelGGL(refitk::plain_nodes, dim3((m.node_count + 255) / 256), dim3(256), 0, s, words, m.nodes,
+                           m.node_count, pnodes);
+        if (tris)
+            hipLaunchKernelGGL(refitk::plain_tris, dim3((tris + 255) / 256), dim3(256), 0, s, words, m.indices,
+                               m.positions, tris, ptris);
+        hipLaunchKernelGGL(refitk::plain_records, dim3((count * 18 + 255) / 256), dim3(256), 0, s, words, count,
+                           reinterpret_cast<uint32_t*>(pb + d.off[0]));
+        if (d.count[1] && d.count[3])
+            RF_TRY(hipMemcpyAsync(pb + d.off[1], words + w[1], (size_t)d.count[3] * 4, hipMemcpyDeviceToDevice, s),
+                   "refit plain top");
+        RF_TRY(hipGetLastError(), "refit plain launch");
     }
-    scene->plain = HostLayout();
+    scene->plain = HostLayout();  // host images are derived again when asked for
     scene->cw8 = HostLayout();
+    if (scene->dev[2].base) (void)hipFree(scene->dev[2].base);
+    scene->dev[2] = DeviceLayout();
+    RF_TRY(hipStreamSynchronize(s), "refit");  // `head` is copied from before it goes
     return RQ_OK;
 }

@@ -1442,13 +1442,16 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
     rq_status s = ensure_ctx();
     if (s != RQ_OK) return s;
     std::string err;
-    if ((layout_mask & 6u) && scene->plain.inst.empty() && scene->words[0] > 0) {
+    if ((s = sync_host_words(scene)) != RQ_OK) return s;
+    const bool want_plain = ((layout_mask & 2u) && !scene->dev[1].ready) ||
+                            ((layout_mask & 4u) && !scene->dev[2].ready && scene->cw8.inst.empty());
+    if (want_plain && scene->plain.inst.empty() && scene->words[0] > 0) {
         if (!build_plain(scene->words, scene->plain, err)) {
             set_error("plain relayout: " + err);
             return RQ_ERR_INVALID_INPUT;
         }
     }
-    if ((layout_mask & 4u) && scene->cw8.inst.empty() && scene->words[0] > 0) {
+    if ((layout_mask & 4u) && !scene->dev[2].ready && scene->cw8.inst.empty() && scene->words[0] > 0) {
         if (!build_cw8(scene->words, scene->plain, scene->cw8, err)) {
             set_error("cw8 relayout: " + err);
             return RQ_ERR_INVALID_INPUT;
@@ -1477,10 +1480,40 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             d.off[2] = at; if ((s = upload(d.base, at, h.nodes)) != RQ_OK) return s;
             d.off[3] = at; if ((s = upload(d.base, at, h.tris)) != RQ_OK) return s;
             d.off[4] = at; if ((s = upload(d.base, at, h.tri_ids)) != RQ_OK) return s;
+            d.count[0] = h.instance_count;
+            d.count[1] = h.top_on ? 1u : 0u;
+            d.count[2] = h.top_root;
+            d.count[3] = (uint32_t)h.top.size();
+            d.mesh_base = scene->plain.mesh_base;
         }
         d.ready = true;
     }
     scene->device = g_ctx.device;
+    return RQ_OK;
+}
+
+extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
+                                            size_t capacity, size_t* bytes) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout > 3 || part > 4 || (capacity && !dst)) {
+        set_error("rq_scene_download_part: bad argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    const DeviceLayout& d = scene->dev[layout];
+    if (!d.ready) {
+        set_error("rq_scene_download_part: layout not uploaded");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    size_t at = 0, n = d.bytes;
+    if (layout == RQ_LAYOUT_PLAIN_F32 || layout == RQ_LAYOUT_CW8_Q8) {
+        at = d.off[part];
+        n = (part < 4 ? d.off[part + 1] : d.bytes) - at;
+    } else if (part != 0) {
+        set_error("rq_scene_download_part: the word layouts have one part");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    *bytes = n;
+    if (capacity < n) return RQ_OK;  // size query
+    HIP_TRY(hipMemcpy(dst, reinterpret_cast<const char*>(d.base) + at, n, hipMemcpyDeviceToHost), "download part");
     return RQ_OK;
 }
 
@@ -1499,7 +1532,7 @@ extern "C" void rq_scene_destroy(rq_scene* scene) {
         if (d.base) (void)hipFree(d.base);
     for (auto& p : scene->refit_plans)
         if (p.order) (void)hipFree(p.order);
-    if (scene->refit_flag) (void)hipFree(scene->refit_flag);
+    if (scene->refit_scratch) (void)hipFree(scene->refit_scratch);
     delete scene;
 }
 
@@ -1539,16 +1572,15 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.top_on = scene->words[1] != 0;
         v.top_root = scene->words[2];
     } else {
-        const HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
         char* b = reinterpret_cast<char*>(d.base);
         v.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
         v.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
         v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
         v.tris = reinterpret_cast<const float*>(b + d.off[3]);
         v.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
-        v.n_inst = h.instance_count;
-        v.top_on = h.top_on;
-        v.top_root = h.top_root;
+        v.n_inst = d.count[0];
+        v.top_on = d.count[1] != 0;
+        v.top_root = d.count[2];
     }
     return v;
 }

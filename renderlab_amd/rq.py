@@ -100,6 +100,7 @@ _sig("rq_instance_bvh_place", _i, _P, _u32, _u32, _u32, _u32, _P, _u32, _i, _P)
 _sig("rq_scene_build", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_gi_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_scene_refit_mesh", _i, _P, _u32, _P, _u32, _P)
+_sig("rq_scene_download_part", _i, _P, _i, _u32, _P, _sz, ctypes.POINTER(_sz))
 _sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
@@ -525,6 +526,15 @@ class Scene:
             return np.zeros(0, np.uint32)
         buf = (ctypes.c_uint32 * (n.value // 4)).from_address(p.value)
         return np.frombuffer(buf, np.uint32).copy()
+
+    def download_part(self, layout, part):
+        """An uploaded layout part, copied back from the device as u32 words (padded)."""
+        n = _sz()
+        lay = LAYOUTS.get(layout, layout)
+        check(lib.rq_scene_download_part(self._h, lay, int(part), None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value // 4, np.uint32)
+        check(lib.rq_scene_download_part(self._h, lay, int(part), _ptr(out), n.value, ctypes.byref(n)))
+        return out
 
     def device_bytes(self, layout):
         n = _sz()

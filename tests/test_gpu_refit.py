@@ -2,7 +2,8 @@
 near_field::refit_bvh (rq_bvh_refit, whose restatement the reference's own refit test pins),
 the instance boxes a fresh pack of the new positions computes, and instance_bvh::place over
 the previous words. The whole packed buffer after a GPU refit must equal that CPU result word
-for word; traversal over it (REF on the refit device words; PLAIN and CW8 uploaded again) must
+for word, and the PLAIN device image refreshed in place must equal a host relayout of those
+words; traversal over them (REF and PLAIN refit on the device, CW8 uploaded again) must
 equal the oracle's bit for bit, and the oracle must agree with the f64 brute force on hit or
 miss over the moved geometry (the refit boxes enclose what they must)."""
 import numpy as np
@@ -138,17 +139,24 @@ def test_refit_equals_cpu_refit_and_repack(setup):
         old = got
 
 
-def test_refit_then_relayout(setup):
+def test_refit_keeps_plain_in_step(setup):
     g, meshes, inst = setup
     scene = rq.Scene.build(meshes, inst)
     scene.upload(("ref", "plain", "cw8"))
-    new = pose(meshes[0][0], 3, g)
-    print("case: refit then PLAIN/CW8 upload", flush=True)
-    scene.refit_mesh(0, to_device(new), len(new))
-    assert scene.device_bytes("plain") == 0 and scene.device_bytes("cw8") == 0
-    scene.upload(("plain", "cw8"))
-    words = scene.words
-    check_traversal(scene, words, rays_at(words, inst, g), ("ref", "plain", "cw8"))
+    for k in (3, 4):
+        new = pose(meshes[0][0], k, g)
+        print(f"case: refit pose {k} with PLAIN resident", flush=True)
+        scene.refit_mesh(0, to_device(new), len(new))
+        assert scene.device_bytes("plain") > 0 and scene.device_bytes("cw8") == 0
+        # the PLAIN device image refreshed in place equals a host relayout of the refit words
+        for part in range(5):
+            host = scene.layout_image("plain", part)
+            dev = scene.download_part("plain", part)
+            assert np.array_equal(dev[:host.size], host), f"PLAIN part {part}"
+        words = scene.words
+        check_traversal(scene, words, rays_at(words, inst, g), ("ref", "plain"))
+    scene.upload(("cw8",))
+    check_traversal(scene, words, rays_at(words, inst, g), ("cw8",))
 
 
 def test_refit_rejects_bad_input(setup):
