@@ -149,6 +149,14 @@ rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_in
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
    would upload, src/gi_cache.rs:4361). */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* rq_scene_build with every mesh's BVH built on the GPU: the same packed words
+   (the SAH BVH4 of near_field::build_bvh_sah, crates/asset/src/near_field.rs:
+   172-430, with its median fallback past depth 15; layout, instance records
+   and top level as rq_scene_build), built level by level for all meshes at
+   once, and the REF layout left uploaded on the current device. Same inputs
+   and errors as rq_scene_build. Synchronizes `stream`. */
+rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                             uint32_t instance_count, int top_level, rq_scene** out, void* stream);
 /* GPU refit of one mesh of a built scene (rq_scene_build) whose REF layout is
    uploaded, for a re-skinned pose: replaces the skin_refit path of
    src/gi_cache.rs:2843-2863 -- near_field::refit_bvh (crates/asset/src/

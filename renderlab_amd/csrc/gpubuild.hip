@@ -1,0 +1,1296 @@
+// GPU build of every mesh's BVH4 for a whole scene, word for word the tree the
+// host builder makes (near_field::build_bvh_sah, crates/asset/src/
+// near_field.rs:172-430), packed into the gi_geometry words as
+// rq_scene_build packs it (src/gi_bvh.rs:33-274, src/gi_cache.rs:4388-4479).
+//
+// The host builder recurses: a group of more than LEAF_TRIANGLES triangles is
+// split twice (binned SAH, 16 bins, or the median when SAH finds no plane) and
+// each of the 2-4 groups recurses; nodes are numbered in post-order, leaves in
+// creation order. Here every mesh is built at once, one BVH4 level at a time:
+// each level's groups are split in two passes over all groups together (a
+// workgroup per group up to 2048 triangles; chunked with global bins above;
+// median splits through a segmented radix sort on (total-order key, triangle)),
+// partitions stable, so each group's triangle order is the host's. After the
+// last level: subtree sizes bottom-up give the post-order ids, a scan over
+// leaf starts gives leaf ids (leaves are contiguous in the final permutation,
+// left to right), and the node boxes come from the refit kernel applied level
+// by level from the deepest -- the host builder's child_bounds and
+// quantized_node on the same children. A mesh whose tree is deeper than
+// MAX_SAH_DEPTH is built again with median splits only (near_field.rs:184-190).
+//
+// Exactness notes: centroids, bins, areas and costs use the host's operations
+// in the host's order (no contraction; correctly rounded division); min and
+// max of centroids and boxes are taken with atomics on total-order keys,
+// which differ from the host's sequential min/max only in the sign of a zero,
+// and no decision or stored word depends on that sign (binning subtracts the
+// bound, the extent of all-zero coordinates is +0 either way, costs compare
+// equal). Duplicate index triplets stay in ascending source order through
+// every stable partition and (key, triangle) sort, so the primitive table is
+// the permutation itself (what map_primitives, src/gi_bvh.rs:108-136, finds).
+#include <hip/hip_runtime.h>
+#include <hipcub/device/device_scan.hpp>
+#include <hipcub/device/device_segmented_radix_sort.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <thread>
+
+#include "bvh_device.h"
+#include "internal.h"
+
+namespace rq {
+namespace gbuild {
+
+using bvhdev::bf;
+using bvhdev::fb;
+
+constexpr uint32_t BINS = 16;
+constexpr uint32_t SMALL_MAX = 2048;  // triangles a workgroup splits on its own
+constexpr uint32_t CHUNK = 4096;      // triangles per workgroup in the chunked path
+constexpr uint32_t MODE_SAH = 0, MODE_MEDIAN = 1;
+
+struct MeshDev {
+    uint32_t tbase, tcount, vbase, vcount;
+    uint32_t mode, keep, node_count, leaf_count;  // counts: set once the layout is known
+    uint32_t nodes, leaves, positions, indices, prims, pad2, pad3, pad4;
+};
+
+struct Seg {
+    uint32_t start, count, mesh, out;
+};
+
+struct MedianReq {
+    uint32_t start, count, mesh, out, axis, pad0, pad1, pad2;
+};
+
+struct Chunk {
+    uint32_t seg, start, count, pad;
+};
+
+__device__ __forceinline__ int32_t okey(float f) {
+    const int32_t k = __float_as_int(f);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+__device__ __forceinline__ float okey_inv(int32_t k) { return __int_as_float(k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1)); }
+__device__ __forceinline__ uint32_t sat_index(float f, uint32_t cap) {
+    if (!(f > 0.0f)) return 0;
+    if (f >= (float)cap) return cap;
+    return (uint32_t)f;
+}
+// near_field.rs:329-331: the bin of a centroid coordinate.
+__device__ __forceinline__ uint32_t bin_of(float c, float lo, float extent) {
+    const float f = (c - lo) / extent * (float)BINS;
+    return min(sat_index(f, 0xffffffffu), BINS - 1);
+}
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float e[3];
+    for (int i = 0; i < 3; ++i) e[i] = bvhdev::mx(hi[i] - lo[i], 0.0f);
+    return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+}
+
+// near_field.rs:337-360: the sweep over one group's bins (counts and box keys
+// laid out [axis][bin] and [axis][bin][lo xyz, hi xyz]).
+__device__ inline bool sah_sweep(const uint32_t* cnt, const int32_t* box, const bool valid[3], uint32_t size,
+                                 int& best_axis, uint32_t& best_plane) {
+    bool found = false;
+    float best_cost = 0.0f;
+    for (int axis = 0; axis < 3; ++axis) {
+        if (!valid[axis]) continue;
+        float right_cost[BINS];
+        float lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = __builtin_inff();
+            hi[i] = -__builtin_inff();
+        }
+        uint32_t n = 0;
+        for (uint32_t b = BINS - 1; b >= 1; --b) {
+            const int32_t* bb = box + (axis * BINS + b) * 6;
+            for (int i = 0; i < 3; ++i) {
+                lo[i] = bvhdev::mn(lo[i], okey_inv(bb[i]));
+                hi[i] = bvhdev::mx(hi[i], okey_inv(bb[3 + i]));
+            }
+            n += cnt[axis * BINS + b];
+            right_cost[b] = half_area(lo, hi) * (float)n;
+        }
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = __builtin_inff();
+            hi[i] = -__builtin_inff();
+        }
+        n = 0;
+        for (uint32_t b = 0; b < BINS - 1; ++b) {
+            const int32_t* bb = box + (axis * BINS + b) * 6;
+            for (int i = 0; i < 3; ++i) {
+                lo[i] = bvhdev::mn(lo[i], okey_inv(bb[i]));
+                hi[i] = bvhdev::mx(hi[i], okey_inv(bb[3 + i]));
+            }
+            n += cnt[axis * BINS + b];
+            if (n == 0 || n == size) continue;
+            const float cost = half_area(lo, hi) * (float)n + right_cost[b + 1];
+            if (!found || cost < best_cost) {
+                found = true;
+                best_cost = cost;
+                best_axis = axis;
+                best_plane = b;
+            }
+        }
+    }
+    return found;
+}
+
+// near_field.rs:367-372: the longest centroid extent, ties to the later axis.
+__device__ __forceinline__ uint32_t median_axis(const float extent[3]) {
+    uint32_t axis = 0;
+    for (uint32_t a = 1; a < 3; ++a)
+        if (okey(extent[a]) >= okey(extent[axis])) axis = a;
+    return axis;
+}
+
+__device__ __forceinline__ void ask_median(MedianReq* med, uint32_t* med_count, const Seg& s, uint32_t axis) {
+    const uint32_t slot = atomicAdd(med_count, 1u);
+    MedianReq r;
+    r.start = s.start;
+    r.count = s.count;
+    r.mesh = s.mesh;
+    r.out = s.out;
+    r.axis = axis;
+    r.pad0 = r.pad1 = r.pad2 = 0;
+    med[slot] = r;
+}
+
+// ---------------------------------------------------------------- set-up
+__global__ __launch_bounds__(256) void prep_tris(const MeshDev* meshes, uint32_t mesh_count, const float* pos,
+                                                 const uint32_t* idx, uint32_t tris, float* cent, float* box,
+                                                 uint32_t* perm) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= tris) return;
+    uint32_t lo = 0, hi = mesh_count;  // last mesh with tbase <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (meshes[mid].tbase <= t) lo = mid;
+        else hi = mid;
+    }
+    const float* p = pos + 3ull * meshes[lo].vbase;
+    const float* a = p + 3ull * idx[3ull * t];
+    const float* b = p + 3ull * idx[3ull * t + 1];
+    const float* c = p + 3ull * idx[3ull * t + 2];
+    for (int x = 0; x < 3; ++x) {  // near_field.rs:213-219
+        float s = a[x] + b[x];
+        s = s + c[x];
+        cent[3ull * t + x] = s / 3.0f;
+        box[6ull * t + x] = bvhdev::mn(bvhdev::mn(a[x], b[x]), c[x]);
+        box[6ull * t + 3 + x] = bvhdev::mx(bvhdev::mx(a[x], b[x]), c[x]);
+    }
+    perm[t] = t;
+}
+
+__global__ __launch_bounds__(256) void fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) p[i] = v;
+}
+
+// Level-0 groups: one per mesh, all its triangles in source order.
+__global__ __launch_bounds__(256) void root_groups(const MeshDev* meshes, uint32_t mesh_count, uint32_t* gstart,
+                                                   uint32_t* gcount, uint32_t* gmesh) {
+    const uint32_t m = blockIdx.x * 256u + threadIdx.x;
+    if (m >= mesh_count) return;
+    gstart[m] = meshes[m].tbase;
+    gcount[m] = meshes[m].tcount;
+    gmesh[m] = m;
+}
+
+// ---------------------------------------------------------------- classify
+__global__ __launch_bounds__(256) void node_flags(const uint32_t* gcount, uint32_t n, uint32_t* flag) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g < n) flag[g] = gcount[g] > LEAF_TRIANGLES ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void make_nodes(const uint32_t* gstart, const uint32_t* gcount, const uint32_t* gmesh,
+                                                  uint32_t* gnode, uint32_t gbase, uint32_t ng, const uint32_t* noff,
+                                                  uint32_t nbase, uint32_t level, uint32_t* node_group,
+                                                  uint32_t* node_mesh, uint32_t* node_level, Seg* segs) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= ng) return;
+    const uint32_t gi = gbase + g;
+    if (gcount[gi] <= LEAF_TRIANGLES) {
+        gnode[gi] = EMPTY;
+        return;
+    }
+    const uint32_t r = noff[g], k = nbase + r;
+    gnode[gi] = k;
+    node_group[k] = gi;
+    node_mesh[k] = gmesh[gi];
+    node_level[k] = level;
+    Seg s;
+    s.start = gstart[gi];
+    s.count = gcount[gi];
+    s.mesh = gmesh[gi];
+    s.out = 3 * r;
+    segs[r] = s;
+}
+
+// Round 1: the halves of each node's group that still hold too many triangles.
+__global__ __launch_bounds__(256) void round1_segs(const uint32_t* gstart, const uint32_t* gcount, const uint32_t* gmesh,
+                                                   const uint32_t* node_group, uint32_t nbase, uint32_t nn,
+                                                   const uint32_t* split_left, Seg* segs, uint32_t* seg_count) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= nn) return;
+    const uint32_t gi = node_group[nbase + r];
+    const uint32_t start = gstart[gi], count = gcount[gi], l0 = split_left[3 * r];
+    const uint32_t hs[2] = {start, start + l0}, hc[2] = {l0, count - l0};
+    for (int h = 0; h < 2; ++h)
+        if (hc[h] > LEAF_TRIANGLES) {
+            Seg s;
+            s.start = hs[h];
+            s.count = hc[h];
+            s.mesh = gmesh[gi];
+            s.out = 3 * r + 1 + h;
+            segs[atomicAdd(seg_count, 1u)] = s;
+        }
+}
+
+__global__ __launch_bounds__(256) void child_counts(const uint32_t* gcount, const uint32_t* node_group, uint32_t nbase,
+                                                    uint32_t nn, const uint32_t* split_left, uint32_t* cc) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= nn) return;
+    const uint32_t count = gcount[node_group[nbase + r]], l0 = split_left[3 * r];
+    cc[r] = (l0 > LEAF_TRIANGLES ? 2u : 1u) + (count - l0 > LEAF_TRIANGLES ? 2u : 1u);
+}
+
+// near_field.rs:289-305: the node's children, halves split once more in order.
+__global__ __launch_bounds__(256) void emit_children(uint32_t* gstart, uint32_t* gcount, uint32_t* gmesh,
+                                                     const uint32_t* node_group, uint32_t nbase, uint32_t nn,
+                                                     const uint32_t* split_left, const uint32_t* coff, uint32_t gnext,
+                                                     uint32_t* child_first, uint32_t* child_count) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= nn) return;
+    const uint32_t k = nbase + r, gi = node_group[k];
+    const uint32_t start = gstart[gi], count = gcount[gi], mesh = gmesh[gi], l0 = split_left[3 * r];
+    const uint32_t hs[2] = {start, start + l0}, hc[2] = {l0, count - l0};
+    uint32_t at = gnext + coff[r], n = 0;
+    for (int h = 0; h < 2; ++h) {
+        if (hc[h] > LEAF_TRIANGLES) {
+            const uint32_t l1 = split_left[3 * r + 1 + h];
+            gstart[at + n] = hs[h];
+            gcount[at + n] = l1;
+            gmesh[at + n] = mesh;
+            ++n;
+            gstart[at + n] = hs[h] + l1;
+            gcount[at + n] = hc[h] - l1;
+            gmesh[at + n] = mesh;
+            ++n;
+        } else {
+            gstart[at + n] = hs[h];
+            gcount[at + n] = hc[h];
+            gmesh[at + n] = mesh;
+            ++n;
+        }
+    }
+    child_first[k] = at;
+    child_count[k] = n;
+}
+
+// ---------------------------------------------------------------- splits
+__global__ __launch_bounds__(256) void route(const Seg* segs, uint32_t n, Seg* small, Seg* large, uint32_t* counts) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const Seg s = segs[i];
+    if (s.count <= SMALL_MAX) small[atomicAdd(&counts[0], 1u)] = s;
+    else large[atomicAdd(&counts[1], 1u)] = s;
+}
+
+// One workgroup splits one group of at most SMALL_MAX triangles in place.
+__global__ __launch_bounds__(256) void split_small(const Seg* segs, const MeshDev* meshes, uint32_t* perm,
+                                                   const float* cent, const float* box, uint32_t* split_left,
+                                                   MedianReq* med, uint32_t* med_count) {
+    __shared__ uint32_t ids[SMALL_MAX];
+    __shared__ int32_t bounds[6];
+    __shared__ uint32_t cnt[3 * BINS];
+    __shared__ int32_t bins[3 * BINS * 6];
+    __shared__ uint32_t runs[256];
+    __shared__ int s_found, s_axis;
+    __shared__ uint32_t s_plane, s_total;
+    const Seg sg = segs[blockIdx.x];
+    const uint32_t n = sg.count, t = threadIdx.x;
+    if (t < 3) {
+        bounds[t] = okey(__builtin_inff());
+        bounds[3 + t] = okey(-__builtin_inff());
+    }
+    for (uint32_t i = t; i < 3 * BINS; i += 256) {
+        cnt[i] = 0;
+        for (int k = 0; k < 3; ++k) {
+            bins[i * 6 + k] = okey(__builtin_inff());
+            bins[i * 6 + 3 + k] = okey(-__builtin_inff());
+        }
+    }
+    __syncthreads();
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t i = t; i < n; i += 256) {
+        const uint32_t id = perm[sg.start + i];
+        ids[i] = id;
+        for (int x = 0; x < 3; ++x) {
+            const float c = cent[3ull * id + x];
+            if (c == c) {
+                lo[x] = fminf(lo[x], c);
+                hi[x] = fmaxf(hi[x], c);
+            }
+        }
+    }
+    for (int x = 0; x < 3; ++x) {
+        atomicMin(&bounds[x], okey(lo[x]));
+        atomicMax(&bounds[3 + x], okey(hi[x]));
+    }
+    __syncthreads();
+    float clo[3], extent[3];
+    bool valid[3];
+    for (int x = 0; x < 3; ++x) {
+        clo[x] = okey_inv(bounds[x]);
+        extent[x] = okey_inv(bounds[3 + x]) - clo[x];
+        valid[x] = extent[x] > 0.0f;  // near_field.rs:325: NaN or <= 0 skips the axis
+    }
+    const bool sah = meshes[sg.mesh].mode == MODE_SAH;
+    if (sah) {
+        for (uint32_t i = t; i < n; i += 256) {
+            const uint32_t id = ids[i];
+            for (int x = 0; x < 3; ++x) {
+                if (!valid[x]) continue;
+                const uint32_t b = x * BINS + bin_of(cent[3ull * id + x], clo[x], extent[x]);
+                atomicAdd(&cnt[b], 1u);
+                for (int k = 0; k < 3; ++k) {
+                    atomicMin(&bins[b * 6 + k], okey(box[6ull * id + k]));
+                    atomicMax(&bins[b * 6 + 3 + k], okey(box[6ull * id + 3 + k]));
+                }
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            int axis = 0;
+            uint32_t plane = 0;
+            s_found = sah_sweep(cnt, bins, valid, n, axis, plane) ? 1 : 0;
+            s_axis = axis;
+            s_plane = plane;
+        }
+        __syncthreads();
+    }
+    if (!sah || !s_found) {
+        if (t == 0) ask_median(med, med_count, sg, median_axis(extent));
+        return;
+    }
+    const int ax = s_axis;
+    const uint32_t plane = s_plane;
+    const uint32_t per = (n + 255) / 256, b0 = min(t * per, n), b1 = min(b0 + per, n);
+    uint32_t nl = 0;
+    for (uint32_t i = b0; i < b1; ++i) nl += bin_of(cent[3ull * ids[i] + ax], clo[ax], extent[ax]) <= plane;
+    runs[t] = nl;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int j = 0; j < 256; ++j) {
+            const uint32_t v = runs[j];
+            runs[j] = acc;
+            acc += v;
+        }
+        s_total = acc;
+    }
+    __syncthreads();
+    const uint32_t nleft = s_total;
+    uint32_t l = runs[t], r = b0 - runs[t];
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t id = ids[i];
+        if (bin_of(cent[3ull * id + ax], clo[ax], extent[ax]) <= plane) perm[sg.start + l++] = id;
+        else perm[sg.start + nleft + r++] = id;
+    }
+    if (t == 0) split_left[sg.out] = nleft;
+}
+
+// Chunked path for groups above SMALL_MAX triangles.
+__global__ __launch_bounds__(256) void large_bounds(const Chunk* chunks, const uint32_t* perm, const float* cent,
+                                                    int32_t* lb) {
+    __shared__ int32_t b[6];
+    const Chunk c = chunks[blockIdx.x];
+    const uint32_t t = threadIdx.x;
+    if (t < 3) {
+        b[t] = okey(__builtin_inff());
+        b[3 + t] = okey(-__builtin_inff());
+    }
+    __syncthreads();
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    for (uint32_t i = t; i < c.count; i += 256) {
+        const uint32_t id = perm[c.start + i];
+        for (int x = 0; x < 3; ++x) {
+            const float v = cent[3ull * id + x];
+            if (v == v) {
+                lo[x] = fminf(lo[x], v);
+                hi[x] = fmaxf(hi[x], v);
+            }
+        }
+    }
+    for (int x = 0; x < 3; ++x) {
+        atomicMin(&b[x], okey(lo[x]));
+        atomicMax(&b[3 + x], okey(hi[x]));
+    }
+    __syncthreads();
+    if (t < 3) {
+        atomicMin(&lb[6 * c.seg + t], b[t]);
+        atomicMax(&lb[6 * c.seg + 3 + t], b[3 + t]);
+    }
+}
+
+__device__ inline void large_axes(const int32_t* lb, float clo[3], float extent[3], bool valid[3]) {
+    for (int x = 0; x < 3; ++x) {
+        clo[x] = okey_inv(lb[x]);
+        extent[x] = okey_inv(lb[3 + x]) - clo[x];
+        valid[x] = extent[x] > 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void large_bins(const Chunk* chunks, const Seg* large, const MeshDev* meshes,
+                                                  const uint32_t* perm, const float* cent, const float* box,
+                                                  const int32_t* lb, uint32_t* gcnt, int32_t* gbins) {
+    __shared__ uint32_t cnt[3 * BINS];
+    __shared__ int32_t bins[3 * BINS * 6];
+    const Chunk c = chunks[blockIdx.x];
+    if (meshes[large[c.seg].mesh].mode != MODE_SAH) return;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < 3 * BINS; i += 256) {
+        cnt[i] = 0;
+        for (int k = 0; k < 3; ++k) {
+            bins[i * 6 + k] = okey(__builtin_inff());
+            bins[i * 6 + 3 + k] = okey(-__builtin_inff());
+        }
+    }
+    __syncthreads();
+    float clo[3], extent[3];
+    bool valid[3];
+    large_axes(lb + 6 * c.seg, clo, extent, valid);
+    for (uint32_t i = t; i < c.count; i += 256) {
+        const uint32_t id = perm[c.start + i];
+        for (int x = 0; x < 3; ++x) {
+            if (!valid[x]) continue;
+            const uint32_t b = x * BINS + bin_of(cent[3ull * id + x], clo[x], extent[x]);
+            atomicAdd(&cnt[b], 1u);
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&bins[b * 6 + k], okey(box[6ull * id + k]));
+                atomicMax(&bins[b * 6 + 3 + k], okey(box[6ull * id + 3 + k]));
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 3 * BINS && cnt[t]) {
+        uint32_t* gc = gcnt + (size_t)c.seg * 3 * BINS;
+        int32_t* gb = gbins + (size_t)c.seg * 3 * BINS * 6;
+        atomicAdd(&gc[t], cnt[t]);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&gb[t * 6 + k], bins[t * 6 + k]);
+            atomicMax(&gb[t * 6 + 3 + k], bins[t * 6 + 3 + k]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void large_eval(const Seg* large, uint32_t n, const MeshDev* meshes, const int32_t* lb,
+                                                 const uint32_t* gcnt, const int32_t* gbins, uint32_t* best,
+                                                 MedianReq* med, uint32_t* med_count) {
+    const uint32_t j = blockIdx.x * 64u + threadIdx.x;
+    if (j >= n) return;
+    const Seg s = large[j];
+    float clo[3], extent[3];
+    bool valid[3];
+    large_axes(lb + 6 * j, clo, extent, valid);
+    int axis = 0;
+    uint32_t plane = 0;
+    if (meshes[s.mesh].mode == MODE_SAH &&
+        sah_sweep(gcnt + (size_t)j * 3 * BINS, gbins + (size_t)j * 3 * BINS * 6, valid, s.count, axis, plane)) {
+        best[j] = ((uint32_t)axis << 8) | plane;
+    } else {
+        best[j] = EMPTY;
+        ask_median(med, med_count, s, median_axis(extent));
+    }
+}
+
+__device__ inline bool large_left(const int32_t* lb, uint32_t best, const float* cent, uint32_t id) {
+    float clo[3], extent[3];
+    bool valid[3];
+    large_axes(lb, clo, extent, valid);
+    const uint32_t ax = best >> 8, plane = best & 0xffu;
+    return bin_of(cent[3ull * id + ax], clo[ax], extent[ax]) <= plane;
+}
+
+__global__ __launch_bounds__(256) void large_count(const Chunk* chunks, const uint32_t* best, const uint32_t* perm,
+                                                   const float* cent, const int32_t* lb, uint32_t* chunk_left) {
+    __shared__ uint32_t sum;
+    const Chunk c = chunks[blockIdx.x];
+    const uint32_t bst = best[c.seg];
+    if (threadIdx.x == 0) sum = 0;
+    __syncthreads();
+    if (bst == EMPTY) {
+        if (threadIdx.x == 0) chunk_left[blockIdx.x] = 0;
+        return;
+    }
+    uint32_t nl = 0;
+    for (uint32_t i = threadIdx.x; i < c.count; i += 256) nl += large_left(lb + 6 * c.seg, bst, cent, perm[c.start + i]);
+    atomicAdd(&sum, nl);
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_left[blockIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(64) void large_total(const Seg* large, uint32_t n, const uint32_t* best,
+                                                  const uint32_t* first_chunk, const uint32_t* chunk_n,
+                                                  const uint32_t* chunk_left, const uint32_t* chunk_off,
+                                                  uint32_t* seg_left, uint32_t* split_left) {
+    const uint32_t j = blockIdx.x * 64u + threadIdx.x;
+    if (j >= n || best[j] == EMPTY) return;
+    const uint32_t f = first_chunk[j], l = f + chunk_n[j] - 1;
+    const uint32_t nl = chunk_off[l] + chunk_left[l] - chunk_off[f];
+    seg_left[j] = nl;
+    split_left[large[j].out] = nl;
+}
+
+__global__ __launch_bounds__(256) void large_scatter(const Chunk* chunks, const Seg* large, const uint32_t* best,
+                                                     const uint32_t* first_chunk, const uint32_t* chunk_off,
+                                                     const uint32_t* seg_left, const uint32_t* perm, const float* cent,
+                                                     const int32_t* lb, uint32_t* tmp) {
+    __shared__ uint32_t runs[256];
+    const Chunk c = chunks[blockIdx.x];
+    const uint32_t bst = best[c.seg];
+    if (bst == EMPTY) return;
+    const Seg s = large[c.seg];
+    const uint32_t t = threadIdx.x, per = (c.count + 255) / 256, b0 = min(t * per, c.count),
+                   b1 = min(b0 + per, c.count);
+    uint32_t nl = 0;
+    for (uint32_t i = b0; i < b1; ++i) nl += large_left(lb + 6 * c.seg, bst, cent, perm[c.start + i]);
+    runs[t] = nl;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int j = 0; j < 256; ++j) {
+            const uint32_t v = runs[j];
+            runs[j] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    const uint32_t lefts_before = chunk_off[blockIdx.x] - chunk_off[first_chunk[c.seg]];
+    const uint32_t rights_before = (c.start - s.start) - lefts_before;
+    uint32_t l = lefts_before + runs[t], r = rights_before + (b0 - runs[t]);
+    const uint32_t nleft = seg_left[c.seg];
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t id = perm[c.start + i];
+        if (large_left(lb + 6 * c.seg, bst, cent, id)) tmp[s.start + l++] = id;
+        else tmp[s.start + nleft + r++] = id;
+    }
+}
+
+__global__ __launch_bounds__(256) void large_copy(const Chunk* chunks, const uint32_t* best, const uint32_t* tmp,
+                                                  uint32_t* perm) {
+    const Chunk c = chunks[blockIdx.x];
+    if (best[c.seg] == EMPTY) return;
+    for (uint32_t i = threadIdx.x; i < c.count; i += 256) perm[c.start + i] = tmp[c.start + i];
+}
+
+// Median splits (near_field.rs:367-379): sort by (total-order key of the
+// centroid on the axis, triangle), lower half left.
+__global__ __launch_bounds__(256) void median_keys(const MedianReq* med, const uint32_t* off, const MeshDev* meshes,
+                                                   const uint32_t* perm, const float* cent, uint64_t* keys) {
+    const MedianReq q = med[blockIdx.x];
+    const uint32_t tb = meshes[q.mesh].tbase;
+    for (uint32_t i = threadIdx.x; i < q.count; i += 256) {
+        const uint32_t id = perm[q.start + i];
+        const uint32_t k = (uint32_t)okey(cent[3ull * id + q.axis]) ^ 0x80000000u;
+        keys[off[blockIdx.x] + i] = ((uint64_t)k << 32) | (id - tb);
+    }
+}
+
+__global__ __launch_bounds__(256) void median_put(const MedianReq* med, const uint32_t* off, const MeshDev* meshes,
+                                                  const uint64_t* keys, uint32_t* perm, uint32_t* split_left) {
+    const MedianReq q = med[blockIdx.x];
+    const uint32_t tb = meshes[q.mesh].tbase;
+    for (uint32_t i = threadIdx.x; i < q.count; i += 256)
+        perm[q.start + i] = tb + (uint32_t)(keys[off[blockIdx.x] + i] & 0xffffffffu);
+    if (threadIdx.x == 0) split_left[q.out] = q.count / 2;
+}
+
+// ---------------------------------------------------------------- numbering
+__global__ __launch_bounds__(256) void node_sizes(uint32_t nbase, uint32_t nn, const uint32_t* child_first,
+                                                  const uint32_t* child_count, const uint32_t* gnode, uint32_t* size) {
+    const uint32_t k = nbase + blockIdx.x * 256u + threadIdx.x;
+    if (k >= nbase + nn) return;
+    uint32_t s = 1;
+    for (uint32_t c = 0; c < child_count[k]; ++c) {
+        const uint32_t g = gnode[child_first[k] + c];
+        if (g != EMPTY) s += size[g];
+    }
+    size[k] = s;
+}
+
+// Post-order: each child's subtree in turn, then the node (near_field.rs:305).
+__global__ __launch_bounds__(256) void node_ids(uint32_t nbase, uint32_t nn, bool roots, const uint32_t* child_first,
+                                                const uint32_t* child_count, const uint32_t* gnode,
+                                                const uint32_t* size, uint32_t* start, uint32_t* id) {
+    const uint32_t k = nbase + blockIdx.x * 256u + threadIdx.x;
+    if (k >= nbase + nn) return;
+    if (roots) start[k] = 0;
+    uint32_t at = start[k];
+    id[k] = at + size[k] - 1;
+    for (uint32_t c = 0; c < child_count[k]; ++c) {
+        const uint32_t g = gnode[child_first[k] + c];
+        if (g == EMPTY) continue;
+        start[g] = at;
+        at += size[g];
+    }
+}
+
+__global__ __launch_bounds__(256) void leaf_marks(const uint32_t* gstart, const uint32_t* gcount, uint32_t n,
+                                                  uint32_t* mark) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g < n && gcount[g] <= LEAF_TRIANGLES) mark[gstart[g]] = 1;
+}
+
+__global__ __launch_bounds__(256) void mesh_summary(const MeshDev* meshes, uint32_t mesh_count, const uint32_t* gnode,
+                                                    const uint32_t* size, const uint32_t* id, const uint32_t* mark,
+                                                    const uint32_t* lrank, uint32_t* out /* 4 per mesh */) {
+    const uint32_t m = blockIdx.x * 256u + threadIdx.x;
+    if (m >= mesh_count) return;
+    const uint32_t root = gnode[m];  // level-0 group m is mesh m
+    const uint32_t tb = meshes[m].tbase, te = tb + meshes[m].tcount - 1;
+    out[4 * m + 0] = root == EMPTY ? 0u : size[root];
+    out[4 * m + 1] = lrank[te] + mark[te] - lrank[tb];
+    out[4 * m + 2] = root == EMPTY ? LEAF_BIT : id[root];
+}
+
+__global__ __launch_bounds__(256) void mesh_depth(const uint32_t* node_mesh, const uint32_t* node_level, uint32_t n,
+                                                  uint32_t* out) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < n) atomicMax(&out[4 * node_mesh[k] + 3], node_level[k] + 1);
+}
+
+// ---------------------------------------------------------------- words
+__device__ __forceinline__ uint32_t leaf_id(const MeshDev& m, const uint32_t* lrank, uint32_t start) {
+    return lrank[start] - lrank[m.tbase];
+}
+
+__global__ __launch_bounds__(256) void write_nodes(uint32_t n, const uint32_t* node_mesh, const MeshDev* meshes,
+                                                   const uint32_t* id, const uint32_t* child_first,
+                                                   const uint32_t* child_count, const uint32_t* gnode,
+                                                   const uint32_t* gstart, const uint32_t* lrank, uint32_t* words) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const MeshDev m = meshes[node_mesh[k]];
+    if (id[k] >= m.node_count) return;  // numbering out of range: left zero, the words will not match
+    uint32_t* w = words + m.nodes + 16ull * id[k];
+    for (uint32_t s = 0; s < 4; ++s) {
+        uint32_t tag = EMPTY;
+        if (s < child_count[k]) {
+            const uint32_t g = child_first[k] + s;
+            if (gnode[g] != EMPTY) {
+                tag = id[gnode[g]];
+                if (tag >= m.node_count) tag = EMPTY;
+            } else {
+                const uint32_t l = leaf_id(m, lrank, gstart[g]);
+                tag = l < m.leaf_count ? (LEAF_BIT | l) : EMPTY;
+            }
+        }
+        w[6 + s] = tag;
+    }
+}
+
+__global__ __launch_bounds__(256) void write_leaves(const uint32_t* gstart, const uint32_t* gcount,
+                                                    const uint32_t* gmesh, uint32_t n, const MeshDev* meshes,
+                                                    const uint32_t* lrank, uint32_t* words) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= n || gcount[g] > LEAF_TRIANGLES) return;
+    const MeshDev m = meshes[gmesh[g]];
+    const uint32_t l = leaf_id(m, lrank, gstart[g]);
+    if (l >= m.leaf_count) return;
+    words[m.leaves + 2ull * l] = 3 * (gstart[g] - m.tbase);
+    words[m.leaves + 2ull * l + 1] = 3 * gcount[g];
+}
+
+__global__ __launch_bounds__(256) void write_tris(const MeshDev* meshes, uint32_t mesh_count, uint32_t tris,
+                                                  const uint32_t* perm, const uint32_t* idx, uint32_t* words) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= tris) return;
+    uint32_t lo = 0, hi = mesh_count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (meshes[mid].tbase <= i) lo = mid;
+        else hi = mid;
+    }
+    const MeshDev m = meshes[lo];
+    const uint32_t p = i - m.tbase, t = perm[i];
+    for (int k = 0; k < 3; ++k) words[m.indices + 3ull * p + k] = idx[3ull * t + k];
+    if (m.keep) words[m.prims + p] = t - m.tbase;
+}
+
+__global__ __launch_bounds__(256) void write_positions(const MeshDev* meshes, uint32_t mesh_count, uint32_t floats,
+                                                       const float* pos, uint32_t* words) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= floats) return;
+    const uint32_t v = i / 3;
+    uint32_t lo = 0, hi = mesh_count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (meshes[mid].vbase <= v) lo = mid;
+        else hi = mid;
+    }
+    const MeshDev m = meshes[lo];
+    words[m.positions + (i - 3 * m.vbase)] = fb(pos[i]);
+}
+
+__global__ __launch_bounds__(256) void refit_nodes(uint32_t nbase, uint32_t nn, const uint32_t* node_mesh,
+                                                   const MeshDev* meshes, const uint32_t* id, uint32_t* words) {
+    const uint32_t k = nbase + blockIdx.x * 256u + threadIdx.x;
+    if (k >= nbase + nn) return;
+    const MeshDev m = meshes[node_mesh[k]];
+    if (id[k] < m.node_count) bvhdev::refit_node(words, m.nodes, m.leaves, m.positions, m.indices, id[k]);
+}
+
+}  // namespace gbuild
+}  // namespace rq
+
+using namespace rq;
+using namespace rq::gbuild;
+
+namespace {
+
+rq_status hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RQ_ERR_OUT_OF_MEMORY : RQ_ERR_DEVICE;
+}
+#define GB_TRY(x, what)                                   \
+    do {                                                  \
+        hipError_t e_ = (x);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, what);  \
+    } while (0)
+#define GB_OK(x)                          \
+    do {                                  \
+        rq_status s_ = (x);               \
+        if (s_ != RQ_OK) return s_;       \
+    } while (0)
+
+struct Buf {
+    void* p = nullptr;
+    size_t n = 0;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    rq_status need(size_t bytes, const char* what) {
+        if (bytes <= n) return RQ_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        GB_TRY(hipMalloc(&p, std::max<size_t>(bytes, 256)), what);
+        n = std::max<size_t>(bytes, 256);
+        return RQ_OK;
+    }
+    template <typename T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+inline dim3 grid_for(uint64_t n, uint32_t block = 256) {
+    return dim3((unsigned)std::max<uint64_t>(1, (n + block - 1) / block));
+}
+
+// The device side of one build: triangles of every mesh, the permutation, and
+// the groups and nodes of every level.
+struct GpuBuild {
+    hipStream_t s = nullptr;
+    uint32_t mesh_count = 0, tris = 0, verts = 0;
+    Buf meshes, pos, idx, cent, box, perm, tmp;
+    Buf gstart, gcount, gmesh, gnode;                // groups, all levels
+    Buf node_group, node_mesh, node_level, child_first, child_count, size, start, id;
+    Buf flag, scan, segs, small, large, split_left, counters, med, med_off, keys, keys_out;
+    Buf chunks, first_chunk, chunk_n, chunk_left, chunk_off, lb, gcnt, gbins, best, seg_left, mark, lrank, summary;
+    Buf cub;
+    std::vector<uint32_t> level_gbase, level_nbase;  // per level; one past the end last
+    uint32_t node_total = 0;
+
+    rq_status exclusive_sum(const uint32_t* in, uint32_t* out, uint32_t n) {
+        size_t bytes = 0;
+        GB_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s), "scan size");
+        GB_OK(cub.need(bytes, "hipMalloc scan"));
+        GB_TRY(hipcub::DeviceScan::ExclusiveSum(cub.p, bytes, in, out, n, s), "scan");
+        return RQ_OK;
+    }
+    rq_status fetch(void* dst, const void* src, size_t bytes) {
+        GB_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "build fetch");
+        GB_TRY(hipStreamSynchronize(s), "build sync");
+        return RQ_OK;
+    }
+    void fill(Buf& b, uint32_t v, uint64_t n) {
+        hipLaunchKernelGGL(fill_u32, dim3((unsigned)std::min<uint64_t>((n + 255) / 256 + 1, 4096)), dim3(256), 0, s,
+                           b.as<uint32_t>(), v, n);
+    }
+
+    // Splits `n` groups listed in segs (one pass: every group once).
+    rq_status split_pass(uint32_t n) {
+        if (!n) return RQ_OK;
+        uint32_t* cnt = counters.as<uint32_t>();
+        fill(counters, 0, 4);
+        hipLaunchKernelGGL(route, grid_for(n), dim3(256), 0, s, segs.as<Seg>(), n, small.as<Seg>(), large.as<Seg>(),
+                           cnt);
+        uint32_t c[4];
+        GB_OK(fetch(c, cnt, 16));
+        const uint32_t n_small = c[0], n_large = c[1];
+        if (n_small)
+            hipLaunchKernelGGL(split_small, dim3(n_small), dim3(256), 0, s, small.as<Seg>(), meshes.as<MeshDev>(),
+                               perm.as<uint32_t>(), cent.as<float>(), box.as<float>(), split_left.as<uint32_t>(),
+                               med.as<MedianReq>(), cnt + 2);
+        if (n_large) GB_OK(large_pass(n_large));
+        GB_TRY(hipGetLastError(), "split launch");
+        uint32_t n_med = 0;
+        GB_OK(fetch(&n_med, cnt + 2, 4));
+        if (n_med) GB_OK(median_pass(n_med));
+        return RQ_OK;
+    }
+
+    rq_status large_pass(uint32_t n) {
+        std::vector<Seg> segs_h(n);
+        GB_OK(fetch(segs_h.data(), large.p, n * sizeof(Seg)));
+        std::vector<Chunk> ch;
+        std::vector<uint32_t> first(n), count(n);
+        for (uint32_t j = 0; j < n; ++j) {
+            first[j] = (uint32_t)ch.size();
+            for (uint32_t o = 0; o < segs_h[j].count; o += CHUNK)
+                ch.push_back(Chunk{j, segs_h[j].start + o, std::min(CHUNK, segs_h[j].count - o), 0});
+            count[j] = (uint32_t)ch.size() - first[j];
+        }
+        const uint32_t nc = (uint32_t)ch.size();
+        GB_OK(chunks.need(nc * sizeof(Chunk), "hipMalloc chunks"));
+        GB_OK(first_chunk.need(n * 4, "hipMalloc chunks"));
+        GB_OK(chunk_n.need(n * 4, "hipMalloc chunks"));
+        GB_OK(chunk_left.need(nc * 4, "hipMalloc chunks"));
+        GB_OK(chunk_off.need(nc * 4, "hipMalloc chunks"));
+        GB_OK(lb.need(n * 6 * 4, "hipMalloc chunks"));
+        GB_OK(gcnt.need((size_t)n * 3 * BINS * 4, "hipMalloc chunks"));
+        GB_OK(gbins.need((size_t)n * 3 * BINS * 6 * 4, "hipMalloc chunks"));
+        GB_OK(best.need(n * 4, "hipMalloc chunks"));
+        GB_OK(seg_left.need(n * 4, "hipMalloc chunks"));
+        GB_TRY(hipMemcpyAsync(chunks.p, ch.data(), nc * sizeof(Chunk), hipMemcpyHostToDevice, s), "chunks");
+        GB_TRY(hipMemcpyAsync(first_chunk.p, first.data(), n * 4, hipMemcpyHostToDevice, s), "chunks");
+        GB_TRY(hipMemcpyAsync(chunk_n.p, count.data(), n * 4, hipMemcpyHostToDevice, s), "chunks");
+        std::vector<int32_t> init(n * 6);
+        std::vector<int32_t> binit((size_t)n * 3 * BINS * 6);
+        const int32_t pinf = (int32_t)0x7f800000, ninf = (int32_t)(0xff800000u ^ 0x7fffffffu);
+        for (uint32_t j = 0; j < n; ++j)
+            for (int k = 0; k < 6; ++k) init[6 * j + k] = k < 3 ? pinf : ninf;
+        for (size_t b = 0; b < (size_t)n * 3 * BINS; ++b)
+            for (int k = 0; k < 6; ++k) binit[b * 6 + k] = k < 3 ? pinf : ninf;
+        GB_TRY(hipMemcpyAsync(lb.p, init.data(), init.size() * 4, hipMemcpyHostToDevice, s), "chunks");
+        GB_TRY(hipMemcpyAsync(gbins.p, binit.data(), binit.size() * 4, hipMemcpyHostToDevice, s), "chunks");
+        fill(gcnt, 0, (uint64_t)n * 3 * BINS);
+        uint32_t* cnt = counters.as<uint32_t>();
+        hipLaunchKernelGGL(large_bounds, dim3(nc), dim3(256), 0, s, chunks.as<Chunk>(), perm.as<uint32_t>(),
+                           cent.as<float>(), lb.as<int32_t>());
+        hipLaunchKernelGGL(large_bins, dim3(nc), dim3(256), 0, s, chunks.as<Chunk>(), large.as<Seg>(),
+                           meshes.as<MeshDev>(), perm.as<uint32_t>(), cent.as<float>(), box.as<float>(),
+                           lb.as<int32_t>(), gcnt.as<uint32_t>(), gbins.as<int32_t>());
+        hipLaunchKernelGGL(large_eval, grid_for(n, 64), dim3(64), 0, s, large.as<Seg>(), n, meshes.as<MeshDev>(),
+                           lb.as<int32_t>(), gcnt.as<uint32_t>(), gbins.as<int32_t>(), best.as<uint32_t>(),
+                           med.as<MedianReq>(), cnt + 2);
+        hipLaunchKernelGGL(large_count, dim3(nc), dim3(256), 0, s, chunks.as<Chunk>(), best.as<uint32_t>(),
+                           perm.as<uint32_t>(), cent.as<float>(), lb.as<int32_t>(), chunk_left.as<uint32_t>());
+        GB_OK(exclusive_sum(chunk_left.as<uint32_t>(), chunk_off.as<uint32_t>(), nc));
+        hipLaunchKernelGGL(large_total, grid_for(n, 64), dim3(64), 0, s, large.as<Seg>(), n, best.as<uint32_t>(),
+                           first_chunk.as<uint32_t>(), chunk_n.as<uint32_t>(), chunk_left.as<uint32_t>(),
+                           chunk_off.as<uint32_t>(), seg_left.as<uint32_t>(), split_left.as<uint32_t>());
+        hipLaunchKernelGGL(large_scatter, dim3(nc), dim3(256), 0, s, chunks.as<Chunk>(), large.as<Seg>(),
+                           best.as<uint32_t>(), first_chunk.as<uint32_t>(), chunk_off.as<uint32_t>(),
+                           seg_left.as<uint32_t>(), perm.as<uint32_t>(), cent.as<float>(), lb.as<int32_t>(),
+                           tmp.as<uint32_t>());
+        hipLaunchKernelGGL(large_copy, dim3(nc), dim3(256), 0, s, chunks.as<Chunk>(), best.as<uint32_t>(),
+                           tmp.as<uint32_t>(), perm.as<uint32_t>());
+        GB_TRY(hipGetLastError(), "large split launch");
+        // the host vectors above must outlive their copies
+        GB_TRY(hipStreamSynchronize(s), "large split");
+        return RQ_OK;
+    }
+
+    rq_status median_pass(uint32_t n) {
+        std::vector<MedianReq> q(n);
+        GB_OK(fetch(q.data(), med.p, n * sizeof(MedianReq)));
+        std::vector<uint32_t> off(n + 1, 0);
+        for (uint32_t j = 0; j < n; ++j) off[j + 1] = off[j] + q[j].count;
+        const uint32_t total = off[n];
+        GB_OK(med_off.need((n + 1) * 4, "hipMalloc median"));
+        GB_OK(keys.need((size_t)total * 8, "hipMalloc median"));
+        GB_OK(keys_out.need((size_t)total * 8, "hipMalloc median"));
+        GB_TRY(hipMemcpyAsync(med_off.p, off.data(), (n + 1) * 4, hipMemcpyHostToDevice, s), "median offsets");
+        hipLaunchKernelGGL(median_keys, dim3(n), dim3(256), 0, s, med.as<MedianReq>(), med_off.as<uint32_t>(),
+                           meshes.as<MeshDev>(), perm.as<uint32_t>(), cent.as<float>(), keys.as<uint64_t>());
+        size_t bytes = 0;
+        const uint32_t* o = med_off.as<uint32_t>();
+        GB_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                                          (int)total, (int)n, o, o + 1, 0, 64, s),
+               "median sort size");
+        GB_OK(cub.need(bytes, "hipMalloc median sort"));
+        GB_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(cub.p, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                                          (int)total, (int)n, o, o + 1, 0, 64, s),
+               "median sort");
+        hipLaunchKernelGGL(median_put, dim3(n), dim3(256), 0, s, med.as<MedianReq>(), med_off.as<uint32_t>(),
+                           meshes.as<MeshDev>(), keys_out.as<uint64_t>(), perm.as<uint32_t>(),
+                           split_left.as<uint32_t>());
+        GB_TRY(hipGetLastError(), "median launch");
+        GB_TRY(hipStreamSynchronize(s), "median");
+        return RQ_OK;
+    }
+
+    // The whole topology: every level's groups, nodes and their children.
+    rq_status topology() {
+        hipLaunchKernelGGL(prep_tris, grid_for(tris), dim3(256), 0, s, meshes.as<MeshDev>(), mesh_count,
+                           pos.as<float>(), idx.as<uint32_t>(), tris, cent.as<float>(), box.as<float>(),
+                           perm.as<uint32_t>());
+        hipLaunchKernelGGL(root_groups, grid_for(mesh_count), dim3(256), 0, s, meshes.as<MeshDev>(), mesh_count,
+                           gstart.as<uint32_t>(), gcount.as<uint32_t>(), gmesh.as<uint32_t>());
+        level_gbase.assign(1, 0);
+        level_nbase.assign(1, 0);
+        uint32_t gb = 0, ng = mesh_count, nb = 0;
+        for (uint32_t level = 0;; ++level) {
+            hipLaunchKernelGGL(node_flags, grid_for(ng), dim3(256), 0, s, gcount.as<uint32_t>() + gb, ng,
+                               flag.as<uint32_t>());
+            GB_OK(exclusive_sum(flag.as<uint32_t>(), scan.as<uint32_t>(), ng));
+            uint32_t last[2];
+            GB_OK(fetch(&last[0], scan.as<uint32_t>() + ng - 1, 4));
+            GB_OK(fetch(&last[1], flag.as<uint32_t>() + ng - 1, 4));
+            const uint32_t nn = last[0] + last[1];
+            hipLaunchKernelGGL(make_nodes, grid_for(ng), dim3(256), 0, s, gstart.as<uint32_t>(), gcount.as<uint32_t>(),
+                               gmesh.as<uint32_t>(), gnode.as<uint32_t>(), gb, ng, scan.as<uint32_t>(), nb, level,
+                               node_group.as<uint32_t>(), node_mesh.as<uint32_t>(), node_level.as<uint32_t>(),
+                               segs.as<Seg>());
+            level_gbase.push_back(gb + ng);
+            if (nn == 0) {
+                level_nbase.push_back(nb);
+                break;
+            }
+            fill(split_left, EMPTY, 3ull * nn);
+            GB_OK(split_pass(nn));  // round 0: each node's group in two
+            fill(counters, 0, 4);
+            hipLaunchKernelGGL(round1_segs, grid_for(nn), dim3(256), 0, s, gstart.as<uint32_t>(),
+                               gcount.as<uint32_t>(), gmesh.as<uint32_t>(), node_group.as<uint32_t>(), nb, nn,
+                               split_left.as<uint32_t>(), segs.as<Seg>(), counters.as<uint32_t>() + 3);
+            uint32_t n1 = 0;
+            GB_OK(fetch(&n1, counters.as<uint32_t>() + 3, 4));
+            GB_OK(split_pass(n1));  // round 1: the halves still too large
+            hipLaunchKernelGGL(child_counts, grid_for(nn), dim3(256), 0, s, gcount.as<uint32_t>(),
+                               node_group.as<uint32_t>(), nb, nn, split_left.as<uint32_t>(), flag.as<uint32_t>());
+            GB_OK(exclusive_sum(flag.as<uint32_t>(), scan.as<uint32_t>(), nn));
+            GB_OK(fetch(&last[0], scan.as<uint32_t>() + nn - 1, 4));
+            GB_OK(fetch(&last[1], flag.as<uint32_t>() + nn - 1, 4));
+            const uint32_t next = last[0] + last[1];
+            hipLaunchKernelGGL(emit_children, grid_for(nn), dim3(256), 0, s, gstart.as<uint32_t>(),
+                               gcount.as<uint32_t>(), gmesh.as<uint32_t>(), node_group.as<uint32_t>(), nb, nn,
+                               split_left.as<uint32_t>(), scan.as<uint32_t>(), gb + ng, child_first.as<uint32_t>(),
+                               child_count.as<uint32_t>());
+            GB_TRY(hipGetLastError(), "level launch");
+            gb += ng;
+            ng = next;
+            nb += nn;
+            level_nbase.push_back(nb);
+        }
+        node_total = nb;
+        return RQ_OK;
+    }
+
+    // Post-order ids, leaf ranks and the per-mesh summary (nodes, leaves, root, depth).
+    rq_status numbering(std::vector<uint32_t>& summary_h) {
+        const size_t levels = level_nbase.size() - 1;  // levels with a node range (the last may be empty)
+        for (size_t l = levels; l-- > 0;) {
+            const uint32_t nb = level_nbase[l], nn = level_nbase[l + 1] - nb;
+            if (nn)
+                hipLaunchKernelGGL(node_sizes, grid_for(nn), dim3(256), 0, s, nb, nn, child_first.as<uint32_t>(),
+                                   child_count.as<uint32_t>(), gnode.as<uint32_t>(), size.as<uint32_t>());
+        }
+        for (size_t l = 0; l < levels; ++l) {
+            const uint32_t nb = level_nbase[l], nn = level_nbase[l + 1] - nb;
+            if (nn)
+                hipLaunchKernelGGL(node_ids, grid_for(nn), dim3(256), 0, s, nb, nn, l == 0, child_first.as<uint32_t>(),
+                                   child_count.as<uint32_t>(), gnode.as<uint32_t>(), size.as<uint32_t>(),
+                                   start.as<uint32_t>(), id.as<uint32_t>());
+        }
+        const uint32_t groups = level_gbase.back();
+        fill(mark, 0, tris);
+        hipLaunchKernelGGL(leaf_marks, grid_for(groups), dim3(256), 0, s, gstart.as<uint32_t>(), gcount.as<uint32_t>(),
+                           groups, mark.as<uint32_t>());
+        GB_OK(exclusive_sum(mark.as<uint32_t>(), lrank.as<uint32_t>(), tris));
+        fill(summary, 0, 4ull * mesh_count);
+        hipLaunchKernelGGL(mesh_summary, grid_for(mesh_count), dim3(256), 0, s, meshes.as<MeshDev>(), mesh_count,
+                           gnode.as<uint32_t>(), size.as<uint32_t>(), id.as<uint32_t>(), mark.as<uint32_t>(),
+                           lrank.as<uint32_t>(), summary.as<uint32_t>());
+        if (node_total)
+            hipLaunchKernelGGL(mesh_depth, grid_for(node_total), dim3(256), 0, s, node_mesh.as<uint32_t>(),
+                               node_level.as<uint32_t>(), node_total, summary.as<uint32_t>());
+        GB_TRY(hipGetLastError(), "numbering launch");
+        summary_h.resize(4ull * mesh_count);
+        return fetch(summary_h.data(), summary.p, summary_h.size() * 4);
+    }
+
+    rq_status allocate() {
+        const uint64_t T = std::max<uint32_t>(tris, 1), G = 2 * T + mesh_count + 16, N = T + 16;
+        GB_OK(cent.need(T * 12, "hipMalloc build"));
+        GB_OK(box.need(T * 24, "hipMalloc build"));
+        GB_OK(perm.need(T * 4, "hipMalloc build"));
+        GB_OK(tmp.need(T * 4, "hipMalloc build"));
+        for (Buf* b : {&gstart, &gcount, &gmesh, &gnode, &flag, &scan}) GB_OK(b->need(G * 4, "hipMalloc build"));
+        for (Buf* b : {&node_group, &node_mesh, &node_level, &child_first, &child_count, &size, &start, &id})
+            GB_OK(b->need(N * 4, "hipMalloc build"));
+        GB_OK(segs.need(N * sizeof(Seg), "hipMalloc build"));
+        GB_OK(small.need(N * sizeof(Seg), "hipMalloc build"));
+        GB_OK(large.need(N * sizeof(Seg), "hipMalloc build"));
+        GB_OK(med.need(N * sizeof(MedianReq), "hipMalloc build"));
+        GB_OK(split_left.need(3 * N * 4, "hipMalloc build"));
+        GB_OK(counters.need(16, "hipMalloc build"));
+        GB_OK(mark.need(T * 4, "hipMalloc build"));
+        GB_OK(lrank.need(T * 4, "hipMalloc build"));
+        GB_OK(summary.need(16ull * mesh_count + 16, "hipMalloc build"));
+        return RQ_OK;
+    }
+};
+
+struct HostMesh {
+    float lo[3], hi[3];
+    std::string err;
+    bool ok = true;
+};
+
+}  // namespace
+
+extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                                        uint32_t instance_count, int top_level, rq_scene** out, void* stream) {
+    if (!out || (!meshes && mesh_count) || (!instances && instance_count)) {
+        set_error("rq_scene_build_gpu: null argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    *out = nullptr;
+    for (uint32_t i = 0; i < instance_count; ++i)
+        if (instances[i].mesh >= mesh_count) {
+            set_error("rq_scene_build_gpu: instance names a missing mesh");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    // The host builder's input checks (near_field.rs:206-212) and the mesh
+    // bounds over all positions (src/gi_bvh.rs:64-87), one pass per mesh.
+    std::vector<HostMesh> hm(mesh_count);
+    {
+        uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 64));
+        std::vector<std::thread> pool;
+        std::atomic_uint next{0};
+        for (uint32_t t = 0; t < std::min(threads, std::max(mesh_count, 1u)); ++t)
+            pool.emplace_back([&]() {
+                for (;;) {
+                    const uint32_t m = next.fetch_add(1);
+                    if (m >= mesh_count) break;
+                    const rq_mesh& src = meshes[m];
+                    HostMesh& h = hm[m];
+                    if (src.index_count == 0 || src.index_count % 3 != 0 || !src.indices) {
+                        h.ok = false;
+                        h.err = "BVH geometry needs a nonempty triangle list";
+                        continue;
+                    }
+                    for (uint32_t k = 0; k < src.index_count && h.ok; ++k)
+                        if (src.indices[k] >= src.vertex_count) {
+                            h.ok = false;
+                            h.err = "BVH geometry has an invalid index or position";
+                        }
+                    for (int a = 0; a < 3; ++a) {
+                        h.lo[a] = __builtin_inff();
+                        h.hi[a] = -__builtin_inff();
+                    }
+                    for (uint64_t v = 0; v < src.vertex_count && h.ok; ++v)
+                        for (int a = 0; a < 3; ++a) {
+                            const float p = src.positions[3 * v + a];
+                            if (!std::isfinite(p)) {
+                                h.ok = false;
+                                h.err = "BVH geometry has an invalid index or position";
+                            }
+                            h.lo[a] = mn(h.lo[a], p);
+                            h.hi[a] = mx(h.hi[a], p);
+                        }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    for (uint32_t m = 0; m < mesh_count; ++m)
+        if (!hm[m].ok) {
+            set_error("indirect geometry is not a valid triangle list: " + hm[m].err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    std::vector<float> inverses(16ull * instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i)
+        if (!mat4_inverse(instances[i].model, &inverses[16ull * i])) {
+            set_error("indirect geometry transform is singular");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    int device = 0;
+    GB_TRY(hipGetDevice(&device), "hipGetDevice");
+    GpuBuild b;
+    b.s = reinterpret_cast<hipStream_t>(stream);
+    b.mesh_count = mesh_count;
+    std::vector<MeshDev> md(mesh_count);
+    uint64_t tris = 0, verts = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        std::memset(&md[m], 0, sizeof(MeshDev));
+        md[m].tbase = (uint32_t)tris;
+        md[m].tcount = meshes[m].index_count / 3;
+        md[m].vbase = (uint32_t)verts;
+        md[m].vcount = meshes[m].vertex_count;
+        md[m].mode = MODE_SAH;
+        md[m].keep = meshes[m].keep_primitives ? 1u : 0u;
+        tris += md[m].tcount;
+        verts += meshes[m].vertex_count;
+    }
+    if (tris >= (1ull << 31) || 3 * verts >= (1ull << 32)) {
+        set_error("rq_scene_build_gpu: too many triangles or vertices for one build");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    b.tris = (uint32_t)tris;
+    b.verts = (uint32_t)verts;
+    GB_OK(b.meshes.need(std::max<size_t>(mesh_count, 1) * sizeof(MeshDev), "hipMalloc build"));
+    GB_OK(b.pos.need(std::max<uint64_t>(verts, 1) * 12, "hipMalloc build"));
+    GB_OK(b.idx.need(std::max<uint64_t>(tris, 1) * 12, "hipMalloc build"));
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        if (meshes[m].vertex_count)
+            GB_TRY(hipMemcpyAsync(b.pos.as<float>() + 3ull * md[m].vbase, meshes[m].positions,
+                                  12ull * meshes[m].vertex_count, hipMemcpyHostToDevice, b.s),
+                   "upload positions");
+        GB_TRY(hipMemcpyAsync(b.idx.as<uint32_t>() + 3ull * md[m].tbase, meshes[m].indices,
+                              4ull * meshes[m].index_count, hipMemcpyHostToDevice, b.s),
+               "upload indices");
+    }
+    GB_OK(b.allocate());
+    std::vector<uint32_t> summary;
+    if (mesh_count) {
+        GB_TRY(hipMemcpyAsync(b.meshes.p, md.data(), mesh_count * sizeof(MeshDev), hipMemcpyHostToDevice, b.s),
+               "upload meshes");
+        GB_OK(b.topology());
+        GB_OK(b.numbering(summary));
+        // near_field.rs:184-190: too deep with SAH, median splits throughout instead
+        bool again = false;
+        for (uint32_t m = 0; m < mesh_count; ++m)
+            if (summary[4 * m + 3] > MAX_SAH_DEPTH) {
+                md[m].mode = MODE_MEDIAN;
+                again = true;
+            }
+        if (again) {
+            GB_TRY(hipMemcpyAsync(b.meshes.p, md.data(), mesh_count * sizeof(MeshDev), hipMemcpyHostToDevice, b.s),
+                   "upload meshes");
+            GB_OK(b.topology());
+            GB_OK(b.numbering(summary));
+        }
+    }
+    // src/gi_bvh.rs:151-179 (layout)
+    const uint32_t* sm = summary.data();
+    uint64_t at = HEADER_WORDS + (uint64_t)instance_count * INSTANCE_WORDS + top_reserve(instance_count);
+    std::vector<uint32_t> offsets(4ull * mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        const uint64_t nodes = at, leaves = nodes + (uint64_t)sm[4 * m] * NODE_WORDS;
+        const uint64_t positions = leaves + 2ull * sm[4 * m + 1], indices = positions + 3ull * md[m].vcount;
+        const uint64_t end = indices + 3ull * md[m].tcount + (md[m].keep ? md[m].tcount : 0);
+        if (end >= 0xffffffffull) {
+            set_error("packed geometry exceeds 2^32 words");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        md[m].nodes = offsets[4 * m] = (uint32_t)nodes;
+        md[m].leaves = offsets[4 * m + 1] = (uint32_t)leaves;
+        md[m].positions = offsets[4 * m + 2] = (uint32_t)positions;
+        md[m].indices = offsets[4 * m + 3] = (uint32_t)indices;
+        md[m].prims = (uint32_t)(indices + 3ull * md[m].tcount);
+        md[m].node_count = sm[4 * m];
+        md[m].leaf_count = sm[4 * m + 1];
+        at = end;
+    }
+    auto scene = new rq_scene();
+    std::vector<uint32_t>& words = scene->words;
+    words.assign(at, 0u);
+    DeviceLayout& d = scene->dev[0];
+    d.bytes = (at * 4 + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&d.base, d.bytes);
+    if (e != hipSuccess) {
+        delete scene;
+        return hip_fail(e, "hipMalloc scene");
+    }
+    auto fail_scene = [&](rq_status st) {
+        rq_scene_destroy(scene);
+        return st;
+    };
+    uint32_t* w = reinterpret_cast<uint32_t*>(d.base);
+    if (hipMemsetAsync(w, 0, d.bytes, b.s) != hipSuccess) return fail_scene(hip_fail(hipGetLastError(), "memset"));
+    if (mesh_count) {
+        if (hipMemcpyAsync(b.meshes.p, md.data(), mesh_count * sizeof(MeshDev), hipMemcpyHostToDevice, b.s) != hipSuccess)
+            return fail_scene(RQ_ERR_DEVICE);
+        if (b.node_total)
+            hipLaunchKernelGGL(write_nodes, grid_for(b.node_total), dim3(256), 0, b.s, b.node_total,
+                               b.node_mesh.as<uint32_t>(), b.meshes.as<MeshDev>(), b.id.as<uint32_t>(),
+                               b.child_first.as<uint32_t>(), b.child_count.as<uint32_t>(), b.gnode.as<uint32_t>(),
+                               b.gstart.as<uint32_t>(), b.lrank.as<uint32_t>(), w);
+        const uint32_t groups = b.level_gbase.back();
+        hipLaunchKernelGGL(write_leaves, grid_for(groups), dim3(256), 0, b.s, b.gstart.as<uint32_t>(),
+                           b.gcount.as<uint32_t>(), b.gmesh.as<uint32_t>(), groups, b.meshes.as<MeshDev>(),
+                           b.lrank.as<uint32_t>(), w);
+        hipLaunchKernelGGL(write_tris, grid_for(b.tris), dim3(256), 0, b.s, b.meshes.as<MeshDev>(), mesh_count, b.tris,
+                           b.perm.as<uint32_t>(), b.idx.as<uint32_t>(), w);
+        hipLaunchKernelGGL(write_positions, grid_for(3ull * b.verts), dim3(256), 0, b.s, b.meshes.as<MeshDev>(),
+                           mesh_count, 3 * b.verts, b.pos.as<float>(), w);
+        // node boxes, deepest level first (children are final before parents)
+        for (size_t l = b.level_nbase.size() - 1; l-- > 0;) {
+            const uint32_t nb = b.level_nbase[l], nn = b.level_nbase[l + 1] - nb;
+            if (nn)
+                hipLaunchKernelGGL(refit_nodes, grid_for(nn), dim3(256), 0, b.s, nb, nn, b.node_mesh.as<uint32_t>(),
+                                   b.meshes.as<MeshDev>(), b.id.as<uint32_t>(), w);
+        }
+        if (hipGetLastError() != hipSuccess) return fail_scene(hip_fail(hipGetLastError(), "build words launch"));
+    }
+    // src/gi_bvh.rs:230-274 (instance_words) + src/gi_cache.rs:4388-4479, as rq_scene_build.
+    std::vector<uint32_t> head(HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS, 0u);
+    head[0] = instance_count;
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        const rq_instance& ins = instances[i];
+        const uint32_t m = ins.mesh;
+        const float* inv = &inverses[16ull * i];
+        const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        for (int row = 0; row < 3; ++row)
+            for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
+        instance_box(ins.model, hm[m].lo, hm[m].hi, &head[r + 12]);
+        for (int k = 0; k < 4; ++k) head[r + 18 + k] = offsets[4 * m + k];
+        head[r + 22] = sm[4 * m + 2];
+        head[r + 23] = 0;
+        head[r + 24] = ins.identity;
+        head[r + 27] = ins.flags & 1u;
+        head[r + 28] = md[m].keep ? md[m].prims : 0u;
+        head[r + 29] = ins.placement;
+    }
+    head[4] = instance_count;
+    top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
+    std::copy(head.begin(), head.end(), words.begin());
+    if (hipMemcpyAsync(w, words.data(), head.size() * 4, hipMemcpyHostToDevice, b.s) != hipSuccess ||
+        hipStreamSynchronize(b.s) != hipSuccess)
+        return fail_scene(hip_fail(hipGetLastError(), "build head"));
+    if (at > head.size()) scene->stale.emplace_back(head.size(), at - head.size());
+    d.ready = true;
+    scene->device = device;
+    uint64_t tri_inst = 0, nodes = 0;
+    uint32_t depth = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        nodes += sm[4 * m];
+        depth = std::max(depth, sm[4 * m + 3]);
+    }
+    for (uint32_t i = 0; i < instance_count; ++i) tri_inst += md[instances[i].mesh].tcount;
+    scene->stats[0] = instance_count;
+    scene->stats[1] = mesh_count;
+    scene->stats[2] = tri_inst;
+    scene->stats[3] = tris;
+    scene->stats[4] = nodes;
+    scene->stats[5] = words[2];
+    scene->stats[6] = depth;
+    scene->models.resize(16ull * instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i)
+        std::copy(instances[i].model, instances[i].model + 16, &scene->models[16ull * i]);
+    scene->mesh_nodes.resize(mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
+    scene->top_force = top_level;
+    scene_bounds(scene);
+    *out = scene;
+    return RQ_OK;
+}

@@ -12,17 +12,14 @@
 #include <cmath>
 #include <cstring>
 
+#include "bvh_device.h"
 #include "internal.h"
 
 namespace rq {
 namespace refitk {
 
-// The host builder's min/max (keep the first argument unless the second is
-// strictly smaller / larger): the refit must round and order like the build.
-__device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }
-__device__ __forceinline__ float mx(float a, float b) { return b > a ? b : a; }
-__device__ __forceinline__ float bf(uint32_t w) { return __uint_as_float(w); }
-__device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+using bvhdev::bf;
+using bvhdev::fb;
 
 struct Args {
     uint32_t* words;
@@ -34,94 +31,7 @@ struct Args {
 __global__ __launch_bounds__(256) void refit_level(Args a) {
     const uint32_t k = a.begin + blockIdx.x * 256u + threadIdx.x;
     if (k >= a.end) return;
-    const uint32_t n = a.order[k];
-    uint32_t* node = a.words + a.nodes + 16ull * n;
-    uint32_t children[4];
-    float blo[4][3], bhi[4][3];
-    uint32_t count = 0;
-    for (int c = 0; c < 4; ++c) {
-        const uint32_t child = node[6 + c];
-        if (child == EMPTY) continue;
-        float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-        float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-        if (child & LEAF_BIT) {
-            const uint32_t l = child & ~LEAF_BIT;
-            const uint32_t off = a.words[a.leaves + 2ull * l], cnt = a.words[a.leaves + 2ull * l + 1];
-            for (uint32_t i = off; i < off + cnt; ++i) {
-                const uint32_t* p = a.words + a.positions + 3ull * a.words[a.indices + i];
-                for (int x = 0; x < 3; ++x) {
-                    lo[x] = mn(lo[x], bf(p[x]));
-                    hi[x] = mx(hi[x], bf(p[x]));
-                }
-            }
-        } else {
-            const uint32_t* cn = a.words + a.nodes + 16ull * child;
-            for (int slot = 0; slot < 4; ++slot) {
-                if (cn[6 + slot] == EMPTY) continue;
-                for (int fused = 0; fused < 2; ++fused)
-                    for (int x = 0; x < 3; ++x) {
-                        const float o = bf(cn[x]), s = bf(cn[3 + x]);
-                        const float ql = (float)((cn[10 + x] >> (8 * slot)) & 0xffu);
-                        const float qh = (float)((cn[13 + x] >> (8 * slot)) & 0xffu);
-                        const float l = fused ? __builtin_fmaf(s, ql, o) : o + s * ql;
-                        const float h = fused ? __builtin_fmaf(s, qh, o) : o + s * qh;
-                        lo[x] = mn(lo[x], l);
-                        hi[x] = mx(hi[x], h);
-                    }
-            }
-        }
-        children[count] = child;
-        for (int x = 0; x < 3; ++x) {
-            blo[count][x] = lo[x];
-            bhi[count][x] = hi[x];
-        }
-        ++count;
-    }
-    // quantized_node
-    float all_lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-    float all_hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-    for (uint32_t c = 0; c < count; ++c)
-        for (int x = 0; x < 3; ++x) {
-            all_lo[x] = mn(all_lo[x], blo[c][x]);
-            all_hi[x] = mx(all_hi[x], bhi[c][x]);
-        }
-    uint32_t out[16];
-    float step[3];
-    const float grow = 1.0f + 1.0f / 1048576.0f;
-    for (int x = 0; x < 3; ++x) {
-        const float extent = all_hi[x] - all_lo[x];
-        float s = mx(extent / 255.0f, mx(1.17549435e-38f, fabsf(all_lo[x]) * 1e-7f));
-        // grow until code 255 covers the top in both decodes (bounded: s grows
-        // by 2^-20 relative per pass and starts within a few ulps of the answer)
-        for (int guard = 0; guard < (1 << 22) && (all_lo[x] + s * 255.0f < all_hi[x] ||
-                                                   __builtin_fmaf(s, 255.0f, all_lo[x]) < all_hi[x]);
-             ++guard)
-            s *= grow;
-        step[x] = s;
-        out[x] = fb(all_lo[x]);
-        out[3 + x] = fb(s);
-    }
-    for (int c = 0; c < 4; ++c) out[6 + c] = c < (int)count ? children[c] : EMPTY;
-    for (int x = 0; x < 3; ++x) {
-        uint32_t lw = 0, hw = 0;
-        const float o = all_lo[x], s = step[x];
-        for (uint32_t c = 0; c < count; ++c) {
-            const float lv = blo[c][x], hv = bhi[c][x];
-            float fl = floorf((lv - o) / s);
-            fl = mn(mx(fl, 0.0f), 255.0f);
-            uint32_t low = (uint32_t)fl;
-            while (low > 0 && (o + s * (float)low > lv || __builtin_fmaf(s, (float)low, o) > lv)) low -= 1;
-            float fh = ceilf((hv - o) / s);
-            fh = mn(mx(fh, 0.0f), 255.0f);
-            uint32_t high = (uint32_t)fh;
-            while (high < 255 && (o + s * (float)high < hv || __builtin_fmaf(s, (float)high, o) < hv)) high += 1;
-            lw |= low << (8 * c);
-            hw |= high << (8 * c);
-        }
-        out[10 + x] = lw;
-        out[13 + x] = hw;
-    }
-    for (int w = 0; w < 16; ++w) node[w] = out[w];
+    bvhdev::refit_node(a.words, a.nodes, a.leaves, a.positions, a.indices, a.order[k]);
 }
 
 constexpr uint32_t BOUNDS_BLOCKS = 512;

@@ -436,10 +436,11 @@ class Scene:
         self._h = handle
 
     @classmethod
-    def build(cls, meshes, instances, top_level=-1, reflection=False):
+    def build(cls, meshes, instances, top_level=-1, reflection=False, gpu=False, stream=None):
         """meshes: [(positions (V,3), indices (3T,), keep_primitives)], instances:
         [dict(model=4x4 row-major or 16 column-major, mesh=i, flags=0, identity=0, placement=i)].
-        reflection=True packs the reflection rays' rl_geometry instead (traced with layout "rs")."""
+        reflection=True packs the reflection rays' rl_geometry instead (traced with layout "rs").
+        gpu=True builds the BVHs on the current GPU (same words; REF layout left uploaded)."""
         keep = []
         md = (_MeshDesc * max(len(meshes), 1))()
         for k, m in enumerate(meshes):
@@ -462,6 +463,12 @@ class Scene:
             idesc[k].identity = int(ins.get("identity", 0))
             idesc[k].placement = int(ins.get("placement", k))
         h = _P()
+        if gpu:
+            if reflection:
+                raise InvalidInput(1, "the GPU build packs GI geometry only")
+            check(lib.rq_scene_build_gpu(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h),
+                                         stream.ptr if stream else None))
+            return cls(h)
         fn = lib.rq_scene_build_reflection if reflection else lib.rq_scene_build
         check(fn(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
         return cls(h)

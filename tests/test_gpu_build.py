@@ -1,0 +1,179 @@
+"""GPU BVH build (rq_scene_build_gpu) against the host build (rq_scene_build, whose BVH4 builder
+restates near_field::build_bvh_sah and is pinned by the reference's own builder tests): the
+packed words must be equal word for word, on the golden fixture scenes and on meshes chosen to
+reach every path of the build -- groups split by one workgroup and by many (above 2048
+triangles), SAH splits and median splits (all centroids equal, in small and large groups), the
+median rebuild of a tree deeper than 15 levels, a mesh of at most 8 triangles (the root is a
+leaf), duplicate triangles with primitive ids, coordinates of both zero signs and far from the
+origin. A GPU-built scene then traces like the oracle and refits on the GPU."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import compare_exact, gpu_trace, to_device
+from renderlab_amd import rq
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def grid(n, z=0.0):
+    pos = np.array([[x, y, z] for y in range(n + 1) for x in range(n + 1)], np.float32) / n
+    idx = []
+    for y in range(n):
+        for x in range(n):
+            i, w = y * (n + 1) + x, n + 1
+            idx += [i, i + 1, i + w + 1, i, i + w + 1, i + w]
+    return pos, np.array(idx, np.uint32)
+
+
+def strip(base, n, size=0.01):
+    xs = base ** np.arange(n)
+    pos = np.array([[x, 0, 0] for x in xs] + [[x + size, 0, 0] for x in xs] + [[x, size, 0] for x in xs], np.float32)
+    idx = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1).reshape(-1).astype(np.uint32)
+    return pos, idx
+
+
+def same_centroid(n, g):
+    """n different triangles that share one centroid: SAH finds no plane, the median splits."""
+    d = g.normal(size=(n, 2, 3)).astype(np.float32)
+    pos = np.concatenate([d[:, 0], d[:, 1], -(d[:, 0] + d[:, 1])]).astype(np.float32)
+    c = pos.reshape(3, n, 3).astype(np.float32)
+    s = (c[0] + c[1]) + c[2]
+    keep = np.all(s == 0, axis=1)  # exactly zero in f32: the same centroid for every kept triangle
+    idx = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1)[keep].reshape(-1).astype(np.uint32)
+    return pos, idx
+
+
+def case_meshes():
+    g = np.random.default_rng(11)
+    meshes = []
+    gp, gi = grid(160)                                     # 51,200 triangles: chunked splits
+    meshes.append((gp, gi, 1))
+    soup = g.normal(size=(3 * 5000, 3)).astype(np.float32) * [4, 1, 0.5]
+    meshes.append((soup.astype(np.float32), np.arange(3 * 5000, dtype=np.uint32), 0))
+    p, i = same_centroid(6000, g)                          # large group, median
+    assert len(i) // 3 > SMALL_LIMIT, len(i) // 3
+    meshes.append((p, i, 0))
+    p, i = same_centroid(900, g)                           # small group, median
+    meshes.append((p, i, 0))
+    meshes.append((*strip(1.5, 210), 0))                   # deeper than 15: median rebuild
+    meshes.append((*strip(1.2, 450), 1))
+    meshes.append((*strip(2.0, 120), 0))                   # deep but within 15
+    tiny = g.normal(size=(15, 3)).astype(np.float32)       # 5 triangles: root is a leaf
+    meshes.append((tiny, np.arange(15, dtype=np.uint32), 1))
+    dp, di = grid(20)                                      # duplicated triangles, primitive ids kept
+    di = np.concatenate([di, di[: 3 * 150], di[3 * 40: 3 * 90]]).astype(np.uint32)
+    meshes.append((dp, di, 1))
+    zp = g.integers(-1, 2, size=(600, 3)).astype(np.float32) * np.float32(0.0)
+    zp[::2] *= -1.0                                        # +0 and -0 coordinates
+    zp[:, 1] = g.normal(size=600).astype(np.float32)
+    meshes.append((zp, g.integers(0, 600, size=3 * 400).astype(np.uint32), 0))
+    far = (g.normal(size=(3000, 3)) * 1e3 + [3e6, -2e6, 5e5]).astype(np.float32)
+    meshes.append((far, np.arange(3000, dtype=np.uint32), 0))
+    return meshes
+
+
+SMALL_LIMIT = 2048
+
+
+def instances(n_meshes, count, g):
+    out = []
+    for k in range(count):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, :3] *= g.uniform(0.5, 2.0)
+        m[:3, 3] = g.uniform(-20, 20, 3)
+        out.append(dict(model=m, mesh=k % n_meshes, flags=int(k % 5 == 2), identity=k, placement=k))
+    return out
+
+
+def assert_same_words(cpu, gpu):
+    a, b = cpu.words, gpu.words
+    assert a.size == b.size, (a.size, b.size)
+    diff = np.nonzero(a != b)[0]
+    assert diff.size == 0, f"{diff.size} words differ, first at {diff[:10]}"
+    assert cpu.stats() == gpu.stats()
+
+
+@pytest.fixture(scope="module")
+def cases():
+    g = np.random.default_rng(4)
+    meshes = case_meshes()
+    return meshes, instances(len(meshes), 48, g)
+
+
+def test_gpu_build_equals_host_build(cases):
+    meshes, inst = cases
+    print(f"case: build {len(meshes)} meshes, {sum(len(m[1]) // 3 for m in meshes)} triangles", flush=True)
+    cpu = rq.Scene.build(meshes, inst)
+    gpu = rq.Scene.build(meshes, inst, gpu=True)
+    assert_same_words(cpu, gpu)
+    assert gpu.device_bytes("ref") > 0
+
+
+@pytest.mark.parametrize("kind", ["grid", "instances", "flat"])
+def test_gpu_build_golden_scenes(kind):
+    z = np.load(os.path.join(GOLDEN, f"traverse_{kind}.npz"))
+    meshes = []
+    k = 0
+    while f"mesh{k}_pos" in z:
+        meshes.append((z[f"mesh{k}_pos"], z[f"mesh{k}_idx"], int(z[f"mesh{k}_prims"][0])))
+        k += 1
+    inst = [dict(model=z["inst_models"][i], mesh=int(z["inst_mesh"][i]), flags=int(z["inst_flags"][i]))
+            for i in range(len(z["inst_mesh"]))]
+    print(f"case: golden {kind}", flush=True)
+    gpu = rq.Scene.build(meshes, inst, gpu=True)
+    assert np.array_equal(gpu.words, z["words"])
+    rays = z["rays"].view(oracle.RAY_DTYPE)
+    h, _ = gpu_trace(gpu, "ref", rays, rq.CLOSEST)
+    assert len(compare_exact(h, z["hits0"].view(oracle.HIT_DTYPE))) == 0
+
+
+def test_gpu_built_scene_traces_and_refits(cases):
+    meshes, inst = cases
+    g = np.random.default_rng(8)
+    gpu = rq.Scene.build(meshes, inst, gpu=True)
+    gpu.upload(("plain", "cw8"))
+    words = gpu.words
+    rays = np.zeros(4000, oracle.RAY_DTYPE)
+    rays["origin"] = g.uniform(-30, 30, (4000, 3))
+    rays["direction"] = g.normal(size=(4000, 3))
+    rays["tmax"] = 1e20
+    print("case: trace GPU-built scene", flush=True)
+    o, _, _ = oracle.traverse(words, rays, 0)
+    for layout in ("ref", "plain", "cw8"):
+        h, _ = gpu_trace(gpu, layout, rays, rq.CLOSEST)
+        assert len(compare_exact(h, o)) == 0, layout
+    new = (meshes[0][0] + g.normal(scale=0.01, size=meshes[0][0].shape)).astype(np.float32)
+    print("case: refit GPU-built scene", flush=True)
+    gpu.refit_mesh(0, to_device(new), len(new))
+    moved = list(meshes)
+    moved[0] = (new, meshes[0][1], meshes[0][2])
+    # refit keeps the built topology: every mesh's words but the refit one match a host build of
+    # the moved meshes only where the tree does not depend on the positions; check traversal instead
+    words = gpu.words
+    o, _, _ = oracle.traverse(words, rays, 0)
+    for layout in ("ref", "plain"):
+        h, _ = gpu_trace(gpu, layout, rays, rq.CLOSEST)
+        assert len(compare_exact(h, o)) == 0, layout
+
+
+def test_gpu_build_rejects_bad_input(cases):
+    meshes, inst = cases
+    bad_idx = [(meshes[7][0], np.array([0, 1, 99], np.uint32), 0)]
+    nan = meshes[7][0].copy()
+    nan[2, 1] = np.nan
+    one = [dict(model=np.eye(4, dtype=np.float32), mesh=0)]
+    print("case: build errors", flush=True)
+    with pytest.raises(rq.InvalidInput):
+        rq.Scene.build(bad_idx, one, gpu=True)
+    with pytest.raises(rq.InvalidInput):
+        rq.Scene.build([(nan, meshes[7][1], 0)], one, gpu=True)
+    with pytest.raises(rq.InvalidInput):
+        rq.Scene.build([(meshes[7][0], np.zeros(0, np.uint32), 0)], one, gpu=True)
+    with pytest.raises(rq.InvalidInput):
+        rq.Scene.build([meshes[7]], [dict(model=np.zeros((4, 4), np.float32), mesh=0)], gpu=True)
+    with pytest.raises(rq.InvalidInput):
+        rq.Scene.build([meshes[7]], [dict(model=np.eye(4, dtype=np.float32), mesh=3)], gpu=True)
