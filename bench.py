@@ -349,6 +349,48 @@ def run_reflection(rq, kind, meshes, tris, frame, reps=3):
             "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words))}
 
 
+def run_refit(rq, n=724, instances=4, reps=20):
+    """GPU refit of one skinned-style mesh (an n x n grid, 2n^2 triangles, drawn by `instances`
+    instances) to new poses: BVH refit, instance boxes, top-level place and the PLAIN refresh,
+    against near_field::refit_bvh (the CPU restatement) on the same mesh and poses."""
+    pos = np.stack(np.meshgrid(np.arange(n + 1), np.arange(n + 1)), -1).reshape(-1, 2) / n
+    pos = np.concatenate([pos, np.zeros((len(pos), 1))], 1).astype(np.float32)
+    q = (np.arange(n)[None, :] + (n + 1) * np.arange(n)[:, None]).reshape(-1)
+    idx = np.stack([q, q + 1, q + n + 2, q, q + n + 2, q + n + 1], 1).reshape(-1).astype(np.uint32)
+
+    def pose(k):
+        p = pos.astype(np.float64).copy()
+        p[:, 2] += 0.15 * np.sin(6.0 * p[:, 0] + k) * np.cos(4.0 * p[:, 1])
+        p[:, :3] += [0.01 * k, 0.0, 0.0]
+        return p.astype(np.float32)
+
+    inst = []
+    for k in range(instances):
+        m = np.eye(4, dtype=np.float32)
+        m[0, 3] = 1.5 * k
+        inst.append(dict(model=m, mesh=0))
+    sc = rq.Scene.build([(pos, idx, 1)], inst)
+    sc.upload(("ref", "plain"))
+    poses = [pose(k) for k in range(3)]
+    dev = [rq.DeviceBuffer.from_array(p) for p in poses]
+    for k in range(3):
+        sc.refit_mesh(0, dev[k], len(pos))
+    t0 = time.perf_counter()
+    for r in range(reps):
+        sc.refit_mesh(0, dev[r % 3], len(pos))
+    gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+    bvh = rq.build_bvh_sah(pos, idx)
+    t0 = time.perf_counter()
+    for r in range(3):
+        rq.refit_bvh(bvh, poses[r % 3])
+    cpu_ms = (time.perf_counter() - t0) / 3 * 1e3
+    tris = len(idx) // 3
+    return {"triangles": tris, "vertices": len(pos), "instances": instances, "gpu_ms": round(gpu_ms, 3),
+            "gpu_mtris_s": round(tris / gpu_ms / 1e3, 1), "cpu_refit_bvh_ms": round(cpu_ms, 2), "cpu_threads": 1,
+            "speedup": round(cpu_ms / gpu_ms, 1),
+            "note": "gpu_ms: host clock around the synchronous call (bounds, refit, records, top level, PLAIN refresh)"}
+
+
 def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
     """The oracle traversal (scalar C port of gi_traverse) on this host's cores, over a bounded sample."""
     import oracle
@@ -393,6 +435,7 @@ def main():
                          "one launch per step with the next frame's primary rays (pipelined)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
     ap.add_argument("--no-reflection", action="store_true", help="skip the reflection-ray (rs_trace) line")
+    ap.add_argument("--no-refit", action="store_true", help="skip the GPU mesh refit line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -412,9 +455,23 @@ def main():
         del src, dst
     kind, meshes, tris, w, h, n_gi, desc = CONFIGS[args.config]
     t0 = time.perf_counter()
-    scene = rq.Scene.generate(kind, 1, meshes, tris)
-    st = scene.stats()
-    log(f"scene {args.config}: {st} built in {time.perf_counter() - t0:.1f}s")
+    host_scene = rq.Scene.generate(kind, 1, meshes, tris)
+    host_s = time.perf_counter() - t0
+    st = host_scene.stats()
+    log(f"scene {args.config}: {st} synthesized and built on the host in {host_s:.2f}s")
+    t0 = time.perf_counter()
+    scene = rq.Scene.generate(kind, 1, meshes, tris, gpu=True)  # the same words, BVHs built on the GPU
+    gpu_s = time.perf_counter() - t0
+    same = np.array_equal(scene.words, host_scene.words)
+    if not same:
+        raise SystemExit("GPU-built scene words differ from the host build")
+    hb, gb = host_scene.stats()["build_us"], scene.stats()["build_us"]
+    build = {"host_build_ms": round(hb / 1e3, 2), "gpu_build_ms": round(gb / 1e3, 2), "speedup": round(hb / max(gb, 1), 2),
+             "words_equal": bool(same), "host_threads": os.cpu_count(),
+             "note": "rq_scene_build (host BVH builder, threads over meshes) vs rq_scene_build_gpu, each timed "
+                     "inside the call (inputs in host memory, packed words out); mesh synthesis excluded"}
+    del host_scene
+    log(f"GPU build of the same scene: {gpu_s:.3f}s with synthesis; build only: {build}")
     layouts = [args.layout] + ([] if args.no_compare or args.layout == "plain" else ["plain"])
     t0 = time.perf_counter()
     scene.upload(tuple(set(layouts) | {"plain", "ref"}))  # REF: original positions for the FP64 check
@@ -534,6 +591,7 @@ def main():
     if verify:
         line["fp64_verify"] = verify
     line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}
+    line["scene_build"] = build
     if not args.no_reflection:
         line["reflection_rs"] = run_reflection(rq, kind, meshes, tris, frame)
         log(f"reflection rays (rs_trace, one launch of primary + GI + shadow): {line['reflection_rs']}")
@@ -547,6 +605,9 @@ def main():
         line["serving_sweep"] = [run_serving(rq, args.layout, [scene], rate=r, n_jobs=int(min(400, max(40, r))),
                                              resolutions=res, mixes=[0, 1]) for r in (50, 200, 500, 1000, 2000)]
         log(f"serving sweep: {line['serving_sweep']}")
+    if not args.no_refit:
+        line["refit_1m"] = run_refit(rq)
+        log(f"mesh refit (1M triangles): {line['refit_1m']}")
     if args.batching:
         line["batching_c3"] = run_batching(rq, args.layout)
         log(f"batching: {line['batching_c3']}")

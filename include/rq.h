@@ -192,7 +192,8 @@ rq_status rq_scene_from_rl_words(const uint32_t* words, size_t word_count, rq_sc
    (host memory owned by the scene). */
 rq_status rq_scene_gi_words(const rq_scene* scene, const uint32_t** words, size_t* word_count);
 /* [0] instances, [1] meshes, [2] triangles (over instances), [3] unique triangles,
-   [4] BLAS nodes, [5] top-level root tag, [6] max BLAS depth */
+   [4] BLAS nodes, [5] top-level root tag, [6] max BLAS depth, [7] microseconds the
+   build took (rq_scene_build / rq_scene_build_gpu; 0 for adopted words) */
 rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]);
 /* Uploads the requested layouts (bit 1<<rq_layout) to the current device. */
 rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
@@ -385,6 +386,9 @@ rq_status rq_event_query(void* event); /* RQ_OK when complete, RQ_ERR_NOT_READY 
            `meshes` meshes and about `triangles` triangles in all. */
 rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
                        rq_scene** out);
+/* rq_gen_scene with the BVHs built by rq_scene_build_gpu (the same words). */
+rq_status rq_gen_scene_gpu(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, rq_scene** out,
+                           void* stream);
 /* The same meshes and placements packed as reflection geometry (rl_geometry). */
 rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
                                   rq_scene** out);

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <thread>
 
@@ -146,6 +147,79 @@ __device__ __forceinline__ uint32_t median_axis(const float extent[3]) {
     return axis;
 }
 
+// One candidate plane of the sweep above, evaluated on its own: c = axis * 15 +
+// plane, the host's visiting order. The left and right boxes are the same
+// min/max folds the host's two sweeps make, so the cost is the host's.
+__device__ inline bool plane_cost(const uint32_t* cnt, const int32_t* box, const bool valid[3], uint32_t size,
+                                  uint32_t c, float& cost) {
+    const uint32_t axis = c / (BINS - 1), plane = c % (BINS - 1);
+    if (c >= 3 * (BINS - 1) || !valid[axis]) return false;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    for (int i = 0; i < 3; ++i) {
+        llo[i] = rlo[i] = __builtin_inff();
+        lhi[i] = rhi[i] = -__builtin_inff();
+    }
+    uint32_t nl = 0, nr = 0;
+    for (uint32_t b = 0; b < BINS; ++b) {
+        const int32_t* bb = box + (axis * BINS + b) * 6;
+        float* lo = b <= plane ? llo : rlo;
+        float* hi = b <= plane ? lhi : rhi;
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = bvhdev::mn(lo[i], okey_inv(bb[i]));
+            hi[i] = bvhdev::mx(hi[i], okey_inv(bb[3 + i]));
+        }
+        (b <= plane ? nl : nr) += cnt[axis * BINS + b];
+    }
+    if (nl == 0 || nl == size) return false;
+    cost = half_area(llo, lhi) * (float)nl + half_area(rlo, rhi) * (float)nr;
+    return true;
+}
+
+// The sequential choice over the 45 candidates (first one taken, then only a
+// strictly smaller cost replaces it) across one wave: the first candidate if
+// its cost is NaN, else the lowest-indexed candidate of least non-NaN cost.
+// Returns the winning lane, or -1 if no lane holds a candidate.
+__device__ inline int pick_plane(bool cand, float cost) {
+    const uint64_t m = __ballot(cand);
+    if (m == 0) return -1;
+    const int first = __builtin_ctzll(m);
+    const float fc = __shfl(cost, first, 64);
+    if (fc != fc) return first;
+    bool ok = cand && cost == cost;
+    float bc = cost;
+    int bl = (int)(threadIdx.x & 63u);
+    for (int o = 32; o > 0; o >>= 1) {
+        const bool ook = __shfl_xor((int)ok, o, 64) != 0;
+        const float oc = __shfl_xor(bc, o, 64);
+        const int ol = __shfl_xor(bl, o, 64);
+        if (ook && (!ok || oc < bc || (oc == bc && ol < bl))) {
+            ok = true;
+            bc = oc;
+            bl = ol;
+        }
+    }
+    return bl;
+}
+
+// Exclusive prefix sum of v over a 256-thread workgroup; *total gets the sum.
+__device__ inline uint32_t block_exclusive(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < 4; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    *total = all;
+    return before + x - v;
+}
+
 __device__ __forceinline__ void ask_median(MedianReq* med, uint32_t* med_count, const Seg& s, uint32_t axis) {
     const uint32_t slot = atomicAdd(med_count, 1u);
     MedianReq r;
@@ -198,6 +272,19 @@ __global__ __launch_bounds__(256) void root_groups(const MeshDev* meshes, uint32
     gmesh[m] = m;
 }
 
+// Appends `s` where `want` holds, one atomic per wave (list order is free:
+// every entry is processed on its own). All lanes of the wave call this.
+__device__ inline void append(bool want, const Seg& s, Seg* list, uint32_t* count) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(m));
+    base = __shfl(base, leader, 64);
+    if (want) list[base + __builtin_popcountll(m & ((1ull << lane) - 1))] = s;
+}
+
 // ---------------------------------------------------------------- classify
 __global__ __launch_bounds__(256) void node_flags(const uint32_t* gcount, uint32_t n, uint32_t* flag) {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
@@ -233,19 +320,24 @@ __global__ __launch_bounds__(256) void round1_segs(const uint32_t* gstart, const
                                                    const uint32_t* node_group, uint32_t nbase, uint32_t nn,
                                                    const uint32_t* split_left, Seg* segs, uint32_t* seg_count) {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r >= nn) return;
-    const uint32_t gi = node_group[nbase + r];
-    const uint32_t start = gstart[gi], count = gcount[gi], l0 = split_left[3 * r];
+    const bool live = r < nn;
+    uint32_t start = 0, count = 0, l0 = 0, mesh = 0;
+    if (live) {
+        const uint32_t gi = node_group[nbase + r];
+        start = gstart[gi];
+        count = gcount[gi];
+        l0 = split_left[3 * r];
+        mesh = gmesh[gi];
+    }
     const uint32_t hs[2] = {start, start + l0}, hc[2] = {l0, count - l0};
-    for (int h = 0; h < 2; ++h)
-        if (hc[h] > LEAF_TRIANGLES) {
-            Seg s;
-            s.start = hs[h];
-            s.count = hc[h];
-            s.mesh = gmesh[gi];
-            s.out = 3 * r + 1 + h;
-            segs[atomicAdd(seg_count, 1u)] = s;
-        }
+    for (int h = 0; h < 2; ++h) {
+        Seg s;
+        s.start = hs[h];
+        s.count = hc[h];
+        s.mesh = mesh;
+        s.out = 3 * r + 1 + h;
+        append(live && hc[h] > LEAF_TRIANGLES, s, segs, seg_count);
+    }
 }
 
 __global__ __launch_bounds__(256) void child_counts(const uint32_t* gcount, const uint32_t* node_group, uint32_t nbase,
@@ -292,10 +384,12 @@ __global__ __launch_bounds__(256) void emit_children(uint32_t* gstart, uint32_t*
 // ---------------------------------------------------------------- splits
 __global__ __launch_bounds__(256) void route(const Seg* segs, uint32_t n, Seg* small, Seg* large, uint32_t* counts) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const Seg s = segs[i];
-    if (s.count <= SMALL_MAX) small[atomicAdd(&counts[0], 1u)] = s;
-    else large[atomicAdd(&counts[1], 1u)] = s;
+    const bool live = i < n;
+    Seg s;
+    if (live) s = segs[i];
+    const bool big = live && s.count > SMALL_MAX;
+    append(live && !big, s, small, &counts[0]);
+    append(big, s, large, &counts[1]);
 }
 
 // One workgroup splits one group of at most SMALL_MAX triangles in place.
@@ -306,9 +400,9 @@ __global__ __launch_bounds__(256) void split_small(const Seg* segs, const MeshDe
     __shared__ int32_t bounds[6];
     __shared__ uint32_t cnt[3 * BINS];
     __shared__ int32_t bins[3 * BINS * 6];
-    __shared__ uint32_t runs[256];
+    __shared__ uint32_t runs[4];
     __shared__ int s_found, s_axis;
-    __shared__ uint32_t s_plane, s_total;
+    __shared__ uint32_t s_plane;
     const Seg sg = segs[blockIdx.x];
     const uint32_t n = sg.count, t = threadIdx.x;
     if (t < 3) {
@@ -363,12 +457,15 @@ __global__ __launch_bounds__(256) void split_small(const Seg* segs, const MeshDe
             }
         }
         __syncthreads();
-        if (t == 0) {
-            int axis = 0;
-            uint32_t plane = 0;
-            s_found = sah_sweep(cnt, bins, valid, n, axis, plane) ? 1 : 0;
-            s_axis = axis;
-            s_plane = plane;
+        if (t < 64) {
+            float cost = 0.0f;
+            const bool cand = plane_cost(cnt, bins, valid, n, t, cost);
+            const int win = pick_plane(cand, cost);
+            if (t == 0) {
+                s_found = win >= 0;
+                s_axis = win >= 0 ? win / (int)(BINS - 1) : 0;
+                s_plane = win >= 0 ? (uint32_t)win % (BINS - 1) : 0;
+            }
         }
         __syncthreads();
     }
@@ -381,20 +478,9 @@ __global__ __launch_bounds__(256) void split_small(const Seg* segs, const MeshDe
     const uint32_t per = (n + 255) / 256, b0 = min(t * per, n), b1 = min(b0 + per, n);
     uint32_t nl = 0;
     for (uint32_t i = b0; i < b1; ++i) nl += bin_of(cent[3ull * ids[i] + ax], clo[ax], extent[ax]) <= plane;
-    runs[t] = nl;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int j = 0; j < 256; ++j) {
-            const uint32_t v = runs[j];
-            runs[j] = acc;
-            acc += v;
-        }
-        s_total = acc;
-    }
-    __syncthreads();
-    const uint32_t nleft = s_total;
-    uint32_t l = runs[t], r = b0 - runs[t];
+    uint32_t nleft;
+    const uint32_t before = block_exclusive(nl, runs, &nleft);
+    uint32_t l = before, r = b0 - before;
     for (uint32_t i = b0; i < b1; ++i) {
         const uint32_t id = ids[i];
         if (bin_of(cent[3ull * id + ax], clo[ax], extent[ax]) <= plane) perm[sg.start + l++] = id;
@@ -550,7 +636,7 @@ __global__ __launch_bounds__(256) void large_scatter(const Chunk* chunks, const 
                                                      const uint32_t* first_chunk, const uint32_t* chunk_off,
                                                      const uint32_t* seg_left, const uint32_t* perm, const float* cent,
                                                      const int32_t* lb, uint32_t* tmp) {
-    __shared__ uint32_t runs[256];
+    __shared__ uint32_t runs[4];
     const Chunk c = chunks[blockIdx.x];
     const uint32_t bst = best[c.seg];
     if (bst == EMPTY) return;
@@ -559,20 +645,11 @@ __global__ __launch_bounds__(256) void large_scatter(const Chunk* chunks, const 
                    b1 = min(b0 + per, c.count);
     uint32_t nl = 0;
     for (uint32_t i = b0; i < b1; ++i) nl += large_left(lb + 6 * c.seg, bst, cent, perm[c.start + i]);
-    runs[t] = nl;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int j = 0; j < 256; ++j) {
-            const uint32_t v = runs[j];
-            runs[j] = acc;
-            acc += v;
-        }
-    }
-    __syncthreads();
+    uint32_t all;
+    const uint32_t before = block_exclusive(nl, runs, &all);
     const uint32_t lefts_before = chunk_off[blockIdx.x] - chunk_off[first_chunk[c.seg]];
     const uint32_t rights_before = (c.start - s.start) - lefts_before;
-    uint32_t l = lefts_before + runs[t], r = rights_before + (b0 - runs[t]);
+    uint32_t l = lefts_before + before, r = rights_before + (b0 - before);
     const uint32_t nleft = seg_left[c.seg];
     for (uint32_t i = b0; i < b1; ++i) {
         const uint32_t id = perm[c.start + i];
@@ -1060,6 +1137,7 @@ struct HostMesh {
 
 extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                                         uint32_t instance_count, int top_level, rq_scene** out, void* stream) {
+    const auto began = std::chrono::steady_clock::now();
     if (!out || (!meshes && mesh_count) || (!instances && instance_count)) {
         set_error("rq_scene_build_gpu: null argument");
         return RQ_ERR_INVALID_INPUT;
@@ -1071,7 +1149,19 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
             return RQ_ERR_INVALID_INPUT;
         }
     // The host builder's input checks (near_field.rs:206-212) and the mesh
-    // bounds over all positions (src/gi_bvh.rs:64-87), one pass per mesh.
+    // bounds over all positions (src/gi_bvh.rs:64-87), one pass per mesh that
+    // also gathers every mesh into one staging array each (two uploads in all).
+    std::vector<uint64_t> vbase(mesh_count + 1, 0), tbase(mesh_count + 1, 0);
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        vbase[m + 1] = vbase[m] + meshes[m].vertex_count;
+        tbase[m + 1] = tbase[m] + meshes[m].index_count / 3;
+    }
+    if (tbase[mesh_count] >= (1ull << 31) || 3 * vbase[mesh_count] >= (1ull << 32)) {
+        set_error("rq_scene_build_gpu: too many triangles or vertices for one build");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    std::vector<float> stage_pos(3 * vbase[mesh_count]);
+    std::vector<uint32_t> stage_idx(3 * tbase[mesh_count]);
     std::vector<HostMesh> hm(mesh_count);
     {
         uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 64));
@@ -1108,6 +1198,9 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
                             h.lo[a] = mn(h.lo[a], p);
                             h.hi[a] = mx(h.hi[a], p);
                         }
+                    if (!h.ok) continue;
+                    std::memcpy(&stage_pos[3 * vbase[m]], src.positions, 12ull * src.vertex_count);
+                    std::memcpy(&stage_idx[3 * tbase[m]], src.indices, 4ull * (src.index_count / 3) * 3);
                 }
             });
         for (auto& th : pool) th.join();
@@ -1129,36 +1222,25 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     b.s = reinterpret_cast<hipStream_t>(stream);
     b.mesh_count = mesh_count;
     std::vector<MeshDev> md(mesh_count);
-    uint64_t tris = 0, verts = 0;
+    const uint64_t tris = tbase[mesh_count], verts = vbase[mesh_count];
     for (uint32_t m = 0; m < mesh_count; ++m) {
         std::memset(&md[m], 0, sizeof(MeshDev));
-        md[m].tbase = (uint32_t)tris;
+        md[m].tbase = (uint32_t)tbase[m];
         md[m].tcount = meshes[m].index_count / 3;
-        md[m].vbase = (uint32_t)verts;
+        md[m].vbase = (uint32_t)vbase[m];
         md[m].vcount = meshes[m].vertex_count;
         md[m].mode = MODE_SAH;
         md[m].keep = meshes[m].keep_primitives ? 1u : 0u;
-        tris += md[m].tcount;
-        verts += meshes[m].vertex_count;
-    }
-    if (tris >= (1ull << 31) || 3 * verts >= (1ull << 32)) {
-        set_error("rq_scene_build_gpu: too many triangles or vertices for one build");
-        return RQ_ERR_INVALID_INPUT;
     }
     b.tris = (uint32_t)tris;
     b.verts = (uint32_t)verts;
     GB_OK(b.meshes.need(std::max<size_t>(mesh_count, 1) * sizeof(MeshDev), "hipMalloc build"));
     GB_OK(b.pos.need(std::max<uint64_t>(verts, 1) * 12, "hipMalloc build"));
     GB_OK(b.idx.need(std::max<uint64_t>(tris, 1) * 12, "hipMalloc build"));
-    for (uint32_t m = 0; m < mesh_count; ++m) {
-        if (meshes[m].vertex_count)
-            GB_TRY(hipMemcpyAsync(b.pos.as<float>() + 3ull * md[m].vbase, meshes[m].positions,
-                                  12ull * meshes[m].vertex_count, hipMemcpyHostToDevice, b.s),
-                   "upload positions");
-        GB_TRY(hipMemcpyAsync(b.idx.as<uint32_t>() + 3ull * md[m].tbase, meshes[m].indices,
-                              4ull * meshes[m].index_count, hipMemcpyHostToDevice, b.s),
-               "upload indices");
-    }
+    if (verts)
+        GB_TRY(hipMemcpyAsync(b.pos.p, stage_pos.data(), 12ull * verts, hipMemcpyHostToDevice, b.s), "upload positions");
+    if (tris)
+        GB_TRY(hipMemcpyAsync(b.idx.p, stage_idx.data(), 12ull * tris, hipMemcpyHostToDevice, b.s), "upload indices");
     GB_OK(b.allocate());
     std::vector<uint32_t> summary;
     if (mesh_count) {
@@ -1291,6 +1373,8 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
     scene->top_force = top_level;
     scene_bounds(scene);
+    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - began).count();
     *out = scene;
     return RQ_OK;
 }

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <chrono>
 #include <thread>
 #include <unordered_map>
 
@@ -208,6 +209,7 @@ using namespace rq;
 
 extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                                     uint32_t instance_count, int top_level, rq_scene** out) {
+    const auto began = std::chrono::steady_clock::now();
     if (!out || (!meshes && mesh_count) || (!instances && instance_count)) {
         set_error("rq_scene_build: null argument");
         return RQ_ERR_INVALID_INPUT;
@@ -344,6 +346,8 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
     for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
     scene->top_force = top_level;
     scene_bounds(scene);
+    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - began).count();
     *out = scene;
     return RQ_OK;
 }

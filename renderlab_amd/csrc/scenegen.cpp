@@ -169,7 +169,7 @@ MeshData cluster(Rng& g, uint32_t n, double median, double sigma) {
 using namespace rq;
 
 static rq_status gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, bool reflection,
-                           rq_scene** out) {
+                           rq_scene** out, bool gpu = false, void* stream = nullptr) {
     if (!out || kind > 1) {
         set_error("rq_gen_scene: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -225,6 +225,7 @@ static rq_status gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64
         ms[m].index_count = (uint32_t)data[m].i.size();
         ms[m].keep_primitives = 1;
     }
+    if (gpu) return rq_scene_build_gpu(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out, stream);
     return reflection
                ? rq_scene_build_reflection(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out)
                : rq_scene_build(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out);
@@ -232,6 +233,11 @@ static rq_status gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64
 
 extern "C" rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, rq_scene** out) {
     return gen_scene(kind, seed, meshes, triangles, false, out);
+}
+
+extern "C" rq_status rq_gen_scene_gpu(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                                      rq_scene** out, void* stream) {
+    return gen_scene(kind, seed, meshes, triangles, false, out, true, stream);
 }
 
 extern "C" rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,

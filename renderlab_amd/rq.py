@@ -104,6 +104,7 @@ _sig("rq_scene_download_part", _i, _P, _i, _u32, _P, _sz, ctypes.POINTER(_sz))
 _sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
+_sig("rq_gen_scene_gpu", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P), _P)
 _sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
@@ -489,8 +490,11 @@ class Scene:
         return cls(h)
 
     @classmethod
-    def generate(cls, kind, seed, meshes=0, triangles=0, reflection=False):
+    def generate(cls, kind, seed, meshes=0, triangles=0, reflection=False, gpu=False):
         h = _P()
+        if gpu:
+            check(lib.rq_gen_scene_gpu(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h), None))
+            return cls(h)
         fn = lib.rq_gen_reflection_scene if reflection else lib.rq_gen_scene
         check(fn(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h)))
         return cls(h)
@@ -507,7 +511,7 @@ class Scene:
         s = np.zeros(8, np.uint64)
         check(lib.rq_scene_stats(self._h, _ptr(s)))
         return dict(instances=int(s[0]), meshes=int(s[1]), triangles=int(s[2]), unique_triangles=int(s[3]),
-                    blas_nodes=int(s[4]), top_root=int(s[5]), max_depth=int(s[6]))
+                    blas_nodes=int(s[4]), top_root=int(s[5]), max_depth=int(s[6]), build_us=int(s[7]))
 
     def refit_mesh(self, mesh, positions, vertex_count=None, stream=None):
         """GPU refit of mesh `mesh` to new positions (DeviceBuffer or device pointer of

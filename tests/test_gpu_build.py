@@ -94,7 +94,9 @@ def assert_same_words(cpu, gpu):
     assert a.size == b.size, (a.size, b.size)
     diff = np.nonzero(a != b)[0]
     assert diff.size == 0, f"{diff.size} words differ, first at {diff[:10]}"
-    assert cpu.stats() == gpu.stats()
+    a, b = cpu.stats(), gpu.stats()
+    a.pop("build_us"), b.pop("build_us")
+    assert a == b
 
 
 @pytest.fixture(scope="module")
@@ -148,11 +150,7 @@ def test_gpu_built_scene_traces_and_refits(cases):
         assert len(compare_exact(h, o)) == 0, layout
     new = (meshes[0][0] + g.normal(scale=0.01, size=meshes[0][0].shape)).astype(np.float32)
     print("case: refit GPU-built scene", flush=True)
-    gpu.refit_mesh(0, to_device(new), len(new))
-    moved = list(meshes)
-    moved[0] = (new, meshes[0][1], meshes[0][2])
-    # refit keeps the built topology: every mesh's words but the refit one match a host build of
-    # the moved meshes only where the tree does not depend on the positions; check traversal instead
+    gpu.refit_mesh(0, to_device(new), len(new))  # the refit keeps the built tree: check by traversal
     words = gpu.words
     o, _, _ = oracle.traverse(words, rays, 0)
     for layout in ("ref", "plain"):
