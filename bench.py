@@ -391,6 +391,61 @@ def run_refit(rq, n=724, instances=4, reps=20):
             "note": "gpu_ms: host clock around the synchronous call (bounds, refit, records, top level, PLAIN refresh)"}
 
 
+def run_gi_cache(rq, scene, layout, cells=32768, reps=5):
+    """The GI cache's ray stages on the benchmark scene: every direction of `cells` seeded cells (the
+    reference's per-frame item capacity at 32,768 cells) as gi_rays items (ray generation + gi_visible
+    into the staged bits), then gi_gather's fixed directions (those left blocked) as closest-hit rays."""
+    from renderlab_amd.gi_table import make_table, random_cells
+    g = np.random.default_rng(7)
+    cell = random_cells(cells, g, box=95.0)
+    for c in cell.values():
+        c["bits"] = 0
+    table = make_table(cell, stride=1)
+    items = np.array([(s << 7) | k for s in range(cells) for k in range(64)], np.uint32)
+    n = len(items)
+    dt, di = rq.DeviceBuffer.from_array(table), rq.DeviceBuffer.from_array(items)
+    work, bits = rq.DeviceBuffer(n * 32), rq.DeviceBuffer(((n + 31) // 32) * 4)
+    start, end = rq.Event(), rq.Event()
+    rq.gi_rays(scene, layout, dt, table.size, di, n, work, bits)
+    rq.synchronize()
+    start.record()
+    for _ in range(reps):
+        rq.gi_rays(scene, layout, dt, table.size, di, n, work, bits)
+    end.record()
+    end.synchronize()
+    vis_ms = start.elapsed_ms(end) / reps
+    staged = bits.download(np.uint32, (n + 31) // 32)
+    # the staged bits become each cell's known-visible directions; the gather traces the rest
+    cold = 64 + (int(table[1]) + 1) * 24
+    cw = max(24, int(table[23]))
+    for s in range(cells):
+        table[cold + s * cw + 12] = staged[2 * s]
+        table[cold + s * cw + 13] = staged[2 * s + 1]
+    dt = rq.DeviceBuffer.from_array(table)
+    slots = rq.DeviceBuffer.from_array(np.arange(cells, dtype=np.uint32))
+    rays, gitems, count = rq.DeviceBuffer(n * 32), rq.DeviceBuffer(n * 4), rq.DeviceBuffer(4)
+    hits, ws = rq.DeviceBuffer(n * 48), rq.DeviceBuffer(rq.gi_gather_workspace(cells))
+
+    def gather():
+        rq.gi_gather_rays(dt, table.size, slots, cells, rays, gitems, n, count, ws)
+        scene.trace_indirect(layout, rays, count.ptr, n, rq.CLOSEST, hits=hits)
+
+    gather()
+    rq.synchronize()
+    start.record()
+    for _ in range(reps):
+        gather()
+    end.record()
+    end.synchronize()
+    gather_ms = start.elapsed_ms(end) / reps
+    m = int(count.download(np.uint32, 1)[0])
+    visible = int(np.unpackbits(staged.view(np.uint8)).sum())
+    return {"cells": cells, "items": n, "visible": visible, "gi_rays_ms": round(vis_ms, 4),
+            "gi_rays_mrays_s": round(n / vis_ms / 1e3, 1), "gather_rays": m, "gather_ms": round(gather_ms, 4),
+            "gather_mrays_s": round(m / gather_ms / 1e3, 1), "layout": layout,
+            "note": "gi_rays: item rays + gi_visible; gather: fixed-direction rays + closest hits (device count)"}
+
+
 def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
     """The oracle traversal (scalar C port of gi_traverse) on this host's cores, over a bounded sample."""
     import oracle
@@ -436,6 +491,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
     ap.add_argument("--no-reflection", action="store_true", help="skip the reflection-ray (rs_trace) line")
     ap.add_argument("--no-refit", action="store_true", help="skip the GPU mesh refit line")
+    ap.add_argument("--no-gi-cache", action="store_true", help="skip the GI cache ray-stage line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one plain device copy of BYTES (the known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -605,6 +661,9 @@ def main():
         line["serving_sweep"] = [run_serving(rq, args.layout, [scene], rate=r, n_jobs=int(min(400, max(40, r))),
                                              resolutions=res, mixes=[0, 1]) for r in (50, 200, 500, 1000, 2000)]
         log(f"serving sweep: {line['serving_sweep']}")
+    if not args.no_gi_cache:
+        line["gi_cache_rays"] = run_gi_cache(rq, scene, args.layout)
+        log(f"GI cache ray stages: {line['gi_cache_rays']}")
     if not args.no_refit:
         line["refit_1m"] = run_refit(rq)
         log(f"mesh refit (1M triangles): {line['refit_1m']}")

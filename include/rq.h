@@ -149,6 +149,36 @@ rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_in
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
    would upload, src/gi_cache.rs:4361). */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* ---- The GI radiance cache's ray stages around the traversal, reading the
+   cache's table words (`gi_table`, shaders/gi_cells.wgsl: 64-word header,
+   24-word hot records, max(24, header[23])-word cold records) in device memory.
+   A slot or item outside the table gives a ray with a NaN origin (a flagged
+   miss: visible, no hit).
+
+   gi_ray (shaders/gi_cache.wgsl:1563-1576): the visibility ray of each item
+   (slot << 7 | k): origin q + n * gi_origin_offset(q, edge), direction k of
+   the cell's hemisphere (gi_direction over gi_basis(n)), t in [0, 1e20]. */
+rq_status rq_gi_item_rays(const uint32_t* table, size_t table_words, const uint32_t* items, uint32_t count,
+                          rq_ray* rays, void* stream);
+/* The gi_rays pass: the item rays into `workspace` (count rays), then
+   gi_visible over them, the bit of item i set in staged_bits when its
+   direction sees the sky (the reference's atomicOr into the staged words). */
+rq_status rq_gi_rays(rq_scene* scene, rq_layout layout, const uint32_t* table, size_t table_words,
+                     const uint32_t* items, uint32_t count, rq_ray* workspace, uint32_t* staged_bits,
+                     void* stream);
+/* gi_gather's fixed directions (shaders/gi_gather_software.wgsl:82-110) for
+   the listed cells: directions phase + j * stride (stride = max(header[21], 1),
+   phase = key & (stride - 1)) whose visibility bit is clear, as closest-hit
+   rays with their items (slot << 7 | k), cell by cell in list order and
+   directions ascending. *count (device memory) receives the total; rays past
+   max_rays are not written. `workspace`: device memory of
+   rq_gi_gather_workspace(slot_count) bytes. Trace them with rq_trace_indirect
+   (count_dev = count). */
+rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words, const uint32_t* slots, uint32_t slot_count,
+                            rq_ray* rays, uint32_t* items, uint32_t max_rays, uint32_t* count, void* workspace,
+                            size_t workspace_bytes, void* stream);
+size_t rq_gi_gather_workspace(uint32_t slot_count);
+
 /* rq_scene_build with every mesh's BVH built on the GPU: the same packed words
    (the SAH BVH4 of near_field::build_bvh_sah, crates/asset/src/near_field.rs:
    172-430, with its median fallback past depth 15; layout, instance records

@@ -44,6 +44,8 @@ _sig("orc_validate_blas", ctypes.c_int, _P, _sz, _u32, _u32, _u32, _u32, _u32, _
 _sig("orc_primary_ray", None, _P, _u32, _u32, _u32, _u32, _P)
 _sig("orc_secondary_rays", _u32, _P, _P, _P, _u32, _u32, _u32, _P)
 _sig("orc_pcg", _u32, _u32)
+_sig("orc_gi_item_rays", None, _P, _sz, _P, _u32, _P)
+_sig("orc_gi_gather_rays", _u32, _P, _sz, _P, _u32, _P, _P, _u32)
 
 
 def _p(a):
@@ -156,3 +158,23 @@ def secondary_rays(primary, hits, sun, seed, n_gi):
 
 def pcg(v):
     return int(lib.orc_pcg(int(v)))
+
+
+def gi_item_rays(table, items):
+    """gi_ray's rays for ray items (slot << 7 | k) of a GI cache table."""
+    t = np.ascontiguousarray(table, np.uint32)
+    it = np.ascontiguousarray(items, np.uint32)
+    out = np.zeros(len(it), RAY_DTYPE)
+    lib.orc_gi_item_rays(_p(t), t.size, _p(it), len(it), _p(out))
+    return out
+
+
+def gi_gather_rays(table, slots):
+    """gi_gather's fixed-direction rays of the listed cells: (rays, items)."""
+    t = np.ascontiguousarray(table, np.uint32)
+    sl = np.ascontiguousarray(slots, np.uint32)
+    n = lib.orc_gi_gather_rays(_p(t), t.size, _p(sl), len(sl), None, None, 0)
+    rays = np.zeros(max(n, 1), RAY_DTYPE)
+    items = np.zeros(max(n, 1), np.uint32)
+    lib.orc_gi_gather_rays(_p(t), t.size, _p(sl), len(sl), _p(rays), _p(items), n)
+    return rays[:n], items[:n]

@@ -1091,3 +1091,137 @@ uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const fl
     }
     return valid;
 }
+
+/* ------------------------------------------------------ GI cache rays */
+/* Table words: shaders/gi_cells.wgsl:18-185 (header 64 words; 24-word hot
+   records; cold records of max(24, header[23]) words), as
+   tests/ara_backend/gi_cache_reference.rs:50-118 decodes them. */
+typedef struct {
+    int ok;
+    float n[3], q[3], edge;
+    uint32_t key_x, key_y;
+    uint64_t bits;
+} gi_cell_t;
+
+static float gi_bits_f(uint32_t w) {
+    float f;
+    memcpy(&f, &w, 4);
+    return f;
+}
+
+/* gi_unpack_normal (shaders/gi_cells.wgsl:481-487): unpack2x16snorm, the
+   octahedral fold, then v / |v|. */
+static void gi_unpack_normal(uint32_t w, float n[3]) {
+    float qx = (float)(int16_t)(uint16_t)(w & 0xffffu) / 32767.0f;
+    float qy = (float)(int16_t)(uint16_t)(w >> 16) / 32767.0f;
+    qx = fminf(fmaxf(qx, -1.0f), 1.0f);
+    qy = fminf(fmaxf(qy, -1.0f), 1.0f);
+    float x = qx, y = qy, z = 1.0f - fabsf(qx) - fabsf(qy);
+    float t = fmaxf(-z, 0.0f);
+    x = x + (x >= 0.0f ? -t : t);
+    y = y + (y >= 0.0f ? -t : t);
+    float len = sqrtf(x * x + y * y + z * z);
+    n[0] = x / len;
+    n[1] = y / len;
+    n[2] = z / len;
+}
+
+static gi_cell_t gi_load_cell(const uint32_t* t, size_t words, uint32_t slot) {
+    gi_cell_t c;
+    memset(&c, 0, sizeof c);
+    if (words <= 64) return c;
+    uint64_t slots = (uint64_t)t[1] + 1, cold_words = t[23] > 24 ? t[23] : 24;
+    if (slot >= slots) return c;
+    uint64_t hot = 64 + (uint64_t)slot * 24, cold = 64 + slots * 24 + (uint64_t)slot * cold_words;
+    if (cold + 24 > words || hot + 24 > words) return c;
+    c.key_x = t[hot] & 0x7fffffffu;
+    c.key_y = t[cold];
+    c.edge = gi_bits_f(t[5]) * exp2f((float)(t[cold + 3] & 0xffu));  /* gi_edge, gi_cells.wgsl:240 */
+    gi_unpack_normal(t[cold + 7], c.n);
+    for (int i = 0; i < 3; ++i) c.q[i] = gi_bits_f(t[cold + 8 + i]);
+    c.bits = (uint64_t)t[cold + 12] | ((uint64_t)t[cold + 13] << 32);
+    c.ok = 1;
+    return c;
+}
+
+/* gi_origin_offset (shaders/gi_cache.wgsl:1064-1068). */
+static float gi_origin_offset(const float q[3], float edge) {
+    float m = fmaxf(fmaxf(fabsf(q[0]), fabsf(q[1])), fabsf(q[2]));
+    float mm = fmaxf(m, 1e-30f);
+    uint32_t b;
+    memcpy(&b, &mm, 4);
+    b += 1;
+    float next;
+    memcpy(&next, &b, 4);
+    float ulp = next - mm;
+    return fmaxf(fmaxf(1e-5f, 32.0f * ulp) + 1e-4f * edge, 1e-4f + 4e-7f * m);
+}
+
+/* gi_direction (shaders/gi_cache.wgsl:1048-1054) through gi_basis
+   (shaders/gi_cells.wgsl:463-469); frame * v summed column by column. */
+static void gi_direction(const gi_cell_t* c, int rotate, uint32_t k, float d[3]) {
+    float z = 1.0f - ((float)k + 0.5f) / 64.0f;
+    float r = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+    float rotation = rotate ? (float)(c->key_y >> 8) * (float)(6.283185307179586 / 16777216.0) : 0.0f;
+    float phi = (float)k * 2.399963229728653f + rotation;
+    float lx = r * cosf(phi), ly = r * sinf(phi), lz = z;
+    const float* n = c->n;
+    float s = n[2] >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n[2]);
+    float b = n[0] * n[1] * a;
+    float c0[3] = {1.0f + s * n[0] * n[0] * a, s * b, -s * n[0]};
+    float c1[3] = {b, s + n[1] * n[1] * a, -n[1]};
+    for (int i = 0; i < 3; ++i) d[i] = c0[i] * lx + c1[i] * ly + n[i] * lz;
+}
+
+static orc_ray gi_cell_ray(const gi_cell_t* c, int rotate, uint32_t k) {
+    orc_ray r;
+    float off = gi_origin_offset(c->q, c->edge), d[3];
+    gi_direction(c, rotate, k, d);
+    for (int i = 0; i < 3; ++i) {
+        r.o[i] = c->q[i] + c->n[i] * off;
+        r.d[i] = d[i];
+    }
+    r.tmin = 0.0f;
+    r.tmax = 1e20f;
+    return r;
+}
+
+void orc_gi_item_rays(const uint32_t* table, size_t words, const uint32_t* items, uint32_t count, orc_ray* rays) {
+    int rotate = words > 2 && (table[2] & 4u) != 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t k = items[i] & 127u;
+        gi_cell_t c = gi_load_cell(table, words, items[i] >> 7);
+        if (c.ok && k < 64) {
+            rays[i] = gi_cell_ray(&c, rotate, k);
+        } else {
+            orc_ray r;
+            for (int a = 0; a < 3; ++a) {
+                r.o[a] = NAN;
+                r.d[a] = 0.0f;
+            }
+            r.tmin = 0.0f;
+            r.tmax = 1e20f;
+            rays[i] = r;
+        }
+    }
+}
+
+uint32_t orc_gi_gather_rays(const uint32_t* table, size_t words, const uint32_t* slots, uint32_t n, orc_ray* rays,
+                            uint32_t* items, uint32_t max_rays) {
+    int rotate = words > 2 && (table[2] & 4u) != 0;
+    uint32_t stride = words > 21 && table[21] > 1 ? table[21] : 1, at = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        gi_cell_t c = gi_load_cell(table, words, slots[i]);
+        if (!c.ok) continue;
+        for (uint32_t k = c.key_x & (stride - 1); k < 64; k += stride) {
+            if ((c.bits >> k) & 1u) continue;  /* the sky is known visible there */
+            if (at < max_rays) {
+                rays[at] = gi_cell_ray(&c, rotate, k);
+                items[at] = (slots[i] << 7) | k;
+            }
+            ++at;
+        }
+    }
+    return at;
+}

@@ -119,6 +119,18 @@ uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const fl
                             uint32_t pixel, uint32_t seed, uint32_t n_gi, orc_ray* out);
 uint32_t orc_pcg(uint32_t v);
 
+/* The GI radiance cache's rays from its table words (shaders/gi_cells.wgsl
+   layout; tests/ara_backend/gi_cache_reference.rs restates it on the CPU):
+   orc_gi_item_rays: gi_ray's ray for each item (slot << 7 | k), NaN origin
+   for an item outside the table (shaders/gi_cache.wgsl:1563-1576, 1048-1068);
+   orc_gi_gather_rays: gi_gather's fixed directions of each listed cell whose
+   visibility bit is clear, cell by cell, k ascending; returns the total and
+   writes at most max_rays (shaders/gi_gather_software.wgsl:82-110). */
+void orc_gi_item_rays(const uint32_t* table, size_t words, const uint32_t* items, uint32_t count,
+                      orc_ray* rays);
+uint32_t orc_gi_gather_rays(const uint32_t* table, size_t words, const uint32_t* slots, uint32_t n,
+                            orc_ray* rays, uint32_t* items, uint32_t max_rays);
+
 #ifdef __cplusplus
 }
 #endif

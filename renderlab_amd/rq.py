@@ -105,6 +105,10 @@ _sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P)
 _sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
 _sig("rq_gen_scene_gpu", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P), _P)
+_sig("rq_gi_item_rays", _i, _P, _sz, _P, _u32, _P, _P)
+_sig("rq_gi_rays", _i, _P, _i, _P, _sz, _P, _u32, _P, _P, _P)
+_sig("rq_gi_gather_rays", _i, _P, _sz, _P, _u32, _P, _P, _u32, _P, _P, _sz, _P)
+_sig("rq_gi_gather_workspace", _sz, _u32)
 _sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
@@ -712,3 +716,30 @@ def serve(scenes, jobs, layout="cw8", slots=4, hbm_budget=8 << 30, sun=(0.3, 0.8
                         primary=r.primary, secondary=r.secondary, status=r.status,
                         latency_ms=(r.end_s - r.arrival_s) * 1e3))
     return out
+
+
+# ------------------------------------------------------ GI cache ray stages
+def gi_item_rays(table, table_words, items, count, rays, stream=None):
+    """gi_ray's rays for `count` items (slot << 7 | k) of a GI cache table (all device buffers)."""
+    check(lib.rq_gi_item_rays(_dptr(table), int(table_words), _dptr(items), int(count), _dptr(rays),
+                              stream.ptr if stream else None))
+
+
+def gi_rays(scene, layout, table, table_words, items, count, workspace, staged_bits, stream=None):
+    """The gi_rays pass: item rays (into `workspace`) traced with gi_visible into `staged_bits`."""
+    check(lib.rq_gi_rays(scene._h, LAYOUTS.get(layout, layout), _dptr(table), int(table_words), _dptr(items),
+                         int(count), _dptr(workspace), _dptr(staged_bits), stream.ptr if stream else None))
+
+
+def gi_gather_workspace(slot_count):
+    return int(lib.rq_gi_gather_workspace(int(slot_count)))
+
+
+def gi_gather_rays(table, table_words, slots, slot_count, rays, items, max_rays, count, workspace,
+                   workspace_bytes=None, stream=None):
+    """gi_gather's fixed-direction rays of the listed cells; the total goes to `count` (device u32).
+    workspace: rq_gi_gather_workspace(slot_count) bytes of device memory."""
+    nbytes = workspace.nbytes if workspace_bytes is None else int(workspace_bytes)
+    check(lib.rq_gi_gather_rays(_dptr(table), int(table_words), _dptr(slots), int(slot_count), _dptr(rays),
+                                _dptr(items), int(max_rays), _dptr(count), _dptr(workspace), nbytes,
+                                stream.ptr if stream else None))
