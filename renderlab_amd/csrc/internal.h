@@ -115,6 +115,8 @@ bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
 // Fetches the stale ranges of the host words from the device REF copy.
 rq_status sync_host_words(const rq_scene* scene);
+// Fetches the stale host words inside [lo, hi) only.
+rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi);
 // Instance record words 12-17: the padded world box of a mesh box under a model
 // (src/gi_bvh.rs instance_words).
 void instance_box(const float model[16], const float lo[3], const float hi[3], uint32_t out[6]);

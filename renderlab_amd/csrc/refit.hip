@@ -221,6 +221,27 @@ rq_status sync_host_words(const rq_scene* scene) {
     scene->stale.clear();
     return RQ_OK;
 }
+
+// Fetches only the stale words inside [lo, hi), keeping the rest stale.
+rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi) {
+    if (!scene || scene->stale.empty() || lo >= hi) return RQ_OK;
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
+    uint32_t* host = const_cast<uint32_t*>(scene->words.data());
+    std::vector<std::pair<uint64_t, uint64_t>> keep;
+    for (const auto& r : scene->stale) {
+        const uint64_t a = r.first, b = r.first + r.second;
+        const uint64_t x = std::max(a, lo), y = std::min(b, hi);
+        if (x >= y) {
+            keep.push_back(r);
+            continue;
+        }
+        RF_TRY(hipMemcpy(host + x, words + x, (y - x) * 4, hipMemcpyDeviceToHost), "fetch words");
+        if (a < x) keep.emplace_back(a, x - a);
+        if (y < b) keep.emplace_back(y, b - y);
+    }
+    scene->stale.swap(keep);
+    return RQ_OK;
+}
 }  // namespace rq
 
 namespace {
@@ -257,11 +278,24 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
     for (uint32_t i = 0; i < count && record == count; ++i)
         if (w[HEADER_WORDS + (size_t)i * INSTANCE_WORDS + 18] == scene->mesh_nodes[mesh]) record = i;
     if (record == count) return RQ_OK;  // no instance draws this mesh: nothing packed to refit
+    // The plan reads the child tags, leaves and indices, which a refit never
+    // changes: fetch them once if a GPU build left them on the device only.
+    const size_t rec = HEADER_WORDS + (size_t)record * INSTANCE_WORDS;
+    bool planned = false;
+    for (const auto& p : scene->refit_plans) planned = planned || p.nodes == w[rec + 18];
+    if (!planned && w[rec + 18] <= w[rec + 20] && w[rec + 20] <= w.size()) {
+        rq_status st = sync_host_range(scene, w[rec + 18], w[rec + 20]);  // nodes and leaves
+        if (st != RQ_OK) return st;
+    }
     MeshView m;
     std::string err;
     if (!mesh_view(w, record, m, err)) {
         set_error("rq_scene_refit_mesh: " + err);
         return RQ_ERR_INVALID_INPUT;
+    }
+    if (!planned) {
+        rq_status st = sync_host_range(scene, m.indices, (uint64_t)m.indices + m.index_count);
+        if (st != RQ_OK) return st;
     }
     if (vertex_count != m.vertex_count || vertex_count == 0) {
         set_error("BVH refit needs finite positions of the same vertices");

@@ -158,6 +158,22 @@ def test_gpu_built_scene_traces_and_refits(cases):
         assert len(compare_exact(h, o)) == 0, layout
 
 
+def test_refit_right_after_gpu_build(cases):
+    """The refit's plan reads child tags, leaves and indices that a GPU build leaves on the device
+    only: a refit called before anything fetched the words must still equal the host path."""
+    meshes, inst = cases
+    g = np.random.default_rng(9)
+    new = (meshes[1][0] + g.normal(scale=0.05, size=meshes[1][0].shape)).astype(np.float32)
+    print("case: refit before any host fetch", flush=True)
+    gpu = rq.Scene.build(meshes, inst, gpu=True)
+    gpu.refit_mesh(1, to_device(new), len(new))
+    host = rq.Scene.build(meshes, inst, gpu=True)
+    host.words  # fetched first this time
+    host.refit_mesh(1, to_device(new), len(new))
+    a, b = gpu.words, host.words
+    assert a.size == b.size and np.array_equal(a, b)
+
+
 def test_gpu_build_rejects_bad_input(cases):
     meshes, inst = cases
     bad_idx = [(meshes[7][0], np.array([0, 1, 99], np.uint32), 0)]
