@@ -149,6 +149,68 @@ rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_in
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
    would upload, src/gi_cache.rs:4361). */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
+/* ---- Cooked geometry read from disk (crates/asset). The cluster BVH sidecar
+   "RLB\0" (crates/asset/src/cluster_bvh/codec.rs:4-188, checked as
+   Bvh::validate, crates/asset/src/cluster_bvh.rs:170-333) and the near field
+   "RNF\0" (crates/asset/src/near_field.rs:540-680, NearField::validate
+   :106-135). max_bytes 0 takes the reference's default limits (150,000,000 and
+   160,000,000). Malformed bytes give RQ_ERR_INVALID_INPUT with the reference's
+   message. Encoders write nothing and report the length when `capacity` is
+   smaller. */
+typedef struct rq_cluster_bvh rq_cluster_bvh;
+typedef struct rq_near_field rq_near_field;
+rq_status rq_cluster_bvh_decode(const uint8_t* bytes, size_t length, size_t max_bytes, rq_cluster_bvh** out);
+rq_status rq_cluster_bvh_encode(const rq_cluster_bvh* bvh, size_t max_bytes, uint8_t* out, size_t capacity,
+                                size_t* length);
+/* near_field::build_bvh / build_bvh_sah (near_field.rs:172-262): the
+   one-level, zero-error cluster BVH of a triangle list. */
+rq_status rq_cluster_bvh_from_mesh(const float* positions, uint32_t vertex_count, const uint32_t* indices,
+                                   uint32_t index_count, int sah, rq_cluster_bvh** out);
+/* nodes, leaves, levels, level leaves, positions, indices */
+rq_status rq_cluster_bvh_counts(const rq_cluster_bvh* bvh, uint32_t counts[6]);
+void rq_cluster_bvh_destroy(rq_cluster_bvh* bvh);
+rq_status rq_near_field_decode(const uint8_t* bytes, size_t length, size_t max_bytes, rq_near_field** out);
+rq_status rq_near_field_encode(const rq_near_field* field, size_t max_bytes, uint8_t* out, size_t capacity,
+                               size_t* length);
+/* counts: meshes, placements; settings: lod_error, reach, skip */
+rq_status rq_near_field_info(const rq_near_field* field, uint32_t counts[2], uint32_t* bins, float settings[3],
+                             uint8_t source[32]);
+void rq_near_field_destroy(rq_near_field* field);
+
+/* One mesh of a scene from exactly one source: a triangle list built at load
+   (set_indirect_geometry), a cooked cluster BVH traced at level `index`
+   (set_indirect_bvh, src/gi_cache.rs:3209-3217) or mesh `index` of a near
+   field (set_indirect_near_field, :3218-3270). */
+typedef struct rq_cooked_mesh {
+    const rq_mesh* mesh;
+    const rq_cluster_bvh* bvh;
+    const rq_near_field* field;
+    uint32_t index;
+    uint32_t reserved;
+} rq_cooked_mesh;
+/* rq_scene_build over such meshes: near-field placements' masks packed after
+   the meshes (src/gi_bvh.rs:151-226), their base in instance word 23 for every
+   instance whose model has the cooked transform's exact bits (:95-104). All
+   near fields with masks must share their bins. Cooked leaves may hold more
+   than 16 triangles: such meshes trace with the REF layout only. */
+rq_status rq_scene_build_cooked(const rq_cooked_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                                uint32_t instance_count, int top_level, rq_scene** out);
+/* gi_near_blocked (shaders/gi_rays_software.wgsl:330-354) over a batch: whether
+   the near-field masks of instance `inst` block direction d at the point
+   (1 - u - v, u, v) of the triangle with position indices tri (every vertex
+   with weight blocks it; `back`: the point is seen from behind). Bit i of
+   blocked_bits (device, zeroed here) is set for a blocked query; an instance
+   without masks blocks nothing. Needs the REF layout uploaded. */
+typedef struct rq_near_query {
+    uint32_t inst;
+    uint32_t tri[3];
+    float bary[2];
+    uint32_t back;
+    float d[3];
+} rq_near_query;
+rq_status rq_near_blocked(const rq_scene* scene, const rq_near_query* queries, uint32_t count,
+                          uint32_t* blocked_bits, void* stream);
+
 /* ---- The GI radiance cache's ray stages around the traversal, reading the
    cache's table words (`gi_table`, shaders/gi_cells.wgsl: 64-word header,
    24-word hot records, max(24, header[23])-word cold records) in device memory.

@@ -45,6 +45,9 @@ _sig("orc_primary_ray", None, _P, _u32, _u32, _u32, _u32, _P)
 _sig("orc_secondary_rays", _u32, _P, _P, _P, _u32, _u32, _u32, _P)
 _sig("orc_pcg", _u32, _u32)
 _sig("orc_gi_item_rays", None, _P, _sz, _P, _u32, _P)
+_sig("orc_near_bin", _u32, _P, _u32)
+_sig("orc_near_bin_direction", None, _u32, _u32, _P)
+_sig("orc_near_blocked", ctypes.c_int, _P, _sz, _u32, _u32, _P, _P, ctypes.c_int, _P)
 _sig("orc_gi_gather_rays", _u32, _P, _sz, _P, _u32, _P, _P, _u32)
 
 
@@ -178,3 +181,27 @@ def gi_gather_rays(table, slots):
     items = np.zeros(max(n, 1), np.uint32)
     lib.orc_gi_gather_rays(_p(t), t.size, _p(sl), len(sl), _p(rays), _p(items), n)
     return rays[:n], items[:n]
+
+
+def near_bin(d, bins):
+    v = np.ascontiguousarray(d, np.float32)
+    return int(lib.orc_near_bin(_p(v), int(bins)))
+
+
+def near_bin_direction(b, bins):
+    out = np.zeros(3, np.float32)
+    lib.orc_near_bin_direction(int(b), int(bins), _p(out))
+    return out
+
+
+def near_blocked(words, bins, queries):
+    """gi_near_blocked per query (records with inst, tri, bary, back, d)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    out = np.zeros(len(queries), bool)
+    for k, q in enumerate(queries):
+        tri = np.ascontiguousarray(q["tri"], np.uint32)
+        bary = np.ascontiguousarray(q["bary"], np.float32)
+        d = np.ascontiguousarray(q["d"], np.float32)
+        out[k] = bool(lib.orc_near_blocked(_p(w), w.size, int(bins), int(q["inst"]), _p(tri), _p(bary),
+                                           int(q["back"]), _p(d)))
+    return out

@@ -1225,3 +1225,51 @@ uint32_t orc_gi_gather_rays(const uint32_t* table, size_t words, const uint32_t*
     }
     return at;
 }
+
+/* ------------------------------------------------------ near-field masks */
+uint32_t orc_near_bin(const float d[3], uint32_t bins) {
+    float s = fabsf(d[0]) + fabsf(d[1]) + fabsf(d[2]);
+    float u = d[0] / s, v = d[1] / s;
+    if (!(d[2] >= 0.0f)) {  /* near_field.rs:160-169 (octahedral) */
+        float fu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        float fv = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = fu;
+        v = fv;
+    }
+    float n = (float)bins;
+    float cu = floorf((u * 0.5f + 0.5f) * n), cv = floorf((v * 0.5f + 0.5f) * n);
+    cu = cu < 0.0f ? 0.0f : (cu > n - 1.0f ? n - 1.0f : cu);
+    cv = cv < 0.0f ? 0.0f : (cv > n - 1.0f ? n - 1.0f : cv);
+    return (uint32_t)cv * bins + (uint32_t)cu;
+}
+
+void orc_near_bin_direction(uint32_t bin, uint32_t bins, float d[3]) {
+    float n = (float)bins;
+    float u = ((float)(bin % bins) + 0.5f) / n * 2.0f - 1.0f;
+    float v = ((float)(bin / bins) + 0.5f) / n * 2.0f - 1.0f;
+    float z = 1.0f - fabsf(u) - fabsf(v);
+    float t = fmaxf(-z, 0.0f);
+    float x = u + (u >= 0.0f ? -t : t);
+    float y = v + (v >= 0.0f ? -t : t);
+    float l = sqrtf(x * x + y * y + z * z);
+    d[0] = x / l;
+    d[1] = y / l;
+    d[2] = z / l;
+}
+
+int orc_near_blocked(const uint32_t* g, size_t n, uint32_t bins, uint32_t inst, const uint32_t tri[3],
+                     const float bary[2], int back, const float d[3]) {
+    if (!bins || n < 16 || inst >= g[0] || 16 + (size_t)inst * 32 + 24 > n) return 0;
+    uint32_t base = g[16 + (size_t)inst * 32 + 23];
+    if (!base) return 0;
+    uint32_t words = bins * bins / 32u, b = orc_near_bin(d, bins);
+    uint64_t at = (uint64_t)base + (back ? words : 0u) + b / 32u, stride = 2ull * words;
+    uint32_t bit = 1u << (b % 32u);
+    float w[3] = {1.0f - bary[0] - bary[1], bary[0], bary[1]};
+    float s = 0.0f;
+    for (int v = 0; v < 3; ++v) {
+        uint64_t k = at + (uint64_t)tri[v] * stride;
+        if (k < n && (g[k] & bit)) s += w[v];
+    }
+    return s >= 0.999f;
+}

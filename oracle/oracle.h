@@ -126,6 +126,16 @@ uint32_t orc_pcg(uint32_t v);
    orc_gi_gather_rays: gi_gather's fixed directions of each listed cell whose
    visibility bit is clear, cell by cell, k ascending; returns the total and
    writes at most max_rays (shaders/gi_gather_software.wgsl:82-110). */
+/* Near-field masks: the octahedral bin of a direction (gi_near_bin,
+   shaders/gi_rays_software.wgsl:330-337; near_field::bin,
+   crates/asset/src/near_field.rs:139-145), the direction through a bin's
+   centre (near_field::bin_direction, :147-158), and gi_near_blocked
+   (shaders/gi_rays_software.wgsl:338-354) read from packed gi words (instance
+   word 23: the mask base; 0 blocks nothing). */
+uint32_t orc_near_bin(const float d[3], uint32_t bins);
+void orc_near_bin_direction(uint32_t bin, uint32_t bins, float d[3]);
+int orc_near_blocked(const uint32_t* g, size_t n, uint32_t bins, uint32_t inst, const uint32_t tri[3],
+                     const float bary[2], int back, const float d[3]);
 void orc_gi_item_rays(const uint32_t* table, size_t words, const uint32_t* items, uint32_t count,
                       orc_ray* rays);
 uint32_t orc_gi_gather_rays(const uint32_t* table, size_t words, const uint32_t* slots, uint32_t n,

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librq.so")
 BUILD = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["api.cpp", "builder.cpp", "toplevel.cpp", "pack.cpp", "layouts.cpp", "scenegen.cpp", "reflect.cpp"]
+HOST_SOURCES = ["api.cpp", "builder.cpp", "toplevel.cpp", "pack.cpp", "layouts.cpp", "scenegen.cpp", "reflect.cpp", "assets.cpp"]
 DEVICE_SOURCES = ["trace.hip", "serve.hip", "sort.hip", "refit.hip", "gpubuild.hip", "gicache.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

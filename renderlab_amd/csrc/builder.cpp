@@ -126,6 +126,7 @@ struct Builder {
         uint32_t id = (uint32_t)bvh->leaf_offset.size();
         bvh->leaf_offset.push_back(offset);
         bvh->leaf_count.push_back((uint32_t)tris.size() * 3u);
+        bvh->leaf_first.push_back(tris[0]);
         return id | LEAF_BIT;
     }
 

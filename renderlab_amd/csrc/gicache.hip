@@ -176,6 +176,53 @@ __global__ __launch_bounds__(256) void gather_write(const uint32_t* table, uint6
     }
 }
 
+// gi_near_bin (shaders/gi_rays_software.wgsl:330-337): the octahedral cell of d
+// among bins x bins, row-major (near_field::bin, crates/asset/src/near_field.rs:139-145).
+__device__ inline uint32_t near_bin(const float d[3], uint32_t bins) {
+    const float s = fabsf(d[0]) + fabsf(d[1]) + fabsf(d[2]);
+    float px = d[0] / s, py = d[1] / s;
+    if (!(d[2] >= 0.0f)) {
+        const float qx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        const float qy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = qx;
+        py = qy;
+    }
+    const float n = (float)bins;
+    const float cx = fminf(fmaxf(floorf((px * 0.5f + 0.5f) * n), 0.0f), n - 1.0f);
+    const float cy = fminf(fmaxf(floorf((py * 0.5f + 0.5f) * n), 0.0f), n - 1.0f);
+    return (uint32_t)cy * bins + (uint32_t)cx;
+}
+
+// gi_near_blocked (:338-354): every vertex with weight at the point blocks d.
+__global__ __launch_bounds__(256) void near_blocked(const uint32_t* words, uint64_t n_words, uint32_t bins,
+                                                    const rq_near_query* q, uint32_t count, uint32_t* out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    bool blocked = false;
+    if (i < count && bins) {
+        const rq_near_query a = q[i];
+        const uint64_t rec = HEADER_WORDS + (uint64_t)a.inst * INSTANCE_WORDS;
+        const uint32_t base = a.inst < words[0] && rec + 24 <= n_words ? words[rec + 23] : 0u;
+        if (base) {
+            const uint32_t per = bins * bins / 32u, b = near_bin(a.d, bins);
+            const uint64_t at = (uint64_t)base + (a.back ? per : 0u) + b / 32u, stride = 2ull * per;
+            const uint32_t bit = 1u << (b % 32u);
+            const float w[3] = {1.0f - a.bary[0] - a.bary[1], a.bary[0], a.bary[1]};
+            float s = 0.0f;
+            for (int v = 0; v < 3; ++v) {
+                const uint64_t k = at + (uint64_t)a.tri[v] * stride;
+                if (k < n_words && (words[k] & bit)) s += w[v];
+            }
+            blocked = s >= 0.999f;
+        }
+    }
+    const uint64_t m = __ballot(blocked);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (i < count && (lane & 31u) == 0) {
+        const uint32_t word = (uint32_t)(m >> (lane & 32u));
+        out[i / 32u] = word;
+    }
+}
+
 }  // namespace gic
 }  // namespace rq
 
@@ -252,4 +299,22 @@ extern "C" size_t rq_gi_gather_workspace(uint32_t slot_count) {
     uint32_t* p = nullptr;
     hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, p, p, std::max<uint32_t>(slot_count, 1), (hipStream_t)0);
     return ((8ull * slot_count + 255) & ~255ull) + scan_bytes + 1024;
+}
+
+extern "C" rq_status rq_near_blocked(const rq_scene* scene, const rq_near_query* queries, uint32_t count,
+                                     uint32_t* blocked_bits, void* stream) {
+    if (!scene || (count && (!queries || !blocked_bits))) {
+        set_error("rq_near_blocked: null argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (scene->kind != RQ_SCENE_GI || !scene->dev[0].ready) {
+        set_error("rq_near_blocked: needs a GI scene with the REF layout uploaded");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    if (!count) return RQ_OK;
+    hipLaunchKernelGGL(gic::near_blocked, dim3((count + 255) / 256), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), reinterpret_cast<const uint32_t*>(scene->dev[0].base),
+                       (uint64_t)scene->words.size(), scene->near_bins, queries, count, blocked_bits);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RQ_OK : gic_fail(e, "rq_near_blocked launch");
 }

@@ -2,6 +2,7 @@
 // the device launchers. Names follow the reference's record vocabulary (nodes,
 // leaves, positions, indices, instance records, top level).
 #pragma once
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -70,6 +71,7 @@ struct Bvh {
     std::vector<Node> nodes;
     std::vector<uint32_t> leaf_offset;  // index units
     std::vector<uint32_t> leaf_count;   // index units
+    std::vector<uint32_t> leaf_first;   // each leaf's first source triangle (a cooked leaf's source offset / 3)
     std::vector<float> positions;       // 3 per vertex
     std::vector<uint32_t> indices;
     uint32_t root = 0;
@@ -109,6 +111,28 @@ void top_place(std::vector<uint32_t>& words, uint32_t header, uint32_t stride, u
                const uint32_t* previous, size_t previous_words, int force);
 
 // pack.cpp
+// One cooked placement's near-field masks and the exact model bits they were
+// cooked under (src/gi_bvh.rs:38-43).
+struct NearMasks {
+    float model[16];
+    std::vector<uint32_t> masks;
+};
+// One mesh's runtime BVH as packed, its bounds over all positions, its primitive
+// table (empty unless kept) and its cooked placements' masks.
+struct Blas {
+    Bvh bvh;
+    float lo[3], hi[3];
+    std::vector<uint32_t> primitives;
+    std::vector<NearMasks> near;
+    size_t words() const {
+        return bvh.nodes.size() * NODE_WORDS + bvh.leaf_offset.size() * 2 + bvh.positions.size() +
+               bvh.indices.size() + primitives.size();
+    }
+};
+rq_status pack_scene(std::vector<Blas>& blas, const rq_instance* instances, uint32_t instance_count,
+                     int top_level, uint32_t near_bins, std::chrono::steady_clock::time_point began,
+                     rq_scene** out);
+void blas_bounds(Blas& b);
 bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* source, uint32_t n,
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
@@ -188,6 +212,7 @@ struct rq_scene {
     std::vector<float> models;
     std::vector<uint32_t> mesh_nodes;
     int top_force = -1;
+    uint32_t near_bins = 0;  // octahedral bins per side of the near-field masks (0: none)
     void* refit_scratch = nullptr;  // device: per-block bounds partials of a refit
     // Word ranges (offset, count) where the device REF words are newer than
     // `words` (a refit wrote them on the device); sync_host_words fetches them.

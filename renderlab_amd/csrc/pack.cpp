@@ -190,19 +190,135 @@ void instance_box(const float model[16], const float blo[3], const float bhi[3],
     }
 }
 
-namespace {
-
-struct Blas {
-    Bvh bvh;
-    float lo[3], hi[3];
-    std::vector<uint32_t> primitives;
-    size_t words() const {
-        return bvh.nodes.size() * NODE_WORDS + bvh.leaf_offset.size() * 2 + bvh.positions.size() +
-               bvh.indices.size() + primitives.size();
+// src/gi_bvh.rs:151-274 (layout, mesh_words, instance_words) + src/gi_cache.rs:4388-4479:
+// the packed words of BLASes already built, their instances and the top level.
+rq_status pack_scene(std::vector<Blas>& blas, const rq_instance* instances, uint32_t instance_count,
+                     int top_level, uint32_t near_bins, std::chrono::steady_clock::time_point began,
+                     rq_scene** out) {
+    const uint32_t mesh_count = (uint32_t)blas.size();
+    // layout: each mesh's arrays, then each mesh's near-field mask blocks
+    uint64_t at = HEADER_WORDS + (uint64_t)instance_count * INSTANCE_WORDS + top_reserve(instance_count);
+    std::vector<uint32_t> offsets(4ull * mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        const Bvh& b = blas[m].bvh;
+        uint64_t nodes = at, leaves = nodes + b.nodes.size() * NODE_WORDS;
+        uint64_t positions = leaves + b.leaf_offset.size() * 2, indices = positions + b.positions.size();
+        if (indices + b.indices.size() + blas[m].primitives.size() >= 0xffffffffull) {
+            set_error("packed geometry exceeds 2^32 words");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        offsets[4 * m] = (uint32_t)nodes;
+        offsets[4 * m + 1] = (uint32_t)leaves;
+        offsets[4 * m + 2] = (uint32_t)positions;
+        offsets[4 * m + 3] = (uint32_t)indices;
+        at += blas[m].words();
     }
-};
+    std::vector<std::vector<uint32_t>> near_base(mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m)
+        for (const NearMasks& n : blas[m].near) {
+            near_base[m].push_back((uint32_t)at);
+            at += n.masks.size();
+        }
+    if (at >= 0xffffffffull) {
+        set_error("packed geometry exceeds 2^32 words");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    auto scene = new rq_scene();
+    std::vector<uint32_t>& words = scene->words;
+    words.assign(at, 0u);
+    // src/gi_bvh.rs:182-226 (mesh_words)
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        const Bvh& b = blas[m].bvh;
+        uint64_t w = offsets[4 * m];
+        for (const Node& n : b.nodes) {
+            std::memcpy(&words[w], &n, sizeof(Node));
+            w += NODE_WORDS;
+        }
+        for (size_t l = 0; l < b.leaf_offset.size(); ++l) {
+            words[w++] = b.leaf_offset[l];
+            words[w++] = b.leaf_count[l];
+        }
+        for (float p : b.positions) words[w++] = f_bits(p);
+        for (uint32_t i : b.indices) words[w++] = i;
+        for (uint32_t p : blas[m].primitives) words[w++] = p;
+        for (size_t k = 0; k < blas[m].near.size(); ++k)
+            std::copy(blas[m].near[k].masks.begin(), blas[m].near[k].masks.end(), words.begin() + near_base[m][k]);
+    }
+    // src/gi_bvh.rs:230-274 (instance_words) + src/gi_cache.rs:4388-4479.
+    std::vector<uint32_t> head(HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS, 0u);
+    head[0] = instance_count;
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        const rq_instance& ins = instances[i];
+        const Blas& b = blas[ins.mesh];
+        float inv[16];
+        if (!mat4_inverse(ins.model, inv)) {
+            delete scene;
+            set_error("indirect geometry transform is singular");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        for (int row = 0; row < 3; ++row)
+            for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
+        instance_box(ins.model, b.lo, b.hi, &head[r + 12]);
+        for (int k = 0; k < 4; ++k) head[r + 18 + k] = offsets[4 * ins.mesh + k];
+        head[r + 22] = b.bvh.root;
+        // src/gi_bvh.rs:97-104 (near_base): the masks cooked under exactly this model's bits
+        uint32_t base = 0;
+        for (size_t k = 0; k < b.near.size() && !base; ++k)
+            if (std::memcmp(b.near[k].model, ins.model, sizeof ins.model) == 0) base = near_base[ins.mesh][k];
+        head[r + 23] = base;
+        uint32_t prim_base =
+            b.primitives.empty() ? 0u : offsets[4 * ins.mesh + 3] + (uint32_t)b.bvh.indices.size();
+        head[r + 24] = ins.identity;
+        head[r + 27] = ins.flags & 1u;
+        head[r + 28] = prim_base;
+        head[r + 29] = ins.placement;
+    }
+    head[4] = instance_count;
+    top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
+    std::copy(head.begin(), head.end(), words.begin());
+    // stats
+    uint64_t tris = 0, uniq = 0, nodes = 0;
+    uint32_t depth = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        uniq += blas[m].bvh.indices.size() / 3;
+        nodes += blas[m].bvh.nodes.size();
+        depth = std::max(depth, bvh_depth(blas[m].bvh, blas[m].bvh.root));
+    }
+    for (uint32_t i = 0; i < instance_count; ++i) tris += blas[instances[i].mesh].bvh.indices.size() / 3;
+    scene->stats[0] = instance_count;
+    scene->stats[1] = mesh_count;
+    scene->stats[2] = tris;
+    scene->stats[3] = uniq;
+    scene->stats[4] = nodes;
+    scene->stats[5] = words[2];
+    scene->stats[6] = depth;
+    scene->models.resize(16ull * instance_count);
+    for (uint32_t i = 0; i < instance_count; ++i) std::copy(instances[i].model, instances[i].model + 16, &scene->models[16ull * i]);
+    scene->mesh_nodes.resize(mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
+    scene->top_force = top_level;
+    scene->near_bins = near_bins;
+    scene_bounds(scene);
+    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - began).count();
+    *out = scene;
+    return RQ_OK;
+}
 
-}  // namespace
+// Bounds over all positions (src/gi_bvh.rs:64-87).
+void blas_bounds(Blas& b) {
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = __builtin_inff();
+        b.hi[a] = -__builtin_inff();
+    }
+    for (size_t v = 0; v + 2 < b.bvh.positions.size(); v += 3)
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = mn(b.lo[a], b.bvh.positions[v + a]);
+            b.hi[a] = mx(b.hi[a], b.bvh.positions[v + a]);
+        }
+}
+
 }  // namespace rq
 
 using namespace rq;
@@ -239,15 +355,7 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
                         ok[m] = 0;
                         continue;
                     }
-                    for (int a = 0; a < 3; ++a) {
-                        blas[m].lo[a] = __builtin_inff();
-                        blas[m].hi[a] = -__builtin_inff();
-                    }
-                    for (uint32_t v = 0; v < src.vertex_count; ++v)
-                        for (int a = 0; a < 3; ++a) {
-                            blas[m].lo[a] = mn(blas[m].lo[a], src.positions[3ull * v + a]);
-                            blas[m].hi[a] = mx(blas[m].hi[a], src.positions[3ull * v + a]);
-                        }
+                    blas_bounds(blas[m]);
                     if (src.keep_primitives)
                         map_primitives(blas[m].bvh.indices, src.indices, src.index_count, blas[m].primitives);
                 }
@@ -259,97 +367,7 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
             set_error("indirect geometry is not a valid triangle list: " + errs[m]);
             return RQ_ERR_INVALID_INPUT;
         }
-    // src/gi_bvh.rs:151-179 (layout)
-    uint64_t at = HEADER_WORDS + (uint64_t)instance_count * INSTANCE_WORDS + top_reserve(instance_count);
-    std::vector<uint32_t> offsets(4ull * mesh_count);
-    for (uint32_t m = 0; m < mesh_count; ++m) {
-        const Bvh& b = blas[m].bvh;
-        uint64_t nodes = at, leaves = nodes + b.nodes.size() * NODE_WORDS;
-        uint64_t positions = leaves + b.leaf_offset.size() * 2, indices = positions + b.positions.size();
-        if (indices + b.indices.size() + blas[m].primitives.size() >= 0xffffffffull) {
-            set_error("packed geometry exceeds 2^32 words");
-            return RQ_ERR_INVALID_INPUT;
-        }
-        offsets[4 * m] = (uint32_t)nodes;
-        offsets[4 * m + 1] = (uint32_t)leaves;
-        offsets[4 * m + 2] = (uint32_t)positions;
-        offsets[4 * m + 3] = (uint32_t)indices;
-        at += blas[m].words();
-    }
-    auto scene = new rq_scene();
-    std::vector<uint32_t>& words = scene->words;
-    words.assign(at, 0u);
-    // src/gi_bvh.rs:182-226 (mesh_words)
-    for (uint32_t m = 0; m < mesh_count; ++m) {
-        const Bvh& b = blas[m].bvh;
-        uint64_t w = offsets[4 * m];
-        for (const Node& n : b.nodes) {
-            std::memcpy(&words[w], &n, sizeof(Node));
-            w += NODE_WORDS;
-        }
-        for (size_t l = 0; l < b.leaf_offset.size(); ++l) {
-            words[w++] = b.leaf_offset[l];
-            words[w++] = b.leaf_count[l];
-        }
-        for (float p : b.positions) words[w++] = f_bits(p);
-        for (uint32_t i : b.indices) words[w++] = i;
-        for (uint32_t p : blas[m].primitives) words[w++] = p;
-    }
-    // src/gi_bvh.rs:230-274 (instance_words) + src/gi_cache.rs:4388-4479.
-    std::vector<uint32_t> head(HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS, 0u);
-    head[0] = instance_count;
-    for (uint32_t i = 0; i < instance_count; ++i) {
-        const rq_instance& ins = instances[i];
-        const Blas& b = blas[ins.mesh];
-        float inv[16];
-        if (!mat4_inverse(ins.model, inv)) {
-            delete scene;
-            set_error("indirect geometry transform is singular");
-            return RQ_ERR_INVALID_INPUT;
-        }
-        size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
-        for (int row = 0; row < 3; ++row)
-            for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
-        instance_box(ins.model, b.lo, b.hi, &head[r + 12]);
-        for (int k = 0; k < 4; ++k) head[r + 18 + k] = offsets[4 * ins.mesh + k];
-        head[r + 22] = b.bvh.root;
-        head[r + 23] = 0;  // no near-field masks
-        uint32_t prim_base =
-            b.primitives.empty() ? 0u : offsets[4 * ins.mesh + 3] + (uint32_t)b.bvh.indices.size();
-        head[r + 24] = ins.identity;
-        head[r + 27] = ins.flags & 1u;
-        head[r + 28] = prim_base;
-        head[r + 29] = ins.placement;
-    }
-    head[4] = instance_count;
-    top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
-    std::copy(head.begin(), head.end(), words.begin());
-    // stats
-    uint64_t tris = 0, uniq = 0, nodes = 0;
-    uint32_t depth = 0;
-    for (uint32_t m = 0; m < mesh_count; ++m) {
-        uniq += blas[m].bvh.indices.size() / 3;
-        nodes += blas[m].bvh.nodes.size();
-        depth = std::max(depth, bvh_depth(blas[m].bvh, blas[m].bvh.root));
-    }
-    for (uint32_t i = 0; i < instance_count; ++i) tris += blas[instances[i].mesh].bvh.indices.size() / 3;
-    scene->stats[0] = instance_count;
-    scene->stats[1] = mesh_count;
-    scene->stats[2] = tris;
-    scene->stats[3] = uniq;
-    scene->stats[4] = nodes;
-    scene->stats[5] = words[2];
-    scene->stats[6] = depth;
-    scene->models.resize(16ull * instance_count);
-    for (uint32_t i = 0; i < instance_count; ++i) std::copy(instances[i].model, instances[i].model + 16, &scene->models[16ull * i]);
-    scene->mesh_nodes.resize(mesh_count);
-    for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
-    scene->top_force = top_level;
-    scene_bounds(scene);
-    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
-                          std::chrono::steady_clock::now() - began).count();
-    *out = scene;
-    return RQ_OK;
+    return pack_scene(blas, instances, instance_count, top_level, 0, began, out);
 }
 
 extern "C" rq_status rq_scene_from_gi_words(const uint32_t* w, size_t n, rq_scene** out) {

@@ -109,6 +109,17 @@ _sig("rq_gi_item_rays", _i, _P, _sz, _P, _u32, _P, _P)
 _sig("rq_gi_rays", _i, _P, _i, _P, _sz, _P, _u32, _P, _P, _P)
 _sig("rq_gi_gather_rays", _i, _P, _sz, _P, _u32, _P, _P, _u32, _P, _P, _sz, _P)
 _sig("rq_gi_gather_workspace", _sz, _u32)
+_sig("rq_cluster_bvh_decode", _i, _P, _sz, _sz, ctypes.POINTER(_P))
+_sig("rq_cluster_bvh_encode", _i, _P, _sz, _P, _sz, ctypes.POINTER(_sz))
+_sig("rq_cluster_bvh_from_mesh", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
+_sig("rq_cluster_bvh_counts", _i, _P, _P)
+_sig("rq_cluster_bvh_destroy", None, _P)
+_sig("rq_near_field_decode", _i, _P, _sz, _sz, ctypes.POINTER(_P))
+_sig("rq_near_field_encode", _i, _P, _sz, _P, _sz, ctypes.POINTER(_sz))
+_sig("rq_near_field_info", _i, _P, _P, _P, _P, _P)
+_sig("rq_near_field_destroy", None, _P)
+_sig("rq_scene_build_cooked", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
+_sig("rq_near_blocked", _i, _P, _P, _u32, _P, _P)
 _sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
@@ -434,6 +445,96 @@ class _InstDesc(ctypes.Structure):
                 ("placement", _u32)]
 
 
+class _CookedDesc(ctypes.Structure):
+    _fields_ = [("mesh", _P), ("bvh", _P), ("field", _P), ("index", _u32), ("reserved", _u32)]
+
+
+def _encode_with(fn, handle, max_bytes):
+    n = _sz()
+    check(fn(handle, int(max_bytes), None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, np.uint8)
+    check(fn(handle, int(max_bytes), _ptr(out), out.size, ctypes.byref(n)))
+    return out.tobytes()
+
+
+class ClusterBvh:
+    """A cooked cluster BVH ("RLB\\0", crates/asset/src/cluster_bvh/codec.rs)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.rq_cluster_bvh_destroy(self._h)
+            self._h = None
+
+    @classmethod
+    def decode(cls, data, max_bytes=0):
+        b = np.frombuffer(bytes(data), np.uint8)
+        h = _P()
+        check(lib.rq_cluster_bvh_decode(_ptr(b) if b.size else None, b.size, int(max_bytes), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_mesh(cls, positions, indices, sah=False):
+        """near_field::build_bvh (sah=False) or build_bvh_sah as a one-level cluster BVH."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        i = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        h = _P()
+        check(lib.rq_cluster_bvh_from_mesh(_ptr(p), p.size // 3, _ptr(i), i.size, 1 if sah else 0, ctypes.byref(h)))
+        return cls(h)
+
+    def encode(self, max_bytes=0):
+        return _encode_with(lib.rq_cluster_bvh_encode, self._h, max_bytes)
+
+    def counts(self):
+        c = np.zeros(6, np.uint32)
+        check(lib.rq_cluster_bvh_counts(self._h, _ptr(c)))
+        return dict(zip(("nodes", "leaves", "levels", "level_leaves", "positions", "indices"), map(int, c)))
+
+
+class NearField:
+    """A cooked near field ("RNF\\0", crates/asset/src/near_field.rs)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.rq_near_field_destroy(self._h)
+            self._h = None
+
+    @classmethod
+    def decode(cls, data, max_bytes=0):
+        b = np.frombuffer(bytes(data), np.uint8)
+        h = _P()
+        check(lib.rq_near_field_decode(_ptr(b) if b.size else None, b.size, int(max_bytes), ctypes.byref(h)))
+        return cls(h)
+
+    def encode(self, max_bytes=0):
+        return _encode_with(lib.rq_near_field_encode, self._h, max_bytes)
+
+    def info(self):
+        counts = np.zeros(2, np.uint32)
+        bins = np.zeros(1, np.uint32)
+        settings = np.zeros(3, np.float32)
+        source = np.zeros(32, np.uint8)
+        check(lib.rq_near_field_info(self._h, _ptr(counts), _ptr(bins), _ptr(settings), _ptr(source)))
+        return dict(meshes=int(counts[0]), placements=int(counts[1]), bins=int(bins[0]),
+                    lod_error=float(settings[0]), reach=float(settings[1]), skip=float(settings[2]),
+                    source=source.tobytes())
+
+
+NEAR_QUERY_DTYPE = np.dtype([("inst", "<u4"), ("tri", "<u4", 3), ("bary", "<f4", 2), ("back", "<u4"),
+                             ("d", "<f4", 3)])
+
+
+def near_blocked(scene, queries, count, blocked_bits, stream=None):
+    """gi_near_blocked over NEAR_QUERY_DTYPE records (device); bit i set where query i is blocked."""
+    check(lib.rq_near_blocked(scene._h, _dptr(queries), int(count), _dptr(blocked_bits),
+                              stream.ptr if stream else None))
+
+
 class Scene:
     """The packed gi_geometry buffer plus its device layouts."""
 
@@ -484,6 +585,40 @@ class Scene:
         w = np.ascontiguousarray(words, np.uint32)
         h = _P()
         check(lib.rq_scene_from_rl_words(_ptr(w), w.size, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def build_cooked(cls, meshes, instances, top_level=-1):
+        """meshes: each ("mesh", (positions, indices, keep)), ("bvh", ClusterBvh, level) or
+        ("field", NearField, mesh index); instances as Scene.build."""
+        keep = []
+        md = (_CookedDesc * max(len(meshes), 1))()
+        raw = (_MeshDesc * max(len(meshes), 1))()
+        for k, (kind, src, *rest) in enumerate(meshes):
+            if kind == "mesh":
+                p = np.ascontiguousarray(src[0], np.float32).reshape(-1)
+                i = np.ascontiguousarray(src[1], np.uint32).reshape(-1)
+                keep += [p, i]
+                raw[k].positions, raw[k].vertex_count = p.ctypes.data, p.size // 3
+                raw[k].indices, raw[k].index_count = i.ctypes.data, i.size
+                raw[k].keep_primitives = int(src[2]) if len(src) > 2 else 0
+                md[k].mesh = ctypes.addressof(raw) + k * ctypes.sizeof(_MeshDesc)
+            elif kind == "bvh":
+                md[k].bvh, md[k].index = src._h, int(rest[0]) if rest else 0
+            else:
+                md[k].field, md[k].index = src._h, int(rest[0]) if rest else 0
+        idesc = (_InstDesc * max(len(instances), 1))()
+        for k, ins in enumerate(instances):
+            model = np.asarray(ins["model"], np.float32)
+            if model.shape == (4, 4):
+                model = model.T.reshape(-1)
+            idesc[k].model[:] = [float(x) for x in model.reshape(-1)]
+            idesc[k].mesh = int(ins["mesh"])
+            idesc[k].flags = int(ins.get("flags", 0))
+            idesc[k].identity = int(ins.get("identity", 0))
+            idesc[k].placement = int(ins.get("placement", k))
+        h = _P()
+        check(lib.rq_scene_build_cooked(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
         return cls(h)
 
     @classmethod
