@@ -15,13 +15,38 @@ __device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float mx(float a, float b) { return b > a ? b : a; }
 __device__ __forceinline__ float bf(uint32_t w) { return __uint_as_float(w); }
 __device__ __forceinline__ uint32_t fb(float f) { return __float_as_uint(f); }
+// f32 total order as a signed integer (atomic min/max of floats).
+__device__ __forceinline__ int32_t okey(float f) {
+    const int32_t k = __float_as_int(f);
+    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+}
+__device__ __forceinline__ float okey_inv(int32_t k) { return __int_as_float(k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1)); }
 
 // Recomputes node n of a mesh in `words` (arrays at word offsets nodes, leaves,
 // positions, indices): each child's box -- a leaf's corners, or the union of
 // both decodes of a child node, which must be final already -- then the
 // origin, steps and outward 8-bit codes quantized_node gives them.
+// A leaf's box: its triangles' corners folded in index order (the host's order,
+// so even the sign of a zero bound matches).
+__device__ inline void leaf_box(const uint32_t* words, uint32_t leaves, uint32_t positions, uint32_t indices,
+                                uint32_t l, float lo[3], float hi[3]) {
+    for (int x = 0; x < 3; ++x) {
+        lo[x] = __builtin_inff();
+        hi[x] = -__builtin_inff();
+    }
+    const uint32_t off = words[leaves + 2ull * l], cnt = words[leaves + 2ull * l + 1];
+    for (uint32_t i = off; i < off + cnt; ++i) {
+        const uint32_t* p = words + positions + 3ull * words[indices + i];
+        for (int x = 0; x < 3; ++x) {
+            lo[x] = mn(lo[x], bf(p[x]));
+            hi[x] = mx(hi[x], bf(p[x]));
+        }
+    }
+}
+
+// `boxes`: the mesh's leaf boxes (6 floats a leaf), or null to fold the triangles here.
 __device__ inline void refit_node(uint32_t* words, uint32_t nodes, uint32_t leaves, uint32_t positions,
-                                  uint32_t indices, uint32_t n) {
+                                  uint32_t indices, uint32_t n, const float* boxes = nullptr) {
     uint32_t* node = words + nodes + 16ull * n;
     uint32_t children[4];
     float blo[4][3], bhi[4][3];
@@ -33,13 +58,13 @@ __device__ inline void refit_node(uint32_t* words, uint32_t nodes, uint32_t leav
         float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
         if (child & LEAF_BIT) {
             const uint32_t l = child & ~LEAF_BIT;
-            const uint32_t off = words[leaves + 2ull * l], cnt = words[leaves + 2ull * l + 1];
-            for (uint32_t i = off; i < off + cnt; ++i) {
-                const uint32_t* p = words + positions + 3ull * words[indices + i];
+            if (boxes) {
                 for (int x = 0; x < 3; ++x) {
-                    lo[x] = mn(lo[x], bf(p[x]));
-                    hi[x] = mx(hi[x], bf(p[x]));
+                    lo[x] = boxes[6ull * l + x];
+                    hi[x] = boxes[6ull * l + 3 + x];
                 }
+            } else {
+                leaf_box(words, leaves, positions, indices, l, lo, hi);
             }
         } else {
             const uint32_t* cn = words + nodes + 16ull * child;

@@ -54,7 +54,7 @@ constexpr uint32_t MODE_SAH = 0, MODE_MEDIAN = 1;
 struct MeshDev {
     uint32_t tbase, tcount, vbase, vcount;
     uint32_t mode, keep, node_count, leaf_count;  // counts: set once the layout is known
-    uint32_t nodes, leaves, positions, indices, prims, pad2, pad3, pad4;
+    uint32_t nodes, leaves, positions, indices, prims, lbase, pad3, pad4;  // lbase: first leaf box
 };
 
 struct Seg {
@@ -69,11 +69,8 @@ struct Chunk {
     uint32_t seg, start, count, pad;
 };
 
-__device__ __forceinline__ int32_t okey(float f) {
-    const int32_t k = __float_as_int(f);
-    return k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
-}
-__device__ __forceinline__ float okey_inv(int32_t k) { return __int_as_float(k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1)); }
+using bvhdev::okey;
+using bvhdev::okey_inv;
 __device__ __forceinline__ uint32_t sat_index(float f, uint32_t cap) {
     if (!(f > 0.0f)) return 0;
     if (f >= (float)cap) return cap;
@@ -814,12 +811,32 @@ __global__ __launch_bounds__(256) void write_positions(const MeshDev* meshes, ui
     words[m.positions + (i - 3 * m.vbase)] = fb(pos[i]);
 }
 
+// Each leaf's box, folded from the written words (bvhdev::leaf_box), at its
+// mesh's slice of `boxes`.
+__global__ __launch_bounds__(256) void leaf_group_boxes(const uint32_t* gstart, const uint32_t* gcount,
+                                                        const uint32_t* gmesh, uint32_t n, const MeshDev* meshes,
+                                                        const uint32_t* lrank, const uint32_t* words, float* boxes) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= n || gcount[g] > LEAF_TRIANGLES) return;
+    const MeshDev m = meshes[gmesh[g]];
+    const uint32_t l = leaf_id(m, lrank, gstart[g]);
+    if (l >= m.leaf_count) return;
+    float lo[3], hi[3];
+    bvhdev::leaf_box(words, m.leaves, m.positions, m.indices, l, lo, hi);
+    for (int x = 0; x < 3; ++x) {
+        boxes[6ull * (m.lbase + l) + x] = lo[x];
+        boxes[6ull * (m.lbase + l) + 3 + x] = hi[x];
+    }
+}
+
 __global__ __launch_bounds__(256) void refit_nodes(uint32_t nbase, uint32_t nn, const uint32_t* node_mesh,
-                                                   const MeshDev* meshes, const uint32_t* id, uint32_t* words) {
+                                                   const MeshDev* meshes, const uint32_t* id, const float* boxes,
+                                                   uint32_t* words) {
     const uint32_t k = nbase + blockIdx.x * 256u + threadIdx.x;
     if (k >= nbase + nn) return;
     const MeshDev m = meshes[node_mesh[k]];
-    if (id[k] < m.node_count) bvhdev::refit_node(words, m.nodes, m.leaves, m.positions, m.indices, id[k]);
+    if (id[k] < m.node_count)
+        bvhdev::refit_node(words, m.nodes, m.leaves, m.positions, m.indices, id[k], boxes + 6ull * m.lbase);
 }
 
 }  // namespace gbuild
@@ -1266,6 +1283,7 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     const uint32_t* sm = summary.data();
     uint64_t at = HEADER_WORDS + (uint64_t)instance_count * INSTANCE_WORDS + top_reserve(instance_count);
     std::vector<uint32_t> offsets(4ull * mesh_count);
+    uint64_t leaf_total = 0;
     for (uint32_t m = 0; m < mesh_count; ++m) {
         const uint64_t nodes = at, leaves = nodes + (uint64_t)sm[4 * m] * NODE_WORDS;
         const uint64_t positions = leaves + 2ull * sm[4 * m + 1], indices = positions + 3ull * md[m].vcount;
@@ -1281,6 +1299,8 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
         md[m].prims = (uint32_t)(indices + 3ull * md[m].tcount);
         md[m].node_count = sm[4 * m];
         md[m].leaf_count = sm[4 * m + 1];
+        md[m].lbase = (uint32_t)leaf_total;
+        leaf_total += sm[4 * m + 1];
         at = end;
     }
     auto scene = new rq_scene();
@@ -1315,12 +1335,17 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
                            b.perm.as<uint32_t>(), b.idx.as<uint32_t>(), w);
         hipLaunchKernelGGL(write_positions, grid_for(3ull * b.verts), dim3(256), 0, b.s, b.meshes.as<MeshDev>(),
                            mesh_count, 3 * b.verts, b.pos.as<float>(), w);
-        // node boxes, deepest level first (children are final before parents)
+        // leaf boxes, then node boxes, deepest level first (children are final before parents)
+        if (b.box.need(std::max<uint64_t>(leaf_total, 1) * 24, "hipMalloc build") != RQ_OK)
+            return fail_scene(RQ_ERR_OUT_OF_MEMORY);
+        hipLaunchKernelGGL(leaf_group_boxes, grid_for(groups), dim3(256), 0, b.s, b.gstart.as<uint32_t>(),
+                           b.gcount.as<uint32_t>(), b.gmesh.as<uint32_t>(), groups, b.meshes.as<MeshDev>(),
+                           b.lrank.as<uint32_t>(), w, b.box.as<float>());
         for (size_t l = b.level_nbase.size() - 1; l-- > 0;) {
             const uint32_t nb = b.level_nbase[l], nn = b.level_nbase[l + 1] - nb;
             if (nn)
                 hipLaunchKernelGGL(refit_nodes, grid_for(nn), dim3(256), 0, b.s, nb, nn, b.node_mesh.as<uint32_t>(),
-                                   b.meshes.as<MeshDev>(), b.id.as<uint32_t>(), w);
+                                   b.meshes.as<MeshDev>(), b.id.as<uint32_t>(), b.box.as<float>(), w);
         }
         if (hipGetLastError() != hipSuccess) return fail_scene(hip_fail(hipGetLastError(), "build words launch"));
     }

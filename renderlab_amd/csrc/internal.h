@@ -171,6 +171,7 @@ struct DeviceLayout {
 struct RefitPlan {
     uint32_t nodes = 0;                  // the mesh's node word offset (its key)
     void* order = nullptr;               // device: node ids, lowest height first
+    void* boxes = nullptr;               // device: the mesh's leaf boxes (6 floats a leaf)
     std::vector<uint32_t> level_start;   // slice of `order` per height
 };
 

@@ -26,21 +26,41 @@ struct Args {
     uint32_t nodes, leaves, positions, indices;  // word offsets of the mesh's arrays
     const uint32_t* order;                       // node ids grouped by height
     uint32_t begin, end;                         // this launch's slice of `order`
+    const float* boxes;                          // the mesh's leaf boxes
 };
 
 __global__ __launch_bounds__(256) void refit_level(Args a) {
     const uint32_t k = a.begin + blockIdx.x * 256u + threadIdx.x;
     if (k >= a.end) return;
-    bvhdev::refit_node(a.words, a.nodes, a.leaves, a.positions, a.indices, a.order[k]);
+    bvhdev::refit_node(a.words, a.nodes, a.leaves, a.positions, a.indices, a.order[k], a.boxes);
+}
+
+// Every leaf's box of one mesh into boxes (6 floats a leaf: lo xyz, hi xyz),
+// so the lowest nodes read one box per leaf child, not its triangles.
+__global__ __launch_bounds__(256) void leaf_boxes(const uint32_t* words, uint32_t leaves, uint32_t positions,
+                                                  uint32_t indices, uint32_t count, float* boxes) {
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    if (l >= count) return;
+    float lo[3], hi[3];
+    bvhdev::leaf_box(words, leaves, positions, indices, l, lo, hi);
+    for (int x = 0; x < 3; ++x) {
+        boxes[6ull * l + x] = lo[x];
+        boxes[6ull * l + 3 + x] = hi[x];
+    }
 }
 
 constexpr uint32_t BOUNDS_BLOCKS = 512;
 
-// Bounds of the new positions (per block: lo xyz, hi xyz, non-finite flag);
-// bounds_finish folds the partials into scratch[BOUNDS_BLOCKS * 8 ...].
-// Min and max of finite floats do not depend on the order they are taken in
-// but for the sign of a zero, which no instance box word depends on.
-__global__ __launch_bounds__(256) void bounds_partial(const float* p, uint32_t vertices, float* part) {
+// Bounds of the new positions and whether all are finite, as seven words:
+// lo xyz and hi xyz as total-order keys (atomic min / max), then a non-finite
+// flag. Min and max of finite floats do not depend on the order they are taken
+// in but for the sign of a zero, which no instance box word depends on.
+__global__ __launch_bounds__(64) void bounds_init(int32_t* out) {
+    const uint32_t t = threadIdx.x;
+    if (t < 7) out[t] = t < 3 ? bvhdev::okey(__builtin_inff()) : t < 6 ? bvhdev::okey(-__builtin_inff()) : 0;
+}
+
+__global__ __launch_bounds__(256) void bounds_reduce(const float* p, uint32_t vertices, int32_t* out) {
     float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
     float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
     uint32_t bad = 0;
@@ -51,42 +71,20 @@ __global__ __launch_bounds__(256) void bounds_partial(const float* p, uint32_t v
             lo[x] = fminf(lo[x], f);
             hi[x] = fmaxf(hi[x], f);
         }
-    __shared__ float slo[3][256], shi[3][256];
-    __shared__ uint32_t sbad[256];
-    for (int x = 0; x < 3; ++x) {
-        slo[x][threadIdx.x] = lo[x];
-        shi[x][threadIdx.x] = hi[x];
-    }
-    sbad[threadIdx.x] = bad;
+    __shared__ int32_t s[7];
+    if (threadIdx.x < 7)
+        s[threadIdx.x] = threadIdx.x < 3 ? bvhdev::okey(__builtin_inff())
+                                         : threadIdx.x < 6 ? bvhdev::okey(-__builtin_inff()) : 0;
     __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int x = 0; x < 3; ++x) {
-                slo[x][threadIdx.x] = fminf(slo[x][threadIdx.x], slo[x][threadIdx.x + w]);
-                shi[x][threadIdx.x] = fmaxf(shi[x][threadIdx.x], shi[x][threadIdx.x + w]);
-                sbad[threadIdx.x] |= sbad[threadIdx.x + w];
-            }
-        __syncthreads();
+    for (int x = 0; x < 3; ++x) {
+        atomicMin(&s[x], bvhdev::okey(lo[x]));
+        atomicMax(&s[3 + x], bvhdev::okey(hi[x]));
     }
-    if (threadIdx.x < 8) {
-        float* o = part + 8ull * blockIdx.x;
-        const uint32_t t = threadIdx.x;
-        o[t] = t < 3 ? slo[t][0] : t < 6 ? shi[t - 3][0] : __uint_as_float(t == 6 ? sbad[0] : 0u);
-    }
-}
-
-__global__ __launch_bounds__(64) void bounds_finish(float* part, uint32_t blocks) {
-    const uint32_t t = threadIdx.x;
-    if (t >= 7) return;
-    float acc = t < 3 ? __builtin_inff() : -__builtin_inff();
-    uint32_t bad = 0;
-    for (uint32_t b = 0; b < blocks; ++b) {
-        const float f = part[8ull * b + t];
-        if (t < 3) acc = fminf(acc, f);
-        else if (t < 6) acc = fmaxf(acc, f);
-        else bad |= __float_as_uint(f);
-    }
-    part[8ull * BOUNDS_BLOCKS + t] = t < 6 ? acc : __uint_as_float(bad);
+    if (bad) atomicOr(&s[6], 1);
+    __syncthreads();
+    if (threadIdx.x < 3) atomicMin(&out[threadIdx.x], s[threadIdx.x]);
+    else if (threadIdx.x < 6) atomicMax(&out[threadIdx.x], s[threadIdx.x]);
+    else if (threadIdx.x == 6 && s[6]) atomicOr(&out[6], 1);
 }
 
 __global__ __launch_bounds__(256) void put_positions(uint32_t* words, uint32_t at, const float* p, uint32_t n) {
@@ -201,6 +199,7 @@ rq_status refit_plan(rq_scene* scene, const MeshView& m, RefitPlan*& out) {
     for (uint32_t l = 0; l <= top; ++l) plan.level_start[l + 1] += plan.level_start[l];
     std::vector<uint32_t> order(m.node_count), fill(plan.level_start.begin(), plan.level_start.end() - 1);
     for (uint32_t n = 0; n < m.node_count; ++n) order[fill[h[n]]++] = n;
+    RF_TRY(hipMalloc(&plan.boxes, std::max<size_t>(m.leaf_count, 1) * 6 * sizeof(float)), "hipMalloc refit plan");
     if (!order.empty()) {
         RF_TRY(hipMalloc(&plan.order, order.size() * 4), "hipMalloc refit plan");
         RF_TRY(hipMemcpy(plan.order, order.data(), order.size() * 4, hipMemcpyHostToDevice), "hipMemcpy refit plan");
@@ -313,22 +312,20 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
     RefitPlan* plan = nullptr;
     rq_status st = refit_plan(scene, m, plan);
     if (st != RQ_OK) return st;
-    if (!scene->refit_scratch)
-        RF_TRY(hipMalloc(&scene->refit_scratch, (refitk::BOUNDS_BLOCKS + 1) * 8 * sizeof(float)), "hipMalloc refit");
+    if (!scene->refit_scratch) RF_TRY(hipMalloc(&scene->refit_scratch, 8 * sizeof(int32_t)), "hipMalloc refit");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     uint32_t* words = reinterpret_cast<uint32_t*>(scene->dev[0].base);
-    float* part = reinterpret_cast<float*>(scene->refit_scratch);
+    int32_t* bnd = reinterpret_cast<int32_t*>(scene->refit_scratch);
     // 1. bounds of the new positions, and whether they are all finite
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((vertex_count + 255) / 256, refitk::BOUNDS_BLOCKS));
-    hipLaunchKernelGGL(refitk::bounds_partial, dim3(blocks), dim3(256), 0, s, positions, vertex_count, part);
-    hipLaunchKernelGGL(refitk::bounds_finish, dim3(1), dim3(64), 0, s, part, blocks);
-    float b[8];
-    RF_TRY(hipMemcpyAsync(b, part + 8ull * refitk::BOUNDS_BLOCKS, 7 * sizeof(float), hipMemcpyDeviceToHost, s),
-           "refit bounds");
+    hipLaunchKernelGGL(refitk::bounds_init, dim3(1), dim3(64), 0, s, bnd);
+    hipLaunchKernelGGL(refitk::bounds_reduce, dim3(blocks), dim3(256), 0, s, positions, vertex_count, bnd);
+    int32_t keys[7];
+    RF_TRY(hipMemcpyAsync(keys, bnd, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, s), "refit bounds");
     RF_TRY(hipStreamSynchronize(s), "refit bounds");
-    uint32_t bad;
-    std::memcpy(&bad, &b[6], 4);
-    if (bad) {
+    float b[6];
+    for (int k = 0; k < 6; ++k) b[k] = bits_f((uint32_t)(keys[k] ^ (int32_t)(((uint32_t)(keys[k] >> 31)) >> 1)));
+    if (keys[6]) {
         set_error("BVH refit needs finite positions of the same vertices");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -343,6 +340,9 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
     a.positions = m.positions;
     a.indices = m.indices;
     a.order = reinterpret_cast<const uint32_t*>(plan->order);
+    a.boxes = reinterpret_cast<const float*>(plan->boxes);
+    hipLaunchKernelGGL(refitk::leaf_boxes, dim3((m.leaf_count + 255) / 256), dim3(256), 0, s, words, m.leaves,
+                       m.positions, m.indices, m.leaf_count, reinterpret_cast<float*>(plan->boxes));
     for (size_t l = 0; l + 1 < plan->level_start.size(); ++l) {
         a.begin = plan->level_start[l];
         a.end = plan->level_start[l + 1];

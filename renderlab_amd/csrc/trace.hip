@@ -1530,8 +1530,10 @@ extern "C" void rq_scene_destroy(rq_scene* scene) {
     if (!scene) return;
     for (auto& d : scene->dev)
         if (d.base) (void)hipFree(d.base);
-    for (auto& p : scene->refit_plans)
+    for (auto& p : scene->refit_plans) {
         if (p.order) (void)hipFree(p.order);
+        if (p.boxes) (void)hipFree(p.boxes);
+    }
     if (scene->refit_scratch) (void)hipFree(scene->refit_scratch);
     delete scene;
 }
