@@ -314,7 +314,7 @@ extern "C" rq_status rq_near_blocked(const rq_scene* scene, const rq_near_query*
     if (!count) return RQ_OK;
     hipLaunchKernelGGL(gic::near_blocked, dim3((count + 255) / 256), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), reinterpret_cast<const uint32_t*>(scene->dev[0].base),
-                       (uint64_t)scene->words.size(), scene->near_bins, queries, count, blocked_bits);
+                       (uint64_t)scene_words(scene), scene->near_bins, queries, count, blocked_bits);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? RQ_OK : gic_fail(e, "rq_near_blocked launch");
 }

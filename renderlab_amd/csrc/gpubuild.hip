@@ -1303,9 +1303,13 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
         leaf_total += sm[4 * m + 1];
         at = end;
     }
+    // The host holds the header, records and top level; the meshes' words stay on
+    // the device until a host reader asks for them (sync_host_words).
     auto scene = new rq_scene();
     std::vector<uint32_t>& words = scene->words;
-    words.assign(at, 0u);
+    const size_t head_words = HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS + top_reserve(instance_count);
+    words.assign(head_words, 0u);
+    scene->total_words = at;
     DeviceLayout& d = scene->dev[0];
     d.bytes = (at * 4 + 255) & ~(size_t)255;
     hipError_t e = hipMalloc(&d.base, d.bytes);
@@ -1370,6 +1374,7 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     }
     head[4] = instance_count;
     top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
+    if (head.size() != head_words) return fail_scene(RQ_ERR_DEVICE);  // top_place appends exactly the reserve
     std::copy(head.begin(), head.end(), words.begin());
     if (hipMemcpyAsync(w, words.data(), head.size() * 4, hipMemcpyHostToDevice, b.s) != hipSuccess ||
         hipStreamSynchronize(b.s) != hipSuccess)

@@ -137,6 +137,8 @@ bool map_primitives(const std::vector<uint32_t>& bvh_indices, const uint32_t* so
                     std::vector<uint32_t>& out);
 bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
+// The packed words' length, on the host or (GPU-built, not yet fetched) the device.
+inline size_t scene_words(const rq_scene* s);
 // Fetches the stale ranges of the host words from the device REF copy.
 rq_status sync_host_words(const rq_scene* scene);
 // Fetches the stale host words inside [lo, hi) only.
@@ -218,5 +220,10 @@ struct rq_scene {
     // Word ranges (offset, count) where the device REF words are newer than
     // `words` (a refit wrote them on the device); sync_host_words fetches them.
     mutable std::vector<std::pair<uint64_t, uint64_t>> stale;
+    // A GPU-built scene keeps only its header, records and top level on the host
+    // until a host reader asks: `words` then grows to total_words (0: words.size()).
+    mutable size_t total_words = 0;
     int device = -1;
 };
+
+inline size_t rq::scene_words(const rq_scene* s) { return s->total_words > s->words.size() ? s->total_words : s->words.size(); }

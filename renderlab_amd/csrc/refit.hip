@@ -211,8 +211,15 @@ rq_status refit_plan(rq_scene* scene, const MeshView& m, RefitPlan*& out) {
 }  // namespace
 
 namespace rq {
+// Grows the host words to their full length (zeros; the new words are stale).
+static void grow_host_words(const rq_scene* scene) {
+    if (scene->total_words > scene->words.size())
+        const_cast<rq_scene*>(scene)->words.resize(scene->total_words, 0u);
+}
+
 rq_status sync_host_words(const rq_scene* scene) {
     if (!scene || scene->stale.empty()) return RQ_OK;
+    grow_host_words(scene);
     const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
     uint32_t* host = const_cast<uint32_t*>(scene->words.data());
     for (const auto& r : scene->stale)
@@ -224,6 +231,7 @@ rq_status sync_host_words(const rq_scene* scene) {
 // Fetches only the stale words inside [lo, hi), keeping the rest stale.
 rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi) {
     if (!scene || scene->stale.empty() || lo >= hi) return RQ_OK;
+    grow_host_words(scene);
     const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
     uint32_t* host = const_cast<uint32_t*>(scene->words.data());
     std::vector<std::pair<uint64_t, uint64_t>> keep;
@@ -282,7 +290,7 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
     const size_t rec = HEADER_WORDS + (size_t)record * INSTANCE_WORDS;
     bool planned = false;
     for (const auto& p : scene->refit_plans) planned = planned || p.nodes == w[rec + 18];
-    if (!planned && w[rec + 18] <= w[rec + 20] && w[rec + 20] <= w.size()) {
+    if (!planned && w[rec + 18] <= w[rec + 20] && w[rec + 20] <= scene_words(scene)) {
         rq_status st = sync_host_range(scene, w[rec + 18], w[rec + 20]);  // nodes and leaves
         if (st != RQ_OK) return st;
     }
