@@ -297,7 +297,7 @@ def run_batching(rq, layout, quick_tune=True):
     return {"rays": total, "batches": len(batches), "one_launch_ms": round(one, 3),
             "one_launch_mrays_s": round(total / one / 1e3, 1), "per_batch_launches_ms": round(sep, 3),
             "per_batch_mrays_s": round(total / sep / 1e3, 1),
-            "tuning": {k: tuned[k] for k in ("block", "stack", "refill", "grid_per_cu")}}
+            "tuning": {k: tuned[k] for k in ("block", "stack", "refill", "grid_per_cu", "speculate")}}
 
 
 def run_serving(rq, layout, scenes, rate, n_jobs, resolutions, mixes, seed=3, slots=4, budget=16 << 30):
@@ -547,7 +547,7 @@ def main():
         if tr is not None and not args.no_tune:
             from renderlab_amd import autotune
             t = autotune.tune_ranges(scene, layout, tr, stream=stream)
-            tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu")}
+            tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu", "speculate")}
             log(f"{layout}: autotuned {t}")
             frame.prime(layout)
         ms, wall, per = time_layout(rq, frame, layout, args.steps, args.warmup)

@@ -429,7 +429,7 @@ bool decode_cluster(const uint8_t* bytes, size_t n, size_t max_bytes, ClusterBvh
         for (float& f : nd.origin) r.f32(f);
         for (float& f : nd.step) r.f32(f);
         for (uint32_t& c : nd.children) r.u32(c);
-        const uint8_t* q;
+        const uint8_t* q = nullptr;  // the length check above covers every read
         for (int a = 0; a < 3; ++a) {
             r.take(4, q);
             std::memcpy(nd.lower[a], q, 4);

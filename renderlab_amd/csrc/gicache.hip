@@ -274,7 +274,8 @@ extern "C" rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words
     }
     size_t scan_bytes = 0;
     uint32_t* counts = nullptr;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, counts, slot_count, s);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, counts, slot_count, s);
+    if (e != hipSuccess) return gic_fail(e, "rq_gi_gather_rays scan size");
     if (workspace_bytes < rq_gi_gather_workspace(slot_count)) {
         set_error("rq_gi_gather_rays: workspace smaller than rq_gi_gather_workspace(slot_count)");
         return RQ_ERR_INVALID_INPUT;
@@ -286,7 +287,7 @@ extern "C" rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words
     const dim3 grid((slot_count + 255) / 256);
     hipLaunchKernelGGL(gic::gather_counts, grid, dim3(256), 0, s, table, (uint64_t)table_words, slots, slot_count,
                        counts);
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, counts, offsets, slot_count, s);
+    e = hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, counts, offsets, slot_count, s);
     if (e != hipSuccess) return gic_fail(e, "rq_gi_gather_rays scan");
     hipLaunchKernelGGL(gic::gather_write, grid, dim3(256), 0, s, table, (uint64_t)table_words, slots, slot_count,
                        counts, offsets, rays, items, max_rays, count);
@@ -297,7 +298,9 @@ extern "C" rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words
 extern "C" size_t rq_gi_gather_workspace(uint32_t slot_count) {
     size_t scan_bytes = 0;
     uint32_t* p = nullptr;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, p, p, std::max<uint32_t>(slot_count, 1), (hipStream_t)0);
+    // the size query only reads its arguments; it cannot fail for a valid count
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, p, p, std::max<uint32_t>(slot_count, 1),
+                                           (hipStream_t)0);
     return ((8ull * slot_count + 255) & ~255ull) + scan_bytes + 1024;
 }
 
