@@ -424,6 +424,29 @@ struct W8Builder {
     }
 };
 
+// Levels of the 8-wide tree under `tag` (0 for a leaf tag), counted without
+// recursion; stops once past `limit`.
+uint32_t w8_levels(const std::vector<uint32_t>& nodes, uint32_t tag, uint32_t limit) {
+    if (tag & LEAF_BIT) return 0;
+    uint32_t deepest = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> todo{{tag, 1u}};
+    while (!todo.empty()) {
+        const auto [node, level] = todo.back();
+        todo.pop_back();
+        deepest = std::max(deepest, level);
+        if (deepest > limit) break;
+        for (size_t s = 0; s < 8; ++s) {
+            const uint32_t c = nodes[(size_t)node * W8_WORDS + 16 + s];
+            if (!(c & LEAF_BIT)) todo.push_back({c, level + 1});
+        }
+    }
+    return deepest;
+}
+
+// The traversal kernel bounds a level's stack by 7 entries a tree level for
+// at most this many levels (it does not check pushes against the bound).
+constexpr uint32_t W8_MAX_LEVELS = 16;
+
 }  // namespace
 
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
@@ -442,6 +465,10 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
             if (d.first == pr) nr = d.second;
         if (nr == EMPTY) {
             if (!b.root(pr, nr)) return false;
+            if (w8_levels(out.nodes, nr, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+                err = "CW8 relayout: a mesh tree deeper than 16 levels";
+                return false;
+            }
             done.push_back({pr, nr});
         }
         out.inst[(size_t)i * INSTANCE_WORDS + 18] = nr;
@@ -453,6 +480,10 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     if (plain.top_on) {
         W8Builder t{&plain.top, &out.top, &err};
         if (!t.root(plain.top_root, out.top_root)) return false;
+        if (w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+            err = "CW8 relayout: a top level deeper than 16 levels";
+            return false;
+        }
     }
     if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
     if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);

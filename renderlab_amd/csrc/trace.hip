@@ -34,11 +34,13 @@ constexpr uint32_t NONE = 0xffffffffu;
 constexpr float FMAXV = 3.4028235e38f;
 // Per-lane stack words: a top-level and a BLAS bound of 48 each for the
 // 4-wide layouts (the reference's; depth <= 15 pushes at most 1 + 3*15), of
-// 112 each for CW8 (at most 1 + 7*15; an 8-wide tree is never deeper than
-// the 4-wide tree it is collapsed from).
+// 112 each for CW8 (7 per level of at most 16; an 8-wide tree is never deeper
+// than the 4-wide tree it is collapsed from, and the relayout rejects one
+// deeper than 16 levels).
 constexpr uint32_t SPILL_WORDS = 224;
 constexpr uint32_t STACK_CAP = 48;
-constexpr uint32_t CW8_CAP = 112;
+constexpr uint32_t CW8_LEVELS = 16;  // the CW8 relayout rejects deeper trees
+constexpr uint32_t CW8_CAP = 7 * CW8_LEVELS;
 constexpr uint32_t LEAF_MASK = (1u << 27) - 1u;
 
 enum : int { REF = 0, PLAIN = 1, CW8 = 2, RS = 3 };
@@ -272,6 +274,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         st[(sp & (S - 1)) * BLOCK] = x;
         sp += c ? 1u : 0u;
     };
+    // Unconditional write at stack position `at` (room() made it free).
+    auto put = [&](uint32_t x, uint32_t at) { st[(at & (S - 1)) * BLOCK] = x; };
     auto pop = [&]() -> uint32_t {
         sp--;
         if (sp < lo) {
@@ -836,17 +840,26 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                         second = tag_of(kmin2 & 7u);
                         hit &= ~(1u << (kmin2 & 7u));
                     }
+                    // Pushes by prefix count: slot k goes above the hit slots beyond it
+                    // (pushed first). A slot that is not pushed writes the word the next
+                    // pushed slot overwrites, or the free word above the top. The CW8
+                    // relayout keeps every tree within CW8_LEVELS levels, so a level's
+                    // stack stays within 7 * CW8_LEVELS = CW8_CAP entries unchecked.
                     room(8);
 #pragma unroll
-                    for (int k = 7; k >= 0; --k) push_if(tag[k], ((hit >> k) & 1u) && sp < cap);
-                    push_if(second, second != NONE && sp < cap);
-                    if (first != NONE && sp < cap) nxt = first;
+                    for (int k = 7; k >= 0; --k) put(tag[k], sp + (uint32_t)__popc(hit >> (k + 1)));
+                    const uint32_t n_hit = (uint32_t)__popc(hit);
+                    put(second, sp + n_hit);
+                    sp += n_hit + (second != NONE ? 1u : 0u);
+                    nxt = first;
                 } else {
                     room(7);
 #pragma unroll
-                    for (int k = 7; k >= 1; --k) push_if(tag[k], ((hit >> k) & 1u) && sp < cap);
-                    if ((hit & 1u) && sp < cap) nxt = tag[0];
+                    for (int k = 7; k >= 1; --k) put(tag[k], sp + (uint32_t)__popc(hit >> (k + 1)));
+                    sp += (uint32_t)__popc(hit >> 1);
+                    if (hit & 1u) nxt = tag[0];
                 }
+                (void)cap;
             }
             }
         }
