@@ -147,7 +147,10 @@ rq_status rq_instance_bvh_place(const uint32_t* words, uint32_t records_words, u
 rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                          uint32_t instance_count, int top_level, rq_scene** out);
 /* Adopts a reference-packed buffer verbatim (the bytes queue.write_buffer
-   would upload, src/gi_cache.rs:4361). */
+   would upload, src/gi_cache.rs:4361). RQ_ERR_INVALID_INPUT when a mesh tree
+   or the top level reaches a node twice, or an id, leaf or index points
+   outside its array: traversal of such words would not end or would read
+   outside the buffer. */
 rq_status rq_scene_from_gi_words(const uint32_t* words, size_t word_count, rq_scene** out);
 /* ---- Cooked geometry read from disk (crates/asset). The cluster BVH sidecar
    "RLB\0" (crates/asset/src/cluster_bvh/codec.rs:4-188, checked as

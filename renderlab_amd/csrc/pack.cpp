@@ -11,6 +11,7 @@
 #include <chrono>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "internal.h"
 
@@ -370,6 +371,81 @@ extern "C" rq_status rq_scene_build(const rq_mesh* meshes, uint32_t mesh_count, 
     return pack_scene(blas, instances, instance_count, top_level, 0, began, out);
 }
 
+namespace {
+
+// Words from outside this library: every traversal reads node, leaf, index and
+// position words by the ids it finds, and ends only if each tree is finite.
+// Checks that each mesh's tree under an instance root and the top level reach
+// no node twice, that every id stays inside its array, and that each reachable
+// leaf's indices stay inside the buffer and name vertices of the mesh.
+bool check_mesh_tree(const std::vector<uint32_t>& w, const MeshView& m, uint32_t root, std::string& err) {
+    std::vector<uint8_t> seen(m.node_count, 0);
+    std::vector<uint32_t> todo{root};
+    while (!todo.empty()) {
+        const uint32_t tag = todo.back();
+        todo.pop_back();
+        if (tag == EMPTY) continue;
+        if (tag & LEAF_BIT) {
+            const uint32_t l = tag & ~LEAF_BIT;
+            if (l >= m.leaf_count) {
+                err = "a leaf id past the mesh's leaves";
+                return false;
+            }
+            const uint64_t first = w[m.leaves + 2ull * l] / 3u * 3u, count = w[m.leaves + 2ull * l + 1] / 3u * 3u;
+            if (m.indices + first + count > w.size()) {
+                err = "a leaf's indices past the end of the buffer";
+                return false;
+            }
+            for (uint64_t k = 0; k < count; ++k)
+                if (w[m.indices + first + k] >= m.vertex_count) {
+                    err = "an index past the mesh's vertices";
+                    return false;
+                }
+            continue;
+        }
+        if (tag >= m.node_count || seen[tag]) {
+            err = tag >= m.node_count ? "a node id past the mesh's nodes" : "a mesh tree reaches a node twice";
+            return false;
+        }
+        seen[tag] = 1;
+        for (int c = 0; c < 4; ++c) todo.push_back(w[m.nodes + (size_t)tag * NODE_WORDS + 6 + c]);
+    }
+    return true;
+}
+
+bool check_top_tree(const std::vector<uint32_t>& w, std::string& err) {
+    const uint32_t count = w[0];
+    if (w[1] == 0) return true;
+    const uint64_t nodes = top_reserve(count) / TOP_WORDS;
+    if ((uint64_t)w[1] + nodes * TOP_WORDS > w.size()) {
+        err = "top level past the end of the buffer";
+        return false;
+    }
+    std::vector<uint8_t> seen(nodes, 0);
+    std::vector<uint32_t> todo{w[2]};
+    while (!todo.empty()) {
+        const uint32_t tag = todo.back();
+        todo.pop_back();
+        if (tag == EMPTY) continue;
+        if (tag & LEAF_BIT) {
+            if ((tag & ~LEAF_BIT) >= count) {
+                err = "a top-level leaf past the instance records";
+                return false;
+            }
+            continue;
+        }
+        if (tag >= nodes || seen[tag]) {
+            err = tag >= nodes ? "a top-level node id past the top level" : "the top level reaches a node twice";
+            return false;
+        }
+        seen[tag] = 1;
+        for (int c = 0; c < 4; ++c) todo.push_back(w[w[1] + (size_t)tag * TOP_WORDS + 24 + c]);
+    }
+    return true;
+}
+
+}  // namespace
+
 extern "C" rq_status rq_scene_from_gi_words(const uint32_t* w, size_t n, rq_scene** out) {
     if (!w || !out || n < HEADER_WORDS) {
         set_error("rq_scene_from_gi_words: buffer too small or null");
@@ -386,12 +462,25 @@ extern "C" rq_status rq_scene_from_gi_words(const uint32_t* w, size_t n, rq_scen
     }
     uint64_t tris = 0, nodes = 0;
     std::vector<uint32_t> seen;
+    std::unordered_set<uint64_t> checked_roots;  // node offset << 32 | root of each mesh tree checked
+    std::string err;
+    if (!check_top_tree(scene->words, err)) {
+        delete scene;
+        set_error("rq_scene_from_gi_words: " + err);
+        return RQ_ERR_INVALID_INPUT;
+    }
     for (uint32_t i = 0; i < count; ++i) {
         MeshView m;
-        std::string err;
         if (!mesh_view(scene->words, i, m, err)) {
             delete scene;
             set_error("rq_scene_from_gi_words: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+        const uint32_t root = scene->words[HEADER_WORDS + (size_t)i * INSTANCE_WORDS + 22];
+        if (checked_roots.insert((uint64_t)m.nodes << 32 | root).second &&
+            !check_mesh_tree(scene->words, m, root, err)) {
+            delete scene;
+            set_error("rq_scene_from_gi_words: instance " + std::to_string(i) + ": " + err);
             return RQ_ERR_INVALID_INPUT;
         }
         tris += m.index_count / 3;

@@ -264,3 +264,40 @@ def test_cw8_rejects_trees_deeper_than_16_levels():
     assert deep.layout_image("plain", 2).size == 120 * 32
     with pytest.raises(rq.InvalidInput, match="deeper than 16"):
         deep.layout_image("cw8", 2)
+
+
+def test_external_words_must_hold_finite_trees_inside_their_arrays():
+    """rq_scene_from_gi_words refuses words whose traversal would read outside the buffer or never
+    end: a node reached twice (here a cycle), ids past their arrays, a leaf past the indices, an
+    index past the vertices, a top level pointing past the instance records."""
+    base = chain_words(8)
+    rq.Scene.from_gi_words(base)
+    r = 16
+    nodes_at, leaves_at, pos_at, idx_at = (int(v) for v in base[r + 18:r + 22])
+
+    def bad(edit, match):
+        w = base.copy()
+        edit(w)
+        with pytest.raises(rq.InvalidInput, match=match):
+            rq.Scene.from_gi_words(w)
+
+    def cycle(w):
+        w[nodes_at + 16 * 3 + 7] = 5  # node 3's inner child becomes its ancestor 5
+    bad(cycle, "twice")
+    bad(lambda w: w.__setitem__(nodes_at + 16 * 2 + 7, 8), "past the mesh's nodes")
+    bad(lambda w: w.__setitem__(r + 22, 99), "past the mesh's nodes")
+    bad(lambda w: w.__setitem__(nodes_at + 6, LEAF | 9), "past the mesh's leaves")
+    bad(lambda w: w.__setitem__(leaves_at + 2 * 4, 3 * 400), "past the end of the buffer")
+    bad(lambda w: w.__setitem__(idx_at + 5, 27), "past the mesh's vertices")
+    two = rq.Scene.build([(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.arange(3, dtype=np.uint32), 0)],
+                         [dict(model=np.eye(4, dtype=np.float32), mesh=0)] * 50).words
+    rq.Scene.from_gi_words(two)
+    assert two[1] != 0
+    w = two.copy()
+    w[int(w[1]) + 24] = LEAF | 50
+    with pytest.raises(rq.InvalidInput, match="top-level leaf"):
+        rq.Scene.from_gi_words(w)
+    w = two.copy()
+    w[int(w[1]) + 24] = 0  # the root lists itself
+    with pytest.raises(rq.InvalidInput, match="twice"):
+        rq.Scene.from_gi_words(w)
