@@ -281,7 +281,9 @@ rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const float* posit
    (primitive, 0, 0), u, v, t, slot = placement slot, normal zero. */
 rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                                     uint32_t instance_count, int top_level, rq_scene** out);
-/* Adopts reference-packed rl_geometry words verbatim (src/gi_cache.rs binding 36 upload). */
+/* Adopts reference-packed rl_geometry words verbatim (src/gi_cache.rs binding 36 upload).
+   RQ_ERR_INVALID_INPUT when a tree reaches a node twice or a node, leaf or
+   top-level leaf points outside the buffer or the records. */
 rq_status rq_scene_from_rl_words(const uint32_t* words, size_t word_count, rq_scene** out);
 /* The packed reference words, gi_geometry or (reflection scene) rl_geometry
    (host memory owned by the scene). */

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <unordered_set>
 
 #include "internal.h"
 
@@ -231,6 +232,44 @@ extern "C" rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t m
     return RQ_OK;
 }
 
+namespace {
+
+// Words from outside this library: traversal reads node and triangle words by
+// the ids it finds and ends only if each tree is finite. Each instance's tree
+// under its root and the top level must reach no node twice and stay inside
+// the buffer (nodes at their base, leaves' triangles at the triangle base,
+// top-level leaves among the records).
+bool check_rs_tree(const uint32_t* w, size_t n, uint64_t base, uint32_t node_words, uint32_t root, uint64_t tri_base,
+                   uint32_t child_word, bool top, std::string& err) {
+    std::unordered_set<uint32_t> seen;
+    std::vector<uint32_t> todo{root};
+    while (!todo.empty()) {
+        const uint32_t tag = todo.back();
+        todo.pop_back();
+        if (tag == EMPTY) continue;
+        if (tag & LEAF_BIT) {
+            if (top) {
+                if ((tag & ~LEAF_BIT) >= w[0]) {
+                    err = "a top-level leaf past the instance records";
+                    return false;
+                }
+            } else if (tri_base + (uint64_t)RS_TRI_WORDS * (((tag & ~LEAF_BIT) >> 4) + (tag & 15u)) > n) {
+                err = "a leaf's triangles past the end of the buffer";
+                return false;
+            }
+            continue;
+        }
+        if (base + (uint64_t)node_words * ((uint64_t)tag + 1) > n || !seen.insert(tag).second) {
+            err = seen.count(tag) ? "a tree reaches a node twice" : "a node past the end of the buffer";
+            return false;
+        }
+        for (int c = 0; c < 4; ++c) todo.push_back(w[base + (uint64_t)node_words * tag + child_word + c]);
+    }
+    return true;
+}
+
+}  // namespace
+
 extern "C" rq_status rq_scene_from_rl_words(const uint32_t* w, size_t n, rq_scene** out) {
     if (!out || !w || n < RS_HEADER_WORDS) {
         set_error("rq_scene_from_rl_words: needs at least the 8 header words");
@@ -250,6 +289,22 @@ extern "C" rq_status rq_scene_from_rl_words(const uint32_t* w, size_t n, rq_scen
         const uint32_t* r = w + RS_HEADER_WORDS + i * RS_RECORD_WORDS;
         if (r[18] > n || r[19] > n || (r[18] & 3u) || (r[19] & 3u)) {
             set_error("rq_scene_from_rl_words: instance " + std::to_string(i) + " names arrays past the end");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    {
+        std::string err;
+        std::unordered_set<uint64_t> checked;  // node base << 32 | root of each tree checked
+        for (uint64_t i = 0; i < count; ++i) {
+            const uint32_t* r = w + RS_HEADER_WORDS + i * RS_RECORD_WORDS;
+            if (checked.insert((uint64_t)r[18] << 32 | r[20]).second &&
+                !check_rs_tree(w, n, r[18], RS_NODE_WORDS, r[20], r[19], 6, false, err)) {
+                set_error("rq_scene_from_rl_words: instance " + std::to_string(i) + ": " + err);
+                return RQ_ERR_INVALID_INPUT;
+            }
+        }
+        if (count && w[1] && !check_rs_tree(w, n, w[1], RS_TOP_NODE_WORDS, w[2], 0, 24, true, err)) {
+            set_error("rq_scene_from_rl_words: " + err);
             return RQ_ERR_INVALID_INPUT;
         }
     }

@@ -217,3 +217,33 @@ def test_rl_words_round_trip_and_checks():
         rq.Scene.from_rl_words(bad)
     with pytest.raises(rq.InvalidInput):
         sc.layout_image("plain", 0)
+
+
+def test_rl_words_must_hold_finite_trees_inside_the_buffer():
+    """External rl_geometry words whose traversal would loop or read outside the buffer are
+    refused: a node reached twice, a node or leaf past the end, a top-level leaf past the
+    records."""
+    pos, idx = grid(6)
+    sc = rq.Scene.build([(pos, idx, 0)], [dict(model=np.eye(4), mesh=0)] * 45, reflection=True)
+    w = sc.words
+    rec = RS_HEADER
+    nodes_at, root = int(w[rec + 18]), int(w[rec + 20])
+    assert not root & rq.LEAF_BIT
+    kids = w[nodes_at + 16 * root + 6:nodes_at + 16 * root + 10]
+    inner = [s for s, c in enumerate(kids) if c != rq.EMPTY and not c & rq.LEAF_BIT]
+    leaf = [s for s, c in enumerate(kids) if c != rq.EMPTY and c & rq.LEAF_BIT]
+
+    def bad(edit, match):
+        b = w.copy()
+        edit(b)
+        with pytest.raises(rq.InvalidInput, match=match):
+            rq.Scene.from_rl_words(b)
+
+    if inner:
+        bad(lambda b: b.__setitem__(nodes_at + 16 * root + 6 + inner[0], root), "twice")
+    slot = (inner + leaf)[0]
+    bad(lambda b: b.__setitem__(nodes_at + 16 * root + 6 + slot, len(w)), "node past the end")
+    bad(lambda b: b.__setitem__(nodes_at + 16 * root + 6 + slot, rq.LEAF_BIT | (len(w) << 4) | 2),
+        "triangles past the end")
+    top = int(w[1])
+    bad(lambda b: b.__setitem__(top + 32 * int(w[2]) + 24, rq.LEAF_BIT | 45), "top-level leaf")
