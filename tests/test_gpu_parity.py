@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import classify_mismatch, compare_exact, gpu_trace, gpu_workload, vis_bits
+from helpers import chain_words, classify_mismatch, compare_exact, gpu_trace, gpu_workload, vis_bits
 from renderlab_amd import rq
 
 pytestmark = pytest.mark.gpu
@@ -76,6 +76,33 @@ def test_cw8_closest_within_tolerance(scene_c0, workload_c0):
         assert (g["u"][same] == o["u"][same]).all() and (g["v"][same] == o["v"][same]).all()
 
 
+def test_cw8_speculation_changes_order_only(scene_c0, workload_c0):
+    """Node-step speculation (lanes with triangle pairs pending join the wave's node steps) only
+    reorders each lane's visits: visibility bits are identical with it off and on, and closest
+    hits differ only between equal-distance candidates; a hit on the same triangle is the same
+    record."""
+    _, prim, _, shadow, gis = workload_c0
+    base = [rq.get_tuning("cw8", m) for m in (0, 1)]
+    out = {}
+    try:
+        for spec in (0, 1):
+            for m, t in enumerate(base):
+                rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=spec)
+            print(f"cw8 speculate={spec}", flush=True)
+            out[spec] = ([gpu_trace(scene_c0, "cw8", r, rq.CLOSEST)[0] for r in (prim, gis[0])],
+                         gpu_trace(scene_c0, "cw8", shadow, 1)[1])
+    finally:
+        for m, t in enumerate(base):
+            rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=t.speculate)
+    assert (out[0][1] == out[1][1]).all()
+    for g0, g1 in zip(out[0][0], out[1][0]):
+        bad, _ = classify_mismatch(g1, g0)
+        assert bad.size == 0, f"{bad.size} non-tie differences, first {bad[:5]}"
+        same = (g0["inst"] == g1["inst"]) & (g0["slot"] == g1["slot"])
+        for k in ("t", "u", "v", "flags"):
+            assert (g0[k][same] == g1[k][same]).all(), k
+
+
 def test_secondary_rays_match_oracle(workload_c0):
     inv, prim, hits, shadow, gis = workload_c0
     masks, rays = oracle.secondary_rays(prim, hits, np.array([0.3, 0.8, 0.2], np.float32) /
@@ -121,3 +148,24 @@ def test_fp64_verification_gi_conditioned(scene_c0, workload_c0):
     print(f"fp64 gi: {v}", flush=True)
     assert v["conditioned"] > 0.5 * v["checked"]
     assert v["max_ulps_conditioned"] < 16
+
+
+def test_cw8_deep_chain_against_the_flat_answer():
+    """A mesh whose 4-wide tree is a 100-node chain collapses into a 15-level CW8 tree: the CW8
+    stack (7 entries a level, no bound check) holds it, and every ray straight down onto a
+    triangle finds that triangle at t = 1, as the tree-free flat loop does."""
+    scene = rq.Scene.from_gi_words(chain_words(100))
+    scene.upload(("cw8",))
+    rays = np.zeros(101, rq.RAY_DTYPE)
+    rays["origin"] = np.stack([np.arange(101) + 0.1, np.full(101, 0.1), np.ones(101)], axis=1)
+    rays["direction"] = [0.0, 0.0, -1.0]
+    rays["tmax"] = 10.0
+    print("cw8 chain(100) n=101", flush=True)
+    g, gv = gpu_trace(scene, "cw8", rays, rq.CLOSEST)
+    f = oracle.flat_f32(scene.words, rays, 0)
+    assert ((g["flags"] & 1) == 1).all() and (g["slot"] == np.arange(101)).all()
+    assert (g["t"] == 1.0).all()
+    assert (g["slot"] == f["slot"]).all() and (g["t"] == f["t"]).all()
+    _, gv = gpu_trace(scene, "cw8", rays, 1)
+    assert (gv == 0).all()  # every ray is blocked
+

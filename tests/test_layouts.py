@@ -5,6 +5,7 @@ every triangle, in leaf order."""
 import numpy as np
 import pytest
 
+from helpers import chain_words
 from renderlab_amd import rq
 
 LEAF, EMPTY = rq.LEAF_BIT, rq.EMPTY
@@ -219,38 +220,6 @@ def test_plain_triangle_records_hold_vertex_and_edges():
             assert [int(x) for x in words[idx + 3 * (k - base):idx + 3 * (k - base) + 3]] == [ia, ib, ic]
             checked += 1
     assert checked > 0
-
-
-def chain_words(depth):
-    """One instance of a mesh whose 4-wide tree is a chain `depth` nodes deep: node k holds leaf k
-    and node k - 1 (node 0 holds leaves 0 and `depth`); every child box is [0, 255]^3."""
-    tris = depth + 1
-    pos = np.zeros((3 * tris, 3), np.float32)
-    for t in range(tris):
-        pos[3 * t:3 * t + 3] = [[t, 0, 0], [t + 0.5, 0, 0], [t, 0.5, 0]]
-    nodes = np.zeros((depth, 16), np.uint32)
-    nodes[:, 3:6] = np.float32(1.0).view(np.uint32)  # origin 0, step 1
-    for k in range(depth):
-        nodes[k, 6] = LEAF | k
-        nodes[k, 7] = (LEAF | depth) if k == 0 else k - 1
-        nodes[k, 8:10] = EMPTY
-        nodes[k, 13:16] = 0xFFFF  # upper codes 255 for children 0 and 1
-    leaves = np.array([[3 * t, 3] for t in range(tris)], np.uint32)
-    rec = np.zeros(32, np.uint32)
-    rec[[0, 5, 10]] = np.float32(1.0).view(np.uint32)  # identity rows
-    rec[15:18] = np.float32(256.0).view(np.uint32)      # world box [0, 256]^3
-    at = 16 + 32 + 32
-    rec[18] = at
-    rec[19] = rec[18] + nodes.size
-    rec[20] = rec[19] + leaves.size
-    rec[21] = rec[20] + pos.size
-    rec[22] = depth - 1
-    header = np.zeros(16, np.uint32)
-    header[0] = header[4] = 1
-    words = np.concatenate([header, rec, np.zeros(32, np.uint32), nodes.reshape(-1), leaves.reshape(-1),
-                            pos.reshape(-1).view(np.uint32), np.arange(3 * tris, dtype=np.uint32)])
-    assert words.size == rec[21] + 3 * tris
-    return words
 
 
 def test_cw8_rejects_trees_deeper_than_16_levels():
