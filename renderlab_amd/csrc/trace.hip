@@ -187,6 +187,7 @@ struct TraceArgs {
     uint32_t n_batches;
     uint32_t count, refill_min;
     uint32_t speculate;  // CW8: lanes with triangles pending take node steps with the wave
+    uint32_t first;      // rays handed out without the cursor (64 per wave of the grid); the cursor counts from here
 };
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
@@ -245,6 +246,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t tri_next = 0, tri_left = 0;  // the current leaf's untested triangles
     uint32_t nxt = NONE;                  // the next item to visit (the top of the stack, in a register)
     bool exhausted = false;
+    bool first_take = true;  // the first refill takes rays [64 gwave, 64 gwave + 64)
+    const uint32_t gwave = blockIdx.x * (uint32_t)(BLOCK / 64) + tid / 64u;
     unsigned long long nb = 0;
     // COUNT: [1] wave iterations, [2] node, [3] triangle-pair, [4] instance
     // steps (lane sums), [5] refill events (per wave)
@@ -370,14 +373,22 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         }
         if (COUNT && refill_now && !exhausted) st_refill++;
         if (!exhausted && refill_now) {
-            uint32_t n = (uint32_t)__popcll(idle);
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(a.counter, n);
-            base = __shfl(base, 0);
+            const uint32_t n = (uint32_t)__popcll(idle);
+            uint64_t base;
+            if (first_take) {
+                // every lane is idle at the start: the wave's own 64 rays, no atomic
+                // (thousands of waves starting at once would queue on one cursor)
+                base = (uint64_t)gwave * 64u;
+                first_take = false;
+            } else {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(a.counter, n);
+                base = (uint64_t)a.first + __shfl(b, 0);
+            }
             if (base + n >= count) exhausted = true;
-            if (ray == NONE) {
-                uint32_t r = base + lane_rank(idle);
-                if (r < count) {
+            if (ray == NONE && base + lane_rank(idle) < count) {
+                {
+                    uint32_t r = (uint32_t)(base + lane_rank(idle));
                     if (!BATCHED && a.order) {
                         r = a.order[r];
                         if (COUNT) nb += 4;
@@ -525,9 +536,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
         }
         if (COUNT) {
-            st_want_node += n_node;
-            st_want_tri += n_tri;
-            st_want_inst += n_inst;
+            // once per wave iteration: by the lowest lane still here (summed over lanes below)
+            if (lane == (uint32_t)__ffsll((unsigned long long)__ballot(true)) - 1u) {
+                st_want_node += n_node;
+                st_want_tri += n_tri;
+                st_want_inst += n_inst;
+            }
             st_node += (kind == 3 && pick == 3) || spec;
             st_tri += kind == 1 && pick == 1;
             st_inst += kind == 2 && pick == 2;
@@ -871,6 +885,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             st_node += __shfl_xor(st_node, off);
             st_tri += __shfl_xor(st_tri, off);
             st_inst += __shfl_xor(st_inst, off);
+            st_want_node += __shfl_xor(st_want_node, off);
+            st_want_tri += __shfl_xor(st_want_tri, off);
+            st_want_inst += __shfl_xor(st_want_inst, off);
         }
         if (lane == 0) {
             atomicAdd(a.bytes, nb);
@@ -1686,6 +1703,7 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     a.count_ptr = count_ptr;
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
+    a.first = (uint32_t)(grid * k->block);
     a.bytes = bytes;
     if (reorder && count >= REORDER_MIN) {
         // keys, indices, sorted keys, order, then the sort's own space
@@ -1797,6 +1815,7 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
     a.spill = res->spill;
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
+    a.first = (uint32_t)(grid * k->block);
     k->rfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "trace ranges launch");
     return RQ_OK;
@@ -1879,6 +1898,7 @@ extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_co
     a.n_batches = batch_count;
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
+    a.first = (uint32_t)(grid * k->block);
     k->bfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "batched trace launch");
     // The table is read by the kernel: the next call on this stream may overwrite it only after.
