@@ -380,7 +380,7 @@ typedef struct rq_tuning {
     uint32_t grid_per_cu;  /* persistent workgroups per CU (0: auto) */
     uint32_t refill;       /* idle lanes a wave gathers before it takes new rays: 1 .. 64 */
     uint32_t speculate;    /* CW8: lanes with triangles pending join the wave's node steps */
-    uint32_t reserved;
+    uint32_t drain;        /* once the rays run out, every live lane steps each iteration */
 } rq_tuning;
 rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out);
 rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* tuning);

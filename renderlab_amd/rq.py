@@ -178,7 +178,7 @@ def _ptr(a):
 
 class RqTuning(ctypes.Structure):
     _fields_ = [("block", _u32), ("stack", _u32), ("persistent", _u32), ("waves_per_eu", _u32),
-                ("grid_per_cu", _u32), ("refill", _u32), ("speculate", _u32), ("reserved", _u32)]
+                ("grid_per_cu", _u32), ("refill", _u32), ("speculate", _u32), ("drain", _u32)]
 
 
 def device_count():
@@ -428,10 +428,10 @@ def get_batch_tuning(layout):
     return t
 
 
-def set_batch_tuning(layout, block, stack, refill, grid_per_cu=0, speculate=0):
+def set_batch_tuning(layout, block, stack, refill, grid_per_cu=0, speculate=0, drain=1):
     t = RqTuning()
     t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
-    t.refill, t.speculate = refill, speculate
+    t.refill, t.speculate, t.drain = refill, speculate, drain
     check(lib.rq_set_batch_tuning(LAYOUTS.get(layout, layout), ctypes.byref(t)))
 
 
@@ -774,10 +774,10 @@ def get_tuning(layout, mode):
     return t
 
 
-def set_tuning(layout, mode, block, stack, refill, grid_per_cu=0, speculate=0):
+def set_tuning(layout, mode, block, stack, refill, grid_per_cu=0, speculate=0, drain=1):
     t = RqTuning()
     t.block, t.stack, t.persistent, t.grid_per_cu = block, stack, 1, grid_per_cu
-    t.refill, t.speculate = refill, speculate
+    t.refill, t.speculate, t.drain = refill, speculate, drain
     check(lib.rq_set_tuning(LAYOUTS.get(layout, layout), int(mode), ctypes.byref(t)))
 
 

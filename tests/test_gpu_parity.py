@@ -87,13 +87,13 @@ def test_cw8_speculation_changes_order_only(scene_c0, workload_c0):
     try:
         for spec in (0, 1):
             for m, t in enumerate(base):
-                rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=spec)
+                rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=spec, drain=t.drain)
             print(f"cw8 speculate={spec}", flush=True)
             out[spec] = ([gpu_trace(scene_c0, "cw8", r, rq.CLOSEST)[0] for r in (prim, gis[0])],
                          gpu_trace(scene_c0, "cw8", shadow, 1)[1])
     finally:
         for m, t in enumerate(base):
-            rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=t.speculate)
+            rq.set_tuning("cw8", m, t.block, t.stack, t.refill, t.grid_per_cu, speculate=t.speculate, drain=t.drain)
     assert (out[0][1] == out[1][1]).all()
     for g0, g1 in zip(out[0][0], out[1][0]):
         bad, _ = classify_mismatch(g1, g0)

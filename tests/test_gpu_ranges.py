@@ -106,4 +106,5 @@ def test_tuner_times_the_launch_on_its_stream(scene, passes):
         print(f"tuned {t}, timed {ms:.4f} ms", flush=True)
         assert 0.33 * ms <= t["ms"] <= 3.0 * ms
     finally:
-        rq.set_tuning("cw8", 0, base.block, base.stack, base.refill, base.grid_per_cu, speculate=base.speculate)
+        rq.set_tuning("cw8", 0, base.block, base.stack, base.refill, base.grid_per_cu, speculate=base.speculate,
+                      drain=base.drain)
