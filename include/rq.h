@@ -474,6 +474,9 @@ rq_status rq_stream_synchronize(void* stream);
 rq_status rq_event_create(void** event);
 rq_status rq_event_destroy(void* event);
 rq_status rq_event_record(void* event, void* stream);
+/* Work queued on `stream` after this call waits until `event` completes
+   (hipStreamWaitEvent): how two streams order their launches. */
+rq_status rq_stream_wait_event(void* stream, void* event);
 rq_status rq_event_synchronize(void* event);
 rq_status rq_event_elapsed_ms(void* start, void* stop, float* ms);
 rq_status rq_event_query(void* event); /* RQ_OK when complete, RQ_ERR_NOT_READY otherwise */

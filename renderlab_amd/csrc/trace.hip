@@ -2143,6 +2143,12 @@ extern "C" rq_status rq_event_record(void* event, void* stream) {
             "hipEventRecord");
     return RQ_OK;
 }
+extern "C" rq_status rq_stream_wait_event(void* stream, void* event) {
+    if (!event) return RQ_ERR_INVALID_INPUT;
+    HIP_TRY(hipStreamWaitEvent(reinterpret_cast<hipStream_t>(stream), reinterpret_cast<hipEvent_t>(event), 0),
+            "hipStreamWaitEvent");
+    return RQ_OK;
+}
 extern "C" rq_status rq_event_synchronize(void* event) {
     HIP_TRY(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
     return RQ_OK;

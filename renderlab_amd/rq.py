@@ -152,6 +152,7 @@ _sig("rq_stream_synchronize", _i, _P)
 _sig("rq_event_create", _i, ctypes.POINTER(_P))
 _sig("rq_event_destroy", _i, _P)
 _sig("rq_event_record", _i, _P, _P)
+_sig("rq_stream_wait_event", _i, _P, _P)
 _sig("rq_event_synchronize", _i, _P)
 _sig("rq_event_elapsed_ms", _i, _P, _P, ctypes.POINTER(ctypes.c_float))
 _sig("rq_event_query", _i, _P)
@@ -251,6 +252,10 @@ class Stream:
 
     def reserve(self, max_rays):
         check(lib.rq_stream_reserve(self.ptr, int(max_rays)))
+
+    def wait(self, event):
+        """Work queued on this stream from now on waits until `event` completes."""
+        check(lib.rq_stream_wait_event(self.ptr, event.ptr))
 
     def __del__(self):
         try:
