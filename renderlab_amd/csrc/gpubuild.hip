@@ -35,6 +35,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <thread>
 
 #include "bvh_device.h"
@@ -1150,6 +1151,67 @@ struct HostMesh {
     bool ok = true;
 };
 
+// Pinned host staging for the build's two uploads, kept between builds:
+// pinning hundreds of megabytes costs more than a build, and an upload from
+// pageable memory runs at a fraction of the pinned rate. The process keeps the
+// largest buffer it has needed (one per concurrent build at most) until it ends.
+struct PinnedPool {
+    std::mutex mu;
+    std::vector<std::pair<void*, size_t>> free;  // (pointer, bytes)
+};
+PinnedPool& pinned_pool() {
+    static PinnedPool* pool = new PinnedPool();  // never destroyed: outlives every build
+    return *pool;
+}
+
+// One build's staging: taken from the pool, returned once the stream has
+// finished reading it (the destructor waits for the stream).
+struct Staging {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipStream_t s = nullptr;
+    bool queued = false;  // an upload from this buffer may still be in flight
+    rq_status take(size_t need) {
+        PinnedPool& pool = pinned_pool();
+        {
+            std::lock_guard<std::mutex> lock(pool.mu);
+            size_t best = pool.free.size();
+            for (size_t i = 0; i < pool.free.size(); ++i)
+                if (pool.free[i].second >= need && (best == pool.free.size() || pool.free[i].second < pool.free[best].second))
+                    best = i;
+            if (best < pool.free.size()) {
+                p = pool.free[best].first;
+                bytes = pool.free[best].second;
+                pool.free.erase(pool.free.begin() + best);
+                return RQ_OK;
+            }
+        }
+        const size_t grain = need <= (1u << 20) ? (1u << 20) : (16u << 20);
+        bytes = (need + grain - 1) & ~(grain - 1);
+        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return hip_fail(e, "hipHostMalloc build staging");
+        }
+        return RQ_OK;
+    }
+    ~Staging() {
+        if (!p) return;
+        if (queued) (void)hipStreamSynchronize(s);
+        PinnedPool& pool = pinned_pool();
+        std::lock_guard<std::mutex> lock(pool.mu);
+        // keep this buffer; drop pooled ones it makes redundant (smaller ones)
+        for (size_t i = 0; i < pool.free.size();)
+            if (pool.free[i].second <= bytes) {
+                (void)hipHostFree(pool.free[i].first);
+                pool.free.erase(pool.free.begin() + i);
+            } else {
+                ++i;
+            }
+        pool.free.emplace_back(p, bytes);
+    }
+};
+
 }  // namespace
 
 extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
@@ -1177,8 +1239,11 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
         set_error("rq_scene_build_gpu: too many triangles or vertices for one build");
         return RQ_ERR_INVALID_INPUT;
     }
-    std::vector<float> stage_pos(3 * vbase[mesh_count]);
-    std::vector<uint32_t> stage_idx(3 * tbase[mesh_count]);
+    Staging staging;
+    staging.s = reinterpret_cast<hipStream_t>(stream);
+    if (vbase[mesh_count] + tbase[mesh_count]) GB_OK(staging.take(12ull * (vbase[mesh_count] + tbase[mesh_count])));
+    float* const stage_pos = static_cast<float*>(staging.p);
+    uint32_t* const stage_idx = stage_pos ? reinterpret_cast<uint32_t*>(stage_pos + 3 * vbase[mesh_count]) : nullptr;
     std::vector<HostMesh> hm(mesh_count);
     {
         uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 64));
@@ -1233,6 +1298,33 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
             set_error("indirect geometry transform is singular");
             return RQ_ERR_INVALID_INPUT;
         }
+    // src/gi_bvh.rs:230-274 (instance_words) + src/gi_cache.rs:4388-4479, as rq_scene_build:
+    // the header, the records and the top level need only the transforms and the mesh
+    // bounds, so a host thread builds them while the device builds the trees; each
+    // record's mesh offsets and root go in once the device has numbered the nodes.
+    std::vector<uint32_t> head(HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS, 0u);
+    struct Joiner {
+        std::thread t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } head_job;
+    head_job.t = std::thread([&]() {
+        head[0] = instance_count;
+        for (uint32_t i = 0; i < instance_count; ++i) {
+            const rq_instance& ins = instances[i];
+            const float* inv = &inverses[16ull * i];
+            const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+            for (int row = 0; row < 3; ++row)
+                for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
+            instance_box(ins.model, hm[ins.mesh].lo, hm[ins.mesh].hi, &head[r + 12]);
+            head[r + 24] = ins.identity;
+            head[r + 27] = ins.flags & 1u;
+            head[r + 29] = ins.placement;
+        }
+        head[4] = instance_count;
+        top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
+    });
     int device = 0;
     GB_TRY(hipGetDevice(&device), "hipGetDevice");
     GpuBuild b;
@@ -1254,10 +1346,11 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     GB_OK(b.meshes.need(std::max<size_t>(mesh_count, 1) * sizeof(MeshDev), "hipMalloc build"));
     GB_OK(b.pos.need(std::max<uint64_t>(verts, 1) * 12, "hipMalloc build"));
     GB_OK(b.idx.need(std::max<uint64_t>(tris, 1) * 12, "hipMalloc build"));
+    staging.queued = true;
     if (verts)
-        GB_TRY(hipMemcpyAsync(b.pos.p, stage_pos.data(), 12ull * verts, hipMemcpyHostToDevice, b.s), "upload positions");
+        GB_TRY(hipMemcpyAsync(b.pos.p, stage_pos, 12ull * verts, hipMemcpyHostToDevice, b.s), "upload positions");
     if (tris)
-        GB_TRY(hipMemcpyAsync(b.idx.p, stage_idx.data(), 12ull * tris, hipMemcpyHostToDevice, b.s), "upload indices");
+        GB_TRY(hipMemcpyAsync(b.idx.p, stage_idx, 12ull * tris, hipMemcpyHostToDevice, b.s), "upload indices");
     GB_OK(b.allocate());
     std::vector<uint32_t> summary;
     if (mesh_count) {
@@ -1353,27 +1446,14 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
         }
         if (hipGetLastError() != hipSuccess) return fail_scene(hip_fail(hipGetLastError(), "build words launch"));
     }
-    // src/gi_bvh.rs:230-274 (instance_words) + src/gi_cache.rs:4388-4479, as rq_scene_build.
-    std::vector<uint32_t> head(HEADER_WORDS + (size_t)instance_count * INSTANCE_WORDS, 0u);
-    head[0] = instance_count;
+    head_job.t.join();
     for (uint32_t i = 0; i < instance_count; ++i) {
-        const rq_instance& ins = instances[i];
-        const uint32_t m = ins.mesh;
-        const float* inv = &inverses[16ull * i];
+        const uint32_t m = instances[i].mesh;
         const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
-        for (int row = 0; row < 3; ++row)
-            for (int c = 0; c < 4; ++c) head[r + row * 4 + c] = f_bits(inv[c * 4 + row]);
-        instance_box(ins.model, hm[m].lo, hm[m].hi, &head[r + 12]);
         for (int k = 0; k < 4; ++k) head[r + 18 + k] = offsets[4 * m + k];
         head[r + 22] = sm[4 * m + 2];
-        head[r + 23] = 0;
-        head[r + 24] = ins.identity;
-        head[r + 27] = ins.flags & 1u;
         head[r + 28] = md[m].keep ? md[m].prims : 0u;
-        head[r + 29] = ins.placement;
     }
-    head[4] = instance_count;
-    top_place(head, HEADER_WORDS, INSTANCE_WORDS, instance_count, nullptr, 0, top_level);
     if (head.size() != head_words) return fail_scene(RQ_ERR_DEVICE);  // top_place appends exactly the reserve
     std::copy(head.begin(), head.end(), words.begin());
     if (hipMemcpyAsync(w, words.data(), head.size() * 4, hipMemcpyHostToDevice, b.s) != hipSuccess ||
