@@ -36,6 +36,7 @@
 #include <chrono>
 #include <cmath>
 #include <mutex>
+#include <new>
 #include <thread>
 
 #include "bvh_device.h"
@@ -1171,6 +1172,7 @@ struct Staging {
     size_t bytes = 0;
     hipStream_t s = nullptr;
     bool queued = false;  // an upload from this buffer may still be in flight
+    std::vector<uint32_t> pageable;  // used instead when pinned memory is refused
     rq_status take(size_t need) {
         PinnedPool& pool = pinned_pool();
         {
@@ -1180,7 +1182,7 @@ struct Staging {
                 if (pool.free[i].second >= need && (best == pool.free.size() || pool.free[i].second < pool.free[best].second))
                     best = i;
             if (best < pool.free.size()) {
-                p = pool.free[best].first;
+                p = stage = pool.free[best].first;
                 bytes = pool.free[best].second;
                 pool.free.erase(pool.free.begin() + best);
                 return RQ_OK;
@@ -1188,16 +1190,26 @@ struct Staging {
         }
         const size_t grain = need <= (1u << 20) ? (1u << 20) : (16u << 20);
         bytes = (need + grain - 1) & ~(grain - 1);
-        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
-        if (e != hipSuccess) {
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+            // pinned memory exhausted: stage in ordinary memory (slower uploads)
+            (void)hipGetLastError();
             p = nullptr;
-            return hip_fail(e, "hipHostMalloc build staging");
+            try {
+                pageable.resize((need + 3) / 4);
+            } catch (const std::bad_alloc&) {
+                set_error("rq_scene_build_gpu: no host memory for the upload staging");
+                return RQ_ERR_OUT_OF_MEMORY;
+            }
+            stage = pageable.data();
+            return RQ_OK;
         }
+        stage = p;
         return RQ_OK;
     }
+    void* stage = nullptr;  // where the meshes are gathered: the pinned buffer or `pageable`
     ~Staging() {
-        if (!p) return;
         if (queued) (void)hipStreamSynchronize(s);
+        if (!p) return;
         PinnedPool& pool = pinned_pool();
         std::lock_guard<std::mutex> lock(pool.mu);
         // keep this buffer; drop pooled ones it makes redundant (smaller ones)
@@ -1242,7 +1254,7 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
     Staging staging;
     staging.s = reinterpret_cast<hipStream_t>(stream);
     if (vbase[mesh_count] + tbase[mesh_count]) GB_OK(staging.take(12ull * (vbase[mesh_count] + tbase[mesh_count])));
-    float* const stage_pos = static_cast<float*>(staging.p);
+    float* const stage_pos = static_cast<float*>(staging.stage);
     uint32_t* const stage_idx = stage_pos ? reinterpret_cast<uint32_t*>(stage_pos + 3 * vbase[mesh_count]) : nullptr;
     std::vector<HostMesh> hm(mesh_count);
     {
