@@ -324,7 +324,7 @@ def run_reflection(rq, kind, meshes, tris, frame, reps=3):
     primary and GI rays as closest-hit queries (rs_closest), the shadow rays as any-hit (rs_blocked)."""
     t0 = time.perf_counter()
     sc = rq.Scene.generate(kind, 1, meshes, tris, reflection=True)
-    sc.upload(("rs",))
+    sc.upload(("rs", "rs_cw8"))
     build_s = time.perf_counter() - t0
     c = frame.counts_host
     n_gi = int(c[1]) if frame.n_gi else 0
@@ -334,19 +334,30 @@ def run_reflection(rq, kind, meshes, tris, frame, reps=3):
     if n_gi:
         ranges.insert(1, (frame.gi, n_gi, rq.CLOSEST, frame.gi_hits, None))
     rays = sum(r[1] for r in ranges)
-    start, end = rq.Event(), rq.Event()
-    sc.trace_ranges("rs", ranges)
+    out = {"rays": int(rays), "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words))}
+    for layout in ("rs", "rs_cw8"):
+        start, end = rq.Event(), rq.Event()
+        sc.trace_ranges(layout, ranges)
+        rq.synchronize()
+        start.record()
+        for _ in range(reps):
+            sc.trace_ranges(layout, ranges)
+        end.record()
+        end.synchronize()
+        ms = start.elapsed_ms(end) / reps
+        nbytes = sum(sc.count_stats(layout, r[0], r[1], r[2])["bytes"] for r in ranges)
+        out[layout] = {"ms": round(ms, 4), "mrays_s": round(rays / ms / 1e3, 1),
+                       "bytes_per_ray": round(nbytes / max(rays, 1), 1),
+                       "achieved_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    # the RS_CW8 records equal the RS ones (rs_trace's closest hit does not depend on the visit order)
+    check = rq.DeviceBuffer(max(frame.n, 1) * 48)
+    sc.trace("rs", frame.prim, frame.n, rq.CLOSEST, hits=hits)
+    sc.trace("rs_cw8", frame.prim, frame.n, rq.CLOSEST, hits=check)
     rq.synchronize()
-    start.record()
-    for _ in range(reps):
-        sc.trace_ranges("rs", ranges)
-    end.record()
-    end.synchronize()
-    ms = start.elapsed_ms(end) / reps
-    nbytes = sum(sc.count_stats("rs", r[0], r[1], r[2])["bytes"] for r in ranges)
-    return {"rays": int(rays), "ms": round(ms, 4), "mrays_s": round(rays / ms / 1e3, 1),
-            "bytes_per_ray": round(nbytes / max(rays, 1), 1), "achieved_gb_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
-            "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words))}
+    out["rs_cw8_records_equal"] = bool(np.array_equal(hits.download(np.uint32, 12 * frame.n),
+                                                      check.download(np.uint32, 12 * frame.n)))
+    out["mrays_s"] = out["rs_cw8"]["mrays_s"]
+    return out
 
 
 def run_refit(rq, n=724, instances=4, reps=20):

@@ -42,7 +42,9 @@ typedef enum rq_layout {
     RQ_LAYOUT_REF_BVH4Q8 = 0, /* the reference gi_geometry words, verbatim */
     RQ_LAYOUT_PLAIN_F32 = 1,  /* BVH4, child boxes pre-decoded to f32, triangles pre-gathered */
     RQ_LAYOUT_CW8_Q8 = 2,     /* 8-wide nodes, 8-bit outward-quantized child boxes, f32 triangles */
-    RQ_LAYOUT_RS_BVH4Q8 = 3   /* the reflection rays' rl_geometry words, verbatim (rs_trace) */
+    RQ_LAYOUT_RS_BVH4Q8 = 3,  /* the reflection rays' rl_geometry words, verbatim (rs_trace) */
+    RQ_LAYOUT_RS_CW8_Q8 = 4   /* rl_geometry's records and triangles over CW8 nodes: rs_trace's
+                                 results (its closest hit does not depend on the visit order) */
 } rq_layout;
 
 /* gi_traverse modes (shaders/gi_rays_software.wgsl:214-216). */
@@ -297,14 +299,17 @@ rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
 rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes);
 /* Copies one part of an uploaded layout back to the host (no reference
    counterpart: a diagnostic for checking device images). PLAIN / CW8 parts as
-   rq_scene_layout_image numbers them (0-4, 256-byte padded); REF / RS part 0
-   is the whole word buffer. *bytes = the part's size; nothing is copied when
+   rq_scene_layout_image numbers them (0-4, 256-byte padded); RS_CW8 parts 0-2
+   likewise; REF / RS part 0 is the whole word buffer. *bytes = the part's size; nothing is copied when
    `capacity` is smaller. */
 rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
                                  size_t capacity, size_t* bytes);
 /* Host image of a relayout (PLAIN or CW8), built on demand without a device:
    part 0 instance records, 1 top-level nodes, 2 BLAS nodes, 3 triangles,
-   4 position indices, 5 {top_on, top_root, instances, 0}. For inspection. */
+   4 position indices, 5 {top_on, top_root, instances, 0}. For inspection.
+   RS_CW8 (a reflection scene): part 0 the words with each record's root
+   replaced by its CW8 root, 1 the CW8 top level, 2 the CW8 BLAS nodes, 5 as
+   above; 3 and 4 are empty. */
 rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
                                 size_t* bytes);
 void rq_scene_destroy(rq_scene* scene);

@@ -162,15 +162,22 @@ extern "C" rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]) {
 
 extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
                                            size_t* bytes) {
-    if (!scene || !data || !bytes || (int)layout < 1 || (int)layout > 2 || part > 5 || scene->kind != RQ_SCENE_GI) {
+    const bool rs = layout == RQ_LAYOUT_RS_CW8_Q8;
+    if (!scene || !data || !bytes || !((int)layout == 1 || (int)layout == 2 || rs) || part > 5 ||
+        scene->kind != (rs ? RQ_SCENE_REFLECTION : RQ_SCENE_GI)) {
         set_error("rq_scene_layout_image: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
-    HostLayout& h = layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
+    HostLayout& h = rs ? scene->rs_cw8 : layout == RQ_LAYOUT_PLAIN_F32 ? scene->plain : scene->cw8;
     std::string err;
     rq_status st = sync_host_words(scene);
     if (st != RQ_OK) return st;
-    if (h.inst.empty() && scene->words[0] > 0) {
+    if (rs) {
+        if (h.inst.empty() && !build_rs_cw8(scene->words, h, err)) {
+            set_error("rs cw8 relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    } else if (h.inst.empty() && scene->words[0] > 0) {
         if (scene->plain.inst.empty() && !build_plain(scene->words, scene->plain, err)) {
             set_error("plain relayout: " + err);
             return RQ_ERR_INVALID_INPUT;

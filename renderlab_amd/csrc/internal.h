@@ -29,6 +29,7 @@ constexpr uint32_t REFITS = 64;             // src/instance_bvh.rs:94
 constexpr uint32_t TRI_FLOATS = 9;
 constexpr uint32_t TRI_ID_WORDS = 6;
 // Reflection geometry (src/reflection_software.rs:26-44).
+constexpr int LAYOUT_COUNT = 5;  // rq_layout values
 constexpr uint32_t RS_HEADER_WORDS = 8;
 constexpr uint32_t RS_RECORD_WORDS = 24;
 constexpr uint32_t RS_NODE_WORDS = 16;
@@ -193,6 +194,10 @@ struct HostLayout {
 bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::string& err);
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out,
                std::string& err);
+// RQ_LAYOUT_RS_CW8_Q8 from rl_geometry words: `inst` holds the words with each
+// record's root (word 20) replaced by its mesh's CW8 root, `nodes` the CW8 BLAS
+// nodes, `top` the CW8 top level.
+bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::string& err);
 
 void set_error(const std::string& msg);
 
@@ -207,8 +212,8 @@ struct rq_scene {
     std::vector<uint32_t> words;  // the reference gi_geometry (or rl_geometry) words
     float bounds[6] = {0, 0, 0, 1, 1, 1};  // union of the instance world boxes (ray reordering keys)
     uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    rq::HostLayout plain, cw8;
-    rq::DeviceLayout dev[4];  // by rq_layout
+    rq::HostLayout plain, cw8, rs_cw8;
+    rq::DeviceLayout dev[rq::LAYOUT_COUNT];  // by rq_layout
     std::vector<rq::RefitPlan> refit_plans;
     // Kept by rq_scene_build for refits: each instance's model (16 floats,
     // column-major), each mesh's node word offset, the top-level switch.

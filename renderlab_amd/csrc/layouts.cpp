@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <tuple>
+#include <unordered_map>
 
 #include "internal.h"
 
@@ -488,6 +489,96 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
     if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);
     (void)words;
+    return true;
+}
+
+bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::string& err) {
+    out = HostLayout();
+    const uint32_t count = words.empty() ? 0u : words[0];
+    out.inst = words;
+    out.instance_count = count;
+    // Each mesh tree (node base, root) as 32-word f32 nodes -- the decode rs_trace makes
+    // (origin + step * code, unfused) -- then collapsed into CW8 nodes. RS leaf tags
+    // (LEAF | first triangle << 4 | count) pass through unchanged.
+    std::vector<uint32_t> plain;
+    W8Builder b{&plain, &out.nodes, &err};
+    std::unordered_map<uint64_t, uint32_t> done;  // node base << 32 | root -> CW8 root tag
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
+        const uint32_t base = words[r + 18], root = words[r + 20];
+        const uint64_t key = (uint64_t)base << 32 | root;
+        auto it = done.find(key);
+        uint32_t cw8_root = EMPTY;
+        if (it != done.end()) {
+            cw8_root = it->second;
+        } else {
+            uint32_t plain_root = root;
+            if (!(root & LEAF_BIT)) {
+                std::unordered_map<uint32_t, uint32_t> id;  // source node id -> f32 node index
+                std::vector<uint32_t> todo{root};
+                id[root] = (uint32_t)(plain.size() / TOP_WORDS);
+                plain.resize(plain.size() + TOP_WORDS, 0u);
+                while (!todo.empty()) {
+                    const uint32_t n = todo.back();
+                    todo.pop_back();
+                    const size_t src = base + (size_t)n * RS_NODE_WORDS;
+                    if (src + RS_NODE_WORDS > words.size()) {
+                        err = "a node past the end of the words";
+                        return false;
+                    }
+                    const uint32_t* w = &words[src];
+                    const size_t o = (size_t)id[n] * TOP_WORDS;
+                    for (int sl = 0; sl < 4; ++sl) {
+                        const uint32_t c = w[6 + sl];
+                        if (c == EMPTY) {
+                            plain[o + 24 + sl] = EMPTY;
+                            continue;
+                        }
+                        float lo[3], hi[3];
+                        decode_unfused(w, sl, lo, hi);
+                        for (int a = 0; a < 3; ++a) {
+                            plain[o + a * 4 + sl] = f_bits(lo[a]);
+                            plain[o + 12 + a * 4 + sl] = f_bits(hi[a]);
+                        }
+                        if (c & LEAF_BIT) {
+                            plain[o + 24 + sl] = c;
+                            continue;
+                        }
+                        auto [at, fresh] = id.emplace(c, (uint32_t)(plain.size() / TOP_WORDS));
+                        if (fresh) {
+                            plain.resize(plain.size() + TOP_WORDS, 0u);
+                            todo.push_back(c);
+                        }
+                        plain[o + 24 + sl] = at->second;
+                    }
+                }
+                plain_root = id[root];
+            }
+            if (!b.root(plain_root, cw8_root)) return false;
+            if (w8_levels(out.nodes, cw8_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+                err = "RS CW8 relayout: a mesh tree deeper than 16 levels";
+                return false;
+            }
+            done.emplace(key, cw8_root);
+        }
+        out.inst[r + 20] = cw8_root;
+    }
+    // Top level: the reference's 4-wide f32 nodes, collapsed the same way.
+    out.top_on = count && words[1] != 0;
+    out.top_root = count ? words[2] : 0u;
+    if (out.top_on) {
+        const size_t at = words[1];
+        const size_t n = std::min<size_t>(top_reserve(count), words.size() - std::min(words.size(), at));
+        std::vector<uint32_t> top(words.begin() + at, words.begin() + at + n);
+        W8Builder t{&top, &out.top, &err};
+        if (!t.root(words[2], out.top_root)) return false;
+        if (w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+            err = "RS CW8 relayout: a top level deeper than 16 levels";
+            return false;
+        }
+    }
+    if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
+    if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);
     return true;
 }
 

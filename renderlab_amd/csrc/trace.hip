@@ -43,7 +43,7 @@ constexpr uint32_t CW8_LEVELS = 16;  // the CW8 relayout rejects deeper trees
 constexpr uint32_t CW8_CAP = 7 * CW8_LEVELS;
 constexpr uint32_t LEAF_MASK = (1u << 27) - 1u;
 
-enum : int { REF = 0, PLAIN = 1, CW8 = 2, RS = 3 };
+enum : int { REF = 0, PLAIN = 1, CW8 = 2, RS = 3, RSW = 4 };  // RSW: rl_geometry over CW8 nodes
 
 __device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 __device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
@@ -228,13 +228,16 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
+    // RSF: reflection words (records, triangles, rs_trace's rules); W8: 8-wide nodes.
+    constexpr bool RSF = LAYOUT == RS || LAYOUT == RSW;
+    constexpr bool W8 = LAYOUT == CW8 || LAYOUT == RSW;
     __shared__ uint32_t lds[S * BLOCK];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     uint32_t* st = lds + tid;
     uint32_t* spill = a.spill + ((uint64_t)blockIdx.x * BLOCK + tid) * SPILL_WORDS;
-    constexpr uint32_t TOP_CAP = LAYOUT == CW8 ? CW8_CAP : STACK_CAP;
-    constexpr uint32_t BLAS_CAP = LAYOUT == CW8 ? CW8_CAP : STACK_CAP;
+    constexpr uint32_t TOP_CAP = W8 ? CW8_CAP : STACK_CAP;
+    constexpr uint32_t BLAS_CAP = W8 ? CW8_CAP : STACK_CAP;
 
     // Lane state.
     uint32_t ray = NONE;
@@ -292,13 +295,13 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     // gi_limit (:64-70); rs_limit keeps equal-distance candidates of the same
     // instance eligible too (lower primitive wins), hence i <= best there.
     auto limit = [&](uint32_t i) -> float {
-        return (bhit && (LAYOUT == RS ? i <= binst : i < binst)) ? bf(fb(bt) + 1u) : bt;
+        return (bhit && (RSF ? i <= binst : i < binst)) ? bf(fb(bt) + 1u) : bt;
     };
     // ---- hit writeback (the record of a finished ray)
     auto writeback = [&]() {
         uint32_t tri0 = 0, tri1 = 0, tri2 = 0, slot = 0;
         V3 n = mk(0.0f, 0.0f, 0.0f);
-        if (LAYOUT == RS && bhit) {
+        if (RSF && bhit) {
             // rs_trace's record (slot + 1, primitive, u, v): the slot, the primitive
             tri0 = btri;
             slot = v.words[RS_HEADER_WORDS + (uint64_t)binst * RS_RECORD_WORDS + 21u];
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     flat_next = 0;
                     tri_left = 0;
                     nxt = (!invalid && v.top_on) ? v.top_root : NONE;
-                    if (LAYOUT == RS && !invalid) {
+                    if (RSF && !invalid) {
                         // rs_trace's scene-bounds pre-cull (:175-187): a ray that misses
                         // the padded union of the instances is a miss; otherwise the best
                         // t starts 2 ulps past the exit so boundary triangles stay eligible.
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     tri_next = v.words[l] / 3u;
                     tri_left = v.words[l + 1] / 3u;
                     if (COUNT) nb += 8;
-                } else if (LAYOUT == RS) {
+                } else if (RSF) {
                     tri_next = (nxt & ~LEAF_BIT) >> 4;  // LEAF | first triangle << 4 | count
                     tri_left = nxt & 15u;
                 } else {
@@ -522,7 +525,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         const bool dual = a.drain != 0u && exhausted;
         bool spec = false;
         uint32_t spec_item = NONE;
-        if (LAYOUT == CW8 && a.speculate && !dual && pick == 3 && kind == 1 && in_blas) {
+        if (W8 && a.speculate && !dual && pick == 3 && kind == 1 && in_blas) {
             if (nxt != NONE) {
                 if (!(nxt & LEAF_BIT)) {
                     spec = true;
@@ -554,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 #pragma unroll
             for (uint32_t j = 0; j < 2; ++j) {
                 if (j >= k || fin) break;
-                if (LAYOUT == RS) {
+                if (RSF) {
                     // rs_trace leaf (:118-141): words 0-2 corner a, 3-5 e1, 6-8 e2, 9 primitive
                     const uint4* tp = reinterpret_cast<const uint4*>(v.words + m_idx + (uint64_t)RS_TRI_WORDS * (tri_next + j));
                     const uint4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
@@ -607,7 +610,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
-                if (LAYOUT == RS) {
+                if (RSF) {
                     // rs_instance (:79-99): record words 0-11 rows, 12-17 world box,
                     // 18 node base, 19 triangle base, 20 root, 21 slot, 22 casts no shadow
                     const uint4* r4 = reinterpret_cast<const uint4*>(v.words + RS_HEADER_WORDS + (uint64_t)i * RS_RECORD_WORDS);
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     oinv = LAYOUT == CW8 ? fast_inverse(od) : safe_inverse(od);
                     cur = i;
                     uint32_t root;
-                    if (LAYOUT == RS) {
+                    if (RSF) {
                         root = rw[18];
                     } else if (LAYOUT == REF) {
                         const uint32_t* r = v.words + 16ull + (uint64_t)i * 32u;
@@ -1241,17 +1244,18 @@ struct Ctx {
     int device = -1;
     int cus = 0;
     std::vector<StreamRes> res;
-    rq_tuning tuning[4][2];
-    rq_tuning batch_tuning[4];
+    rq_tuning tuning[LAYOUT_COUNT][2];
+    rq_tuning batch_tuning[LAYOUT_COUNT];
     bool tuned = false;
 };
 Ctx g_ctx;
 
-// A layout traces a scene of its kind: REF, PLAIN and CW8 the GI words, RS the
-// reflection words.
+// A layout traces a scene of its kind: REF, PLAIN and CW8 the GI words, RS and
+// RS_CW8 the reflection words.
 bool layout_ok(const rq_scene* s, int layout) {
-    if (!s || layout < 0 || layout > 3) return false;
-    return s->kind == RQ_SCENE_REFLECTION ? layout == RQ_LAYOUT_RS_BVH4Q8 : layout != RQ_LAYOUT_RS_BVH4Q8;
+    if (!s || layout < 0 || layout >= LAYOUT_COUNT) return false;
+    const bool rs = layout == RQ_LAYOUT_RS_BVH4Q8 || layout == RQ_LAYOUT_RS_CW8_Q8;
+    return s->kind == RQ_SCENE_REFLECTION ? rs : !rs;
 }
 
 rq_status hip_fail(hipError_t e, const char* what) {
@@ -1272,13 +1276,13 @@ void default_tuning(rq_tuning& t, int layout, int any) {
     t.waves_per_eu = 0;
     t.grid_per_cu = 0;
     t.refill = 32u;  // refill threshold (lanes): the autotuner's pick on every measured workload
-    t.speculate = layout == RQ_LAYOUT_CW8_Q8 ? 1u : 0u;  // 2-3% faster on the measured frames
+    t.speculate = layout == RQ_LAYOUT_CW8_Q8 || layout == RQ_LAYOUT_RS_CW8_Q8 ? 1u : 0u;  // 2-3% on the measured frames
     t.drain = 1u;                                        // 0.4-1% faster on the measured frames
 }
 
 void init_tuning() {
     if (g_ctx.tuned) return;
-    for (int l = 0; l < 4; ++l) {
+    for (int l = 0; l < LAYOUT_COUNT; ++l) {
         for (int m = 0; m < 2; ++m) default_tuning(g_ctx.tuning[l][m], l, m);
         default_tuning(g_ctx.batch_tuning[l], l, 0);
     }
@@ -1310,12 +1314,12 @@ void launch_one(dim3 g, dim3 b, hipStream_t s, const dev::TraceArgs& a) {
 
 struct KernelEntry {
     uint32_t block, stack;
-    Launch fn[4][2];    // by layout, any-hit
-    void* ptr[4][2];
-    Launch bfn[4];      // batched, mode per batch
-    void* bptr[4];
-    Launch rfn[4];      // ranges of one scene, mode per range
-    void* rptr[4];
+    Launch fn[LAYOUT_COUNT][2];  // by layout, any-hit
+    void* ptr[LAYOUT_COUNT][2];
+    Launch bfn[LAYOUT_COUNT];    // batched, mode per batch
+    void* bptr[LAYOUT_COUNT];
+    Launch rfn[LAYOUT_COUNT];    // ranges of one scene, mode per range
+    void* rptr[LAYOUT_COUNT];
 };
 
 template <int L, int B, int S>
@@ -1339,6 +1343,7 @@ KernelEntry make_entry() {
     fill_layout<dev::PLAIN, B, S>(e);
     fill_layout<dev::CW8, B, S>(e);
     fill_layout<dev::RS, B, S>(e);
+    fill_layout<dev::RSW, B, S>(e);
     return e;
 }
 
@@ -1366,6 +1371,7 @@ const KernelEntry& count_kernel() {
         put(std::integral_constant<int, dev::PLAIN>());
         put(std::integral_constant<int, dev::CW8>());
         put(std::integral_constant<int, dev::RS>());
+        put(std::integral_constant<int, dev::RSW>());
         return k;
     }();
     return e;
@@ -1486,13 +1492,14 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
         set_error("rq_scene_upload: null scene");
         return RQ_ERR_INVALID_INPUT;
     }
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < LAYOUT_COUNT; ++l)
         if ((layout_mask & (1u << l)) && !layout_ok(scene, l)) {
-            set_error(scene->kind == RQ_SCENE_REFLECTION ? "rq_scene_upload: a reflection scene has only the RS layout"
-                                                        : "rq_scene_upload: the RS layout needs a reflection scene");
+            set_error(scene->kind == RQ_SCENE_REFLECTION
+                          ? "rq_scene_upload: a reflection scene has only the RS layouts"
+                          : "rq_scene_upload: the RS layouts need a reflection scene");
             return RQ_ERR_INVALID_INPUT;
         }
-    if (layout_mask & ~15u) {
+    if (layout_mask & ~((1u << LAYOUT_COUNT) - 1u)) {
         set_error("rq_scene_upload: unknown layout bits");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1515,10 +1522,29 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             return RQ_ERR_INVALID_INPUT;
         }
     }
-    for (int l = 0; l < 4; ++l) {
+    if ((layout_mask & 16u) && !scene->dev[RQ_LAYOUT_RS_CW8_Q8].ready && scene->rs_cw8.inst.empty()) {
+        if (!build_rs_cw8(scene->words, scene->rs_cw8, err)) {
+            set_error("rs cw8 relayout: " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    for (int l = 0; l < LAYOUT_COUNT; ++l) {
         if (!(layout_mask & (1u << l)) || scene->dev[l].ready) continue;
         DeviceLayout& d = scene->dev[l];
-        if (l == 0 || l == 3) {  // the reference words verbatim
+        if (l == RQ_LAYOUT_RS_CW8_Q8) {  // reflection words, CW8 top level, CW8 BLAS nodes
+            HostLayout& h = scene->rs_cw8;
+            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes);
+            HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
+            uint64_t at = 0;
+            d.off[0] = at; if ((s = upload(d.base, at, h.inst)) != RQ_OK) return s;
+            d.off[1] = at; if ((s = upload(d.base, at, h.top)) != RQ_OK) return s;
+            d.off[2] = at; if ((s = upload(d.base, at, h.nodes)) != RQ_OK) return s;
+            d.off[3] = d.off[4] = at;
+            d.count[0] = h.instance_count;
+            d.count[1] = h.top_on ? 1u : 0u;
+            d.count[2] = h.top_root;
+            d.count[3] = (uint32_t)h.top.size();
+        } else if (l == 0 || l == 3) {  // the reference words verbatim
             d.bytes = padded(scene->words);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
@@ -1552,7 +1578,7 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
 
 extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
                                             size_t capacity, size_t* bytes) {
-    if (!scene || !bytes || (int)layout < 0 || (int)layout > 3 || part > 4 || (capacity && !dst)) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || part > 4 || (capacity && !dst)) {
         set_error("rq_scene_download_part: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1565,6 +1591,13 @@ extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout lay
     if (layout == RQ_LAYOUT_PLAIN_F32 || layout == RQ_LAYOUT_CW8_Q8) {
         at = d.off[part];
         n = (part < 4 ? d.off[part + 1] : d.bytes) - at;
+    } else if (layout == RQ_LAYOUT_RS_CW8_Q8) {  // parts: words, top, nodes
+        if (part > 2) {
+            set_error("rq_scene_download_part: the RS CW8 layout has three parts");
+            return RQ_ERR_INVALID_INPUT;
+        }
+        at = d.off[part];
+        n = (part < 2 ? d.off[part + 1] : d.bytes) - at;
     } else if (part != 0) {
         set_error("rq_scene_download_part: the word layouts have one part");
         return RQ_ERR_INVALID_INPUT;
@@ -1576,7 +1609,7 @@ extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout lay
 }
 
 extern "C" rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes) {
-    if (!scene || !bytes || (int)layout < 0 || (int)layout > 3) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout >= LAYOUT_COUNT) {
         set_error("rq_scene_device_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1597,7 +1630,7 @@ extern "C" void rq_scene_destroy(rq_scene* scene) {
 }
 
 extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* out) {
-    if (!out || (int)layout < 0 || (int)layout > 3) {
+    if (!out || (int)layout < 0 || (int)layout >= LAYOUT_COUNT) {
         set_error("rq_get_tuning: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1608,7 +1641,7 @@ extern "C" rq_status rq_get_tuning(rq_layout layout, uint32_t mode, rq_tuning* o
 }
 
 extern "C" rq_status rq_set_tuning(rq_layout layout, uint32_t mode, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 3 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+    if (!t || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || !find_kernel(t->block, t->stack) || t->refill == 0 ||
         t->refill > 64) {
         set_error("rq_set_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;
@@ -1631,6 +1664,15 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.n_inst = scene->words[0];
         v.top_on = scene->words[1] != 0;
         v.top_root = scene->words[2];
+    } else if (layout == RQ_LAYOUT_RS_CW8_Q8) {
+        // the reflection words (records' roots rewritten), CW8 top level, CW8 BLAS nodes
+        char* b = reinterpret_cast<char*>(d.base);
+        v.words = reinterpret_cast<const uint32_t*>(b + d.off[0]);
+        v.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
+        v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
+        v.n_inst = d.count[0];
+        v.top_on = d.count[1] != 0;
+        v.top_root = d.count[2];
     } else {
         char* b = reinterpret_cast<char*>(d.base);
         v.inst = reinterpret_cast<const uint32_t*>(b + d.off[0]);
@@ -1828,7 +1870,7 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
 
 extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
                                       const rq_batch* batches, uint32_t batch_count, void* stream) {
-    if (!scenes || !batches || (int)layout < 0 || (int)layout > 3 || batch_count == 0) {
+    if (!scenes || !batches || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || batch_count == 0) {
         set_error("rq_trace_batches: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1925,7 +1967,7 @@ extern "C" rq_status rq_stream_reserve(void* stream, uint32_t max_rays) {
 }
 
 extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
-    if (!out || (int)layout < 0 || (int)layout > 3) {
+    if (!out || (int)layout < 0 || (int)layout >= LAYOUT_COUNT) {
         set_error("rq_get_batch_tuning: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1936,7 +1978,7 @@ extern "C" rq_status rq_get_batch_tuning(rq_layout layout, rq_tuning* out) {
 }
 
 extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
-    if (!t || (int)layout < 0 || (int)layout > 3 || !find_kernel(t->block, t->stack) || t->refill == 0 ||
+    if (!t || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || !find_kernel(t->block, t->stack) || t->refill == 0 ||
         t->refill > 64) {
         set_error("rq_set_batch_tuning: unsupported block/stack/refill combination");
         return RQ_ERR_INVALID_INPUT;

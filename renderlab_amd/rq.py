@@ -32,7 +32,8 @@ REF_BVH4Q8 = 0
 PLAIN_F32 = 1
 CW8_Q8 = 2
 RS_BVH4Q8 = 3
-LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8, "rs": RS_BVH4Q8}
+RS_CW8_Q8 = 4
+LAYOUTS = {"ref": REF_BVH4Q8, "plain": PLAIN_F32, "cw8": CW8_Q8, "rs": RS_BVH4Q8, "rs_cw8": RS_CW8_Q8}
 
 CLOSEST = 0
 ANY = 1

@@ -169,3 +169,22 @@ def test_cw8_deep_chain_against_the_flat_answer():
     _, gv = gpu_trace(scene, "cw8", rays, 1)
     assert (gv == 0).all()  # every ray is blocked
 
+
+@pytest.mark.parametrize("count", [0, 1, 63, 64, 65, 1000])
+def test_small_launches_match_oracle(scene_c0, workload_c0, count):
+    """Launches smaller than a wave or than the grid's first handout (each wave starts on its own
+    64 rays without the cursor): REF and PLAIN records bit for bit, CW8 within its tie rule."""
+    _, _, _, _, gis = workload_c0
+    rays = gis[0][:count]
+    o, ov, _ = oracle.traverse(scene_c0.words, rays, 0, want_vis=False) if count else (None, None, None)
+    for layout in ("ref", "plain", "cw8"):
+        print(f"closest {layout} n={count}", flush=True)
+        g, _ = gpu_trace(scene_c0, layout, rays, rq.CLOSEST)
+        if not count:
+            continue
+        if layout == "cw8":
+            bad, _ = classify_mismatch(g, o)
+        else:
+            bad = compare_exact(g, o)
+        assert bad.size == 0, f"{layout}: {bad.size} records differ, first {bad[:5]}"
+

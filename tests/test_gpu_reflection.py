@@ -1,7 +1,9 @@
-"""The reflection rays on the GPU (layout RS over rl_geometry words): closest hits equal the
-oracle's rs_trace records bit for bit (record index, primitive, u, v, t, slot), any-hit
-visibility bits equal, also through the multi-range launch and the reordered launch; a scene
-kind and a layout that do not belong together are refused."""
+"""The reflection rays on the GPU (layout RS over rl_geometry words, and RS_CW8: the same records
+and triangles over CW8 nodes): closest hits equal the oracle's rs_trace records bit for bit
+(record index, primitive, u, v, t, slot) -- rs_trace's closest hit does not depend on the visit
+order, so the 8-wide tree gives the same records -- any-hit visibility bits equal, also through
+the multi-range launch and the reordered launch; a scene kind and a layout that do not belong
+together are refused."""
 import numpy as np
 import pytest
 
@@ -16,9 +18,9 @@ pytestmark = pytest.mark.gpu
 def scenes():
     print("workload: reflection scenes (box scene, lognormal 60K)", flush=True)
     box = rq.Scene.generate(0, 1, reflection=True)
-    box.upload(("rs",))
+    box.upload(("rs", "rs_cw8"))
     logn = rq.Scene.generate(1, 5, 80, 60000, reflection=True)
-    logn.upload(("rs",))
+    logn.upload(("rs", "rs_cw8"))
     gi = rq.Scene.generate(1, 5, 80, 60000)
     gi.upload(("plain",))
     return box, logn, gi
@@ -39,42 +41,45 @@ def rays(scenes):
     return np.concatenate([p, gis[0], rnd]), s
 
 
+@pytest.mark.parametrize("layout", ["rs", "rs_cw8"])
 @pytest.mark.parametrize("which", ["box", "lognormal"])
-def test_rs_closest_bit_exact(scenes, rays, which):
+def test_rs_closest_bit_exact(scenes, rays, which, layout):
     sc = scenes[0] if which == "box" else scenes[1]
     r, _ = rays
-    print(f"case: rs closest {which} n={len(r)}", flush=True)
-    h, _ = gpu_trace(sc, "rs", r, rq.CLOSEST)
+    print(f"case: {layout} closest {which} n={len(r)}", flush=True)
+    h, _ = gpu_trace(sc, layout, r, rq.CLOSEST)
     o, _, _ = oracle.rs_traverse(sc.words, r, 0)
     assert ((o["flags"] & 1) == 1).sum() > 1000
     assert len(compare_exact(h, o)) == 0
 
 
+@pytest.mark.parametrize("layout", ["rs", "rs_cw8"])
 @pytest.mark.parametrize("which", ["box", "lognormal"])
-def test_rs_visibility_bits(scenes, rays, which):
+def test_rs_visibility_bits(scenes, rays, which, layout):
     sc = scenes[0] if which == "box" else scenes[1]
     _, s = rays
-    print(f"case: rs any-hit {which} n={len(s)}", flush=True)
-    _, v = gpu_trace(sc, "rs", s, rq.ANY)
+    print(f"case: {layout} any-hit {which} n={len(s)}", flush=True)
+    _, v = gpu_trace(sc, layout, s, rq.ANY)
     _, ov, _ = oracle.rs_traverse(sc.words, s, 1, want_hits=False, want_vis=True)
     assert np.array_equal(v, ov)
 
 
-def test_rs_ranges_and_reorder(scenes, rays):
+@pytest.mark.parametrize("layout", ["rs", "rs_cw8"])
+def test_rs_ranges_and_reorder(scenes, rays, layout):
     sc = scenes[1]
     r, s = rays
-    print("case: rs ranges + reorder", flush=True)
+    print(f"case: {layout} ranges + reorder", flush=True)
     dr, ds = to_device(r), to_device(s)
     dh = rq.DeviceBuffer(len(r) * 48)
     dv = rq.DeviceBuffer(((len(s) + 31) // 32) * 4)
-    sc.trace_ranges("rs", [(dr, len(r), rq.CLOSEST, dh, None), (ds, len(s), rq.ANY, None, dv)])
+    sc.trace_ranges(layout, [(dr, len(r), rq.CLOSEST, dh, None), (ds, len(s), rq.ANY, None, dv)])
     rq.synchronize()
     o, _, _ = oracle.rs_traverse(sc.words, r, 0)
     _, ov, _ = oracle.rs_traverse(sc.words, s, 1, want_hits=False, want_vis=True)
     assert len(compare_exact(dh.download(rq.HIT_DTYPE, len(r)), o)) == 0
     assert np.array_equal(dv.download(np.uint32, (len(s) + 31) // 32), ov)
     dh2 = rq.DeviceBuffer(len(r) * 48)
-    sc.trace("rs", dr, len(r), rq.CLOSEST | rq.REORDER, hits=dh2)
+    sc.trace(layout, dr, len(r), rq.CLOSEST | rq.REORDER, hits=dh2)
     rq.synchronize()
     assert len(compare_exact(dh2.download(rq.HIT_DTYPE, len(r)), o)) == 0
 
@@ -85,8 +90,12 @@ def test_kind_and_layout_must_match(scenes):
         box.upload(("plain",))
     with pytest.raises(rq.InvalidInput):
         gi.upload(("rs",))
+    with pytest.raises(rq.InvalidInput):
+        gi.upload(("rs_cw8",))
     d = to_device(np.zeros(1, oracle.RAY_DTYPE))
     with pytest.raises(rq.InvalidInput):
         gi.trace("rs", d, 1, rq.CLOSEST)
     with pytest.raises(rq.InvalidInput):
         box.trace("cw8", d, 1, rq.CLOSEST)
+    with pytest.raises(rq.InvalidInput):
+        gi.trace("rs_cw8", d, 1, rq.CLOSEST)

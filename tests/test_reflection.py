@@ -247,3 +247,77 @@ def test_rl_words_must_hold_finite_trees_inside_the_buffer():
         "triangles past the end")
     top = int(w[1])
     bad(lambda b: b.__setitem__(top + 32 * int(w[2]) + 24, rq.LEAF_BIT | 45), "top-level leaf")
+
+
+def rs_node_children(words, base, n):
+    """Children of an rl_geometry BVH4 node: (tag, lo, hi) decoded as rs_trace does (o + s * q)."""
+    w = words[base + 16 * n:base + 16 * n + 16]
+    f = w.view(np.float32)
+    out = []
+    for s in range(4):
+        c = int(w[6 + s])
+        if c == rq.EMPTY:
+            continue
+        lo = np.array([f[a] + f[3 + a] * np.float32((int(w[10 + a]) >> (8 * s)) & 0xFF) for a in range(3)], np.float32)
+        hi = np.array([f[a] + f[3 + a] * np.float32((int(w[13 + a]) >> (8 * s)) & 0xFF) for a in range(3)], np.float32)
+        out.append((c, lo, hi))
+    return out
+
+
+def test_rs_cw8_relayout_encloses_every_leaf_once():
+    """The RS_CW8 image: each record's root becomes a CW8 tree reaching exactly the reflection
+    tree's leaves (their tags unchanged), every CW8 box enclosing the rs_trace box of each leaf
+    below it; the CW8 top level reaches every instance once; the rest of the words is unchanged."""
+    from test_layouts import w8_children
+    meshes, inst, _ = random_reflection_scene(3, instances=60)
+    sc = rq.Scene.build(meshes, inst, reflection=True)
+    w = sc.words
+    img = sc.layout_image("rs_cw8", 0)
+    nodes = sc.layout_image("rs_cw8", 2)
+    top = sc.layout_image("rs_cw8", 1)
+    meta = sc.layout_image("rs_cw8", 5)
+    count = int(w[0])
+    keep = np.ones(len(w), bool)
+    keep[RS_HEADER + RS_REC * np.arange(count) + 20] = False
+    assert len(img) == len(w) and np.array_equal(img[keep], w[keep])
+
+    def rs_leaves(base, tag):
+        if tag & rq.LEAF_BIT:
+            return {}
+        out = {}
+        for c, lo, hi in rs_node_children(w, base, tag):
+            if c & rq.LEAF_BIT:
+                out[c] = (lo, hi)
+            else:
+                out.update(rs_leaves(base, c))
+        return out
+
+    def cw8_leaves(tag):
+        """(leaf tag, [CW8 boxes on the path to it]) under `tag`."""
+        got = []
+        for c, lo, hi in w8_children(nodes, tag):
+            if c & rq.LEAF_BIT:
+                got.append((c, [(lo, hi)]))
+            else:
+                got += [(t, [(lo, hi)] + path) for t, path in cw8_leaves(c)]
+        return got
+
+    for i in range(count):
+        r = RS_HEADER + RS_REC * i
+        want = rs_leaves(int(w[r + 18]), int(w[r + 20]))
+        got = cw8_leaves(int(img[r + 20]))
+        tags = [t for t, _ in got]
+        assert sorted(tags) == sorted(want) and len(set(tags)) == len(tags)
+        for t, path in got:  # every CW8 box on the way down encloses the leaf's rs_trace box
+            for lo, hi in path:
+                assert (lo <= want[t][0]).all() and (hi >= want[t][1]).all()
+
+    assert meta[0] == 1  # 60 instances: a top level
+    def top_leaves(tag):
+        if tag & rq.LEAF_BIT:
+            return [tag & ~rq.LEAF_BIT]
+        out = []
+        for c, _, _ in w8_children(top, tag):
+            out += top_leaves(c)
+        return out
+    assert sorted(top_leaves(int(meta[1]))) == list(range(count))
