@@ -264,7 +264,8 @@ std::vector<Cand> wide_children(const std::vector<uint32_t>& nodes, uint32_t nod
 // Collapses a 4-wide tree into 8-wide nodes of 24 words (96 B):
 //   0-2 origin (f32), 3 exponents x|y<<8|z<<16 and valid-slot mask <<24,
 //   4-15 byte planes (lo x, lo y, lo z, hi x, hi y, hi z; 8 codes, 2 words each),
-//   16-23 child tags (node index, leaf tag unchanged from the source, EMPTY).
+//   16-23 child tags (node index, leaf tag unchanged from the source); an empty
+//   slot holds an inverted box and a copy of the first child's tag.
 // A box decodes as origin + 2^e*q, which is a representable f32 for every
 // code (the origin is aligned and the step chosen for that).
 struct W8Builder {
@@ -403,8 +404,19 @@ struct W8Builder {
                 todo.push_back({ni, c.child});
             }
         }
+        // Empty slots: an inverted box (low codes 255, high codes 0) that fails the
+        // traversal's slab test but for rays whose origin is about 2^19 node extents away
+        // (the culling slack then exceeds the node), and a copy of the first child's tag -- if such a ray enters it,
+        // it visits that child twice, which costs time and changes no result. The
+        // traversal then needs no per-slot valid test; the valid mask stays in word 3.
+        uint32_t dup = EMPTY;
+        for (size_t s = 0; s < 8 && dup == EMPTY; ++s)
+            if (occupied[s]) dup = w[16 + s];
         for (size_t s = 0; s < 8; ++s)
-            if (!occupied[s]) w[16 + s] = EMPTY;
+            if (!occupied[s]) {
+                w[16 + s] = dup;
+                for (int a = 0; a < 3; ++a) w[4 + 2 * a + s / 4] |= 0xffu << (8 * (s % 4));
+            }
         w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16) |
                (valid << 24);
         std::copy(w, w + W8_WORDS, out->begin() + (size_t)index * W8_WORDS);

@@ -798,7 +798,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
                 uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4], q5 = w4[5];
                 if (COUNT) nb += 96;
-                const uint32_t valid = q0.w >> 24;
+                // An empty slot holds an inverted box (low codes 255, high codes 0), which fails
+                // the slab test unless the ray's origin is about 2^19 node extents away (the
+                // slack then exceeds the node), and a copy of a real child's tag: visiting it is extra work, never a
+                // wrong answer. So no slot needs the valid mask.
                 const V3 scl = mk(bf((q0.w & 0xffu) << 23) * ri.x, bf(((q0.w >> 8) & 0xffu) << 23) * ri.y,
                                   bf(((q0.w >> 16) & 0xffu) << 23) * ri.z);
                 const V3 base = mk((bf(q0.x) - ro.x) * ri.x, (bf(q0.y) - ro.y) * ri.y, (bf(q0.z) - ro.z) * ri.z);
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     const float tfz = __builtin_fmaf((float)((wfz >> sh) & 0xffu), scl.z, bfr.z);
                     const float near = fmx(fmx(fmx(tnx, tny), tnz), tmin);
                     const float far = fmn(fmn(fmn(tfx, tfy), tfz), lim);
-                    const bool h = near <= far && ((valid >> s) & 1u);
+                    const bool h = near <= far;
                     if (h) hit |= 1u << s;
                     if (order) {
                         const uint32_t key = h ? ((fb(near) & 0x7ffffff8u) | (uint32_t)s) : 0xffffffffu;
