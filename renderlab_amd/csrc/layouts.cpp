@@ -460,6 +460,30 @@ uint32_t w8_levels(const std::vector<uint32_t>& nodes, uint32_t tag, uint32_t li
 // at most this many levels (it does not check pushes against the bound).
 constexpr uint32_t W8_MAX_LEVELS = 16;
 
+// Levels of a 4-wide f32 tree (32-word nodes, children in words 24-27) under
+// `tag`, counted without recursion; stops once past `limit` (or at a node past
+// the array). An 8-wide node spans at most 7 levels of a tree whose nodes have
+// two or more children (each opened child adds a slot), so such a source deeper
+// than 7 * W8_MAX_LEVELS cannot collapse within the bound. Checked before the
+// collapse, which recurses once per 8-wide level: that recursion then stays
+// shallow whatever the input.
+uint32_t wide_levels(const std::vector<uint32_t>& nodes, uint32_t tag, uint32_t limit) {
+    if (tag & LEAF_BIT) return 0;
+    uint32_t deepest = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> todo{{tag, 1u}};
+    while (!todo.empty()) {
+        const auto [node, level] = todo.back();
+        todo.pop_back();
+        deepest = std::max(deepest, level);
+        if (deepest > limit || ((size_t)node + 1) * TOP_WORDS > nodes.size()) return limit + 1;
+        for (size_t s = 0; s < 4; ++s) {
+            const uint32_t c = nodes[(size_t)node * TOP_WORDS + 24 + s];
+            if (!(c & LEAF_BIT)) todo.push_back({c, level + 1});
+        }
+    }
+    return deepest;
+}
+
 }  // namespace
 
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
@@ -477,6 +501,10 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
         for (auto& d : done)
             if (d.first == pr) nr = d.second;
         if (nr == EMPTY) {
+            if (wide_levels(plain.nodes, pr, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS) {
+                err = "CW8 relayout: a mesh tree deeper than 16 levels";
+                return false;
+            }
             if (!b.root(pr, nr)) return false;
             if (w8_levels(out.nodes, nr, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
                 err = "CW8 relayout: a mesh tree deeper than 16 levels";
@@ -492,6 +520,10 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     out.top_root = plain.top_root;
     if (plain.top_on) {
         W8Builder t{&plain.top, &out.top, &err};
+        if (wide_levels(plain.top, plain.top_root, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS) {
+            err = "CW8 relayout: a top level deeper than 16 levels";
+            return false;
+        }
         if (!t.root(plain.top_root, out.top_root)) return false;
         if (w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
             err = "CW8 relayout: a top level deeper than 16 levels";
@@ -566,9 +598,9 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
                 }
                 plain_root = id[root];
             }
-            if (!b.root(plain_root, cw8_root)) return false;
-            if (w8_levels(out.nodes, cw8_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
-                err = "RS CW8 relayout: a mesh tree deeper than 16 levels";
+            if (wide_levels(plain, plain_root, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS ||
+                !b.root(plain_root, cw8_root) || w8_levels(out.nodes, cw8_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+                if (err.empty()) err = "RS CW8 relayout: a mesh tree deeper than 16 levels";
                 return false;
             }
             done.emplace(key, cw8_root);
@@ -583,9 +615,9 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
         const size_t n = std::min<size_t>(top_reserve(count), words.size() - std::min(words.size(), at));
         std::vector<uint32_t> top(words.begin() + at, words.begin() + at + n);
         W8Builder t{&top, &out.top, &err};
-        if (!t.root(words[2], out.top_root)) return false;
-        if (w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
-            err = "RS CW8 relayout: a top level deeper than 16 levels";
+        if (wide_levels(top, words[2], 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS ||
+            !t.root(words[2], out.top_root) || w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
+            if (err.empty()) err = "RS CW8 relayout: a top level deeper than 16 levels";
             return false;
         }
     }
