@@ -61,8 +61,10 @@ def profiled_traffic(config, layout, launch):
 class Frame:
     """Device buffers of one frame and the launches of one step."""
 
-    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream, reorder=False, schedule="fused"):
+    def __init__(self, rq, scene, kind, seed, w, h, n_gi, stream, reorder=False, schedule="fused", tiled=False):
         self.rq, self.scene, self.w, self.h, self.n_gi, self.seed = rq, scene, w, h, n_gi, seed
+        # primary rays in 8x8 screen-tile order (a wave's 64 rays through one tile) or row-major
+        self.tiled = tiled and w % 8 == 0 and h % 8 == 0
         # secondary launches sorted by (octant, origin Morton code) before tracing
         self.sec = rq.REORDER if reorder else 0
         # fused: shadow and GI rays as ranges of one persistent launch (they do not depend on each
@@ -97,7 +99,7 @@ class Frame:
             # this frame's primary hits are in stage `parity` (traced by the previous step)
             cur_p, cur_h = self.stage(self.parity)
             nxt_p, nxt_h = self.stage(1 - self.parity)
-            rq.gen_primary(self.inv, self.w, self.h, nxt_p, stream=s)
+            rq.gen_primary(self.inv, self.w, self.h, nxt_p, stream=s, tiled=self.tiled)
             rec("raygen")
             rq.gen_secondary(cur_p, cur_h, self.n, SUN, self.seed, self.n_gi, self.shadow,
                              self.gi if self.n_gi else None, self.counts, stream=s)
@@ -107,7 +109,7 @@ class Frame:
             rec("frame")
             self.parity = 1 - self.parity
             return
-        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=s)
+        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=s, tiled=self.tiled)
         rec("raygen")
         self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=s)
         rec("primary")
@@ -166,7 +168,7 @@ class Frame:
         schedule, leaves the first frame's primary hits in stage 0."""
         rq = self.rq
         self.parity = 0
-        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=self.stream)
+        rq.gen_primary(self.inv, self.w, self.h, self.prim, stream=self.stream, tiled=self.tiled)
         self.scene.trace(layout, self.prim, self.n, rq.CLOSEST, hits=self.hits, stream=self.stream)
         rq.gen_secondary(self.prim, self.hits, self.n, SUN, self.seed, self.n_gi, self.shadow,
                          self.gi if self.n_gi else None, self.counts, stream=self.stream)
@@ -500,6 +502,8 @@ def main():
                     help="shadow and GI rays as ranges of one launch (fused), one launch per pass (separate), or "
                          "one launch per step with the next frame's primary rays (pipelined)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
+    ap.add_argument("--primary-order", default="tiles", choices=["tiles", "rows"],
+                    help="primary rays in 8x8 screen-tile order (rq_gen_primary_tiled) or row-major")
     ap.add_argument("--no-reflection", action="store_true", help="skip the reflection-ray (rs_trace) line")
     ap.add_argument("--no-refit", action="store_true", help="skip the GPU mesh refit line")
     ap.add_argument("--no-gi-cache", action="store_true", help="skip the GI cache ray-stage line")
@@ -554,7 +558,7 @@ def main():
         ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
     stream = rq.Stream()
     frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary",
-                  schedule=args.secondary)
+                  schedule=args.secondary, tiled=args.primary_order == "tiles")
     rays = frame.prime(args.layout)
     log(f"rays per step: {rays} (primary {frame.n}, secondary {list(frame.counts_host)})")
 
@@ -649,6 +653,7 @@ def main():
         "dtype": "f32", "data": "synthetic (seeded scene and camera)",
         "config": {"workload": desc, "layout": args.layout, "rays_per_step": rays, "triangles": st["triangles"],
                    "instances": st["instances"], "reorder": args.reorder, "secondary": args.secondary,
+                   "primary_order": "tiles" if frame.tiled else "rows",
                    "tuning": tuned.get(args.layout),
                    "parallelism": "rays across one GPU"},
         "roofline": roofline, "cpu_baseline": cpu,

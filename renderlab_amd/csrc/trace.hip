@@ -928,11 +928,19 @@ struct Mat16 {
     float m[16];
 };
 
-// shaders/primary_rays.wgsl:50-66
+// shaders/primary_rays.wgsl:50-66. TILED: ray i is pixel k = i % 64 of 8x8 tile
+// t = i / 64 (tiles row-major, pixels row-major within a tile; w and h multiples
+// of 8), so a wave's 64 rays leave through one screen tile.
+template <bool TILED>
 __global__ void primary_kernel(Mat16 inv, uint32_t w, uint32_t h, rq_ray* rays) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
     uint32_t px = i % w, py = i / w;
+    if (TILED) {
+        const uint32_t t = i >> 6, k = i & 63u, tiles_x = w >> 3;
+        px = (t % tiles_x) * 8u + (k & 7u);
+        py = (t / tiles_x) * 8u + (k >> 3);
+    }
     float nx = __builtin_fmaf(((float)px + 0.5f) / (float)w, 2.0f, -1.0f);
     float ny = __builtin_fmaf(((float)py + 0.5f) / (float)h, -2.0f, 1.0f);
     float xy[4], A[4], B[4];
@@ -2077,19 +2085,39 @@ extern "C" rq_status rq_device_copy(void* dst, const void* src, size_t bytes, vo
     return RQ_OK;
 }
 
-extern "C" rq_status rq_gen_primary(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays,
-                                    void* stream) {
-    if (!inverse || !rays || width == 0 || height == 0) {
-        set_error("rq_gen_primary: bad argument");
+namespace rq {
+namespace {
+rq_status gen_primary(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays, void* stream,
+                      bool tiled) {
+    if (!inverse || !rays || width == 0 || height == 0 || (uint64_t)width * height >= (1ull << 32) ||
+        (tiled && (width % 8 || height % 8))) {
+        set_error(tiled ? "rq_gen_primary_tiled: bad argument (width and height must be multiples of 8)"
+                        : "rq_gen_primary: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
     dev::Mat16 m;
     std::memcpy(m.m, inverse, sizeof m.m);
     uint64_t n = (uint64_t)width * height;
-    hipLaunchKernelGGL(dev::primary_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), m, width, height, rays);
+    if (tiled)
+        hipLaunchKernelGGL(dev::primary_kernel<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           reinterpret_cast<hipStream_t>(stream), m, width, height, rays);
+    else
+        hipLaunchKernelGGL(dev::primary_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           reinterpret_cast<hipStream_t>(stream), m, width, height, rays);
     HIP_TRY(hipGetLastError(), "primary launch");
     return RQ_OK;
+}
+}  // namespace
+}  // namespace rq
+
+extern "C" rq_status rq_gen_primary(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays,
+                                    void* stream) {
+    return rq::gen_primary(inverse, width, height, rays, stream, false);
+}
+
+extern "C" rq_status rq_gen_primary_tiled(const float inverse[16], uint32_t width, uint32_t height, rq_ray* rays,
+                                          void* stream) {
+    return rq::gen_primary(inverse, width, height, rays, stream, true);
 }
 
 extern "C" rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t count, const float sun[3],

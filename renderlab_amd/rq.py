@@ -139,6 +139,7 @@ _sig("rq_get_tuning", _i, _i, _u32, _P)
 _sig("rq_set_tuning", _i, _i, _u32, _P)
 _sig("rq_verify_fp64", _i, _P, _P, _P, _u32, _P, _P)
 _sig("rq_gen_primary", _i, _P, _u32, _u32, _P, _P)
+_sig("rq_gen_primary_tiled", _i, _P, _u32, _u32, _P, _P)
 _sig("rq_gen_secondary", _i, _P, _P, _u32, _P, _u32, _u32, _P, _P, _P, _P)
 _sig("rq_malloc", _i, _sz, ctypes.POINTER(_P))
 _sig("rq_free", _i, _P)
@@ -793,9 +794,12 @@ def camera(kind, seed, width, height):
     return inv
 
 
-def gen_primary(inverse, width, height, rays, stream=None):
+def gen_primary(inverse, width, height, rays, stream=None, tiled=False):
+    """Primary rays, one a pixel: row-major, or (tiled) 8x8 screen tiles row-major with the
+    pixels of a tile row-major (width and height multiples of 8)."""
     inv = np.ascontiguousarray(inverse, np.float32)
-    check(lib.rq_gen_primary(_ptr(inv), int(width), int(height), rays.ptr, stream.ptr if stream else None))
+    fn = lib.rq_gen_primary_tiled if tiled else lib.rq_gen_primary
+    check(fn(_ptr(inv), int(width), int(height), rays.ptr, stream.ptr if stream else None))
 
 
 def gen_secondary(primary, hits, count, sun, seed, n_gi, shadow, gi, counts, stream=None):

@@ -37,6 +37,24 @@ def test_primary_rays_match_oracle_bits(workload_c0):
     assert (prim.view(np.uint32) == want.view(np.uint32)).all()
 
 
+def test_tiled_primary_rays_are_the_pixels_in_tile_order(workload_c0):
+    """rq_gen_primary_tiled: ray i is pixel i % 64 of 8x8 tile i / 64 (tiles and pixels
+    row-major), bit for bit the oracle's ray of that pixel; sizes not multiples of 8 refused."""
+    inv, _, _, _, _ = workload_c0
+    want = oracle.primary_rays(inv, W, H)
+    d = rq.DeviceBuffer(W * H * 32)
+    rq.gen_primary(inv, W, H, d, tiled=True)
+    rq.synchronize()
+    got = d.download(rq.RAY_DTYPE, W * H)
+    i = np.arange(W * H)
+    t, k = i // 64, i % 64
+    px = (t % (W // 8)) * 8 + k % 8
+    py = (t // (W // 8)) * 8 + k // 8
+    assert (got.view(np.uint32).reshape(-1, 8) == want[py * W + px].view(np.uint32).reshape(-1, 8)).all()
+    with pytest.raises(rq.InvalidInput):
+        rq.gen_primary(inv, W - 4, H, d, tiled=True)
+
+
 @pytest.mark.parametrize("layout", ["ref", "plain"])
 def test_closest_bit_exact(scene_c0, workload_c0, layout):
     _, prim, _, _, gis = workload_c0
