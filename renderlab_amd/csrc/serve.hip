@@ -11,7 +11,9 @@
 // fit in the budget arena (first fit); completion is polled on events.
 // Every job's outputs are folded on the device into a 64-bit digest
 // (position-keyed, order-free sum), so a job's result can be compared alone
-// and in the mix without copying frames back.
+// and in the mix without copying frames back. Primary rays are generated in
+// 8x8 screen-tile order when the job's size is a multiple of 8 (positions
+// are ray indices).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -189,7 +191,11 @@ extern "C" rq_status rq_serve(rq_scene* const* scenes, uint32_t scene_count, con
         if ((st = rq_gen_camera(j.scene_kind, j.seed, j.width, j.height, inv)) != RQ_OK) return st;
         void* s = sl.stream;
         (void)hipEventRecord(sl.start, sl.stream);
-        if ((st = rq_gen_primary(inv, j.width, j.height, prim, s)) != RQ_OK) return st;
+        // screen-tile order when the size allows (a wave's rays through one 8x8 tile)
+        const bool tiled = j.width % 8 == 0 && j.height % 8 == 0;
+        if ((st = tiled ? rq_gen_primary_tiled(inv, j.width, j.height, prim, s)
+                        : rq_gen_primary(inv, j.width, j.height, prim, s)) != RQ_OK)
+            return st;
         if ((st = rq_trace(sc, cfg->layout, prim, (uint32_t)n, RQ_MODE_CLOSEST, hits, nullptr, s)) != RQ_OK) return st;
         if ((st = rq_gen_secondary(prim, hits, (uint32_t)n, cfg->sun, (uint32_t)j.seed, j.n_gi, shadow, gi, counts, s)) !=
             RQ_OK)
