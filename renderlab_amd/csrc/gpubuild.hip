@@ -35,6 +35,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -285,6 +286,12 @@ __device__ inline void append(bool want, const Seg& s, Seg* list, uint32_t* coun
 }
 
 // ---------------------------------------------------------------- classify
+// The total of an exclusive scan (last prefix + last flag) in one word, so a
+// level reads it back with one copy.
+__global__ void scan_total(const uint32_t* scan, const uint32_t* flag, uint32_t n, uint32_t* out) {
+    if (threadIdx.x == 0) out[0] = scan[n - 1] + flag[n - 1];
+}
+
 __global__ __launch_bounds__(256) void node_flags(const uint32_t* gcount, uint32_t n, uint32_t* flag) {
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
     if (g < n) flag[g] = gcount[g] > LEAF_TRIANGLES ? 1u : 0u;
@@ -912,10 +919,26 @@ struct GpuBuild {
         GB_TRY(hipcub::DeviceScan::ExclusiveSum(cub.p, bytes, in, out, n, s), "scan");
         return RQ_OK;
     }
+    uint32_t* pinned = nullptr;  // host words for the small per-level readbacks
+    ~GpuBuild() {
+        if (pinned) (void)hipHostFree(pinned);
+    }
     rq_status fetch(void* dst, const void* src, size_t bytes) {
-        GB_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "build fetch");
+        if (bytes <= 256 && !pinned && hipHostMalloc(reinterpret_cast<void**>(&pinned), 256, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            pinned = nullptr;
+        }
+        void* to = bytes <= 256 && pinned ? static_cast<void*>(pinned) : dst;
+        GB_TRY(hipMemcpyAsync(to, src, bytes, hipMemcpyDeviceToHost, s), "build fetch");
         GB_TRY(hipStreamSynchronize(s), "build sync");
+        if (to != dst) std::memcpy(dst, to, bytes);
         return RQ_OK;
+    }
+    // scan[n - 1] + flag[n - 1]: the number of flagged entries, read back once
+    rq_status fetch_total(const Buf& scan_b, const Buf& flag_b, uint32_t n, uint32_t& out) {
+        hipLaunchKernelGGL(scan_total, dim3(1), dim3(64), 0, s, scan_b.as<uint32_t>(), flag_b.as<uint32_t>(), n,
+                           counters.as<uint32_t>() + 8);
+        return fetch(&out, counters.as<uint32_t>() + 8, 4);
     }
     void fill(Buf& b, uint32_t v, uint64_t n) {
         hipLaunchKernelGGL(fill_u32, dim3((unsigned)std::min<uint64_t>((n + 255) / 256 + 1, 4096)), dim3(256), 0, s,
@@ -1049,10 +1072,8 @@ struct GpuBuild {
             hipLaunchKernelGGL(node_flags, grid_for(ng), dim3(256), 0, s, gcount.as<uint32_t>() + gb, ng,
                                flag.as<uint32_t>());
             GB_OK(exclusive_sum(flag.as<uint32_t>(), scan.as<uint32_t>(), ng));
-            uint32_t last[2];
-            GB_OK(fetch(&last[0], scan.as<uint32_t>() + ng - 1, 4));
-            GB_OK(fetch(&last[1], flag.as<uint32_t>() + ng - 1, 4));
-            const uint32_t nn = last[0] + last[1];
+            uint32_t nn = 0;
+            GB_OK(fetch_total(scan, flag, ng, nn));
             hipLaunchKernelGGL(make_nodes, grid_for(ng), dim3(256), 0, s, gstart.as<uint32_t>(), gcount.as<uint32_t>(),
                                gmesh.as<uint32_t>(), gnode.as<uint32_t>(), gb, ng, scan.as<uint32_t>(), nb, level,
                                node_group.as<uint32_t>(), node_mesh.as<uint32_t>(), node_level.as<uint32_t>(),
@@ -1074,9 +1095,8 @@ struct GpuBuild {
             hipLaunchKernelGGL(child_counts, grid_for(nn), dim3(256), 0, s, gcount.as<uint32_t>(),
                                node_group.as<uint32_t>(), nb, nn, split_left.as<uint32_t>(), flag.as<uint32_t>());
             GB_OK(exclusive_sum(flag.as<uint32_t>(), scan.as<uint32_t>(), nn));
-            GB_OK(fetch(&last[0], scan.as<uint32_t>() + nn - 1, 4));
-            GB_OK(fetch(&last[1], flag.as<uint32_t>() + nn - 1, 4));
-            const uint32_t next = last[0] + last[1];
+            uint32_t next = 0;
+            GB_OK(fetch_total(scan, flag, nn, next));
             hipLaunchKernelGGL(emit_children, grid_for(nn), dim3(256), 0, s, gstart.as<uint32_t>(),
                                gcount.as<uint32_t>(), gmesh.as<uint32_t>(), node_group.as<uint32_t>(), nb, nn,
                                split_left.as<uint32_t>(), scan.as<uint32_t>(), gb + ng, child_first.as<uint32_t>(),
@@ -1138,7 +1158,7 @@ struct GpuBuild {
         GB_OK(large.need(N * sizeof(Seg), "hipMalloc build"));
         GB_OK(med.need(N * sizeof(MedianReq), "hipMalloc build"));
         GB_OK(split_left.need(3 * N * 4, "hipMalloc build"));
-        GB_OK(counters.need(16, "hipMalloc build"));
+        GB_OK(counters.need(64, "hipMalloc build"));  // [3] round-1 count, [8] scan totals
         GB_OK(mark.need(T * 4, "hipMalloc build"));
         GB_OK(lrank.need(T * 4, "hipMalloc build"));
         GB_OK(summary.need(16ull * mesh_count + 16, "hipMalloc build"));
