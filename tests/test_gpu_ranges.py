@@ -108,3 +108,24 @@ def test_tuner_times_the_launch_on_its_stream(scene, passes):
     finally:
         rq.set_tuning("cw8", 0, base.block, base.stack, base.refill, base.grid_per_cu, speculate=base.speculate,
                       drain=base.drain)
+
+
+def test_stream_wait_orders_two_streams(scene, passes):
+    """rq_stream_wait_event: a copy queued on stream B after waiting on an event recorded on stream
+    A behind a trace copies the finished records (equal to a plain trace's)."""
+    p, _, _, _ = passes
+    want, _ = gpu_trace(scene, "cw8", p, rq.CLOSEST)
+    a, b = rq.Stream(), rq.Stream()
+    dr = to_device(p)
+    dh = rq.DeviceBuffer(len(p) * 48)
+    dh.zero()
+    out = rq.DeviceBuffer(len(p) * 48)
+    done = rq.Event()
+    print("case: trace on A, copy on B after A's event", flush=True)
+    scene.trace("cw8", dr, len(p), rq.CLOSEST, hits=dh, stream=a)
+    done.record(a)
+    b.wait(done)
+    rq.device_copy(out, dh, len(p) * 48, stream=b)
+    b.synchronize()
+    assert len(compare_exact(out.download(rq.HIT_DTYPE, len(p)), want)) == 0
+    a.synchronize()
