@@ -325,7 +325,13 @@ def run_reflection(rq, kind, meshes, tris, frame, reps=3):
     """The reflection rays' traversal (rs_trace over rl_geometry) on the same seeded scene: the frame's
     primary and GI rays as closest-hit queries (rs_closest), the shadow rays as any-hit (rs_blocked)."""
     t0 = time.perf_counter()
-    sc = rq.Scene.generate(kind, 1, meshes, tris, reflection=True)
+    host = rq.Scene.generate(kind, 1, meshes, tris, reflection=True)
+    rq.Scene.generate(kind, 1, meshes, tris, reflection=True, gpu=True)  # first GPU build: pins staging
+    sc = rq.Scene.generate(kind, 1, meshes, tris, reflection=True, gpu=True)
+    build = {"host_build_ms": round(host.stats()["build_us"] / 1e3, 2),
+             "gpu_build_ms": round(sc.stats()["build_us"] / 1e3, 2),
+             "words_equal": bool(np.array_equal(host.words, sc.words))}
+    del host
     sc.upload(("rs", "rs_cw8"))
     build_s = time.perf_counter() - t0
     c = frame.counts_host
@@ -336,7 +342,8 @@ def run_reflection(rq, kind, meshes, tris, frame, reps=3):
     if n_gi:
         ranges.insert(1, (frame.gi, n_gi, rq.CLOSEST, frame.gi_hits, None))
     rays = sum(r[1] for r in ranges)
-    out = {"rays": int(rays), "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words))}
+    out = {"rays": int(rays), "build_and_upload_s": round(build_s, 2), "words": int(len(sc.words)),
+           "scene_build": build}
     for layout in ("rs", "rs_cw8"):
         start, end = rq.Event(), rq.Event()
         sc.trace_ranges(layout, ranges)

@@ -254,6 +254,14 @@ size_t rq_gi_gather_workspace(uint32_t slot_count);
    and errors as rq_scene_build. Synchronizes `stream`. */
 rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                              uint32_t instance_count, int top_level, rq_scene** out, void* stream);
+/* rq_scene_build_reflection with the BVHs built on the GPU (no reference
+   counterpart, as rq_scene_build_gpu): the same rl_geometry words, from the
+   GPU build of the meshes (trees, triangle order and primitive ids) with the
+   reflection leaf tags and 12-word triangles written on the device; records,
+   top level and scene bounds on the host. The words are left uploaded for
+   RQ_LAYOUT_RS_BVH4Q8 and copied to the host. */
+rq_status rq_scene_build_reflection_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
+                                        uint32_t instance_count, int top_level, rq_scene** out, void* stream);
 /* GPU refit of one mesh of a built scene (rq_scene_build) whose REF layout is
    uploaded, for a re-skinned pose: replaces the skin_refit path of
    src/gi_cache.rs:2843-2863 -- near_field::refit_bvh (crates/asset/src/
@@ -502,6 +510,9 @@ rq_status rq_gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t t
 /* rq_gen_scene with the BVHs built by rq_scene_build_gpu (the same words). */
 rq_status rq_gen_scene_gpu(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles, rq_scene** out,
                            void* stream);
+/* rq_gen_reflection_scene with the BVHs built on the GPU (the same words). */
+rq_status rq_gen_reflection_scene_gpu(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                                      rq_scene** out, void* stream);
 /* The same meshes and placements packed as reflection geometry (rl_geometry). */
 rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
                                   rq_scene** out);

@@ -1513,7 +1513,279 @@ extern "C" rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_cou
         std::copy(instances[i].model, instances[i].model + 16, &scene->models[16ull * i]);
     scene->mesh_nodes.resize(mesh_count);
     for (uint32_t m = 0; m < mesh_count; ++m) scene->mesh_nodes[m] = offsets[4 * m];
+    scene->mesh_map.resize(9ull * mesh_count);
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        uint32_t* o = &scene->mesh_map[9ull * m];
+        o[0] = md[m].nodes;
+        o[1] = md[m].leaves;
+        o[2] = md[m].positions;
+        o[3] = md[m].indices;
+        o[4] = md[m].keep ? md[m].prims : 0u;
+        o[5] = md[m].node_count;
+        o[6] = md[m].leaf_count;
+        o[7] = md[m].tcount;
+        o[8] = sm[4 * m + 2];  // root tag
+    }
     scene->top_force = top_level;
+    scene_bounds(scene);
+    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - began).count();
+    *out = scene;
+    return RQ_OK;
+}
+
+// ----------------------------------------------------------------- reflection words
+// rq_scene_build_reflection's words (src/reflection_software.rs:92-388) from the
+// GPU build of the same meshes: the same SAH trees (node numbering unchanged),
+// RS leaf tags LEAF | first triangle << 4 | count, and 12-word triangles
+// (corner a, b - a, c - a, source primitive, 0, 0) gathered in BVH order.
+namespace rq {
+namespace gbuild {
+
+struct RsMesh {
+    uint32_t nodes, leaves, positions, indices, prims;  // GI word offsets of the mesh
+    uint32_t rs_nodes, rs_tris;                          // RS word offsets
+    uint32_t node_first, tri_first;                      // first global node / triangle
+    uint32_t root;                                       // GI root tag
+};
+
+__device__ __forceinline__ uint32_t rs_mesh_of(const RsMesh* ms, uint32_t n, uint32_t i, bool tri) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if ((tri ? ms[mid].tri_first : ms[mid].node_first) <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// A GI child tag as the reflection words tag it (leaves by triangle range).
+__device__ __forceinline__ uint32_t rs_tag(const uint32_t* gi, const RsMesh& m, uint32_t c) {
+    if (c == EMPTY || !(c & LEAF_BIT)) return c;
+    const uint32_t l = c & ~LEAF_BIT;
+    const uint32_t first = gi[m.leaves + 2ull * l], count = gi[m.leaves + 2ull * l + 1];
+    return LEAF_BIT | ((first / 3u) << 4) | (count / 3u);
+}
+
+__global__ __launch_bounds__(256) void rs_nodes(const RsMesh* ms, uint32_t mesh_count, uint32_t total,
+                                                const uint32_t* gi, uint32_t* rs) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const RsMesh m = ms[rs_mesh_of(ms, mesh_count, i, false)];
+    const uint64_t n = i - m.node_first;
+    const uint32_t* src = gi + m.nodes + 16ull * n;
+    uint32_t* dst = rs + m.rs_nodes + 16ull * n;
+    for (int k = 0; k < 16; ++k) dst[k] = (k >= 6 && k < 10) ? rs_tag(gi, m, src[k]) : src[k];
+}
+
+__global__ __launch_bounds__(256) void rs_tris(const RsMesh* ms, uint32_t mesh_count, uint32_t total,
+                                               const uint32_t* gi, uint32_t* rs) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= total) return;
+    const RsMesh m = ms[rs_mesh_of(ms, mesh_count, i, true)];
+    const uint64_t t = i - m.tri_first;
+    const float* P = reinterpret_cast<const float*>(gi + m.positions);
+    const uint32_t a = gi[m.indices + 3 * t], b = gi[m.indices + 3 * t + 1], c = gi[m.indices + 3 * t + 2];
+    uint32_t* dst = rs + m.rs_tris + 12ull * t;
+    for (int k = 0; k < 3; ++k) {
+        const float pa = P[3ull * a + k];
+        dst[k] = __float_as_uint(pa);
+        dst[3 + k] = __float_as_uint(P[3ull * b + k] - pa);
+        dst[6 + k] = __float_as_uint(P[3ull * c + k] - pa);
+    }
+    dst[9] = m.prims ? gi[m.prims + t] : (uint32_t)t;
+    dst[10] = 0u;
+    dst[11] = 0u;
+}
+
+__global__ __launch_bounds__(256) void rs_roots(const RsMesh* ms, uint32_t mesh_count, const uint32_t* gi,
+                                                uint32_t* out) {
+    const uint32_t m = blockIdx.x * 256u + threadIdx.x;
+    if (m < mesh_count) out[m] = rs_tag(gi, ms[m], ms[m].root);
+}
+
+}  // namespace gbuild
+}  // namespace rq
+
+extern "C" rq_status rq_scene_build_reflection_gpu(const rq_mesh* meshes, uint32_t mesh_count,
+                                                   const rq_instance* instances, uint32_t instance_count,
+                                                   int top_level, rq_scene** out, void* stream) {
+    using namespace rq::gbuild;
+    const auto began = std::chrono::steady_clock::now();
+    if (!out || (!meshes && mesh_count) || (!instances && instance_count)) {
+        set_error("rq_scene_build_reflection_gpu: null argument");
+        return RQ_ERR_INVALID_INPUT;
+    }
+    *out = nullptr;
+    for (uint32_t i = 0; i < instance_count; ++i)
+        if (instances[i].mesh >= mesh_count) {
+            set_error("rq_scene_build_reflection_gpu: instance names a missing mesh");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    // The trees: the GI build of the meshes alone, every mesh with its primitive table.
+    std::vector<rq_mesh> with_prims(meshes, meshes + mesh_count);
+    for (auto& m : with_prims) m.keep_primitives = 1;
+    rq_scene* gi = nullptr;
+    rq_status st = rq_scene_build_gpu(with_prims.data(), mesh_count, nullptr, 0, -1, &gi, stream);
+    if (st != RQ_OK) return st;
+    struct Drop {
+        rq_scene* s;
+        ~Drop() { rq_scene_destroy(s); }
+    } drop{gi};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // Object bounds over the referenced corners (bounds(indices.map(positions))), per mesh.
+    std::vector<float> bounds(6ull * mesh_count);
+    {
+        const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 64));
+        std::vector<std::thread> pool;
+        std::atomic_uint next{0};
+        for (uint32_t t = 0; t < std::min(threads, std::max(mesh_count, 1u)); ++t)
+            pool.emplace_back([&]() {
+                for (;;) {
+                    const uint32_t m = next.fetch_add(1);
+                    if (m >= mesh_count) break;
+                    float* b = &bounds[6ull * m];
+                    for (int a = 0; a < 3; ++a) {
+                        b[a] = __builtin_inff();
+                        b[3 + a] = -__builtin_inff();
+                    }
+                    for (uint32_t k = 0; k < meshes[m].index_count; ++k)
+                        for (int a = 0; a < 3; ++a) {
+                            const float p = meshes[m].positions[3ull * meshes[m].indices[k] + a];
+                            b[a] = mn(b[a], p);
+                            b[3 + a] = mx(b[3 + a], p);
+                        }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    // pack (src/reflection_software.rs:285-305): each mesh's nodes then triangles
+    // after the header, records, top level and scene bounds.
+    std::vector<RsMesh> rm(mesh_count);
+    uint64_t at = rs_records_words(instance_count), node_total = 0, tri_total = 0;
+    for (uint32_t m = 0; m < mesh_count; ++m) {
+        const uint32_t* g = &gi->mesh_map[9ull * m];
+        RsMesh& r = rm[m];
+        r.nodes = g[0];
+        r.leaves = g[1];
+        r.positions = g[2];
+        r.indices = g[3];
+        r.prims = g[4];
+        r.root = g[8];
+        r.node_first = (uint32_t)node_total;
+        r.tri_first = (uint32_t)tri_total;
+        r.rs_nodes = (uint32_t)at;
+        at += 16ull * g[5];
+        r.rs_tris = (uint32_t)at;
+        at += 12ull * g[7];
+        node_total += g[5];
+        tri_total += g[7];
+        if (at >= 0xffffffffull) {
+            set_error("reflection geometry exceeds 2^32 words");
+            return RQ_ERR_INVALID_INPUT;
+        }
+    }
+    rq::DeviceLayout d;
+    d.bytes = (at * 4 + 255) & ~(size_t)255;
+    GB_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc reflection scene");
+    struct Own {
+        void* p;
+        ~Own() {
+            if (p) (void)hipFree(p);
+        }
+    } own{d.base};
+    uint32_t* rs = reinterpret_cast<uint32_t*>(d.base);
+    const uint32_t* gw = reinterpret_cast<const uint32_t*>(gi->dev[0].base);
+    Buf dev_meshes, roots;
+    GB_OK(dev_meshes.need(std::max<size_t>(mesh_count, 1) * sizeof(RsMesh), "hipMalloc reflection meshes"));
+    GB_OK(roots.need(std::max<size_t>(mesh_count, 1) * 4, "hipMalloc reflection roots"));
+    GB_TRY(hipMemsetAsync(rs, 0, d.bytes, s), "memset reflection scene");
+    std::vector<uint32_t> root_tags(mesh_count);
+    if (mesh_count) {
+        GB_TRY(hipMemcpyAsync(dev_meshes.p, rm.data(), mesh_count * sizeof(RsMesh), hipMemcpyHostToDevice, s),
+               "upload reflection meshes");
+        if (node_total)
+            hipLaunchKernelGGL(rs_nodes, grid_for(node_total), dim3(256), 0, s, dev_meshes.as<RsMesh>(), mesh_count,
+                               (uint32_t)node_total, gw, rs);
+        if (tri_total)
+            hipLaunchKernelGGL(rs_tris, grid_for(tri_total), dim3(256), 0, s, dev_meshes.as<RsMesh>(), mesh_count,
+                               (uint32_t)tri_total, gw, rs);
+        hipLaunchKernelGGL(rs_roots, grid_for(mesh_count), dim3(256), 0, s, dev_meshes.as<RsMesh>(), mesh_count, gw,
+                           roots.as<uint32_t>());
+        GB_TRY(hipGetLastError(), "reflection words launch");
+        GB_TRY(hipMemcpyAsync(root_tags.data(), roots.p, mesh_count * 4ull, hipMemcpyDeviceToHost, s), "roots");
+        GB_TRY(hipStreamSynchronize(s), "reflection words");
+    }
+    // instance_words (src/reflection_software.rs:318-345) as rq_scene_build_reflection:
+    // singular models are skipped; then the top level and the scene bounds.
+    std::vector<uint32_t> head(RS_HEADER_WORDS, 0u);
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < instance_count; ++i) {
+        const rq_instance& ins = instances[i];
+        float inv[16];
+        if (!mat4_inverse(ins.model, inv)) continue;
+        for (int row = 0; row < 3; ++row)
+            for (int c = 0; c < 4; ++c) head.push_back(f_bits(inv[c * 4 + row]));
+        float lo[3], hi[3];
+        rs_world_bounds(ins.model, &bounds[6ull * ins.mesh], &bounds[6ull * ins.mesh + 3], lo, hi);
+        for (int a = 0; a < 3; ++a) head.push_back(f_bits(lo[a]));
+        for (int a = 0; a < 3; ++a) head.push_back(f_bits(hi[a]));
+        head.push_back(rm[ins.mesh].rs_nodes);
+        head.push_back(rm[ins.mesh].rs_tris);
+        head.push_back(root_tags[ins.mesh]);
+        head.push_back(ins.placement);
+        head.push_back(ins.flags & 1u);
+        head.push_back(0u);
+        ++count;
+    }
+    head[0] = count;
+    top_place(head, RS_HEADER_WORDS, RS_RECORD_WORDS, instance_count, nullptr, 0, top_level);
+    {
+        float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+        float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        for (uint32_t i = 0; i < count; ++i) {
+            const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = mn(lo[a], bits_f(head[r + 12 + a]));
+                hi[a] = mx(hi[a], bits_f(head[r + 15 + a]));
+            }
+        }
+        head[4] = (uint32_t)head.size();
+        for (int a = 0; a < 3; ++a) head.push_back(f_bits(lo[a]));
+        head.push_back(0u);
+        for (int a = 0; a < 3; ++a) head.push_back(f_bits(hi[a]));
+        head.push_back(0u);
+    }
+    if (head.size() > at) {
+        set_error("rq_scene_build_reflection_gpu: header past the mesh words");
+        return RQ_ERR_DEVICE;
+    }
+    auto scene = new rq_scene();
+    scene->kind = RQ_SCENE_REFLECTION;
+    scene->words.assign(at, 0u);
+    GB_TRY(hipMemcpyAsync(rs, head.data(), head.size() * 4, hipMemcpyHostToDevice, s), "reflection head");
+    GB_TRY(hipMemcpyAsync(scene->words.data(), rs, at * 4, hipMemcpyDeviceToHost, s), "reflection words");
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        delete scene;
+        return hip_fail(e, "reflection words");
+    }
+    int device = 0;
+    (void)hipGetDevice(&device);
+    scene->device = device;
+    rq::DeviceLayout& dl = scene->dev[RQ_LAYOUT_RS_BVH4Q8];
+    dl.base = d.base;
+    dl.bytes = d.bytes;
+    dl.ready = true;
+    own.p = nullptr;
+    uint64_t tris = 0;
+    for (uint32_t i = 0; i < instance_count; ++i) tris += gi->mesh_map[9ull * instances[i].mesh + 7];
+    scene->stats[0] = count;
+    scene->stats[1] = mesh_count;
+    scene->stats[2] = tris;
+    scene->stats[3] = tri_total;
+    scene->stats[4] = node_total;
+    scene->stats[5] = scene->words[2];
     scene_bounds(scene);
     scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
                           std::chrono::steady_clock::now() - began).count();

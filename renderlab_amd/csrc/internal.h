@@ -149,6 +149,8 @@ rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi);
 void instance_box(const float model[16], const float lo[3], const float hi[3], uint32_t out[6]);
 // reflect.cpp: words of the reflection header, records, top level and scene bounds
 uint32_t rs_records_words(uint32_t instances);
+// world_bounds (src/reflection_software.rs:372-388): an instance's padded world box.
+void rs_world_bounds(const float* M, const float lo_in[3], const float hi_in[3], float lo[3], float hi[3]);
 
 // Per-mesh facts recovered from an instance record of packed words.
 struct MeshView {
@@ -219,6 +221,10 @@ struct rq_scene {
     // column-major), each mesh's node word offset, the top-level switch.
     std::vector<float> models;
     std::vector<uint32_t> mesh_nodes;
+    // GPU-built scenes: per mesh, 9 words -- node, leaf, position, index and
+    // primitive-table word offsets (0: none), node count, leaf count, triangles,
+    // root tag.
+    std::vector<uint32_t> mesh_map;
     int top_force = -1;
     uint32_t near_bins = 0;  // octahedral bins per side of the near-field masks (0: none)
     void* refit_scratch = nullptr;  // device: per-block bounds partials of a refit

@@ -12,6 +12,7 @@
 // Every record starts on a 16-byte boundary (src/reflection_software.rs:26-44).
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -78,6 +79,8 @@ bool rs_blas(const rq_mesh& m, RsBlas& out, std::string& err) {
     return true;
 }
 
+}  // namespace
+
 // world_bounds (src/reflection_software.rs:372-388): the 8 transformed corners
 // (glam transform_point3 order) widened by 1e-5 of the largest of |lo|, |hi|
 // and the extent over the axes.
@@ -106,8 +109,6 @@ void rs_world_bounds(const float* M, const float lo_in[3], const float hi_in[3],
     }
 }
 
-}  // namespace
-
 uint32_t rs_records_words(uint32_t instances) {
     return RS_HEADER_WORDS + instances * RS_RECORD_WORDS + top_reserve(instances) + 8u;
 }
@@ -119,6 +120,7 @@ using namespace rq;
 extern "C" rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t mesh_count,
                                                const rq_instance* instances, uint32_t instance_count, int top_level,
                                                rq_scene** out) {
+    const auto began = std::chrono::steady_clock::now();
     if (!out || (!meshes && mesh_count) || (!instances && instance_count)) {
         set_error("rq_scene_build_reflection: null argument");
         return RQ_ERR_INVALID_INPUT;
@@ -228,6 +230,8 @@ extern "C" rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t m
     scene->stats[4] = nodes;
     scene->stats[5] = words[2];
     scene_bounds(scene);
+    scene->stats[7] = (uint64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                          std::chrono::steady_clock::now() - began).count();
     *out = scene;
     return RQ_OK;
 }

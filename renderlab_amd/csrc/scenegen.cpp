@@ -225,7 +225,11 @@ static rq_status gen_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64
         ms[m].index_count = (uint32_t)data[m].i.size();
         ms[m].keep_primitives = 1;
     }
-    if (gpu) return rq_scene_build_gpu(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out, stream);
+    if (gpu)
+        return reflection ? rq_scene_build_reflection_gpu(ms.data(), (uint32_t)ms.size(), inst.data(),
+                                                          (uint32_t)inst.size(), -1, out, stream)
+                          : rq_scene_build_gpu(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1,
+                                               out, stream);
     return reflection
                ? rq_scene_build_reflection(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out)
                : rq_scene_build(ms.data(), (uint32_t)ms.size(), inst.data(), (uint32_t)inst.size(), -1, out);
@@ -243,6 +247,11 @@ extern "C" rq_status rq_gen_scene_gpu(uint32_t kind, uint64_t seed, uint32_t mes
 extern "C" rq_status rq_gen_reflection_scene(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
                                              rq_scene** out) {
     return gen_scene(kind, seed, meshes, triangles, true, out);
+}
+
+extern "C" rq_status rq_gen_reflection_scene_gpu(uint32_t kind, uint64_t seed, uint32_t meshes, uint64_t triangles,
+                                                 rq_scene** out, void* stream) {
+    return gen_scene(kind, seed, meshes, triangles, true, out, true, stream);
 }
 
 extern "C" rq_status rq_gen_camera(uint32_t kind, uint64_t seed, uint32_t width, uint32_t height, float inverse[16]) {

@@ -106,6 +106,7 @@ _sig("rq_scene_build_reflection", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P)
 _sig("rq_scene_from_rl_words", _i, _P, _sz, ctypes.POINTER(_P))
 _sig("rq_gen_reflection_scene", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P))
 _sig("rq_gen_scene_gpu", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P), _P)
+_sig("rq_gen_reflection_scene_gpu", _i, _u32, _u64, _u32, _u64, ctypes.POINTER(_P), _P)
 _sig("rq_gi_item_rays", _i, _P, _sz, _P, _u32, _P, _P)
 _sig("rq_gi_rays", _i, _P, _i, _P, _sz, _P, _u32, _P, _P, _P)
 _sig("rq_gi_gather_rays", _i, _P, _sz, _P, _u32, _P, _P, _u32, _P, _P, _sz, _P)
@@ -577,10 +578,9 @@ class Scene:
             idesc[k].placement = int(ins.get("placement", k))
         h = _P()
         if gpu:
-            if reflection:
-                raise InvalidInput(1, "the GPU build packs GI geometry only")
-            check(lib.rq_scene_build_gpu(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h),
-                                         stream.ptr if stream else None))
+            fn = lib.rq_scene_build_reflection_gpu if reflection else lib.rq_scene_build_gpu
+            check(fn(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h),
+                     stream.ptr if stream else None))
             return cls(h)
         fn = lib.rq_scene_build_reflection if reflection else lib.rq_scene_build
         check(fn(md, len(meshes), idesc, len(instances), int(top_level), ctypes.byref(h)))
@@ -639,7 +639,8 @@ class Scene:
     def generate(cls, kind, seed, meshes=0, triangles=0, reflection=False, gpu=False):
         h = _P()
         if gpu:
-            check(lib.rq_gen_scene_gpu(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h), None))
+            fn = lib.rq_gen_reflection_scene_gpu if reflection else lib.rq_gen_scene_gpu
+            check(fn(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h), None))
             return cls(h)
         fn = lib.rq_gen_reflection_scene if reflection else lib.rq_gen_scene
         check(fn(int(kind), int(seed), int(meshes), int(triangles), ctypes.byref(h)))

@@ -191,3 +191,31 @@ def test_gpu_build_rejects_bad_input(cases):
         rq.Scene.build([meshes[7]], [dict(model=np.zeros((4, 4), np.float32), mesh=0)], gpu=True)
     with pytest.raises(rq.InvalidInput):
         rq.Scene.build([meshes[7]], [dict(model=np.eye(4, dtype=np.float32), mesh=3)], gpu=True)
+
+
+def test_gpu_reflection_build_equals_host_build(cases):
+    """rq_scene_build_reflection_gpu: the rl_geometry words of rq_scene_build_reflection word for
+    word (the same trees, RS leaf tags, 12-word triangles with the source primitive, records with
+    a singular model skipped, top level, scene bounds), left uploaded for the RS layout, and traced
+    like the oracle's rs_trace."""
+    meshes, inst = cases
+    inst = list(inst)
+    inst.append(dict(model=np.zeros((4, 4), np.float32), mesh=0))  # singular: skipped by both
+    print(f"case: reflection build {len(meshes)} meshes, {len(inst)} instances", flush=True)
+    cpu = rq.Scene.build(meshes, inst, reflection=True)
+    gpu = rq.Scene.build(meshes, inst, reflection=True, gpu=True)
+    assert_same_words(cpu, gpu)
+    assert gpu.device_bytes("rs") > 0
+    for kind, meshes_n, tris in ((0, 0, 0), (1, 60, 40000)):
+        print(f"case: generated reflection scene kind {kind}", flush=True)
+        a = rq.Scene.generate(kind, 3, meshes_n, tris, reflection=True)
+        b = rq.Scene.generate(kind, 3, meshes_n, tris, reflection=True, gpu=True)
+        assert_same_words(a, b)
+    g = np.random.default_rng(11)
+    rays = np.zeros(3000, oracle.RAY_DTYPE)
+    rays["origin"] = g.uniform(-40, 40, (3000, 3))
+    rays["direction"] = g.normal(size=(3000, 3))
+    rays["tmax"] = 1e20
+    h, _ = gpu_trace(b, "rs", rays, rq.CLOSEST)
+    o, _, _ = oracle.rs_traverse(b.words, rays, 0)
+    assert len(compare_exact(h, o)) == 0
