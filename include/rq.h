@@ -259,7 +259,7 @@ rq_status rq_scene_build_gpu(const rq_mesh* meshes, uint32_t mesh_count, const r
    GPU build of the meshes (trees, triangle order and primitive ids) with the
    reflection leaf tags and 12-word triangles written on the device; records,
    top level and scene bounds on the host. The words are left uploaded for
-   RQ_LAYOUT_RS_BVH4Q8 and copied to the host. */
+   RQ_LAYOUT_RS_BVH4Q8; the mesh words reach the host when a host reader asks. */
 rq_status rq_scene_build_reflection_gpu(const rq_mesh* meshes, uint32_t mesh_count, const rq_instance* instances,
                                         uint32_t instance_count, int top_level, rq_scene** out, void* stream);
 /* GPU refit of one mesh of a built scene (rq_scene_build) whose REF layout is

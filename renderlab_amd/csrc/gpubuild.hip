@@ -1760,11 +1760,14 @@ extern "C" rq_status rq_scene_build_reflection_gpu(const rq_mesh* meshes, uint32
         set_error("rq_scene_build_reflection_gpu: header past the mesh words");
         return RQ_ERR_DEVICE;
     }
+    // The host holds the header, records, top level and bounds; the mesh words stay
+    // on the device until a host reader asks for them (sync_host_words).
     auto scene = new rq_scene();
     scene->kind = RQ_SCENE_REFLECTION;
-    scene->words.assign(at, 0u);
+    scene->words = head;
+    scene->total_words = at;
+    if (at > head.size()) scene->stale.emplace_back(head.size(), at - head.size());
     GB_TRY(hipMemcpyAsync(rs, head.data(), head.size() * 4, hipMemcpyHostToDevice, s), "reflection head");
-    GB_TRY(hipMemcpyAsync(scene->words.data(), rs, at * 4, hipMemcpyDeviceToHost, s), "reflection words");
     const hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         delete scene;

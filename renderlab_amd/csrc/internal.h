@@ -140,7 +140,8 @@ bool mat4_inverse(const float m[16], float inv[16]);
 void scene_bounds(rq_scene* scene);
 // The packed words' length, on the host or (GPU-built, not yet fetched) the device.
 inline size_t scene_words(const rq_scene* s);
-// Fetches the stale ranges of the host words from the device REF copy.
+// Fetches the stale ranges of the host words from the device copy (REF, or RS
+// for a reflection scene).
 rq_status sync_host_words(const rq_scene* scene);
 // Fetches the stale host words inside [lo, hi) only.
 rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi);

@@ -212,6 +212,11 @@ rq_status refit_plan(rq_scene* scene, const MeshView& m, RefitPlan*& out) {
 
 namespace rq {
 // Grows the host words to their full length (zeros; the new words are stale).
+// The device copy of a scene's words: REF for GI scenes, RS for reflection scenes.
+static int word_layout(const rq_scene* scene) {
+    return scene->kind == RQ_SCENE_REFLECTION ? RQ_LAYOUT_RS_BVH4Q8 : RQ_LAYOUT_REF_BVH4Q8;
+}
+
 static void grow_host_words(const rq_scene* scene) {
     if (scene->total_words > scene->words.size())
         const_cast<rq_scene*>(scene)->words.resize(scene->total_words, 0u);
@@ -220,7 +225,7 @@ static void grow_host_words(const rq_scene* scene) {
 rq_status sync_host_words(const rq_scene* scene) {
     if (!scene || scene->stale.empty()) return RQ_OK;
     grow_host_words(scene);
-    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[word_layout(scene)].base);
     uint32_t* host = const_cast<uint32_t*>(scene->words.data());
     for (const auto& r : scene->stale)
         RF_TRY(hipMemcpy(host + r.first, words + r.first, r.second * 4, hipMemcpyDeviceToHost), "fetch refit words");
@@ -232,7 +237,7 @@ rq_status sync_host_words(const rq_scene* scene) {
 rq_status sync_host_range(const rq_scene* scene, uint64_t lo, uint64_t hi) {
     if (!scene || scene->stale.empty() || lo >= hi) return RQ_OK;
     grow_host_words(scene);
-    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[0].base);
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(scene->dev[word_layout(scene)].base);
     uint32_t* host = const_cast<uint32_t*>(scene->words.data());
     std::vector<std::pair<uint64_t, uint64_t>> keep;
     for (const auto& r : scene->stale) {
