@@ -322,6 +322,30 @@ rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part
                                 size_t* bytes);
 void rq_scene_destroy(rq_scene* scene);
 
+/* ---- The buffer form of a hit (shaders/primary_rays.wgsl:19; the reflection
+   rays' RsHit.record, shaders/reflections_software_bvh4.wgsl:42, 90):
+   vec4<u32>(placement + 1, primitive, bits(u), bits(v)), all zero on a miss.
+   For gi_geometry scenes, placement is gi_hit_placement (instance word 29,
+   shaders/gi_rays_software.wgsl:314-316) and primitive is gi_hit_primitive
+   (:310-313: tri[0] when the instance has a primitive table, word 28 nonzero,
+   else 0xffffffff); for rl_geometry scenes they are the hit's placement slot
+   and primitive, as rs_closest returns them (:201). */
+typedef struct rq_hit_record {
+    uint32_t placement_plus1;
+    uint32_t primitive;
+    float u, v;
+} rq_hit_record;
+/* One hit on the host: its record and gi_hit_identity (:297-309: instance
+   word 24, or for a merged world the identity + 1 of its placement from the
+   position table at words 25-26; 0 for reflection scenes, misses and
+   see-through records). identity may be NULL. */
+rq_status rq_gi_hit_decode(const rq_scene* scene, const rq_hit* hit, uint32_t record[4], uint32_t* identity);
+/* The same for `count` device hits into device arrays (records and identities
+   may each be NULL): one kernel reading the scene's REF words (gi_geometry
+   scenes need RQ_LAYOUT_REF_BVH4Q8 uploaded). */
+rq_status rq_hit_records(const rq_scene* scene, const rq_hit* hits, uint32_t count, rq_hit_record* records,
+                         uint32_t* identities, void* stream);
+
 /* ---- Trace: gi_closest / gi_visible / gi_visible_within over a batch.
    rays, hits and vis_bits are device pointers; hits and vis_bits may each be
    NULL. vis_bits gets bit i set when ray i is unoccluded (gi_visible's

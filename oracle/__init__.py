@@ -44,6 +44,12 @@ _sig("orc_validate_blas", ctypes.c_int, _P, _sz, _u32, _u32, _u32, _u32, _u32, _
 _sig("orc_primary_ray", None, _P, _u32, _u32, _u32, _u32, _P)
 _sig("orc_secondary_rays", _u32, _P, _P, _P, _u32, _u32, _u32, _P)
 _sig("orc_pcg", _u32, _u32)
+_sig("orc_pcg_batch", None, _P, _u32, _P)
+_sig("orc_rand_sequence", None, _u32, _u32, _P)
+_sig("orc_basis", None, _P, _P)
+_sig("orc_cosine_dir", None, _P, ctypes.c_float, ctypes.c_float, _P)
+_sig("orc_offset", None, _P, _P, _P)
+_sig("orc_hit_records", None, _P, _sz, ctypes.c_int, _P, _u32, _P, _P)
 _sig("orc_gi_item_rays", None, _P, _sz, _P, _u32, _P)
 _sig("orc_near_bin", _u32, _P, _u32)
 _sig("orc_near_bin_direction", None, _u32, _u32, _P)
@@ -161,6 +167,52 @@ def secondary_rays(primary, hits, sun, seed, n_gi):
 
 def pcg(v):
     return int(lib.orc_pcg(int(v)))
+
+
+def pcg_batch(values):
+    v = np.ascontiguousarray(values, np.uint32)
+    out = np.zeros(v.size, np.uint32)
+    lib.orc_pcg_batch(_p(v), v.size, _p(out))
+    return out
+
+
+def rand_sequence(state, count):
+    out = np.zeros(count, np.float32)
+    lib.orc_rand_sequence(int(state) & 0xFFFFFFFF, int(count), _p(out))
+    return out
+
+
+def basis(n):
+    """basis(n) as a 3x3 array whose columns are c0, c1, c2."""
+    v = np.ascontiguousarray(n, np.float32)
+    out = np.zeros(9, np.float32)
+    lib.orc_basis(_p(v), _p(out))
+    return out.reshape(3, 3).T
+
+
+def cosine_dir(n, u1, u2):
+    v = np.ascontiguousarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.orc_cosine_dir(_p(v), float(u1), float(u2), _p(out))
+    return out
+
+
+def offset(p, ng):
+    a = np.ascontiguousarray(p, np.float32)
+    b = np.ascontiguousarray(ng, np.float32)
+    out = np.zeros(3, np.float32)
+    lib.orc_offset(_p(a), _p(b), _p(out))
+    return out
+
+
+def hit_records(words, hits, reflection=False):
+    """(records (n, 4) u32: placement + 1, primitive, bits(u), bits(v); identities (n,) u32)."""
+    w = np.ascontiguousarray(words, np.uint32)
+    h = np.ascontiguousarray(hits, HIT_DTYPE)
+    rec = np.zeros((len(h), 4), np.uint32)
+    ids = np.zeros(len(h), np.uint32)
+    lib.orc_hit_records(_p(w), w.size, int(bool(reflection)), _p(h), len(h), _p(rec), _p(ids))
+    return rec, ids
 
 
 def gi_item_rays(table, items):

@@ -1044,52 +1044,132 @@ uint32_t orc_pcg(uint32_t v) {
     uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
     return (word >> 22u) ^ word;
 }
+void orc_pcg_batch(const uint32_t* in, uint32_t count, uint32_t* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = orc_pcg(in[i]);
+}
+/* reference_path_tracer.wgsl:154-158 (rand): the state advances by one hash and
+   its top 24 bits become a float in [0, 1). */
 static inline float rnd(uint32_t* s) {
     *s = orc_pcg(*s);
     return (float)(*s >> 8u) * (1.0f / 16777216.0f);
 }
+void orc_rand_sequence(uint32_t state, uint32_t count, float* out) {
+    for (uint32_t i = 0; i < count; ++i) out[i] = rnd(&state);
+}
 
+/* reference_path_tracer.wgsl:160-167 (basis, Duff et al. 2017): columns c0, c1,
+   c2 = n, stored out[3c + row]. */
+void orc_basis(const float n[3], float out[9]) {
+    float sg = n[2] >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sg + n[2]);
+    float b = n[0] * n[1] * a;
+    float c[9] = {1.0f + sg * n[0] * n[0] * a, sg * b, -sg * n[0],
+                  b, sg + n[1] * n[1] * a, -n[1],
+                  n[0], n[1], n[2]};
+    memcpy(out, c, sizeof c);
+}
+
+/* reference_path_tracer.wgsl:466-468: the cosine-weighted direction of the
+   random pair (u1, u2) about n: r = sqrt(u1), phi = 2 pi u2, basis(n) applied to
+   (r cos phi, r sin phi, sqrt(max(0, 1 - r^2))), the matrix-vector product
+   summed column by column. */
+void orc_cosine_dir(const float n[3], float u1, float u2, float out[3]) {
+    float c[9];
+    orc_basis(n, c);
+    float rr = sqrtf(u1);
+    float phi = 6.2831855f * u2;
+    float l[3] = {rr * cosf(phi), rr * sinf(phi), sqrtf(mx(0.0f, 1.0f - rr * rr))};
+    for (int k = 0; k < 3; ++k) out[k] = c[k] * l[0] + c[3 + k] * l[1] + c[6 + k] * l[2];
+}
+
+/* reference_path_tracer.wgsl:237-240 (offset): p pushed along ng by
+   1e-4 + 4e-7 max(|p.x|, |p.y|, |p.z|). */
+void orc_offset(const float p[3], const float ng[3], float out[3]) {
+    float m = mx(mx(fabsf(p[0]), fabsf(p[1])), fabsf(p[2]));
+    float k = 1e-4f + 4e-7f * m;
+    for (int i = 0; i < 3; ++i) out[i] = p[i] + ng[i] * k;
+}
+
+/* The rays leaving one primary hit. The hit point is origin + direction * t;
+   the geometric normal is turned against the incoming ray
+   (reference_path_tracer.wgsl:224); the shadow ray toward the sun starts at the
+   offset point and is valid when the sun lies on the normal's side (:249-256);
+   each GI bounce draws two rand() values for its cosine direction (:466-470),
+   valid when it leaves on the normal's side. The random state of pixel p
+   starts at p ^ (seed * 0x9E3779B9). */
 uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const float sun[3], uint32_t pixel,
                             uint32_t seed, uint32_t n_gi, orc_ray* out) {
     if (!(hit->flags & 1u)) return 0u;
     float t = hit->t;
     v3 o = mk(primary->o[0], primary->o[1], primary->o[2]);
     v3 d = mk(primary->d[0], primary->d[1], primary->d[2]);
-    v3 p = mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
-    v3 ng = mk(hit->n[0], hit->n[1], hit->n[2]);
-    if (dot3(ng, d) > 0.0f) ng = mk(-ng.x, -ng.y, -ng.z);
-    /* reference_path_tracer.wgsl:237-240 */
-    float m = mx(mx(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
-    float k = 1e-4f + 4e-7f * m;
-    v3 start = mk(p.x + ng.x * k, p.y + ng.y * k, p.z + ng.z * k);
+    float p[3] = {o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+    v3 ngv = mk(hit->n[0], hit->n[1], hit->n[2]);
+    if (dot3(ngv, d) > 0.0f) ngv = mk(-ngv.x, -ngv.y, -ngv.z);
+    float ng[3] = {ngv.x, ngv.y, ngv.z}, start[3];
+    orc_offset(p, ng, start);
     uint32_t valid = 0;
     v3 s = mk(sun[0], sun[1], sun[2]);
     orc_ray* r = &out[0];
-    r->o[0] = start.x; r->o[1] = start.y; r->o[2] = start.z;
+    memcpy(r->o, start, sizeof start);
     r->d[0] = s.x; r->d[1] = s.y; r->d[2] = s.z;
     r->tmin = 0.0f; r->tmax = 1e20f;
-    if (dot3(ng, s) > 0.0f) valid |= 1u;
-    /* reference_path_tracer.wgsl:160-167 (Duff et al. 2017) and :466-470. */
-    float sg = ng.z >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (sg + ng.z);
-    float b = ng.x * ng.y * a;
-    v3 c0 = mk(1.0f + sg * ng.x * ng.x * a, sg * b, -sg * ng.x);
-    v3 c1 = mk(b, sg + ng.y * ng.y * a, -ng.y);
-    v3 c2 = ng;
+    if (dot3(ngv, s) > 0.0f) valid |= 1u;
     uint32_t state = pixel ^ (seed * 0x9E3779B9u);
     for (uint32_t gi = 0; gi < n_gi; ++gi) {
-        float rr = sqrtf(rnd(&state));
-        float phi = 6.2831855f * rnd(&state);
-        v3 l = mk(rr * cosf(phi), rr * sinf(phi), sqrtf(mx(0.0f, 1.0f - rr * rr)));
-        v3 wi = mk(c0.x * l.x + c1.x * l.y + c2.x * l.z, c0.y * l.x + c1.y * l.y + c2.y * l.z,
-                   c0.z * l.x + c1.z * l.y + c2.z * l.z);
+        float u1 = rnd(&state);
+        float u2 = rnd(&state);
+        float wi[3];
+        orc_cosine_dir(ng, u1, u2, wi);
         orc_ray* q = &out[1 + gi];
-        q->o[0] = start.x; q->o[1] = start.y; q->o[2] = start.z;
-        q->d[0] = wi.x; q->d[1] = wi.y; q->d[2] = wi.z;
+        memcpy(q->o, start, sizeof start);
+        memcpy(q->d, wi, sizeof wi);
         q->tmin = 0.0f; q->tmax = 1e20f;
-        if (dot3(ng, wi) > 0.0f) valid |= 1u << (1u + gi);
+        if (dot3(ngv, mk(wi[0], wi[1], wi[2])) > 0.0f) valid |= 1u << (1u + gi);
     }
     return valid;
+}
+
+/* ------------------------------------------------------ hit records */
+/* The 16-byte hit record vec4<u32>(placement + 1, primitive, bits(u), bits(v)),
+   zero on a miss (shaders/primary_rays.wgsl:19; RsHit.record,
+   shaders/reflections_software_bvh4.wgsl:42, 90), with gi_hit_identity,
+   gi_hit_primitive and gi_hit_placement as shaders/gi_rays_software.wgsl:297-316
+   write them: the identity table is scanned in full and its last pair whose
+   first position is <= tri.x wins. A hit on a record past header word 4 (none
+   exists) gives zeros. Reflection hits carry slot and primitive themselves. */
+void orc_hit_records(const uint32_t* g, size_t n, int reflection, const orc_hit* hits, uint32_t count,
+                     uint32_t* records, uint32_t* identities) {
+    for (uint32_t k = 0; k < count; ++k) {
+        const orc_hit* h = &hits[k];
+        uint32_t rec[4] = {0, 0, 0, 0}, identity = 0;
+        if (h->flags & 1u) {
+            if (reflection) {
+                rec[0] = h->slot + 1u;
+                rec[1] = h->tri[0];
+                rec[2] = fb(h->u);
+                rec[3] = fb(h->v);
+            } else if (n >= 16 && h->inst < g[4] && 16 + (uint64_t)h->inst * IW + IW <= n) {
+                uint64_t r = 16 + (uint64_t)h->inst * IW;
+                int opaque = h->inst < g[0];
+                rec[0] = g[r + 29] + 1u;                                          /* gi_hit_placement */
+                rec[1] = (!opaque || g[r + 28] == 0u) ? 0xffffffffu : h->tri[0];  /* gi_hit_primitive */
+                rec[2] = fb(h->u);
+                rec[3] = fb(h->v);
+                if (opaque) {                                                      /* gi_hit_identity */
+                    identity = g[r + 24];
+                    if (identity == 0u) {
+                        uint64_t map = g[r + 25];
+                        uint32_t pairs = g[r + 26];
+                        for (uint32_t i = 0; i < pairs; ++i)
+                            if (map + 2ull * i + 1 < n && g[map + 2ull * i] <= h->tri[0]) identity = g[map + 2ull * i + 1];
+                    }
+                }
+            }
+        }
+        if (records) memcpy(&records[4ull * k], rec, sizeof rec);
+        if (identities) identities[k] = identity;
+    }
 }
 
 /* ------------------------------------------------------ GI cache rays */

@@ -23,7 +23,12 @@
  *                        on one mesh's runtime records.
  *   orc_primary_ray      shaders/primary_rays.wgsl:50-66 (primary_ray).
  *   orc_secondary_rays   shaders/reference_path_tracer.wgsl:149-167, 237-256,
- *                        466-470 (pcg, rand, basis, offset, cosine bounce).
+ *                        466-470 (pcg, rand, basis, offset, cosine bounce),
+ *                        built from orc_pcg, orc_basis, orc_offset and
+ *                        orc_cosine_dir, each exported so it can be pinned alone.
+ *   orc_hit_record       shaders/primary_rays.wgsl:19 (the 16-byte hit record)
+ *                        and gi_hit_identity / gi_hit_primitive /
+ *                        gi_hit_placement (shaders/gi_rays_software.wgsl:297-316).
  *
  * Arithmetic convention (fixed here because WGSL leaves it to the driver):
  * no contraction (built with -ffp-contract=off), IEEE division and square
@@ -118,6 +123,21 @@ void orc_primary_ray(const float inverse[16], uint32_t px, uint32_t py, uint32_t
 uint32_t orc_secondary_rays(const orc_ray* primary, const orc_hit* hit, const float sun[3],
                             uint32_t pixel, uint32_t seed, uint32_t n_gi, orc_ray* out);
 uint32_t orc_pcg(uint32_t v);
+void orc_pcg_batch(const uint32_t* in, uint32_t count, uint32_t* out);
+/* rand() `count` times from `state`: out[i] in [0, 1), a multiple of 2^-24. */
+void orc_rand_sequence(uint32_t state, uint32_t count, float* out);
+/* basis(n): columns c0, c1, c2 = n at out[0..2], out[3..5], out[6..8]. */
+void orc_basis(const float n[3], float out[9]);
+/* The cosine-weighted bounce direction about n for the random pair (u1, u2). */
+void orc_cosine_dir(const float n[3], float u1, float u2, float out[3]);
+/* offset(p, ng): the shadow and bounce origin. */
+void orc_offset(const float p[3], const float ng[3], float out[3]);
+
+/* Hit records (4 words per hit, may be NULL) and gi_hit_identity (may be NULL)
+   of `count` hits over gi_geometry words, or rl_geometry hits when
+   `reflection` is nonzero. */
+void orc_hit_records(const uint32_t* g, size_t n, int reflection, const orc_hit* hits, uint32_t count,
+                     uint32_t* records, uint32_t* identities);
 
 /* The GI radiance cache's rays from its table words (shaders/gi_cells.wgsl
    layout; tests/ara_backend/gi_cache_reference.rs restates it on the CPU):

@@ -1622,6 +1622,12 @@ extern "C" rq_status rq_scene_build_reflection_gpu(const rq_mesh* meshes, uint32
             set_error("rq_scene_build_reflection_gpu: instance names a missing mesh");
             return RQ_ERR_INVALID_INPUT;
         }
+    for (uint32_t m = 0; m < mesh_count; ++m)
+        if (meshes[m].index_count / 3u >= RS_MAX_MESH_TRIS) {
+            set_error("rq_scene_build_reflection_gpu: a mesh of 2^27 or more triangles (the leaf tag holds the "
+                      "first triangle in 27 bits)");
+            return RQ_ERR_INVALID_INPUT;
+        }
     // The trees: the GI build of the meshes alone, every mesh with its primitive table.
     std::vector<rq_mesh> with_prims(meshes, meshes + mesh_count);
     for (auto& m : with_prims) m.keep_primitives = 1;

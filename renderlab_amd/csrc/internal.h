@@ -35,6 +35,8 @@ constexpr uint32_t RS_RECORD_WORDS = 24;
 constexpr uint32_t RS_NODE_WORDS = 16;
 constexpr uint32_t RS_TRI_WORDS = 12;
 constexpr uint32_t RS_TOP_NODE_WORDS = 32;
+// RS leaf tags are LEAF | first triangle << 4 | count: a mesh's triangles must stay below 2^27.
+constexpr uint32_t RS_MAX_MESH_TRIS = 1u << 27;
 // Scene kinds: the GI rays' gi_geometry words, or the reflection rays' rl_geometry words.
 constexpr int RQ_SCENE_GI = 0;
 constexpr int RQ_SCENE_REFLECTION = 1;

@@ -391,7 +391,13 @@ bool check_mesh_tree(const std::vector<uint32_t>& w, const MeshView& m, uint32_t
                 err = "a leaf id past the mesh's leaves";
                 return false;
             }
-            const uint64_t first = w[m.leaves + 2ull * l] / 3u * 3u, count = w[m.leaves + 2ull * l + 1] / 3u * 3u;
+            const uint64_t first = w[m.leaves + 2ull * l], count = w[m.leaves + 2ull * l + 1];
+            if (first % 3u || count % 3u) {
+                // gi_rays_software.wgsl:127 reads indices first + k; every builder writes whole
+                // triangles, and the device layouts step by triangles
+                err = "a leaf's index offset or count is not a multiple of 3";
+                return false;
+            }
             if (m.indices + first + count > w.size()) {
                 err = "a leaf's indices past the end of the buffer";
                 return false;
@@ -481,6 +487,30 @@ extern "C" rq_status rq_scene_from_gi_words(const uint32_t* w, size_t n, rq_scen
             !check_mesh_tree(scene->words, m, root, err)) {
             delete scene;
             set_error("rq_scene_from_gi_words: instance " + std::to_string(i) + ": " + err);
+            return RQ_ERR_INVALID_INPUT;
+        }
+        // Words 24-29 read after a hit (gi_rays_software.wgsl:297-316, the closest-hit
+        // primitive remap :147-148): the primitive table holds one word per triangle, and a
+        // merged world's identity table (word 24 zero) holds word-26 pairs in position order.
+        const size_t r = HEADER_WORDS + (size_t)i * INSTANCE_WORDS;
+        const uint64_t prim_base = scene->words[r + 28];
+        if (prim_base != 0 && prim_base + m.index_count / 3 > n) {
+            err = "instance " + std::to_string(i) + ": its primitive table (word 28) runs past the buffer";
+        } else if (scene->words[r + 24] == 0 && scene->words[r + 26] != 0) {
+            const uint64_t map = scene->words[r + 25], pairs = scene->words[r + 26];
+            if (map + 2 * pairs > n) {
+                err = "instance " + std::to_string(i) + ": its identity table (words 25-26) runs past the buffer";
+            } else {
+                for (uint64_t k = 1; k < pairs; ++k)
+                    if (scene->words[map + 2 * k] < scene->words[map + 2 * k - 2]) {
+                        err = "instance " + std::to_string(i) + ": its identity table is not in position order";
+                        break;
+                    }
+            }
+        }
+        if (!err.empty()) {
+            delete scene;
+            set_error("rq_scene_from_gi_words: " + err);
             return RQ_ERR_INVALID_INPUT;
         }
         tris += m.index_count / 3;

@@ -131,6 +131,12 @@ extern "C" rq_status rq_scene_build_reflection(const rq_mesh* meshes, uint32_t m
             set_error("rq_scene_build_reflection: instance names a missing mesh");
             return RQ_ERR_INVALID_INPUT;
         }
+    for (uint32_t m = 0; m < mesh_count; ++m)
+        if (meshes[m].index_count / 3u >= RS_MAX_MESH_TRIS) {
+            set_error("rq_scene_build_reflection: a mesh of 2^27 or more triangles (the leaf tag holds the first "
+                      "triangle in 27 bits)");
+            return RQ_ERR_INVALID_INPUT;
+        }
     std::vector<RsBlas> blas(mesh_count);
     std::vector<std::string> errs(mesh_count);
     std::vector<char> ok(mesh_count, 1);

@@ -122,6 +122,8 @@ _sig("rq_near_field_info", _i, _P, _P, _P, _P, _P)
 _sig("rq_near_field_destroy", None, _P)
 _sig("rq_scene_build_cooked", _i, _P, _u32, _P, _u32, _i, ctypes.POINTER(_P))
 _sig("rq_near_blocked", _i, _P, _P, _u32, _P, _P)
+_sig("rq_gi_hit_decode", _i, _P, _P, _P, _P)
+_sig("rq_hit_records", _i, _P, _P, _u32, _P, _P, _P)
 _sig("rq_scene_gi_words", _i, _P, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_scene_stats", _i, _P, _P)
 _sig("rq_scene_upload", _i, _P, _u32)
@@ -756,6 +758,20 @@ class Scene:
         return dict(zip(["bytes", "iterations", "node_steps", "tri_steps", "inst_steps", "refills", "live_lanes",
                          "want_node", "want_tri", "want_inst"],
                         (int(x) for x in out)))
+
+    def hit_record(self, hit):
+        """The reference's buffer form of one host hit record (HIT_DTYPE): (record (4,) u32: placement + 1,
+        primitive, bits(u), bits(v); gi_hit_identity)."""
+        h = np.ascontiguousarray(np.asarray(hit, HIT_DTYPE).reshape(1))
+        rec = np.zeros(4, np.uint32)
+        ident = ctypes.c_uint32(0)
+        check(lib.rq_gi_hit_decode(self._h, h.ctypes.data_as(_P), rec.ctypes.data_as(_P), ctypes.byref(ident)))
+        return rec, int(ident.value)
+
+    def hit_records(self, hits, count, records=None, identities=None, stream=None):
+        """rq_hit_records: device hits -> device records (16 B each) and identities (4 B each)."""
+        check(lib.rq_hit_records(self._h, _dptr(hits), int(count), _dptr(records), _dptr(identities),
+                                 stream.ptr if stream else None))
 
     def verify_fp64(self, rays, hits, count, stream=None):
         out = np.zeros(8, np.float64)

@@ -39,7 +39,7 @@ def gpu_workload(scene, kind, seed, width, height, n_gi, layout="plain"):
     p = prim.download(rq.RAY_DTYPE, n)
     h = hits.download(rq.HIT_DTYPE, n)
     s = shadow.download(rq.RAY_DTYPE, int(c[0]))
-    g = gi.download(rq.RAY_DTYPE, n_gi * n).reshape(max(n_gi, 1), n)
+    g = gi.download(rq.RAY_DTYPE, n_gi * n).reshape(n_gi, n) if n_gi else None
     gis = [g[b, : int(c[1 + b])] for b in range(n_gi)]
     return inv, p, h, s, gis
 

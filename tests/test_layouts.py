@@ -262,6 +262,31 @@ def test_external_words_must_hold_finite_trees_inside_their_arrays():
     bad(lambda w: w.__setitem__(nodes_at + 6, LEAF | 9), "past the mesh's leaves")
     bad(lambda w: w.__setitem__(leaves_at + 2 * 4, 3 * 400), "past the end of the buffer")
     bad(lambda w: w.__setitem__(idx_at + 5, 27), "past the mesh's vertices")
+    # a leaf whose index offset or count is not whole triangles (the reference reads indices
+    # first + k, the device layouts step by triangles: such words would trace differently)
+    bad(lambda w: w.__setitem__(leaves_at + 2 * 4, 3 * 4 + 1), "multiple of 3")
+    bad(lambda w: w.__setitem__(leaves_at + 2 * 4 + 1, 2), "multiple of 3")
+    # the primitive table (word 28) and a merged world's identity table (words 24-26) are read
+    # after hits; both must lie inside the buffer, the identity table in position order
+    bad(lambda w: w.__setitem__(r + 28, len(base) - 3), "primitive table")
+    bad(lambda w: w.__setitem__(r + 28, 0xFFFFFFF0), "primitive table")
+    ok = base.copy()
+    ok[r + 28] = len(base) - 9  # a table of the mesh's 9 triangles ending at the last word
+    rq.Scene.from_gi_words(ok)
+
+    def table(first, n):
+        def edit(w):
+            w[r + 24], w[r + 25], w[r + 26] = 0, first, n
+        return edit
+    bad(table(len(base) - 4, 3), "identity table")
+
+    def descending(w):
+        table(idx_at, 2)(w)
+        w[idx_at], w[idx_at + 2] = 5, 1  # pairs (5, .) then (1, .); still valid vertex indices
+    bad(descending, "position order")
+    ok = base.copy()
+    table(len(base) - 6, 3)(ok)  # the last index words as pairs (21, 22), (23, 24), (25, 26)
+    rq.Scene.from_gi_words(ok)
     two = rq.Scene.build([(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.arange(3, dtype=np.uint32), 0)],
                          [dict(model=np.eye(4, dtype=np.float32), mesh=0)] * 50).words
     rq.Scene.from_gi_words(two)
