@@ -354,6 +354,20 @@ rq_status rq_hit_records(const rq_scene* scene, const rq_hit* hits, uint32_t cou
 rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                    uint32_t mode, rq_hit* hits, uint32_t* vis_bits, void* stream);
 
+/* rq_trace without allocation. No counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc). The ray cursor and
+   the stack spill area come from one static device array of
+   rq_trace_noalloc_scratch_bytes() = 112 MiB + 256 B per GPU (held until the
+   process ends) instead of the stream's own memory, and the persistent grid
+   is capped to the 131,072 lanes that array holds; the ray count is not
+   limited. RQ_REORDER is refused (RQ_ERR_UNSUPPORTED): its sort space grows
+   with the ray count. Results equal rq_trace's. Calls that overlap on one GPU
+   (two streams or host threads) share the array and race: order them. */
+rq_status rq_trace_noalloc(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
+                           rq_hit* hits, uint32_t* vis_bits, void* stream);
+size_t rq_trace_noalloc_scratch_bytes(void);
+
 /* rq_trace with the ray count read on the device at launch time:
    min(*count_dev, max_count) rays (count_dev: a device word written by an
    earlier launch on the same stream, such as rq_gen_secondary's counts), so a

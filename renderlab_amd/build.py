@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "librq.so")
 BUILD = os.path.join(HERE, "_build")
 
 HOST_SOURCES = ["api.cpp", "builder.cpp", "toplevel.cpp", "pack.cpp", "layouts.cpp", "scenegen.cpp", "reflect.cpp", "assets.cpp"]
-DEVICE_SOURCES = ["trace.hip", "serve.hip", "sort.hip", "refit.hip", "gpubuild.hip", "gicache.hip", "hits.hip"]
+DEVICE_SOURCES = ["trace.hip", "serve.hip", "sort.hip", "refit.hip", "gpubuild.hip", "gicache.hip", "hits.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RQ_OFFLOAD_ARCH", "gfx950")

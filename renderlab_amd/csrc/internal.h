@@ -206,6 +206,20 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
 
 void set_error(const std::string& msg);
 
+// The traversal's per-lane stack spill area: words per lane of a persistent grid.
+constexpr uint32_t TRACE_SPILL_WORDS = 224;
+// Device memory a trace launch uses in place of its stream's own (no allocation):
+// the ray cursor (one word, cleared by the launch) and the spill area of `lanes` lanes.
+struct FixedScratch {
+    uint32_t* counter;
+    uint32_t* spill;
+    size_t lanes;
+};
+// rq_trace / rq_trace_indirect / rq_trace_count_bytes, and rq_trace_noalloc with `fixed`.
+rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
+                       rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
+                       const uint32_t* count_ptr = nullptr, const FixedScratch* fixed = nullptr);
+
 }  // namespace rq
 
 struct rq_bvh {

@@ -37,7 +37,7 @@ constexpr float FMAXV = 3.4028235e38f;
 // 112 each for CW8 (7 per level of at most 16; an 8-wide tree is never deeper
 // than the 4-wide tree it is collapsed from, and the relayout rejects one
 // deeper than 16 levels).
-constexpr uint32_t SPILL_WORDS = 224;
+constexpr uint32_t SPILL_WORDS = TRACE_SPILL_WORDS;
 constexpr uint32_t STACK_CAP = 48;
 constexpr uint32_t CW8_LEVELS = 16;  // the CW8 relayout rejects deeper trees
 constexpr uint32_t CW8_CAP = 7 * CW8_LEVELS;
@@ -1698,14 +1698,25 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
     return v;
 }
 
-rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
-                       rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
-                       const uint32_t* count_ptr = nullptr) {
+}  // namespace
+}  // namespace rq
+
+// `fixed`, when set, is device memory the launch uses instead of the stream's
+// own (no allocation): the ray cursor, then the stack spill area of at most
+// fixed->lanes lanes; the grid is capped to that many lanes.
+rq_status rq::launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
+                           rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
+                           const uint32_t* count_ptr, const rq::FixedScratch* fixed) {
     const bool reorder = (mode & RQ_REORDER) != 0;
     mode &= ~RQ_REORDER;
     if (!scene || !layout_ok(scene, layout) || mode > 2 || (count && !rays)) {
         set_error("rq_trace: bad argument (or a layout of the other scene kind)");
         return RQ_ERR_INVALID_INPUT;
+    }
+    if (fixed && (reorder || bytes || count_ptr)) {
+        set_error("rq_trace_noalloc: RQ_REORDER is not available without allocation (its sort space grows with "
+                  "the ray count)");
+        return RQ_ERR_UNSUPPORTED;
     }
     if (!scene->dev[layout].ready) {
         set_error("rq_trace: layout not uploaded (rq_scene_upload)");
@@ -1742,10 +1753,24 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     uint64_t grid = (uint64_t)g_ctx.cus * per_cu;
     uint64_t need = ((uint64_t)count + k->block - 1) / k->block;
     grid = std::max<uint64_t>(1, std::min(grid, need));
-    StreamRes* res = nullptr;
-    if ((st = res_for(s, &res)) != RQ_OK) return st;
-    if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
-    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
+    uint32_t* counter = nullptr;
+    uint32_t* spill = nullptr;
+    if (fixed) {
+        if (fixed->lanes < k->block) {
+            set_error("rq_trace_noalloc: the fixed scratch holds fewer lanes than one workgroup");
+            return RQ_ERR_UNSUPPORTED;
+        }
+        grid = std::min<uint64_t>(grid, fixed->lanes / k->block);
+        counter = fixed->counter;
+        spill = fixed->spill;
+    } else {
+        StreamRes* res = nullptr;
+        if ((st = res_for(s, &res)) != RQ_OK) return st;
+        if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
+        counter = res->counter;
+        spill = res->spill;
+    }
+    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, counter, 0u, (uint64_t)1);
 
     dev::TraceArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1754,8 +1779,8 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     a.hits = hits;
     a.vis = vis_bits;
     a.count = count;
-    a.counter = res->counter;
-    a.spill = res->spill;
+    a.counter = counter;
+    a.spill = spill;
     a.count_ptr = count_ptr;
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
@@ -1768,6 +1793,8 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
         size_t temp = 0;
         HIP_TRY(ray_order(rays, count, count_ptr, scene->bounds, nullptr, nullptr, nullptr, nullptr, nullptr, temp, s),
                 "ray order (size)");
+        StreamRes* res = nullptr;
+        if ((st = res_for(s, &res)) != RQ_OK) return st;
         if ((st = ensure_order(*res, 4 * arr + temp)) != RQ_OK) return st;
         char* b = reinterpret_cast<char*>(res->order);
         uint32_t* keys = reinterpret_cast<uint32_t*>(b);
@@ -1782,8 +1809,6 @@ rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, ui
     HIP_TRY(hipGetLastError(), "trace launch");
     return RQ_OK;
 }
-}  // namespace
-}  // namespace rq
 
 extern "C" rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
                               rq_hit* hits, uint32_t* vis_bits, void* stream) {

@@ -132,6 +132,8 @@ _sig("rq_scene_destroy", None, _P)
 _sig("rq_scene_layout_image", _i, _P, _i, _u32, ctypes.POINTER(_P), ctypes.POINTER(_sz))
 _sig("rq_trace", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
 _sig("rq_trace_indirect", _i, _P, _i, _P, _P, _u32, _u32, _P, _P, _P)
+_sig("rq_trace_noalloc", _i, _P, _i, _P, _u32, _u32, _P, _P, _P)
+_sig("rq_trace_noalloc_scratch_bytes", _sz)
 _sig("rq_trace_batches", _i, _P, _u32, _i, _P, _u32, _P)
 _sig("rq_trace_ranges", _i, _P, _i, _P, _u32, _P)
 _sig("rq_get_batch_tuning", _i, _i, _P)
@@ -707,6 +709,11 @@ class Scene:
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
         check(lib.rq_trace(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits), p(vis),
                            stream.ptr if stream else None))
+
+    def trace_noalloc(self, layout, rays, count, mode, hits=None, vis=None, stream=None):
+        """rq_trace_noalloc: rq_trace with the cursor and stack spill from a static device array."""
+        check(lib.rq_trace_noalloc(self._h, LAYOUTS[layout], _dptr(rays), int(count), int(mode), _dptr(hits),
+                                   _dptr(vis), stream.ptr if stream else None))
 
     def trace_indirect(self, layout, rays, count_dev, max_count, mode, hits=None, vis=None, stream=None):
         """rq_trace with the ray count read from device memory (count_dev: device pointer)."""
