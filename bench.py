@@ -5,12 +5,14 @@ the device: primary rays -> closest hit -> sun-shadow and cosine GI rays from
 every hit -> any-hit shadow visibility and closest-hit GI. `value` is the
 rays traced per second over the K timed steps (inputs resident in HBM).
 
-Workload (default): 1M-triangle seeded scene (5,000 lognormal-edge clusters in
-a 200 m cube plus ground), 1920x1080 primary + 1 sun shadow + 1 GI bounce per
-primary hit, traced through the 8-wide quantized layout; the plain FP32 layout
-is timed on the same rays for the quantized-vs-plain ratio.
+Workload (default, the metric's precision form): 4M-triangle seeded scene
+(5,000 lognormal-edge clusters in a 200 m cube plus ground), 3840x2160 primary
++ 1 sun shadow + 2 cosine GI bounces per primary hit, traced through the 8-wide
+quantized layout; the plain FP32 layout is timed on the same rays for the
+quantized-vs-plain ratio, and both trace one frame's identical rays for the
+hit-ID agreement. The 1M-triangle 1080p frame (c1) is reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c0]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c1|c0]
 """
 import argparse
 import json
@@ -491,15 +493,175 @@ def cpu_baseline(scene_words, rays_sets, budget_s=12.0):
     return total / secs / 1e6, threads, total
 
 
+def build_scene(rq, cfg, compare_builds):
+    """The configuration's seeded scene built on the GPU; with compare_builds, also on the host (timed,
+    words checked equal) and a first GPU build (which pins the host staging buffer)."""
+    kind, meshes, tris = CONFIGS[cfg][:3]
+    build = None
+    if compare_builds:
+        t0 = time.perf_counter()
+        host_scene = rq.Scene.generate(kind, 1, meshes, tris)
+        log(f"scene {cfg}: {host_scene.stats()} synthesized and built on the host in {time.perf_counter() - t0:.2f}s")
+        first = rq.Scene.generate(kind, 1, meshes, tris, gpu=True)
+        gb_first = first.stats()["build_us"]
+        del first
+    scene = rq.Scene.generate(kind, 1, meshes, tris, gpu=True)
+    if compare_builds:
+        same = np.array_equal(scene.words, host_scene.words)
+        if not same:
+            raise SystemExit("GPU-built scene words differ from the host build")
+        hb, gb = host_scene.stats()["build_us"], scene.stats()["build_us"]
+        build = {"host_build_ms": round(hb / 1e3, 2), "gpu_build_ms": round(gb / 1e3, 2),
+                 "speedup": round(hb / max(gb, 1), 2), "gpu_build_first_ms": round(gb_first / 1e3, 2),
+                 "words_equal": bool(same), "host_threads": os.cpu_count(),
+                 "note": "rq_scene_build (host BVH builder, threads over meshes) vs rq_scene_build_gpu, each timed "
+                         "inside the call (inputs in host memory, packed words out); mesh synthesis excluded; "
+                         "gpu_build_ms is a process's second build, gpu_build_first_ms its first (which also pins "
+                         "the host staging buffer)"}
+        del host_scene
+        log(f"GPU build of the same scene: {build}")
+    return scene, build
+
+
+def time_config(rq, scene, cfg, layouts, args, stream):
+    """Autotune and time each layout on the configuration's frame; returns (frame, rays per step,
+    {layout: (ms/step, wall s, per-launch ms)}, {layout: tuning})."""
+    kind, _, _, w, h, n_gi, _ = CONFIGS[cfg]
+    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary",
+                  schedule=args.secondary, tiled=args.primary_order == "tiles")
+    rays = frame.prime(layouts[0])
+    log(f"{cfg}: rays per step {rays} (primary {frame.n}, secondary {[int(x) for x in frame.counts_host]})")
+    results, tuned = {}, {}
+    for layout in layouts:
+        frame.prime(layout)
+        tr = frame.tuning_ranges()
+        if tr is not None and not args.no_tune:
+            from renderlab_amd import autotune
+            t = autotune.tune_ranges(scene, layout, tr, stream=stream)
+            tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu", "speculate")}
+            log(f"{cfg} {layout}: autotuned {t}")
+            frame.prime(layout)
+        ms, wall, per = time_layout(rq, frame, layout, args.steps, args.warmup)
+        results[layout] = (ms, wall, per)
+        log(f"{cfg} {layout}: {ms:.3f} ms/step, {rays / ms / 1e3:.1f} Mrays/s, per launch {per}")
+    return frame, rays, results, tuned
+
+
+def dominant_launch(scene, frame, layout, per):
+    """The longest trace launch of a step, its rays and the kernel's step statistics on them (counted
+    pass by pass: a ray's traversal and the bytes it moves do not depend on which launch carries it)."""
+    names = [k for k in per if k in ("primary", "shadow", "secondary", "frame") or k.startswith("gi")]
+    dom = max(names, key=lambda k: per[k])
+    parts = frame.launch_parts(dom)
+    tot = None
+    for ptr, n, mode in parts:
+        st1 = scene.count_stats(layout, ptr, n, mode)
+        tot = st1 if tot is None else {k: tot[k] + st1[k] for k in tot}
+    return dom, sum(n for _, n, _ in parts), tot
+
+
+def compare_hits(q, p, rel=1e-5):
+    """Quantized-layout hits q against plain-layout hits p of the same rays: hit-ID agreement (instance and
+    BVH-order triangle), ties (|dt|/t <= rel), genuine mismatches, and t / barycentric differences where both
+    found the same triangle."""
+    hq, hp = (q["flags"] & 1) == 1, (p["flags"] & 1) == 1
+    same_tri = hq & hp & (q["inst"] == p["inst"]) & (q["slot"] == p["slot"])
+    diff = (hq != hp) | (hp & ~same_tri)
+    dt = np.abs(q["t"].astype(np.float64) - p["t"]) / np.maximum(np.abs(p["t"].astype(np.float64)), 1e-30)
+    tie = diff & hq & hp & (dt <= rel)
+    out = {"rays": int(len(p)), "hits": int(hp.sum()), "differing_ids": int(diff.sum()), "ties": int(tie.sum()),
+           "mismatches": int((diff & ~tie).sum())}
+    if same_tri.any():
+        out["max_t_rel_same_triangle"] = float(dt[same_tri].max())
+        out["max_bary_same_triangle"] = float(max(np.abs(q["u"][same_tri] - p["u"][same_tri]).max(),
+                                                  np.abs(q["v"][same_tri] - p["v"][same_tri]).max()))
+    return out
+
+
+def hit_id_agreement(rq, scene, frame, layout, base="plain"):
+    """One frame's rays through both layouts: the primary rays, then shadow and GI rays generated from the
+    base layout's primary hits, so both layouts trace identical rays. Closest hits are compared by hit ID,
+    shadow rays by visibility bit."""
+    n, n_gi = frame.n, frame.n_gi
+    rq.gen_primary(frame.inv, frame.w, frame.h, frame.prim, tiled=frame.tiled)
+    hb, hq = rq.DeviceBuffer(n * 48), rq.DeviceBuffer(n * 48)
+    scene.trace(base, frame.prim, n, rq.CLOSEST, hits=hb)
+    scene.trace(layout, frame.prim, n, rq.CLOSEST, hits=hq)
+    rq.gen_secondary(frame.prim, hb, n, SUN, frame.seed, n_gi, frame.shadow, frame.gi if n_gi else None,
+                     frame.counts)
+    rq.synchronize()
+    counts = frame.counts.download(np.uint32)
+    passes = {"primary": compare_hits(hq.download(rq.HIT_DTYPE, n), hb.download(rq.HIT_DTYPE, n))}
+    for b in range(n_gi):
+        m = int(counts[1 + b])
+        rays = frame.gi.view(b * n * 32, m * 32)
+        scene.trace(base, rays, m, rq.CLOSEST, hits=hb)
+        scene.trace(layout, rays, m, rq.CLOSEST, hits=hq)
+        rq.synchronize()
+        passes[f"gi{b}"] = compare_hits(hq.download(rq.HIT_DTYPE, m), hb.download(rq.HIT_DTYPE, m))
+    m = int(counts[0])
+    vb, vq = rq.DeviceBuffer(((m + 31) // 32) * 4 + 4), rq.DeviceBuffer(((m + 31) // 32) * 4 + 4)
+    scene.trace(base, frame.shadow, m, rq.ANY, vis=vb)
+    scene.trace(layout, frame.shadow, m, rq.ANY, vis=vq)
+    rq.synchronize()
+    words = (m + 31) // 32
+    xor = vb.download(np.uint32, words) ^ vq.download(np.uint32, words)
+    shadow_diff = int(np.unpackbits(xor.view(np.uint8)).sum())
+    closest = list(passes.values())
+    out = {"layouts": f"{layout} vs {base}", "closest_rays": sum(p["rays"] for p in closest),
+           "differing_ids": sum(p["differing_ids"] for p in closest), "ties": sum(p["ties"] for p in closest),
+           "mismatches": sum(p["mismatches"] for p in closest), "shadow_rays": m,
+           "shadow_visibility_mismatches": shadow_diff,
+           "max_t_rel_same_triangle": max(p.get("max_t_rel_same_triangle", 0.0) for p in closest),
+           "max_bary_same_triangle": max(p.get("max_bary_same_triangle", 0.0) for p in closest),
+           "passes": passes,
+           "note": "hit ID = (instance, triangle); ties: a different triangle at |dt|/t <= 1e-5 (an equal-distance "
+                   "candidate reached in another visit order); mismatches: any other difference"}
+    return out
+
+
+def fp64_all(rq, scene, frame, layout):
+    """rq_verify_fp64 over every closest hit of one frame (primary and each GI bounce) in `layout`."""
+    n, n_gi = frame.n, frame.n_gi
+    rq.gen_primary(frame.inv, frame.w, frame.h, frame.prim, tiled=frame.tiled)
+    scene.trace(layout, frame.prim, n, rq.CLOSEST, hits=frame.hits)
+    rq.gen_secondary(frame.prim, frame.hits, n, SUN, frame.seed, n_gi, frame.shadow, frame.gi if n_gi else None,
+                     frame.counts)
+    rq.synchronize()
+    counts = frame.counts.download(np.uint32)
+    frame.counts_host = counts  # the frame's secondary buffers now hold these rays
+    sets = [(frame.prim, frame.hits, n)]
+    for b in range(n_gi):
+        m = int(counts[1 + b])
+        rays, hits = frame.gi.view(b * n * 32, m * 32), frame.gi_hits.view(b * n * 48, m * 48)
+        scene.trace(layout, rays, m, rq.CLOSEST, hits=hits)
+        sets.append((rays, hits, m))
+    out = None
+    for rays, hits, m in sets:
+        v = scene.verify_fp64(rays, hits, m)
+        if out is None:
+            out = dict(v)
+        else:
+            for k in ("max_t_rel", "max_bary_abs", "max_t_scaled", "max_ulps_conditioned"):
+                out[k] = max(out[k], v[k])
+            for k in ("checked", "over_1e5", "conditioned"):
+                out[k] += v[k]
+    out["rays"] = "primary + " + ", ".join(f"GI bounce {b}" for b in range(n_gi)) if n_gi else "primary"
+    out["layout"] = layout
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--layout", default="cw8", choices=["ref", "plain", "cw8"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the plain layout")
+    ap.add_argument("--no-agreement", action="store_true", help="skip the hit-ID agreement check against plain")
+    ap.add_argument("--no-extra-c1", action="store_true", help="skip the 1M-triangle 1080p line (c1)")
     ap.add_argument("--batching", action="store_true", help="also run the 8-scene, 256-batch one-launch form")
     ap.add_argument("--no-serving", action="store_true", help="skip the short serving run")
     ap.add_argument("--serving-sweep", action="store_true", help="sweep Poisson rates 50..2000 jobs/s")
@@ -533,88 +695,30 @@ def main():
         del src, dst
     kind, meshes, tris, w, h, n_gi, desc = CONFIGS[args.config]
     rq.Scene.generate(1, 2, 20, 20000, gpu=True)  # load the build kernels before anything is timed
-    t0 = time.perf_counter()
-    host_scene = rq.Scene.generate(kind, 1, meshes, tris)
-    host_s = time.perf_counter() - t0
-    st = host_scene.stats()
-    log(f"scene {args.config}: {st} synthesized and built on the host in {host_s:.2f}s")
-    # the first GPU build of a process also pins its host staging buffer; the second is the steady state
-    first = rq.Scene.generate(kind, 1, meshes, tris, gpu=True)
-    gb_first = first.stats()["build_us"]
-    del first
-    t0 = time.perf_counter()
-    scene = rq.Scene.generate(kind, 1, meshes, tris, gpu=True)  # the same words, BVHs built on the GPU
-    gpu_s = time.perf_counter() - t0
-    same = np.array_equal(scene.words, host_scene.words)
-    if not same:
-        raise SystemExit("GPU-built scene words differ from the host build")
-    hb, gb = host_scene.stats()["build_us"], scene.stats()["build_us"]
-    build = {"host_build_ms": round(hb / 1e3, 2), "gpu_build_ms": round(gb / 1e3, 2), "speedup": round(hb / max(gb, 1), 2),
-             "gpu_build_first_ms": round(gb_first / 1e3, 2),
-             "words_equal": bool(same), "host_threads": os.cpu_count(),
-             "note": "rq_scene_build (host BVH builder, threads over meshes) vs rq_scene_build_gpu, each timed "
-                     "inside the call (inputs in host memory, packed words out); mesh synthesis excluded; "
-                     "gpu_build_ms is a process's second build, gpu_build_first_ms its first (which also pins "
-                     "the host staging buffer)"}
-    del host_scene
-    log(f"GPU build of the same scene: {gpu_s:.3f}s with synthesis; build only: {build}")
+    scene, build = build_scene(rq, args.config, compare_builds=True)
+    st = scene.stats()
     layouts = [args.layout] + ([] if args.no_compare or args.layout == "plain" else ["plain"])
     t0 = time.perf_counter()
     scene.upload(tuple(set(layouts) | {"plain", "ref"}))  # REF: original positions for the FP64 check
     log(f"uploaded {layouts} in {time.perf_counter() - t0:.1f}s; bytes " +
         ", ".join(f"{l}={scene.device_bytes(l)}" for l in set(layouts) | {"plain"}))
     stream = rq.Stream()
-    frame = Frame(rq, scene, kind, 1, w, h, n_gi, stream, reorder=args.reorder == "secondary",
-                  schedule=args.secondary, tiled=args.primary_order == "tiles")
-    rays = frame.prime(args.layout)
-    log(f"rays per step: {rays} (primary {frame.n}, secondary {list(frame.counts_host)})")
-
-    results, tuned = {}, {}
-    for layout in layouts:
-        frame.prime(layout)
-        tr = frame.tuning_ranges()
-        if tr is not None and not args.no_tune:
-            from renderlab_amd import autotune
-            t = autotune.tune_ranges(scene, layout, tr, stream=stream)
-            tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu", "speculate")}
-            log(f"{layout}: autotuned {t}")
-            frame.prime(layout)
-        ms, wall, per = time_layout(rq, frame, layout, args.steps, args.warmup)
-        results[layout] = (ms, wall, per)
-        log(f"{layout}: {ms:.3f} ms/step, {rays / ms / 1e3:.1f} Mrays/s, per launch {per}")
+    frame, rays, results, tuned = time_config(rq, scene, args.config, layouts, args, stream)
     ms, wall, per = results[args.layout]
     value = rays / (ms * 1e-3) / 1e6
 
-    # Dominant trace launch and its algorithmic bytes (counted on the same rays, pass by pass: a ray's
-    # traversal and the bytes it moves do not depend on which launch carries it).
-    trace_names = [k for k in per if k in ("primary", "shadow", "secondary", "frame") or k.startswith("gi")]
-    dom = max(trace_names, key=lambda k: per[k])
-    parts = frame.launch_parts(dom)
-    dom_rays = sum(n for _, n, _ in parts)
-
-    def counted(layout):
-        tot = None
-        for ptr, n, mode in parts:
-            st1 = scene.count_stats(layout, ptr, n, mode)
-            tot = st1 if tot is None else {k: tot[k] + st1[k] for k in tot}
-        return tot
-
-    counts = frame.counts_host
-    stats = counted(args.layout)
+    dom, dom_rays, stats = dominant_launch(scene, frame, args.layout, per)
     nbytes = stats["bytes"]
     lanes = 64 * stats["iterations"]
     log(f"dominant launch {dom}: {dom_rays} rays, steps/ray node {stats['node_steps'] / dom_rays:.2f} "
         f"tri-pair {stats['tri_steps'] / dom_rays:.2f} inst {stats['inst_steps'] / dom_rays:.2f}; lane use "
         f"{(stats['node_steps'] + stats['tri_steps'] + stats['inst_steps']) / max(lanes, 1):.3f} (pass by pass); "
-        f"iterations/wave-ray {stats['iterations'] * 64 / dom_rays:.1f}; per iteration: live lanes "
-        f"{stats['live_lanes'] / max(stats['iterations'], 1):.1f}, wanting node {stats['want_node'] / max(stats['iterations'], 1):.1f} "
-        f"tri {stats['want_tri'] / max(stats['iterations'], 1):.1f} inst {stats['want_inst'] / max(stats['iterations'], 1):.1f}")
+        f"per iteration: live lanes {stats['live_lanes'] / max(stats['iterations'], 1):.1f}, wanting node "
+        f"{stats['want_node'] / max(stats['iterations'], 1):.1f} tri {stats['want_tri'] / max(stats['iterations'], 1):.1f} "
+        f"inst {stats['want_inst'] / max(stats['iterations'], 1):.1f}")
     plain_stats = None
     if "plain" in results and args.layout != "plain":
-        plain_stats = counted("plain")
-        log(f"plain on the same launch: steps/ray node {plain_stats['node_steps'] / dom_rays:.2f} tri-pair "
-            f"{plain_stats['tri_steps'] / dom_rays:.2f} inst {plain_stats['inst_steps'] / dom_rays:.2f}; bytes/ray "
-            f"{plain_stats['bytes'] / dom_rays:.0f}; iterations/wave-ray {plain_stats['iterations'] * 64 / dom_rays:.1f}")
+        _, _, plain_stats = dominant_launch(scene, frame, "plain", per)
     achieved = nbytes / (per[dom] * 1e-3)
     # profile labels: primary, shadow, gi, ranges (the multi-range launch of the fused and pipelined schedules)
     kind_of = "gi" if dom.startswith("gi") else ("ranges" if dom in ("secondary", "frame") else dom)
@@ -627,20 +731,12 @@ def main():
                 "bytes_per_ray": round(nbytes / max(dom_rays, 1), 1), "launch_ms": round(per[dom], 4),
                 "lane_use": round((stats["node_steps"] + stats["tri_steps"] + stats["inst_steps"]) /
                                   max(64 * stats["iterations"], 1), 3),
-                "live_lanes": round(stats["live_lanes"] / max(64 * stats["iterations"], 1), 3)}
-
-    # FP64 verification of the GI hits (outside the timed region).
-    verify = None
-    if n_gi:
-        vr = rq.DeviceBuffer.__new__(rq.DeviceBuffer)
-        vr.ptr, vr.nbytes = rq._P(frame.gi.ptr.value), int(counts[1]) * 32
-        vh = rq.DeviceBuffer.__new__(rq.DeviceBuffer)
-        vh.ptr, vh.nbytes = rq._P(frame.gi_hits.ptr.value), int(counts[1]) * 48
-        verify = scene.verify_fp64(vr, vh, int(counts[1]))
-        vr.ptr = vh.ptr = None
+                "live_lanes": round(stats["live_lanes"] / max(64 * stats["iterations"], 1), 3),
+                "steps_per_ray": {k: round(stats[f"{k}_steps"] / max(dom_rays, 1), 2) for k in ("node", "tri", "inst")}}
 
     cpu = None
     if not args.no_cpu_baseline:
+        counts = frame.counts_host
         prim = frame.prim.download(rq.RAY_DTYPE, frame.n)
         sh = frame.shadow.download(rq.RAY_DTYPE, int(counts[0]))
         sets = [(0, prim), (1, sh)]
@@ -668,17 +764,25 @@ def main():
     if "plain" in results and args.layout != "plain":
         pm = results["plain"][0]
         line["plain_f32"] = {"value": round(rays / (pm * 1e-3) / 1e6, 2), "ms_per_step": round(pm, 4),
-                             "quantized_speedup": round(pm / ms, 3),
+                             "quantized_speedup": round(pm / ms, 3), "tuning": tuned.get("plain"),
                              "per_launch_ms": {k: round(v, 4) for k, v in results["plain"][2].items()}}
         if plain_stats:
             line["plain_f32"]["dominant_steps_per_ray"] = {
                 k: round(plain_stats[f"{k}_steps"] / max(dom_rays, 1), 2) for k in ("node", "tri", "inst")}
-        line["roofline"]["steps_per_ray"] = {k: round(stats[f"{k}_steps"] / max(dom_rays, 1), 2)
-                                             for k in ("node", "tri", "inst")}
-    if verify:
-        line["fp64_verify"] = verify
+            line["plain_f32"]["dominant_lane_use"] = round(
+                (plain_stats["node_steps"] + plain_stats["tri_steps"] + plain_stats["inst_steps"]) /
+                max(64 * plain_stats["iterations"], 1), 3)
+        if not args.no_agreement:
+            line["hit_id_agreement"] = hit_id_agreement(rq, scene, frame, args.layout)
+            log(f"hit-ID agreement: {line['hit_id_agreement']}")
+    # FP64 re-intersection of every closest hit of one frame (outside the timed region).
+    line["fp64_verify"] = fp64_all(rq, scene, frame, args.layout)
+    log(f"FP64 verification: {line['fp64_verify']}")
     line["per_launch_ms"] = {k: round(v, 4) for k, v in per.items()}
     line["scene_build"] = build
+    if args.config != "c1" and not args.no_extra_c1:
+        line["c1"] = run_extra(rq, "c1", layouts, args)
+        log(f"c1: {line['c1']}")
     if not args.no_reflection:
         line["reflection_rs"] = run_reflection(rq, kind, meshes, tris, frame)
         log(f"reflection rays (rs_trace, one launch of primary + GI + shadow): {line['reflection_rs']}")
@@ -702,6 +806,26 @@ def main():
         line["batching_c3"] = run_batching(rq, args.layout)
         log(f"batching: {line['batching_c3']}")
     print(json.dumps(line), flush=True)
+
+
+def run_extra(rq, cfg, layouts, args):
+    """Another configuration's frame, timed the same way (its own scene, tuning and rays)."""
+    scene, _ = build_scene(rq, cfg, compare_builds=False)
+    scene.upload(tuple(set(layouts)))
+    stream = rq.Stream()
+    frame, rays, results, tuned = time_config(rq, scene, cfg, layouts, args, stream)
+    ms, _, per = results[layouts[0]]
+    dom, dom_rays, stats = dominant_launch(scene, frame, layouts[0], per)
+    out = {"workload": CONFIGS[cfg][6], "layout": layouts[0], "value": round(rays / (ms * 1e-3) / 1e6, 2),
+           "ms_per_step": round(ms, 4), "rays_per_step": rays, "tuning": tuned.get(layouts[0]),
+           "bytes_per_ray": round(stats["bytes"] / max(dom_rays, 1), 1),
+           "roofline_frac": round(stats["bytes"] / (per[dom] * 1e-3) / HBM_PEAK, 4)}
+    if "plain" in results and layouts[0] != "plain":
+        pm = results["plain"][0]
+        out["plain_f32"] = round(rays / (pm * 1e-3) / 1e6, 2)
+        out["quantized_speedup"] = round(pm / ms, 3)
+    del frame, scene
+    return out
 
 
 if __name__ == "__main__":

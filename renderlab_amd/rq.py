@@ -227,14 +227,21 @@ class DeviceBuffer:
     def zero(self):
         check(lib.rq_memset(self.ptr, 0, self.nbytes))
 
+    def view(self, offset, nbytes):
+        """A non-owning buffer over bytes [offset, offset + nbytes) of this one (kept alive by the view)."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        v = DeviceBuffer.__new__(DeviceBuffer)
+        v.nbytes, v.ptr, v._owner = int(nbytes), _P(self.ptr.value + int(offset)), self
+        return v
+
     def free(self):
-        if self.ptr:
+        if self.ptr and getattr(self, "_owner", None) is None:
             check(lib.rq_free(self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
-            if self.ptr:
+            if self.ptr and getattr(self, "_owner", None) is None:
                 lib.rq_free(self.ptr)
         except Exception:
             pass
