@@ -270,7 +270,7 @@ def c3_batches(rq, scenes, layout, n_batches=256, seed=7):
         hits = rq.DeviceBuffer(n * 48) if mode == rq.CLOSEST else None
         vis = rq.DeviceBuffer(((n + 31) // 32) * 4) if mode != rq.CLOSEST else None
         outs.append((hits, vis))
-        batches.append((si, mode, n, buf.ptr.value + 32 * start, hits, vis))
+        batches.append((si, mode, n, buf.view(32 * start, 32 * n), hits, vis))
     return batches, pools, outs
 
 

@@ -1,0 +1,124 @@
+"""The benchmark configurations at their full sizes on the GPU, against the oracle.
+
+- c0: 2,000 boxes + 8 UV spheres + ground, 960x640 primary + sun shadow: every
+  ray against the oracle.
+- c1: 1M triangles, 1920x1080 primary + shadow + 1 GI bounce (5.2M rays); c2:
+  4M triangles, 3840x2160 primary + shadow + 2 GI bounces (28.5M rays). Every
+  ray of the frame is traced in the REF, PLAIN and CW8 layouts and by the
+  oracle: REF and PLAIN records equal the oracle's bit for bit, CW8 differs
+  only on ties, visibility bits equal.
+- c3: 8 resident scenes of 100K-4M triangles and 256 lognormal batches
+  (4K-2M rays; 40% primary, 30% shadow, 30% GI; 32M rays) in one persistent
+  launch: every batch equal to its own launch and to the oracle on every
+  ray.
+
+The secondary rays of a frame come from its REF primary hits (the oracle's, bit
+for bit), so every layout traces the same rays."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import SUN, classify_mismatch, compare_exact, vis_bits
+from renderlab_amd import rq
+
+pytestmark = pytest.mark.gpu
+
+
+
+def frame_rays(scene, kind, w, h, n_gi, seed=1):
+    """(primary, shadow, [gi...]) device buffers and counts of one frame, from REF primary hits."""
+    n = w * h
+    inv = rq.camera(kind, seed, w, h)
+    prim = rq.DeviceBuffer(n * 32)
+    rq.gen_primary(inv, w, h, prim, tiled=w % 8 == 0 and h % 8 == 0)
+    hits = rq.DeviceBuffer(n * 48)
+    scene.trace("ref", prim, n, rq.CLOSEST, hits=hits)
+    shadow = rq.DeviceBuffer(n * 32)
+    gi = rq.DeviceBuffer(max(n_gi, 1) * n * 32)
+    counts = rq.DeviceBuffer(4 * (n_gi + 1))
+    rq.gen_secondary(prim, hits, n, SUN, seed, n_gi, shadow, gi if n_gi else None, counts)
+    rq.synchronize()
+    c = counts.download(np.uint32)
+    passes = [("primary", prim, n, rq.CLOSEST), ("shadow", shadow, int(c[0]), rq.ANY)]
+    passes += [(f"gi{b}", gi.view(b * n * 32, int(c[1 + b]) * 32), int(c[1 + b]), rq.CLOSEST) for b in range(n_gi)]
+    return passes, (prim, hits, shadow, gi)
+
+
+def trace_all(scene, layout, rays, n, mode):
+    if mode == rq.CLOSEST:
+        out = rq.DeviceBuffer(max(n, 1) * 48)
+        scene.trace(layout, rays, n, mode, hits=out)
+        rq.synchronize()
+        return out.download(rq.HIT_DTYPE, n)
+    out = rq.DeviceBuffer(max((n + 31) // 32, 1) * 4)
+    scene.trace(layout, rays, n, mode, vis=out)
+    rq.synchronize()
+    return vis_bits(out.download(np.uint32, (n + 31) // 32), n)
+
+
+def check_frame(scene, passes):
+    for name, rays, n, mode in passes:
+        got = {layout: trace_all(scene, layout, rays, n, mode) for layout in ("ref", "plain", "cw8")}
+        host = rays.download(rq.RAY_DTYPE, n)
+        print(f"{name}: {n} rays", flush=True)
+        if mode == rq.CLOSEST:
+            want, _, _ = oracle.traverse(scene.words, host, 0)
+            assert len(compare_exact(got["ref"], want)) == 0, name
+            assert len(compare_exact(got["plain"], want)) == 0, name
+            bad, ties = classify_mismatch(got["cw8"], want)
+            assert len(bad) == 0, (name, bad[:10])
+            print(f"{name}: cw8 ties {len(ties)}", flush=True)
+            assert (want["flags"] & 1).sum() > n // 10
+        else:
+            _, want, _ = oracle.traverse(scene.words, host, 1, want_hits=False, want_vis=True)
+            want = vis_bits(want, n)
+            for layout in ("ref", "plain", "cw8"):
+                assert np.array_equal(got[layout], want), (name, layout)
+
+
+def test_c0_full_frame_equals_the_oracle_on_every_ray():
+    scene = rq.Scene.generate(0, 1)
+    assert scene.stats()["triangles"] == 31682
+    scene.upload(("ref", "plain", "cw8"))
+    passes, keep = frame_rays(scene, 0, 960, 640, 0)
+    check_frame(scene, passes)
+
+
+@pytest.mark.parametrize("cfg", ["c1", "c2"])
+def test_full_frame_at_scale(cfg):
+    tris, w, h, n_gi = {"c1": (1_000_000, 1920, 1080, 1), "c2": (4_000_000, 3840, 2160, 2)}[cfg]
+    scene = rq.Scene.generate(1, 1, 5000, tris, gpu=True)
+    assert abs(scene.stats()["triangles"] - tris) < tris // 100
+    scene.upload(("ref", "plain", "cw8"))
+    passes, keep = frame_rays(scene, 1, w, h, n_gi)
+    check_frame(scene, passes)
+
+
+def test_c3_eight_scenes_and_256_batches_in_one_launch():
+    from bench import c3_batches, c3_scenes
+    layout = "cw8"
+    scenes = c3_scenes(rq, layout)
+    for sc in scenes:
+        sc.upload(("ref",))
+    batches, pools, outs = c3_batches(rq, scenes, layout)
+    assert len(batches) == 256 and len({b[0] for b in batches}) == 8
+    sizes = np.array([b[2] for b in batches])
+    assert sizes.min() >= 4096 and sizes.max() <= 2_000_000
+    print(f"c3: {len(batches)} batches, {sizes.sum()} rays", flush=True)
+    rq.trace_batches(scenes, layout, batches)
+    rq.synchronize()
+    for k, (si, mode, n, rays, hits, vis) in enumerate(batches):
+        r = rays.download(rq.RAY_DTYPE, n)
+        alone = trace_all(scenes[si], layout, rays, n, mode)
+        if mode == rq.CLOSEST:
+            got = hits.download(rq.HIT_DTYPE, n)
+            bad, _ = classify_mismatch(got, alone)
+            assert len(bad) == 0, k
+            want, _, _ = oracle.traverse(scenes[si].words, r, 0)
+            bad, _ = classify_mismatch(got, want)
+            assert len(bad) == 0, k
+        else:
+            got = vis_bits(vis.download(np.uint32, (n + 31) // 32), n)
+            assert np.array_equal(got, alone), k
+            _, want, _ = oracle.traverse(scenes[si].words, r, 1, want_hits=False, want_vis=True)
+            assert np.array_equal(got, vis_bits(want, n)), k
