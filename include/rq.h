@@ -57,6 +57,15 @@ typedef enum rq_mode {
    least 65536 rays in (direction octant, origin Morton code) order, for
    incoherent secondary rays. Results keep each ray's index and bits. */
 #define RQ_REORDER 0x100u
+/* Or'ed into the mode of rq_trace / rq_trace_indirect / rq_trace_noalloc
+   (REF, PLAIN and CW8 layouts): the watertight leaf test (Woop, Benthin and
+   Wald, JCGT 2013) in place of the reference's Moller-Trumbore
+   (shaders/gi_rays_software.wgsl:41-58). A ray through an edge or vertex shared
+   by two triangles hits at least one of them, where Moller-Trumbore can miss
+   both; hits elsewhere are the same triangles, with t and (u, v) from the
+   watertight arithmetic (within a few f32 roundings of Moller-Trumbore's). No
+   counterpart in the reference: an opt-in mode, off by default. */
+#define RQ_WATERTIGHT 0x200u
 
 /* One ray, 32 bytes: origin, tmin, direction (need not be unit), tmax. */
 typedef struct rq_ray {
@@ -307,14 +316,15 @@ rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask);
 rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t* bytes);
 /* Copies one part of an uploaded layout back to the host (no reference
    counterpart: a diagnostic for checking device images). PLAIN / CW8 parts as
-   rq_scene_layout_image numbers them (0-4, 256-byte padded); RS_CW8 parts 0-2
+   rq_scene_layout_image numbers them (0-4 and 6, 256-byte padded); RS_CW8 parts 0-2
    likewise; REF / RS part 0 is the whole word buffer. *bytes = the part's size; nothing is copied when
    `capacity` is smaller. */
 rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
                                  size_t capacity, size_t* bytes);
 /* Host image of a relayout (PLAIN or CW8), built on demand without a device:
    part 0 instance records, 1 top-level nodes, 2 BLAS nodes, 3 triangles,
-   4 position indices, 5 {top_on, top_root, instances, 0}. For inspection.
+   4 position indices, 5 {top_on, top_root, instances, 0}, 6 the triangles'
+   vertices (a, b, c: the watertight test's records). For inspection.
    RS_CW8 (a reflection scene): part 0 the words with each record's root
    replaced by its CW8 root, 1 the CW8 top level, 2 the CW8 BLAS nodes, 5 as
    above; 3 and 4 are empty. */

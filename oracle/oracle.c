@@ -48,6 +48,7 @@ typedef struct ctx {
     size_t n;
     orc_counts c;
     int bad;
+    int wt;  /* the watertight leaf test (mode bit ORC_WATERTIGHT) */
 } ctx;
 static inline uint32_t W(ctx* x, uint64_t i) {
     x->c.words++;
@@ -64,13 +65,23 @@ static inline float safe_inv1(float d) {
 }
 static inline v3 safe_inverse(v3 d) { return mk(safe_inv1(d.x), safe_inv1(d.y), safe_inv1(d.z)); }
 
-/* gi_rays_software.wgsl:35-40 */
-static inline float slab_near(v3 o, v3 inv, v3 lo, v3 hi, float tmin, float tmax) {
+/* The watertight mode's culling (Ize, "Robust BVH Ray Traversal", JCGT 2(2),
+   2013): a box's exit distance scaled by 1 + 4 * 2^-23 >= 1 + 2 gamma_3 before
+   near <= far, conservative against the roundings of (b - o) * inv. */
+#define ROBUST_FAR 1.00000048f
+
+/* gi_rays_software.wgsl:35-40; robust: the exit distance scaled by ROBUST_FAR. */
+static inline float slab_near_r(v3 o, v3 inv, v3 lo, v3 hi, float tmin, float tmax, int robust) {
     v3 a = mul3(sub3(lo, o), inv);
     v3 b = mul3(sub3(hi, o), inv);
     float near = mx(mx(mx(mn(a.x, b.x), mn(a.y, b.y)), mn(a.z, b.z)), tmin);
-    float far = mn(mn(mn(mx(a.x, b.x), mx(a.y, b.y)), mx(a.z, b.z)), tmax);
+    float exit = mn(mn(mx(a.x, b.x), mx(a.y, b.y)), mx(a.z, b.z));
+    if (robust) exit = exit * ROBUST_FAR;
+    float far = mn(exit, tmax);
     return near <= far ? near : FMAXV;
+}
+static inline float slab_near(v3 o, v3 inv, v3 lo, v3 hi, float tmin, float tmax) {
+    return slab_near_r(o, inv, lo, hi, tmin, tmax, 0);
 }
 
 /* gi_rays_software.wgsl:42-58: Moller-Trumbore, both faces; t<0 is a miss. */
@@ -90,6 +101,53 @@ static inline v3 tri_test(v3 o, v3 d, v3 a, v3 b, v3 c, float tmin, float tmax) 
     float t = dot3(e2, q) * inv;
     if (t > tmin && t < tmax) return mk(t, u, v);
     return none;
+}
+
+/* Kahan's difference of products a*b - c*d: within 2 ulps of the exact value
+   (Jeannerod, Louvet and Muller, Math. Comp. 82, 2013), so its sign is exact
+   and it is zero exactly when a*b == c*d. */
+static inline float diff_of_products(float a, float b, float c, float d) {
+    float w = c * d;
+    float e = fmaf(-c, d, w);
+    float f = fmaf(a, b, -w);
+    return f + e;
+}
+static inline float comp(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+/* Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight
+   Ray/Triangle Intersection", JCGT 2(1), 2013), both faces, in f32: translate
+   to the ray origin, shear along the dominant axis (kz the largest |d|, the
+   lowest axis on ties; kx, ky the next two, swapped when d[kz] < 0), the three
+   edge functions by diff_of_products in place of the paper's f64 retry, reject
+   mixed signs and det = U + V + W == 0, t = T / det with T = U*Az + V*Bz + W*Cz
+   summed in that order, and (u, v) = (V, W) / det, the weights of b and c. Not
+   the reference's test (gi_rays_software.wgsl:41-58 is Moller-Trumbore): the
+   opt-in watertight mode of this engine. t < 0 is a miss. */
+static inline v3 wt_tri_test(v3 o, v3 d, v3 a, v3 b, v3 c, float tmin, float tmax) {
+    v3 none = mk(-1.0f, 0.0f, 0.0f);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(d, kz);
+    if (dz < 0.0f) { int k = kx; kx = ky; ky = k; }
+    float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+    v3 A = sub3(a, o), B = sub3(b, o), C = sub3(c, o);
+    float az_ = comp(A, kz), bz_ = comp(B, kz), cz_ = comp(C, kz);
+    float Ax = comp(A, kx) - sx * az_, Ay = comp(A, ky) - sy * az_;
+    float Bx = comp(B, kx) - sx * bz_, By = comp(B, ky) - sy * bz_;
+    float Cx = comp(C, kx) - sx * cz_, Cy = comp(C, ky) - sy * cz_;
+    float U = diff_of_products(Cx, By, Cy, Bx);
+    float V = diff_of_products(Ax, Cy, Ay, Cx);
+    float W = diff_of_products(Bx, Ay, By, Ax);
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return none;
+    float det = U + V + W;
+    if (det == 0.0f) return none;
+    float T = U * (sz * az_) + V * (sz * bz_) + W * (sz * cz_);
+    float inv = 1.0f / det;
+    float t = T * inv;
+    if (!(t > tmin && t < tmax)) return none;
+    return mk(t, V * inv, W * inv);
 }
 
 /* shaders/scene.wgsl:86-90 */
@@ -130,7 +188,7 @@ static int gi_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, floa
     uint64_t r = 16u + (uint64_t)i * IW;
     x->c.inst_tests++;
     if (any_hit && (W(x, r + 27) & 1u) != 0u) return 0;
-    if (!(slab_near(origin, world_inv, GV(x, r + 12), GV(x, r + 15), tmin, limit_of(best, i)) <=
+    if (!(slab_near_r(origin, world_inv, GV(x, r + 12), GV(x, r + 15), tmin, limit_of(best, i), x->wt) <=
           limit_of(best, i)))
         return 0;
     x->c.inst_entries++;
@@ -163,7 +221,8 @@ static int gi_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, floa
                 v3 a2 = GV(x, positions + (uint64_t)ja * 3u), b2 = GV(x, positions + (uint64_t)jb * 3u),
                    c2 = GV(x, positions + (uint64_t)jc * 3u);
                 x->c.tri_tests++;
-                v3 h = tri_test(o, d, a, b, c, tmin, limit_of(best, i));
+                v3 h = x->wt ? wt_tri_test(o, d, a, b, c, tmin, limit_of(best, i))
+                             : tri_test(o, d, a, b, c, tmin, limit_of(best, i));
                 if (h.x >= 0.0f) {
                     best->hit = 1; best->t = h.x; best->u = h.y; best->v = h.z; best->inst = i;
                     best->n = hit_normal(r0, r1, r2, a, b, c);
@@ -177,7 +236,8 @@ static int gi_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, floa
                 }
                 if (two) {
                     x->c.tri_tests++;
-                    v3 h2 = tri_test(o, d, a2, b2, c2, tmin, limit_of(best, i));
+                    v3 h2 = x->wt ? wt_tri_test(o, d, a2, b2, c2, tmin, limit_of(best, i))
+                                  : tri_test(o, d, a2, b2, c2, tmin, limit_of(best, i));
                     if (h2.x >= 0.0f) {
                         best->hit = 1; best->t = h2.x; best->u = h2.y; best->v = h2.z; best->inst = i;
                         best->n = hit_normal(r0, r1, r2, a2, b2, c2);
@@ -207,7 +267,7 @@ static int gi_instance(ctx* x, uint32_t i, v3 origin, v3 dir, v3 world_inv, floa
             v3 hq = mk((float)((hx >> sh) & 0xffu), (float)((hy >> sh) & 0xffu), (float)((hz >> sh) & 0xffu));
             v3 lo = mk(oq.x + st.x * lq.x, oq.y + st.y * lq.y, oq.z + st.z * lq.z);
             v3 hi = mk(oq.x + st.x * hq.x, oq.y + st.y * hq.y, oq.z + st.z * hq.z);
-            t[s] = slab_near(o, inv, lo, hi, tmin, limit_of(best, i));
+            t[s] = slab_near_r(o, inv, lo, hi, tmin, limit_of(best, i), x->wt);
         }
         /* Sorting network on (bits(t), child), descending (:189-197). */
         uint32_t k0[2] = {fb(t[0]), c[0]}, k1[2] = {fb(t[1]), c[1]}, k2[2] = {fb(t[2]), c[2]},
@@ -256,6 +316,8 @@ int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mo
     ctx x;
     memset(&x, 0, sizeof x);
     x.g = g; x.n = n;
+    x.wt = (mode & ORC_WATERTIGHT) != 0u;
+    mode &= 0xffu;
     best_t best;
     init_best(&best, ray->tmax);
     if (ray_invalid(ray)) {
@@ -299,7 +361,9 @@ int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mo
                 float az = (F(&x, nb + 8 + l) - origin.z) * world_inv.z;
                 float bz = (F(&x, nb + 20 + l) - origin.z) * world_inv.z;
                 float near = mx(mx(mx(mn(ax, bx), mn(ay, by)), mn(az, bz)), tmin);
-                float far = mn(mn(mn(mx(ax, bx), mx(ay, by)), mx(az, bz)), bound);
+                float exit = mn(mn(mx(ax, bx), mx(ay, by)), mx(az, bz));
+                if (x.wt) exit = exit * ROBUST_FAR;
+                float far = mn(exit, bound);
                 ids[l] = W(&x, nb + 24 + l);
                 dist[l] = (near <= far && ids[l] != EMPTY) ? near : FMAXV;
             }
@@ -611,11 +675,12 @@ int orc_flat_f32(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t coun
         ctx x;
         memset(&x, 0, sizeof x);
         x.g = g; x.n = n;
+        x.wt = (mode & ORC_WATERTIGHT) != 0u;
         const orc_ray* ray = &rays[q];
         best_t best;
         init_best(&best, ray->tmax);
         if (ray_invalid(ray)) { write_hit(&best, &hits[q]); hits[q].flags = 2u; continue; }
-        int any_hit = mode != 0u;
+        int any_hit = (mode & 0xffu) != 0u;
         v3 origin = mk(ray->o[0], ray->o[1], ray->o[2]);
         v3 dir = mk(ray->d[0], ray->d[1], ray->d[2]);
         uint32_t instances = W(&x, 0);
@@ -639,7 +704,8 @@ int orc_flat_f32(const uint32_t* g, size_t n, const orc_ray* rays, uint32_t coun
                 uint32_t ia = W(&x, indices + k), ib = W(&x, indices + k + 1), ic = W(&x, indices + k + 2);
                 v3 a = GV(&x, positions + (uint64_t)ia * 3u), b = GV(&x, positions + (uint64_t)ib * 3u),
                    c = GV(&x, positions + (uint64_t)ic * 3u);
-                v3 h = tri_test(o, d, a, b, c, ray->tmin, limit_of(&best, i));
+                v3 h = x.wt ? wt_tri_test(o, d, a, b, c, ray->tmin, limit_of(&best, i))
+                            : tri_test(o, d, a, b, c, ray->tmin, limit_of(&best, i));
                 if (h.x >= 0.0f) {
                     best.hit = 1; best.t = h.x; best.u = h.y; best.v = h.z; best.inst = i;
                     best.n = hit_normal(r0, r1, r2, a, b, c);

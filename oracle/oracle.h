@@ -8,6 +8,10 @@
  *   orc_gi_traverse      shaders/gi_rays_software.wgsl:217-279 (gi_traverse),
  *                        :98-207 (gi_instance), :27-70 (helpers), with
  *                        shaders/scene.wgsl:86-90 (unit_normal).
+ *   wt_tri_test          (mode ORC_WATERTIGHT) the watertight test of Woop,
+ *                        Benthin and Wald (JCGT 2013) in place of gi_triangle:
+ *                        the engine's opt-in mode, pinned by watertightness
+ *                        and closed forms rather than by the reference.
  *   orc_flat_f32         the flat instance loop of the same file (:225-228)
  *                        with every triangle of every instance tested in leaf
  *                        order: the tree-free answer gi_traverse must equal.
@@ -73,6 +77,10 @@ typedef struct orc_counts {
     uint64_t tri_steps;
     uint64_t tri_tests;
 } orc_counts;
+
+/* Or'ed into `mode` of orc_gi_traverse(_batch) and orc_flat_f32: the
+   watertight leaf test (wt_tri_test in oracle.c) in place of Moller-Trumbore. */
+#define ORC_WATERTIGHT 0x200u
 
 /* One ray through gi_traverse. Returns 0, or -1 if a read left the buffer. */
 int orc_gi_traverse(const uint32_t* g, size_t n, const orc_ray* ray, uint32_t mode,

@@ -191,6 +191,7 @@ struct HostLayout {
     std::vector<uint32_t> mesh_base;  // (node word offset, first node, first triangle) per mesh
     std::vector<float> tris;       // TRI_FLOATS per triangle: a, b - a, c - a
     std::vector<uint32_t> tri_ids; // TRI_ID_WORDS per triangle: ia, ib, ic, slot, primitive, 0
+    std::vector<float> verts;      // TRI_FLOATS per triangle: the vertices a, b, c (watertight tests)
     uint32_t top_root = 0;
     uint32_t top_on = 0;
     uint32_t instance_count = 0;

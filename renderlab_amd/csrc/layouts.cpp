@@ -101,7 +101,7 @@ void decode_unfused(const uint32_t* w, int slot, float lo[3], float hi[3]) {
 }
 
 void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint32_t slot_first,
-                      uint32_t n, std::vector<float>& tris, std::vector<uint32_t>& ids) {
+                      uint32_t n, std::vector<float>& tris, std::vector<uint32_t>& ids, std::vector<float>& verts) {
     for (uint32_t k = slot_first; k < slot_first + n; ++k) {
         // Record (TRI_FLOATS): vertex a, then the edges b - a and c - a rounded to
         // f32 -- the same roundings the intersection test would make, done once.
@@ -114,6 +114,10 @@ void gather_triangles(const std::vector<uint32_t>& words, const MeshMap& m, uint
         for (int a = 0; a < 3; ++a) tris.push_back(p[0][a]);
         for (int e = 1; e < 3; ++e)
             for (int a = 0; a < 3; ++a) tris.push_back(p[e][a] - p[0][a]);
+        // The vertices themselves, for the watertight test: a vertex shared by two
+        // triangles must be the same floats in both (a + (b - a) need not be b).
+        for (int v = 0; v < 3; ++v)
+            for (int a = 0; a < 3; ++a) verts.push_back(p[v][a]);
         // Side record (TRI_ID_WORDS), read once per hit: the three position
         // indices above, the BVH-order slot, the primitive id, zero.
         ids.push_back(k);
@@ -177,6 +181,7 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
     out.nodes.assign((size_t)node_total * TOP_WORDS, 0u);
     out.tris.reserve((size_t)tri_total * TRI_FLOATS);
     out.tri_ids.reserve((size_t)tri_total * TRI_ID_WORDS);
+    out.verts.reserve((size_t)tri_total * TRI_FLOATS);
     std::vector<uint32_t> roots(meshes.size());
     for (size_t mi = 0; mi < meshes.size(); ++mi) {
         const MeshMap& m = meshes[mi];
@@ -218,7 +223,7 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
             if (err.empty()) err = "mesh root out of range";
             return false;
         }
-        gather_triangles(words, m, 0, m.view.index_count / 3, out.tris, out.tri_ids);
+        gather_triangles(words, m, 0, m.view.index_count / 3, out.tris, out.tri_ids, out.verts);
     }
     instance_records(words, meshes, instance_mesh, roots, out.inst);
     copy_top(words, out);
@@ -491,6 +496,7 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     out.inst = plain.inst;
     out.tris = plain.tris;
     out.tri_ids = plain.tri_ids;
+    out.verts = plain.verts;
     out.instance_count = plain.instance_count;
     // BLAS: every mesh root reachable from an instance record, collapsed once.
     W8Builder b{&plain.nodes, &out.nodes, &err};

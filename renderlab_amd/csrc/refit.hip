@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void plain_nodes(const uint32_t* words, uint32
 }
 
 __global__ __launch_bounds__(256) void plain_tris(const uint32_t* words, uint32_t indices, uint32_t positions,
-                                                  uint32_t count, float* out) {
+                                                  uint32_t count, float* out, float* verts) {
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     if (k >= count) return;
     float p[3][3];
@@ -125,6 +125,9 @@ __global__ __launch_bounds__(256) void plain_tris(const uint32_t* words, uint32_
     for (int x = 0; x < 3; ++x) o[x] = p[0][x];
     for (int e = 1; e < 3; ++e)
         for (int x = 0; x < 3; ++x) o[3 * e + x] = p[e][x] - p[0][x];
+    float* q = verts + (size_t)TRI_FLOATS * k;
+    for (int v = 0; v < 3; ++v)
+        for (int x = 0; x < 3; ++x) q[3 * v + x] = p[v][x];
 }
 
 // Instance record words 0-17 (rows and world box) into the PLAIN records.
@@ -383,12 +386,13 @@ extern "C" rq_status rq_scene_refit_mesh(rq_scene* scene, uint32_t mesh, const f
         char* pb = reinterpret_cast<char*>(d.base);
         uint32_t* pnodes = reinterpret_cast<uint32_t*>(pb + d.off[2]) + 32ull * d.mesh_base[k + 1];
         float* ptris = reinterpret_cast<float*>(pb + d.off[3]) + (size_t)TRI_FLOATS * d.mesh_base[k + 2];
+        float* pverts = reinterpret_cast<float*>(pb + d.off[5]) + (size_t)TRI_FLOATS * d.mesh_base[k + 2];
         const uint32_t tris = m.index_count / 3;
         hipLaunchKernelGGL(refitk::plain_nodes, dim3((m.node_count + 255) / 256), dim3(256), 0, s, words, m.nodes,
                            m.node_count, pnodes);
         if (tris)
             hipLaunchKernelGGL(refitk::plain_tris, dim3((tris + 255) / 256), dim3(256), 0, s, words, m.indices,
-                               m.positions, tris, ptris);
+                               m.positions, tris, ptris, pverts);
         hipLaunchKernelGGL(refitk::plain_records, dim3((count * 18 + 255) / 256), dim3(256), 0, s, words, count,
                            reinterpret_cast<uint32_t*>(pb + d.off[0]));
         if (d.count[1] && d.count[3])

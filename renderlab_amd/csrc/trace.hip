@@ -72,13 +72,22 @@ __device__ __forceinline__ float fast_inv1(float d) {
 }
 __device__ __forceinline__ V3 fast_inverse(V3 d) { return mk(fast_inv1(d.x), fast_inv1(d.y), fast_inv1(d.z)); }
 
-// gi_slab_near (:35-40)
+// A box's exit distance scaled by 1 + 4 * 2^-23 >= 1 + 2 gamma_3 before the
+// near <= far test makes it conservative against the roundings of (b - o) * inv
+// (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013): the watertight mode's
+// culling, so a ray leaving through a box face exactly at a vertex keeps the box.
+constexpr float ROBUST_FAR = 1.00000048f;
+
+// gi_slab_near (:35-40); ROBUST: the exit distance scaled by ROBUST_FAR
+template <bool ROBUST = false>
 __device__ __forceinline__ float slab_near(V3 o, V3 inv, V3 lo, V3 hi, float tmin, float tmax) {
     float ax = (lo.x - o.x) * inv.x, bx = (hi.x - o.x) * inv.x;
     float ay = (lo.y - o.y) * inv.y, by = (hi.y - o.y) * inv.y;
     float az = (lo.z - o.z) * inv.z, bz = (hi.z - o.z) * inv.z;
     float near = fmx(fmx(fmx(fmn(ax, bx), fmn(ay, by)), fmn(az, bz)), tmin);
-    float far = fmn(fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz)), tmax);
+    float exit = fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz));
+    if (ROBUST) exit *= ROBUST_FAR;
+    float far = fmn(exit, tmax);
     return near <= far ? near : FMAXV;
 }
 
@@ -103,6 +112,60 @@ __device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 e1, V3 e2, float 
         return t;
     }
     return -1.0f;
+}
+
+// Kahan's difference of products a*b - c*d: the rounding error of c*d taken back
+// with one fma. The result lies within 2 ulps of the exact value (Jeannerod,
+// Louvet and Muller, "Further analysis of Kahan's algorithm for the accurate
+// computation of 2x2 determinants", Math. Comp. 82, 2013), so its sign is the
+// exact sign and it is zero exactly when a*b == c*d (no overflow or underflow).
+__device__ __forceinline__ float diff_of_products(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = __builtin_fmaf(-c, d, w);
+    const float f = __builtin_fmaf(a, b, -w);
+    return f + e;
+}
+__device__ __forceinline__ float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013), both faces, all in f32: the vertices are
+// translated to the ray origin and sheared so the ray runs along its dominant
+// axis; the three 2D edge functions then decide the hit. Computed with the
+// exactly-signed difference of products (in place of the paper's f64 retry),
+// an edge shared by two triangles gives opposite signs or zero in both, so a
+// ray through the edge or a shared vertex cannot miss both. Barycentrics are
+// the weights of b and c, as gi_triangle's (u, v); t < 0 on a miss.
+__device__ __forceinline__ float wt_tri_test(V3 o, V3 d, V3 a, V3 b, V3 c, float tmin, float tmax, float& u_out,
+                                             float& v_out) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const int kz = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dz = comp(d, kz);
+    if (dz < 0.0f) {  // keep the winding
+        const int k = kx;
+        kx = ky;
+        ky = k;
+    }
+    const float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+    const V3 A = sub3(a, o), B = sub3(b, o), C = sub3(c, o);
+    const float az_ = comp(A, kz), bz_ = comp(B, kz), cz_ = comp(C, kz);
+    const float Ax = comp(A, kx) - sx * az_, Ay = comp(A, ky) - sy * az_;
+    const float Bx = comp(B, kx) - sx * bz_, By = comp(B, ky) - sy * bz_;
+    const float Cx = comp(C, kx) - sx * cz_, Cy = comp(C, ky) - sy * cz_;
+    const float U = diff_of_products(Cx, By, Cy, Bx);
+    const float V = diff_of_products(Ax, Cy, Ay, Cx);
+    const float W = diff_of_products(Bx, Ay, By, Ax);
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return -1.0f;
+    const float det = U + V + W;
+    if (det == 0.0f) return -1.0f;
+    const float T = U * (sz * az_) + V * (sz * bz_) + W * (sz * cz_);
+    const float inv = 1.0f / det;
+    const float t = T * inv;
+    if (!(t > tmin && t < tmax)) return -1.0f;
+    u_out = V * inv;
+    v_out = W * inv;
+    return t;
 }
 
 // rs_trace's triangle test (shaders/reflections_software_bvh4.wgsl:124-135): the
@@ -154,6 +217,7 @@ struct SceneView {
     const uint32_t* nodes;  // PLAIN 32 words, CW8 24 words a node
     const float* tris;      // TRI_FLOATS a triangle: a, b - a, c - a
     const uint32_t* tri_ids;  // TRI_ID_WORDS a triangle: ia, ib, ic, slot, primitive, 0
+    const float* verts;     // TRI_FLOATS a triangle: a, b, c (the watertight test)
     uint32_t n_inst, top_on, top_root, pad;
 };
 
@@ -225,7 +289,10 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // whole leaf loop for one lane while the others wait on nodes, and the lanes
 // whose rays finished are refilled at the top of every iteration. Each ray
 // still visits exactly the sequence a one-item-at-a-time traversal visits.
-template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false>
+// WT: the watertight leaf test (wt_tri_test) in place of Moller-Trumbore, on the
+// triangles' own vertices (REF: through the indices; PLAIN/CW8: their vertex
+// records).
+template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false, bool WT = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
     // RSF: reflection words (records, triangles, rs_trace's rules); W8: 8-wide nodes.
@@ -575,26 +642,36 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     continue;
                 }
                 V3 va, e1, e2;
-                uint32_t slot_id;
+                const uint32_t slot_id = tri_next + j;
+                float u, vv, t;
                 if (LAYOUT == REF) {
                     uint64_t x = (uint64_t)m_idx + 3ull * (tri_next + j);
                     uint32_t ia = v.words[x], ib = v.words[x + 1], ic = v.words[x + 2];
                     const uint32_t* P = v.words + m_pos;
                     va = mk(bf(P[3ull * ia]), bf(P[3ull * ia + 1]), bf(P[3ull * ia + 2]));
-                    e1 = sub3(mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2])), va);
-                    e2 = sub3(mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2])), va);
-                    slot_id = tri_next + j;
+                    const V3 vb = mk(bf(P[3ull * ib]), bf(P[3ull * ib + 1]), bf(P[3ull * ib + 2]));
+                    const V3 vc = mk(bf(P[3ull * ic]), bf(P[3ull * ic + 1]), bf(P[3ull * ic + 2]));
+                    if (WT) {
+                        t = wt_tri_test(oo, od, va, vb, vc, tmin, limit(cur), u, vv);
+                    } else {
+                        e1 = sub3(vb, va);
+                        e2 = sub3(vc, va);
+                        t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
+                    }
+                } else if (WT) {
+                    const float3* tp = reinterpret_cast<const float3*>(v.verts + (uint64_t)TRI_FLOATS * (tri_next + j));
+                    const float3 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    t = wt_tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), tmin,
+                                    limit(cur), u, vv);
                 } else {
                     const float3* tp = reinterpret_cast<const float3*>(v.tris + (uint64_t)TRI_FLOATS * (tri_next + j));
                     const float3 t0 = tp[0], t1 = tp[1], t2 = tp[2];
                     va = mk(t0.x, t0.y, t0.z);
                     e1 = mk(t1.x, t1.y, t1.z);
                     e2 = mk(t2.x, t2.y, t2.z);
-                    slot_id = tri_next + j;
+                    t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
                 }
                 if (COUNT) nb += LAYOUT == REF ? 48 : 4 * TRI_FLOATS;
-                float u, vv;
-                float t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
                 if (t >= 0.0f) {
                     bt = t; bu = u; bv = vv; binst = cur; bhit = true;
                     btri = slot_id;
@@ -642,7 +719,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 }
                 const bool skip = any_hit && (rw[19] & 1u);
                 const float lim = limit(i);
-                if (!skip && slab_near(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
+                if (!skip && slab_near<WT>(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
                                        mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim) {
                     V3 r0 = mk(bf(rw[0]), bf(rw[1]), bf(rw[2])), r1 = mk(bf(rw[4]), bf(rw[5]), bf(rw[6])),
                        r2 = mk(bf(rw[8]), bf(rw[9]), bf(rw[10]));
@@ -689,7 +766,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                         float ay = (bf(tw[4 + l]) - wo.y) * winv.y, by = (bf(tw[16 + l]) - wo.y) * winv.y;
                         float az = (bf(tw[8 + l]) - wo.z) * winv.z, bz = (bf(tw[20 + l]) - wo.z) * winv.z;
                         float near = fmx(fmx(fmx(fmn(ax, bx), fmn(ay, by)), fmn(az, bz)), tmin);
-                        float far = fmn(fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz)), bound);
+                        float exit = fmn(fmn(fmx(ax, bx), fmx(ay, by)), fmx(az, bz));
+                        if (WT) exit *= ROBUST_FAR;
+                        float far = fmn(exit, bound);
                         ids[l] = tw[24 + l];
                         dist[l] = (near <= far && ids[l] != EMPTY) ? near : FMAXV;
                     }
@@ -735,7 +814,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                                    (float)((nw[15] >> sh) & 0xffu));
                         V3 lo3 = mk(oq.x + stp.x * lq.x, oq.y + stp.y * lq.y, oq.z + stp.z * lq.z);
                         V3 hi3 = mk(oq.x + stp.x * hq.x, oq.y + stp.y * hq.y, oq.z + stp.z * hq.z);
-                        t[s] = slab_near(oo, oinv, lo3, hi3, tmin, lim);
+                        t[s] = slab_near<WT>(oo, oinv, lo3, hi3, tmin, lim);
                     }
                 }
                 uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
@@ -773,7 +852,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     c[s] = nw[24 + s];
                     t[s] = FMAXV;
                     if (c[s] != EMPTY)
-                        t[s] = slab_near(ro, ri, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
+                        t[s] = slab_near<WT>(ro, ri, mk(bf(nw[s]), bf(nw[4 + s]), bf(nw[8 + s])),
                                          mk(bf(nw[12 + s]), bf(nw[16 + s]), bf(nw[20 + s])), tmin, lim);
                 }
                 uint32_t k0 = fb(t[0]), k1 = fb(t[1]), k2 = fb(t[2]), k3 = fb(t[3]);
@@ -1312,15 +1391,15 @@ rq_status ensure_ctx() {
     return RQ_OK;
 }
 
-template <int L, int B, int S, int AM, bool C = false, bool BT = false>
+template <int L, int B, int S, int AM, bool C = false, bool BT = false, bool WT = false>
 void* kernel_ptr() {
-    return reinterpret_cast<void*>(&dev::trace_kernel<L, B, S, AM, C, BT>);
+    return reinterpret_cast<void*>(&dev::trace_kernel<L, B, S, AM, C, BT, WT>);
 }
 
 typedef void (*Launch)(dim3, dim3, hipStream_t, const dev::TraceArgs&);
-template <int L, int B, int S, int AM, bool C = false, bool BT = false>
+template <int L, int B, int S, int AM, bool C = false, bool BT = false, bool WT = false>
 void launch_one(dim3 g, dim3 b, hipStream_t s, const dev::TraceArgs& a) {
-    hipLaunchKernelGGL((dev::trace_kernel<L, B, S, AM, C, BT>), g, b, 0, s, a);
+    hipLaunchKernelGGL((dev::trace_kernel<L, B, S, AM, C, BT, WT>), g, b, 0, s, a);
 }
 
 struct KernelEntry {
@@ -1383,6 +1462,28 @@ const KernelEntry& count_kernel() {
         put(std::integral_constant<int, dev::CW8>());
         put(std::integral_constant<int, dev::RS>());
         put(std::integral_constant<int, dev::RSW>());
+        return k;
+    }();
+    return e;
+}
+
+// The watertight leaf test (RQ_WATERTIGHT): REF, PLAIN and CW8, closest and any hit.
+const KernelEntry& watertight_kernel() {
+    static KernelEntry e = [] {
+        KernelEntry k;
+        std::memset(&k, 0, sizeof k);
+        k.block = 256;
+        k.stack = 16;
+        auto put = [&](auto lc) {
+            constexpr int L = decltype(lc)::value;
+            k.fn[L][0] = &launch_one<L, 256, 16, 0, false, false, true>;
+            k.fn[L][1] = &launch_one<L, 256, 16, 1, false, false, true>;
+            k.ptr[L][0] = kernel_ptr<L, 256, 16, 0, false, false, true>();
+            k.ptr[L][1] = kernel_ptr<L, 256, 16, 1, false, false, true>();
+        };
+        put(std::integral_constant<int, dev::REF>());
+        put(std::integral_constant<int, dev::PLAIN>());
+        put(std::integral_constant<int, dev::CW8>());
         return k;
     }();
     return e;
@@ -1567,7 +1668,9 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             if (h.nodes.empty()) h.nodes.assign(32, 0u);
             if (h.tris.empty()) h.tris.assign(TRI_FLOATS, 0.0f);
             if (h.tri_ids.empty()) h.tri_ids.assign(TRI_ID_WORDS, 0u);
-            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.tris) + padded(h.tri_ids);
+            if (h.verts.empty()) h.verts.assign(TRI_FLOATS, 0.0f);
+            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.tris) + padded(h.tri_ids) +
+                      padded(h.verts);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
             d.off[0] = at; if ((s = upload(d.base, at, h.inst)) != RQ_OK) return s;
@@ -1575,6 +1678,7 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             d.off[2] = at; if ((s = upload(d.base, at, h.nodes)) != RQ_OK) return s;
             d.off[3] = at; if ((s = upload(d.base, at, h.tris)) != RQ_OK) return s;
             d.off[4] = at; if ((s = upload(d.base, at, h.tri_ids)) != RQ_OK) return s;
+            d.off[5] = at; if ((s = upload(d.base, at, h.verts)) != RQ_OK) return s;
             d.count[0] = h.instance_count;
             d.count[1] = h.top_on ? 1u : 0u;
             d.count[2] = h.top_root;
@@ -1589,7 +1693,8 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
 
 extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
                                             size_t capacity, size_t* bytes) {
-    if (!scene || !bytes || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || part > 4 || (capacity && !dst)) {
+    if (!scene || !bytes || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || part > 6 || part == 5 ||
+        (capacity && !dst)) {
         set_error("rq_scene_download_part: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
@@ -1600,8 +1705,9 @@ extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout lay
     }
     size_t at = 0, n = d.bytes;
     if (layout == RQ_LAYOUT_PLAIN_F32 || layout == RQ_LAYOUT_CW8_Q8) {
-        at = d.off[part];
-        n = (part < 4 ? d.off[part + 1] : d.bytes) - at;
+        const uint32_t p = part == 6 ? 5u : part;  // part 6: the vertex records, stored after part 4
+        at = d.off[p];
+        n = (p < 5 ? d.off[p + 1] : d.bytes) - at;
     } else if (layout == RQ_LAYOUT_RS_CW8_Q8) {  // parts: words, top, nodes
         if (part > 2) {
             set_error("rq_scene_download_part: the RS CW8 layout has three parts");
@@ -1691,6 +1797,7 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
         v.tris = reinterpret_cast<const float*>(b + d.off[3]);
         v.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
+        v.verts = reinterpret_cast<const float*>(b + d.off[5]);
         v.n_inst = d.count[0];
         v.top_on = d.count[1] != 0;
         v.top_root = d.count[2];
@@ -1708,10 +1815,15 @@ rq_status rq::launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays
                            rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,
                            const uint32_t* count_ptr, const rq::FixedScratch* fixed) {
     const bool reorder = (mode & RQ_REORDER) != 0;
-    mode &= ~RQ_REORDER;
+    const bool watertight = (mode & RQ_WATERTIGHT) != 0;
+    mode &= ~(RQ_REORDER | RQ_WATERTIGHT);
     if (!scene || !layout_ok(scene, layout) || mode > 2 || (count && !rays)) {
         set_error("rq_trace: bad argument (or a layout of the other scene kind)");
         return RQ_ERR_INVALID_INPUT;
+    }
+    if (watertight && (bytes || layout > RQ_LAYOUT_CW8_Q8)) {
+        set_error("rq_trace: RQ_WATERTIGHT is for the REF, PLAIN and CW8 layouts (not counted launches)");
+        return RQ_ERR_UNSUPPORTED;
     }
     if (fixed && (reorder || bytes || count_ptr)) {
         set_error("rq_trace_noalloc: RQ_REORDER is not available without allocation (its sort space grows with "
@@ -1738,7 +1850,7 @@ rq_status rq::launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays
     if (count == 0) return RQ_OK;
     const int any = mode != 0 ? 1 : 0;
     const rq_tuning& t = g_ctx.tuning[layout][any];
-    const KernelEntry* k = bytes ? &count_kernel() : find_kernel(t.block, t.stack);
+    const KernelEntry* k = bytes ? &count_kernel() : watertight ? &watertight_kernel() : find_kernel(t.block, t.stack);
     if (!k) {
         set_error("rq_trace: no kernel for the tuning");
         return RQ_ERR_INVALID_INPUT;

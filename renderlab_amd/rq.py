@@ -39,6 +39,7 @@ CLOSEST = 0
 ANY = 1
 ANY_OPAQUE = 2
 REORDER = 0x100  # or'ed into a trace mode: sort the launch's rays before tracing
+WATERTIGHT = 0x200  # or'ed into a trace mode: the watertight leaf test instead of Moller-Trumbore
 
 LEAF_BIT = 0x80000000
 EMPTY = 0xFFFFFFFF
@@ -686,7 +687,8 @@ class Scene:
         check(lib.rq_scene_upload(self._h, mask))
 
     def layout_image(self, layout, part):
-        """Host image of a relayout part (0 inst, 1 top, 2 nodes, 3 tris, 4 ids, 5 meta) as u32 words."""
+        """Host image of a relayout part (0 inst, 1 top, 2 nodes, 3 tris, 4 ids, 5 meta, 6 vertices) as u32
+        words."""
         p = _P()
         n = _sz()
         check(lib.rq_scene_layout_image(self._h, LAYOUTS.get(layout, layout), int(part), ctypes.byref(p),

@@ -149,7 +149,7 @@ def test_refit_keeps_plain_in_step(setup):
         scene.refit_mesh(0, to_device(new), len(new))
         assert scene.device_bytes("plain") > 0 and scene.device_bytes("cw8") == 0
         # the PLAIN device image refreshed in place equals a host relayout of the refit words
-        for part in range(5):
+        for part in (0, 1, 2, 3, 4, 6):
             host = scene.layout_image("plain", part)
             dev = scene.download_part("plain", part)
             assert np.array_equal(dev[:host.size], host), f"PLAIN part {part}"
