@@ -535,7 +535,14 @@ def time_config(rq, scene, cfg, layouts, args, stream):
     for layout in layouts:
         frame.prime(layout)
         tr = frame.tuning_ranges()
-        if tr is not None and not args.no_tune:
+        fixed = dict(t.split("=", 1) for t in args.tuning).get(layout)
+        if fixed:
+            block, stack, refill, grid, spec = (int(x) for x in fixed.split(","))
+            rq.set_tuning(layout, rq.CLOSEST, block, stack, refill, grid_per_cu=grid, speculate=spec)
+            rq.set_tuning(layout, rq.ANY, block, stack, refill, grid_per_cu=grid, speculate=spec)
+            tuned[layout] = {"block": block, "stack": stack, "refill": refill, "grid_per_cu": grid, "speculate": spec}
+            log(f"{cfg} {layout}: fixed tuning {tuned[layout]}")
+        elif tr is not None and not args.no_tune:
             from renderlab_amd import autotune
             t = autotune.tune_ranges(scene, layout, tr, stream=stream)
             tuned[layout] = {k: t[k] for k in ("block", "stack", "refill", "grid_per_cu", "speculate")}
@@ -671,6 +678,9 @@ def main():
                     help="shadow and GI rays as ranges of one launch (fused), one launch per pass (separate), or "
                          "one launch per step with the next frame's primary rays (pipelined)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning (no autotuner sweep)")
+    ap.add_argument("--tuning", action="append", default=[], metavar="LAYOUT=B,S,R,G,SPEC",
+                    help="fixed launch tuning of a layout instead of the autotuner's (block, stack, refill, "
+                         "workgroups per CU, speculation), e.g. cw8=128,16,32,0,1")
     ap.add_argument("--primary-order", default="tiles", choices=["tiles", "rows"],
                     help="primary rays in 8x8 screen-tile order (rq_gen_primary_tiled) or row-major")
     ap.add_argument("--no-reflection", action="store_true", help="skip the reflection-ray (rs_trace) line")

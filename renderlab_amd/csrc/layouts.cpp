@@ -19,6 +19,7 @@
 //   as origin + 2^e*q with no rounding. Leaves and triangles are the
 //   PLAIN layout's, in the same order.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <tuple>
 #include <unordered_map>
@@ -277,8 +278,86 @@ struct W8Builder {
     const std::vector<uint32_t>* src;
     std::vector<uint32_t>* out;
     std::string* err;
+    // GI mesh trees: the CW8 copy of the triangle records, whose leaves may be split
+    // (null for the top level and for reflection trees, whose leaf tags differ)
+    HostLayout* tris = nullptr;
     std::vector<Cand> cands_by_slot_;
     std::vector<char> slots_used_;
+
+    // Exact bounds of triangles [first, first + n) of the CW8 copy (their vertices).
+    Bounds tri_bounds(uint32_t first, uint32_t n) const {
+        Bounds b = empty_bounds();
+        for (uint32_t k = first; k < first + n; ++k)
+            for (int v = 0; v < 3; ++v)
+                for (int a = 0; a < 3; ++a) {
+                    const float x = tris->verts[(size_t)TRI_FLOATS * k + 3 * v + a];
+                    b.lo[a] = mn(b.lo[a], x);
+                    b.hi[a] = mx(b.hi[a], x);
+                }
+        return b;
+    }
+
+    // An 8-wide node tests all 8 slots whatever it holds, so free slots cost nothing
+    // more when filled. While slots are free, the leaf with the most triangles (3 or
+    // more) is split in two at the median of its triangles' centroids along their
+    // longest axis, each part with the exact box of its vertices: a ray then tests
+    // only the part it meets. The triangles move within the leaf's range of this
+    // layout's own triangle records (the side record keeps each triangle's slot and
+    // indices for the hit record). The first part has an even count, so a ray that
+    // meets both parts takes no more pair steps than it did. Within an instance a
+    // closest hit can then differ only between equal-distance triangles.
+    void split_leaves(std::vector<Cand>& cands) {
+        while (cands.size() < 8) {
+            int pick = -1;
+            uint32_t best = 2;
+            for (int i = 0; i < (int)cands.size(); ++i) {
+                const uint32_t t = cands[i].child;
+                if (!(t & LEAF_BIT)) continue;
+                const uint32_t n = ((t >> 27) & 0xfu) + 1u;
+                if (n > best || (n == best && pick >= 0 && half_area(cands[i].box) > half_area(cands[pick].box))) {
+                    pick = i;
+                    best = n;
+                }
+            }
+            if (pick < 0) break;
+            const uint32_t tag = cands[pick].child, first = tag & ((1u << 27) - 1u), n = ((tag >> 27) & 0xfu) + 1u;
+            std::vector<std::array<float, 3>> ctr(n);
+            Bounds cb = empty_bounds();
+            for (uint32_t k = 0; k < n; ++k) {
+                const float* p = &tris->verts[(size_t)TRI_FLOATS * (first + k)];
+                for (int a = 0; a < 3; ++a) {
+                    ctr[k][a] = (p[a] + p[3 + a] + p[6 + a]) / 3.0f;
+                    cb.lo[a] = mn(cb.lo[a], ctr[k][a]);
+                    cb.hi[a] = mx(cb.hi[a], ctr[k][a]);
+                }
+            }
+            int axis = 0;
+            for (int a = 1; a < 3; ++a)
+                if (cb.hi[a] - cb.lo[a] > cb.hi[axis] - cb.lo[axis]) axis = a;
+            std::vector<uint32_t> order(n);
+            for (uint32_t k = 0; k < n; ++k) order[k] = k;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t x, uint32_t y) { return ctr[x][axis] < ctr[y][axis]; });
+            auto permute = [&](auto& v, size_t width) {
+                std::vector<typename std::decay_t<decltype(v)>::value_type> tmp(v.begin() + width * first,
+                                                                                v.begin() + width * (first + n));
+                for (uint32_t k = 0; k < n; ++k)
+                    std::copy(tmp.begin() + width * order[k], tmp.begin() + width * (order[k] + 1),
+                              v.begin() + width * (first + k));
+            };
+            permute(tris->tris, TRI_FLOATS);
+            permute(tris->tri_ids, TRI_ID_WORDS);
+            permute(tris->verts, TRI_FLOATS);
+            const uint32_t m = 2u * ((n + 2u) / 4u);  // even, about half: 3 -> 2+1, 5 -> 2+3, 6 -> 4+2, 8 -> 4+4
+            Cand a, b;
+            a.child = LEAF_BIT | ((m - 1u) << 27) | first;
+            a.box = tri_bounds(first, m);
+            b.child = LEAF_BIT | ((n - m - 1u) << 27) | (first + m);
+            b.box = tri_bounds(first + m, n - m);
+            cands[pick] = a;
+            cands.insert(cands.begin() + pick + 1, b);
+        }
+    }
 
     uint32_t alloc() {
         uint32_t index = (uint32_t)(out->size() / W8_WORDS);
@@ -300,6 +379,7 @@ struct W8Builder {
             cands.erase(cands.begin() + pick);
             cands.insert(cands.begin() + pick, kids.begin(), kids.end());
         }
+        if (tris) split_leaves(cands);
         Bounds all = empty_bounds();
         for (auto& c : cands) all = merge(all, c.box);
         // Slot assignment by ray octant (Ylitie et al. 2017, sec. 3.2): slot s
@@ -500,6 +580,7 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     out.instance_count = plain.instance_count;
     // BLAS: every mesh root reachable from an instance record, collapsed once.
     W8Builder b{&plain.nodes, &out.nodes, &err};
+    b.tris = &out;
     std::vector<std::pair<uint32_t, uint32_t>> done;  // plain root -> cw8 root
     for (uint32_t i = 0; i < plain.instance_count; ++i) {
         uint32_t pr = plain.inst[(size_t)i * INSTANCE_WORDS + 18];

@@ -1,7 +1,7 @@
 """Relayouts, checked on the host: PLAIN boxes are the reference decode bit
-for bit; every CW8 child box (decoded exactly as the kernel decodes it)
-encloses the PLAIN box of the subtree it holds, at every level; both keep
-every triangle, in leaf order."""
+for bit; CW8 leaves are parts of PLAIN leaves (split to fill free slots) whose
+boxes, decoded exactly as the kernel decodes them, enclose their triangles;
+both keep every triangle exactly once."""
 import numpy as np
 import pytest
 
@@ -75,35 +75,25 @@ def test_plain_boxes_are_the_reference_decode(kind, seed, meshes, tris):
 
 
 @pytest.mark.parametrize("kind,seed,meshes,tris", [(0, 1, 0, 0), (1, 5, 60, 20000)])
-def test_cw8_boxes_enclose_the_plain_subtrees(kind, seed, meshes, tris):
+def test_cw8_leaves_split_plain_leaves_and_their_boxes_enclose_their_triangles(kind, seed, meshes, tris):
+    """CW8 keeps PLAIN's triangles, each moved at most within its PLAIN leaf's range (its triangle
+    record, side record and vertices together); each CW8 leaf is a contiguous part of one PLAIN leaf,
+    and the CW8 leaves under a mesh root cover the PLAIN leaves' triangles exactly once; every CW8
+    child box encloses its leaf's vertices (inner children: their subtrees' leaves)."""
     sc = rq.Scene.generate(kind, seed, meshes, tris)
     pn, cn = sc.layout_image("plain", 2), sc.layout_image("cw8", 2)
     pi, ci = sc.layout_image("plain", 0).reshape(-1, 32), sc.layout_image("cw8", 0).reshape(-1, 32)
-    ptris, ctris = sc.layout_image("plain", 3), sc.layout_image("cw8", 3)
-    assert np.array_equal(ptris, ctris)
+    ptris, ctris = sc.layout_image("plain", 3).reshape(-1, 9), sc.layout_image("cw8", 3).reshape(-1, 9)
+    pids, cids = sc.layout_image("plain", 4).reshape(-1, 6), sc.layout_image("cw8", 4).reshape(-1, 6)
+    pv, cv = sc.layout_image("plain", 6).reshape(-1, 9), sc.layout_image("cw8", 6).reshape(-1, 9)
+    assert ptris.shape == ctris.shape
+    # one row per triangle: its record, side record (indices, slot, primitive) and vertices
+    prow = np.concatenate([ptris, pids, pv], axis=1)
+    crow = np.concatenate([ctris, cids, cv], axis=1)
 
-    def plain_leaves(tag):
-        if tag & LEAF:
-            return [tag]
-        out = []
-        for t, _, _ in plain_children(pn, tag):
-            out += plain_leaves(t)
-        return out
-
-    def plain_box_of_leaf(root):
-        """The box each leaf tag has in its PLAIN parent."""
-        boxes = {}
-        stack = [root]
-        while stack:
-            t = stack.pop()
-            if t & LEAF:
-                continue
-            for c, lo, hi in plain_children(pn, t):
-                if c & LEAF:
-                    boxes[c] = (lo, hi)
-                else:
-                    stack.append(c)
-        return boxes
+    def span(tag):
+        first = tag & ((1 << 27) - 1)
+        return first, first + ((tag >> 27) & 0xF) + 1
 
     checked = 0
     for i in range(len(pi)):
@@ -111,23 +101,34 @@ def test_cw8_boxes_enclose_the_plain_subtrees(kind, seed, meshes, tris):
         if proot & LEAF:
             assert croot == proot
             continue
-        leaf_box = plain_box_of_leaf(proot)
-        want = sorted(plain_leaves(proot))
-        got = []
-        stack = [(croot, None)]
+        plain_spans = []
+        stack = [proot]
         while stack:
-            t, box = stack.pop()
-            if t & LEAF:
-                got.append(t)
-                lo, hi = leaf_box[t]
-                assert (box[0] <= lo).all() and (box[1] >= hi).all()
-                checked += 1
-                continue
+            t = stack.pop()
+            for c, _, _ in plain_children(pn, t):
+                (plain_spans.append(span(c)) if c & LEAF else stack.append(c))
+        got = []
+        stack = [croot]
+        while stack:
+            t = stack.pop()
             for c, lo, hi in w8_children(cn, t):
-                if box is not None:
-                    pass
-                stack.append((c, (lo, hi)))
-        assert sorted(got) == want
+                if not c & LEAF:
+                    stack.append(c)
+                    continue
+                a, b = span(c)
+                got.append((a, b))
+                home = [(pa, pb) for pa, pb in plain_spans if pa <= a and b <= pb]
+                assert len(home) == 1  # inside one PLAIN leaf
+                v = cv[a:b].reshape(-1, 3).view(np.float32)
+                assert (lo <= v.min(axis=0)).all() and (hi >= v.max(axis=0)).all()
+                checked += 1
+        covered = np.zeros(max(b for _, b in plain_spans), int)
+        for a, b in got:
+            covered[a:b] += 1
+        for a, b in plain_spans:
+            assert (covered[a:b] == 1).all()
+            # the PLAIN leaf's triangles, moved only among themselves (rows moved whole)
+            assert np.array_equal(np.unique(crow[a:b], axis=0), np.unique(prow[a:b], axis=0))
         if i > 40:
             break
     assert checked > 0
