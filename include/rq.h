@@ -34,7 +34,8 @@ typedef enum rq_status {
     RQ_ERR_DEVICE = 3,        /* HIP error, or no GPU present */
     RQ_ERR_UNSUPPORTED = 4,
     RQ_ERR_BUDGET = 5,        /* serving: admission refused by the HBM budget */
-    RQ_ERR_NOT_READY = 6
+    RQ_ERR_NOT_READY = 6,
+    RQ_ERR_DEADLINE = 7       /* serving: admission refused by the latency budget */
 } rq_status;
 
 /* Device layouts of one scene. */
@@ -489,9 +490,11 @@ rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t c
                            const float sun[3], uint32_t seed, uint32_t n_gi, rq_ray* shadow,
                            rq_ray* gi, uint32_t* counts, void* stream);
 
-/* ---- Serving: render-query jobs admitted by an HBM budget with continuous
-   batching. No counterpart in the reference (its renderer traces once a
-   frame on one queue); this is the request loop around the ray query. */
+/* ---- Serving: render-query jobs admitted by an HBM budget and a latency
+   budget, traced with continuous batching. No counterpart in the reference
+   (its renderer traces once a frame on one queue; its admission is a bounded
+   wait, src/engine/mod.rs:1247-1262); this is the request loop around the ray
+   query. */
 typedef struct rq_job {
     uint32_t scene;       /* index into the scenes array */
     uint32_t scene_kind;  /* camera orbit of rq_gen_camera (0 or 1) */
@@ -503,22 +506,31 @@ typedef struct rq_job {
 } rq_job;
 typedef struct rq_serve_config {
     rq_layout layout;
-    uint32_t slots;       /* jobs in flight at once (one HIP stream each), 1..16 */
+    uint32_t slots;       /* jobs admitted and not finished at once, 1..256 */
     uint64_t hbm_budget;  /* bytes of job buffers in flight at once */
     float sun[3];         /* unit direction toward the sun */
-    uint32_t pad;
+    uint32_t latency_budget_us; /* 0: none; else a job whose predicted completion is later than its
+                                   arrival plus this is refused (RQ_ERR_DEADLINE) */
 } rq_serve_config;
 typedef struct rq_job_result {
     double arrival_s, start_s, end_s; /* host clock, seconds after the run starts */
     double gpu_ms;                    /* first to last launch of the job, HIP events */
     uint64_t digest;                  /* order-free hash of the job's hits and visibility bits */
     uint64_t primary, secondary;      /* rays traced */
-    uint32_t status;                  /* RQ_OK, or RQ_ERR_BUDGET when the job alone exceeds the budget */
+    uint32_t status;                  /* RQ_OK; RQ_ERR_BUDGET: the job alone exceeds the HBM budget;
+                                         RQ_ERR_DEADLINE: refused by the latency budget */
     uint32_t pad;
 } rq_job_result;
-/* Runs `jobs` in real time: a job is admitted once it has arrived, a slot is
-   free and its buffers fit in the budget (first come, first served; first
-   fit in one arena of hbm_budget bytes). Returns when every job is done. */
+/* Runs `jobs` in real time. A job is admitted once it has arrived, a slot is
+   free and its buffers fit in the budget (first come, first served; first fit
+   in one arena of hbm_budget bytes), unless its predicted completion (the
+   running launch, the next launch with its primary rays, the one after with
+   its secondary rays; each a fixed drain cost plus rays over the measured
+   throughput) misses arrival + latency_budget_us: then it is refused, on
+   arrival or while it waits. One persistent launch (rq_trace_batches, CW8
+   speculation off) runs at a time, carrying the primary rays of the jobs
+   admitted since the previous launch and the shadow and GI rays of the jobs
+   whose primary rays that launch traced. Returns when every job is done. */
 rq_status rq_serve(rq_scene* const* scenes, uint32_t scene_count, const rq_serve_config* config,
                    const rq_job* jobs, uint32_t job_count, rq_job_result* results);
 

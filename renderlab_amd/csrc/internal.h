@@ -216,6 +216,10 @@ struct FixedScratch {
     uint32_t* spill;
     size_t lanes;
 };
+// rq_trace_batches with the CW8 node-step speculation forced on (1) or off (0),
+// or as the batch tuning says (-1).
+rq_status trace_batches_impl(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout, const rq_batch* batches,
+                             uint32_t batch_count, void* stream, int speculate);
 // rq_trace / rq_trace_indirect / rq_trace_count_bytes, and rq_trace_noalloc with `fixed`.
 rq_status launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count, uint32_t mode,
                        rq_hit* hits, uint32_t* vis_bits, void* stream, unsigned long long* bytes,

@@ -2018,6 +2018,11 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
 
 extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
                                       const rq_batch* batches, uint32_t batch_count, void* stream) {
+    return rq::trace_batches_impl(scenes, scene_count, layout, batches, batch_count, stream, -1);
+}
+
+rq_status rq::trace_batches_impl(rq_scene* const* scenes, uint32_t scene_count, rq_layout layout,
+                                 const rq_batch* batches, uint32_t batch_count, void* stream, int speculate) {
     if (!scenes || !batches || (int)layout < 0 || (int)layout >= LAYOUT_COUNT || batch_count == 0) {
         set_error("rq_trace_batches: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -2092,7 +2097,7 @@ extern "C" rq_status rq_trace_batches(rq_scene* const* scenes, uint32_t scene_co
     a.prefix = reinterpret_cast<const uint32_t*>(tb + poff);
     a.n_batches = batch_count;
     a.refill_min = t.refill ? t.refill : 1u;
-    a.speculate = t.speculate;
+    a.speculate = speculate < 0 ? t.speculate : (uint32_t)speculate;
     a.drain = t.drain;
     a.first = (uint32_t)(grid * k->block);
     k->bfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);

@@ -27,6 +27,7 @@ RQ_ERR_DEVICE = 3
 RQ_ERR_UNSUPPORTED = 4
 RQ_ERR_BUDGET = 5
 RQ_ERR_NOT_READY = 6
+RQ_ERR_DEADLINE = 7
 
 REF_BVH4Q8 = 0
 PLAIN_F32 = 1
@@ -863,7 +864,8 @@ class Job(ctypes.Structure):
 
 
 class ServeConfig(ctypes.Structure):
-    _fields_ = [("layout", _i), ("slots", _u32), ("hbm_budget", _u64), ("sun", ctypes.c_float * 3), ("pad", _u32)]
+    _fields_ = [("layout", _i), ("slots", _u32), ("hbm_budget", _u64), ("sun", ctypes.c_float * 3),
+                ("latency_budget_us", _u32)]
 
 
 class JobResult(ctypes.Structure):
@@ -872,8 +874,10 @@ class JobResult(ctypes.Structure):
                 ("status", _u32), ("pad", _u32)]
 
 
-def serve(scenes, jobs, layout="cw8", slots=4, hbm_budget=8 << 30, sun=(0.3, 0.8, 0.2)):
-    """jobs: [dict(scene, kind, width, height, n_gi, seed, arrival_s)] -> list of result dicts."""
+def serve(scenes, jobs, layout="cw8", slots=4, hbm_budget=8 << 30, sun=(0.3, 0.8, 0.2), latency_budget_ms=0.0):
+    """jobs: [dict(scene, kind, width, height, n_gi, seed, arrival_s)] -> list of result dicts.
+    latency_budget_ms > 0 refuses jobs predicted to finish later than arrival + budget (status
+    RQ_ERR_DEADLINE)."""
     arr = (ctypes.c_void_p * len(scenes))(*[sc._h.value for sc in scenes])
     js = (Job * max(len(jobs), 1))()
     for k, j in enumerate(jobs):
@@ -882,6 +886,7 @@ def serve(scenes, jobs, layout="cw8", slots=4, hbm_budget=8 << 30, sun=(0.3, 0.8
         js[k].seed, js[k].arrival_s = int(j["seed"]), float(j["arrival_s"])
     cfg = ServeConfig()
     cfg.layout, cfg.slots, cfg.hbm_budget = LAYOUTS.get(layout, layout), int(slots), int(hbm_budget)
+    cfg.latency_budget_us = int(round(latency_budget_ms * 1e3))
     sv = np.asarray(sun, np.float64)
     sv = sv / np.linalg.norm(sv)
     cfg.sun[:] = [float(x) for x in sv.astype(np.float32)]
