@@ -1,6 +1,15 @@
 """Builds librq.so (host C++ with g++, device HIP for gfx950 with hipcc).
 
 Usage: python -m renderlab_amd.build   (or build() from Python)
+       python -m renderlab_amd.build --sanitize
+           builds renderlab_amd/_san/librq_san.so: the host C++ sources (the
+           builders, packers, relayouts and the parsers of external words and
+           cooked files) compiled by g++ with AddressSanitizer and
+           UndefinedBehaviorSanitizer, linked with the usual device objects. Run
+           the CPU tests against it with
+             RQ_LIBRARY=renderlab_amd/_san/librq_san.so \
+             LD_PRELOAD=$(gcc -print-file-name=libasan.so) ASAN_OPTIONS=detect_leaks=0 \
+             python -m pytest tests -m "not gpu"
 
 The kernels are compiled without floating-point contraction and with IEEE
 division and square root, so they compute the same f32 arithmetic as the
@@ -68,6 +77,33 @@ def build(verbose=False):
     return OUT
 
 
+SAN = os.path.join(HERE, "_san")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized():
+    """librq_san.so: sanitized host objects (g++) with the ordinary device objects, linked by g++."""
+    build()  # the device objects
+    os.makedirs(SAN, exist_ok=True)
+    objs, procs = [], []
+    for src in HOST_SOURCES:
+        obj = os.path.join(SAN, src + ".o")
+        cmd = ["g++", *HOST_FLAGS, *SAN_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            raise RuntimeError("command failed: " + " ".join(cmd))
+    objs += [os.path.join(BUILD, src + ".o") for src in DEVICE_SOURCES]
+    out = os.path.join(SAN, "librq_san.so")
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    _run(["g++", "-shared", *SAN_FLAGS, "-o", out, *objs, "-pthread", f"-L{rocm}/lib", f"-Wl,-rpath,{rocm}/lib",
+          "-lamdhip64"])
+    return out
+
+
 def build_oracle():
     oracle = os.path.join(os.path.dirname(HERE), "oracle")
     _run(["make", "-s", "-C", oracle])
@@ -75,5 +111,8 @@ def build_oracle():
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
-    print(build_oracle())
+    if "--sanitize" in sys.argv:
+        print(build_sanitized())
+    else:
+        print(build(verbose=True))
+        print(build_oracle())

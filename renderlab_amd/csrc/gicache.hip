@@ -17,7 +17,7 @@
 // level in the low byte, 7 the packed normal, 8-10 the query point, 12-13 the
 // visibility bits).
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_scan.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -223,6 +223,15 @@ __global__ __launch_bounds__(256) void near_blocked(const uint32_t* words, uint6
     }
 }
 
+__global__ void zero_word(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+// Exclusive prefix sum of n words (rocPRIM); a size query when temp is null.
+hipError_t exclusive_sum(void* temp, size_t& bytes, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t s) {
+    return rocprim::exclusive_scan(temp, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s);
+}
+
 }  // namespace gic
 }  // namespace rq
 
@@ -269,12 +278,13 @@ extern "C" rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (!slot_count) {
-        const hipError_t e = hipMemsetAsync(count, 0, 4, s);
+        hipLaunchKernelGGL(gic::zero_word, dim3(1), dim3(64), 0, s, count);
+        const hipError_t e = hipGetLastError();
         return e == hipSuccess ? RQ_OK : gic_fail(e, "rq_gi_gather_rays");
     }
     size_t scan_bytes = 0;
     uint32_t* counts = nullptr;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, counts, slot_count, s);
+    hipError_t e = gic::exclusive_sum(nullptr, scan_bytes, counts, counts, slot_count, s);
     if (e != hipSuccess) return gic_fail(e, "rq_gi_gather_rays scan size");
     if (workspace_bytes < rq_gi_gather_workspace(slot_count)) {
         set_error("rq_gi_gather_rays: workspace smaller than rq_gi_gather_workspace(slot_count)");
@@ -287,7 +297,7 @@ extern "C" rq_status rq_gi_gather_rays(const uint32_t* table, size_t table_words
     const dim3 grid((slot_count + 255) / 256);
     hipLaunchKernelGGL(gic::gather_counts, grid, dim3(256), 0, s, table, (uint64_t)table_words, slots, slot_count,
                        counts);
-    e = hipcub::DeviceScan::ExclusiveSum(temp, scan_bytes, counts, offsets, slot_count, s);
+    e = gic::exclusive_sum(temp, scan_bytes, counts, offsets, slot_count, s);
     if (e != hipSuccess) return gic_fail(e, "rq_gi_gather_rays scan");
     hipLaunchKernelGGL(gic::gather_write, grid, dim3(256), 0, s, table, (uint64_t)table_words, slots, slot_count,
                        counts, offsets, rays, items, max_rays, count);
@@ -299,8 +309,7 @@ extern "C" size_t rq_gi_gather_workspace(uint32_t slot_count) {
     size_t scan_bytes = 0;
     uint32_t* p = nullptr;
     // the size query only reads its arguments; it cannot fail for a valid count
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, p, p, std::max<uint32_t>(slot_count, 1),
-                                           (hipStream_t)0);
+    (void)gic::exclusive_sum(nullptr, scan_bytes, p, p, std::max<uint32_t>(slot_count, 1), (hipStream_t)0);
     return ((8ull * slot_count + 255) & ~255ull) + scan_bytes + 1024;
 }
 

@@ -28,8 +28,8 @@
 // every stable partition and (key, triangle) sort, so the primitive table is
 // the permutation itself (what map_primitives, src/gi_bvh.rs:108-136, finds).
 #include <hip/hip_runtime.h>
-#include <hipcub/device/device_scan.hpp>
-#include <hipcub/device/device_segmented_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -914,9 +914,10 @@ struct GpuBuild {
 
     rq_status exclusive_sum(const uint32_t* in, uint32_t* out, uint32_t n) {
         size_t bytes = 0;
-        GB_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s), "scan size");
+        GB_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s),
+               "scan size");
         GB_OK(cub.need(bytes, "hipMalloc scan"));
-        GB_TRY(hipcub::DeviceScan::ExclusiveSum(cub.p, bytes, in, out, n, s), "scan");
+        GB_TRY(rocprim::exclusive_scan(cub.p, bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), s), "scan");
         return RQ_OK;
     }
     uint32_t* pinned = nullptr;  // host words for the small per-level readbacks
@@ -1043,12 +1044,12 @@ struct GpuBuild {
                            meshes.as<MeshDev>(), perm.as<uint32_t>(), cent.as<float>(), keys.as<uint64_t>());
         size_t bytes = 0;
         const uint32_t* o = med_off.as<uint32_t>();
-        GB_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
-                                                          (int)total, (int)n, o, o + 1, 0, 64, s),
+        GB_TRY(rocprim::segmented_radix_sort_keys(nullptr, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(), total,
+                                                  n, o, o + 1, 0u, 64u, s),
                "median sort size");
         GB_OK(cub.need(bytes, "hipMalloc median sort"));
-        GB_TRY(hipcub::DeviceSegmentedRadixSort::SortKeys(cub.p, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
-                                                          (int)total, (int)n, o, o + 1, 0, 64, s),
+        GB_TRY(rocprim::segmented_radix_sort_keys(cub.p, bytes, keys.as<uint64_t>(), keys_out.as<uint64_t>(), total, n,
+                                                  o, o + 1, 0u, 64u, s),
                "median sort");
         hipLaunchKernelGGL(median_put, dim3(n), dim3(256), 0, s, med.as<MedianReq>(), med_off.as<uint32_t>(),
                            meshes.as<MeshDev>(), keys_out.as<uint64_t>(), perm.as<uint32_t>(),

@@ -192,6 +192,10 @@ bool build_plain(const std::vector<uint32_t>& words, HostLayout& out, std::strin
                 uint32_t l = child & ~LEAF_BIT;
                 if (l >= m.view.leaf_count) return EMPTY;
                 uint32_t first = words[m.view.leaves + 2ull * l], cnt = words[m.view.leaves + 2ull * l + 1];
+                if ((uint64_t)first / 3 + cnt / 3 > m.view.index_count / 3) {
+                    err = "a leaf's triangles past the mesh's index count";
+                    return EMPTY;
+                }
                 return leaf_tag(m.tri_base + first / 3, cnt / 3, err);
             }
             if (child >= m.view.node_count) return EMPTY;
@@ -321,6 +325,7 @@ struct W8Builder {
             }
             if (pick < 0) break;
             const uint32_t tag = cands[pick].child, first = tag & ((1u << 27) - 1u), n = ((tag >> 27) & 0xfu) + 1u;
+            if ((size_t)first + n > tris->verts.size() / TRI_FLOATS) break;  // the relayout checked every leaf
             std::vector<std::array<float, 3>> ctr(n);
             Bounds cb = empty_bounds();
             for (uint32_t k = 0; k < n; ++k) {

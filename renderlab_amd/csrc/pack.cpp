@@ -402,6 +402,12 @@ bool check_mesh_tree(const std::vector<uint32_t>& w, const MeshView& m, uint32_t
                 err = "a leaf's indices past the end of the buffer";
                 return false;
             }
+            if (first + count > m.index_count) {
+                // the relayouts gather each mesh's index_count / 3 triangles and address a
+                // leaf's triangles inside that range
+                err = "a leaf's indices past the mesh's index count";
+                return false;
+            }
             for (uint64_t k = 0; k < count; ++k)
                 if (w[m.indices + first + k] >= m.vertex_count) {
                     err = "an index past the mesh's vertices";

@@ -1,12 +1,12 @@
 // Ray reordering for incoherent batches: a 30-bit key per ray (direction
 // octant, then a 9-bit-per-axis Morton code of the origin inside the scene's
-// bounds) sorted with rocPRIM's radix sort through hipCUB. The traversal then
+// bounds) sorted with rocPRIM's radix sort. The traversal then
 // takes rays in key order, so a wave's lanes start from nearby origins in the
 // same octant and stay in step longer; every ray's result is still written
 // to its own index.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "internal.h"
 
@@ -52,8 +52,7 @@ hipError_t ray_order(const rq_ray* rays, uint32_t n, const uint32_t* count_ptr, 
                      uint32_t* keys_out, uint32_t* vals_in, uint32_t* order, void* temp, size_t& temp_bytes,
                      hipStream_t s) {
     if (!temp) {
-        return hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys_in, keys_out, vals_in, order, (int)n, 0, 30,
-                                                  s);
+        return rocprim::radix_sort_pairs(nullptr, temp_bytes, keys_in, keys_out, vals_in, order, (size_t)n, 0u, 30u, s);
     }
     float sx = 511.0f / fmaxf(bounds[3] - bounds[0], 1e-20f);
     float sy = 511.0f / fmaxf(bounds[4] - bounds[1], 1e-20f);
@@ -63,7 +62,7 @@ hipError_t ray_order(const rq_ray* rays, uint32_t n, const uint32_t* count_ptr, 
                        bounds[2], sx, sy, sz, keys_in, vals_in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, order, (int)n, 0, 30, s);
+    return rocprim::radix_sort_pairs(temp, temp_bytes, keys_in, keys_out, vals_in, order, (size_t)n, 0u, 30u, s);
 }
 
 }  // namespace rq

@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "librq.so")
+# RQ_LIBRARY selects another build of the same library (the sanitized host build of
+# `python -m renderlab_amd.build --sanitize`); it never selects a fallback.
+_LIB_PATH = os.environ.get("RQ_LIBRARY") or os.path.join(_HERE, "librq.so")
 
 RQ_OK = 0
 RQ_ERR_INVALID_INPUT = 1

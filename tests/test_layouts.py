@@ -267,6 +267,12 @@ def test_external_words_must_hold_finite_trees_inside_their_arrays():
     # first + k, the device layouts step by triangles: such words would trace differently)
     bad(lambda w: w.__setitem__(leaves_at + 2 * 4, 3 * 4 + 1), "multiple of 3")
     bad(lambda w: w.__setitem__(leaves_at + 2 * 4 + 1, 2), "multiple of 3")
+    # a leaf running past its mesh's indices (into words after them, still valid vertex indices): the
+    # relayouts address a leaf's triangles inside the mesh's own range
+    w = np.concatenate([base, np.zeros(60, np.uint32)])
+    w[leaves_at + 2 * 2 + 1] = 3 * 15
+    with pytest.raises(rq.InvalidInput, match="mesh's index count"):
+        rq.Scene.from_gi_words(w)
     # the primitive table (word 28) and a merged world's identity table (words 24-26) are read
     # after hits; both must lie inside the buffer, the identity table in position order
     bad(lambda w: w.__setitem__(r + 28, len(base) - 3), "primitive table")
