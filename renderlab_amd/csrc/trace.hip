@@ -577,7 +577,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         // ---- the wave runs one kind of step at a time: whichever most lanes want
         // (1 triangle pair, 2 instance entry, 3 node). The others wait a step; each
         // lane's own sequence of steps is unchanged.
-        const int kind = fin ? 0 : (tri_left > 0 ? 1 : ((nxt & LEAF_BIT) ? 2 : 3));
+        // The 8-wide layouts enter an instance and visit its BLAS root in one node step
+        // (kind 3 covers both), so instance entries share the node steps' iterations;
+        // each lane's sequence of visits is unchanged. (PLAIN measured slower merged.)
+        constexpr bool MERGE = W8;
+        const int kind = fin ? 0 : (tri_left > 0 ? 1 : ((nxt & LEAF_BIT) && !MERGE ? 2 : 3));
         const uint32_t n_tri = (uint32_t)__popcll(__ballot(kind == 1));
         const uint32_t n_inst = (uint32_t)__popcll(__ballot(kind == 2));
         const uint32_t n_node = (uint32_t)__popcll(__ballot(kind == 3));
@@ -614,9 +618,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 st_want_tri += n_tri;
                 st_want_inst += n_inst;
             }
-            st_node += (kind == 3 && (pick == 3 || dual)) || spec;
-            st_tri += kind == 1 && (pick == 1 || dual);
-            st_inst += kind == 2 && (pick == 2 || dual);
+            st_tri += kind == 1 && (pick == 1 || dual);  // node and instance steps: counted where they run
         }
         if (kind == 1 && (pick == 1 || dual)) {
             // ---- triangle step (:126-168): the next two triangles in leaf order
@@ -681,9 +683,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             tri_next += k;
             tri_left -= k;
         } else if ((kind >= 2 && (kind == pick || dual)) || spec) {
-            const uint32_t item = spec ? spec_item : nxt;
+            uint32_t item = spec ? spec_item : nxt;
             nxt = NONE;  // consumed (a speculating lane's pending node, or its stack top already popped)
-            if (kind == 2) {
+            // an instance (a top-level or flat-loop leaf; speculation takes nodes only) or a node
+            bool node_step = !(item & LEAF_BIT);
+            if (!node_step) {
+                if (COUNT) st_inst++;
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
@@ -744,8 +749,16 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     }
                     floor_ = sp;
                     in_blas = true;
-                    nxt = root;  // pushed and popped at once
+                    if (MERGE && !(root & LEAF_BIT)) {
+                        item = root;  // the BLAS root: visited in this same step
+                        node_step = true;
+                    } else {
+                        nxt = root;  // pushed and popped at once
+                    }
                 }
+            }
+            if (node_step && COUNT) st_node++;
+            if (!node_step) {
             } else if (LAYOUT == REF || LAYOUT == RS) {
                 // RS: rs_trace's top level and BLAS nodes are the same records; its
                 // closest hit does not depend on the visit order (rs_limit + the

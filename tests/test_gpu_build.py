@@ -219,3 +219,25 @@ def test_gpu_reflection_build_equals_host_build(cases):
     h, _ = gpu_trace(b, "rs", rays, rq.CLOSEST)
     o, _, _ = oracle.rs_traverse(b.words, rays, 0)
     assert len(compare_exact(h, o)) == 0
+
+
+def test_gpu_built_reflection_scene_uploads_rs_cw8_before_any_host_read():
+    """A GPU-built reflection scene keeps its mesh words on the device until a host reader asks;
+    uploading RS_CW8 first must fetch them (the relayout reads them) and trace like rs_trace."""
+    g = np.random.default_rng(12)
+    scene = rq.Scene.generate(1, 5, 80, 60000, reflection=True, gpu=True)
+    scene.upload(("rs_cw8",))  # before scene.words or any other host read
+    rays = np.zeros(4000, oracle.RAY_DTYPE)
+    rays["origin"] = g.uniform(-60, 60, (4000, 3))
+    rays["direction"] = g.normal(size=(4000, 3))
+    rays["tmax"] = 1e20
+    print("case: rs_cw8 upload of a GPU-built reflection scene", flush=True)
+    h, _ = gpu_trace(scene, "rs_cw8", rays, rq.CLOSEST)
+    _, vis = gpu_trace(scene, "rs_cw8", rays, rq.ANY)
+    host = rq.Scene.generate(1, 5, 80, 60000, reflection=True)
+    assert_same_words(host, scene)
+    o, ov, _ = oracle.rs_traverse(host.words, rays, 0)
+    assert len(compare_exact(h, o)) == 0
+    assert (h["flags"] & 1).sum() > 500
+    _, ov, _ = oracle.rs_traverse(host.words, rays, 1, want_hits=False, want_vis=True)
+    assert np.array_equal(vis, ov)
