@@ -13,14 +13,21 @@
 //   per hit.
 // CW8_Q8: 8-wide nodes with 8-bit child boxes on a power-of-two grid (after
 //   Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
-//   Through Compressed Wide BVHs", HPG 2017), collapsed from the 4-wide
-//   trees, for the BLAS and the top level alike. A child's box encloses the
-//   PLAIN box of the subtree it holds, rounded outward, and decodes exactly
-//   as origin + 2^e*q with no rounding. Leaves and triangles are the
-//   PLAIN layout's, in the same order.
+//   Through Compressed Wide BVHs", HPG 2017), for the BLAS and the top level
+//   alike. Each mesh's tree and the top level are built again from the
+//   triangles and the instances' world boxes (binned SAH, then that paper's
+//   cost-optimal collapse into 8-wide nodes), or, for words whose trees do not
+//   hold each triangle or instance once, collapsed from the 4-wide trees. A
+//   child's box encloses its subtree's triangles (or instance boxes), rounded
+//   outward, and decodes exactly as origin + 2^e*q with no rounding. The
+//   triangles are PLAIN's, reordered within their mesh so each leaf is a
+//   contiguous range; the side records keep each triangle's slot and indices.
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <atomic>
+#include <limits>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 
@@ -285,6 +292,9 @@ struct W8Builder {
     // GI mesh trees: the CW8 copy of the triangle records, whose leaves may be split
     // (null for the top level and for reflection trees, whose leaf tags differ)
     HostLayout* tris = nullptr;
+    // Trees built from the triangles: each source node is a list of at most 8
+    // children (leaf tags, or indices of further lists), taken as they are.
+    const std::vector<std::vector<Cand>>* lists = nullptr;
     std::vector<Cand> cands_by_slot_;
     std::vector<char> slots_used_;
 
@@ -370,9 +380,13 @@ struct W8Builder {
         return index;
     }
 
+    std::vector<Cand> source_children(uint32_t node) const {
+        return lists ? (*lists)[node] : wide_children(*src, node);
+    }
+
     bool fill(uint32_t index, std::vector<Cand> cands) {
         // Open the largest inner candidate while its children fit in 8 slots.
-        for (;;) {
+        for (; !lists;) {
             int pick = -1;
             for (int i = 0; i < (int)cands.size(); ++i) {
                 if (cands[i].child & LEAF_BIT) continue;
@@ -384,7 +398,7 @@ struct W8Builder {
             cands.erase(cands.begin() + pick);
             cands.insert(cands.begin() + pick, kids.begin(), kids.end());
         }
-        if (tris) split_leaves(cands);
+        if (tris && !lists) split_leaves(cands);
         Bounds all = empty_bounds();
         for (auto& c : cands) all = merge(all, c.box);
         // Slot assignment by ray octant (Ylitie et al. 2017, sec. 3.2): slot s
@@ -511,7 +525,7 @@ struct W8Builder {
                (valid << 24);
         std::copy(w, w + W8_WORDS, out->begin() + (size_t)index * W8_WORDS);
         for (auto& t : todo)
-            if (!fill(t.first, wide_children(*src, t.second))) return false;
+            if (!fill(t.first, source_children(t.second))) return false;
         return true;
     }
 
@@ -523,9 +537,267 @@ struct W8Builder {
         }
         uint32_t index = alloc();
         out_root = index;
-        return fill(index, wide_children(*src, src_root));
+        return fill(index, source_children(src_root));
     }
 };
+
+// ---- CW8 mesh trees built from the triangles
+// The 4-wide trees of the words are the reference builder's; the CW8 layout
+// needs only their triangles. A mesh's CW8 tree is built again from them: a
+// binary tree by binned SAH over the triangles' box centres (32 bins, every
+// axis), then collapsed into nodes of at most 8 children by the dynamic program
+// of Ylitie, Karras and Laine (HPG 2017, sec. 3.1), which finds the collapse of
+// least SAH cost. C(n, i) is the cost of binary node n spread over at most i
+// child slots:
+//   C(n, 1) = min(leaf: A(n) * C_PAIR * ceil(T(n) / 2) when T(n) <= LEAF_MAX,
+//                 node: A(n) * C_NODE + D(n, 8))
+//   C(n, i) = min(D(n, i), C(n, i - 1)),   D(n, i) = min_k C(l, k) + C(r, i - k)
+// with A the box's half area and T the triangle count. Leaves are priced per
+// triangle-pair step (the traversal tests two triangles a step); a node step
+// costs about twice a pair step on the kernel's measured instruction counts.
+// The mesh's triangles are reordered so each leaf is a contiguous range.
+constexpr int SAH_BINS = 32;
+constexpr uint32_t SAH_DEPTH = 64;     // deeper splits take the median (bounds the depth)
+constexpr uint32_t DP_LEAF_MAX = 8;
+// C_PAIR = C_NODE: a pair step and a node step each take a wave iteration; the
+// measured optimum on the 4M-triangle frame was flat over C_PAIR in [1, 3].
+// Top-level leaves are instances (one a leaf), priced like a node step (the
+// kernel enters an instance and visits its root in one step).
+constexpr double DP_C_NODE = 1.0, DP_C_PAIR = 1.0, DP_C_INST = 1.0;
+
+struct BinNode {
+    Bounds box;
+    uint32_t left = 0, right = 0;   // 0: a single triangle
+    uint32_t first = 0, count = 0;  // triangles order[first, first + count)
+};
+
+double area2(const Bounds& b) {
+    double e[3];
+    for (int a = 0; a < 3; ++a) e[a] = std::max((double)b.hi[a] - (double)b.lo[a], 0.0);
+    return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+}
+
+// Bounds of triangles [0, n) of `verts` (TRI_FLOATS a triangle).
+std::vector<Bounds> triangle_boxes(const float* verts, uint32_t n) {
+    std::vector<Bounds> tb(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        Bounds b = empty_bounds();
+        for (int v = 0; v < 3; ++v)
+            for (int a = 0; a < 3; ++a) {
+                const float x = verts[(size_t)TRI_FLOATS * k + 3 * v + a];
+                b.lo[a] = mn(b.lo[a], x);
+                b.hi[a] = mx(b.hi[a], x);
+            }
+        tb[k] = b;
+    }
+    return tb;
+}
+
+// Binary tree over primitives with boxes `tb` (finite); `order` receives the
+// primitive order its nodes' ranges refer to. Node 0 is the root; children
+// always follow their parent.
+std::vector<BinNode> sah_binary(const std::vector<Bounds>& tb, std::vector<uint32_t>& order) {
+    const uint32_t n = (uint32_t)tb.size();
+    std::vector<std::array<float, 3>> c(n);
+    for (uint32_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) c[k][a] = 0.5f * tb[k].lo[a] + 0.5f * tb[k].hi[a];
+    order.resize(n);
+    for (uint32_t k = 0; k < n; ++k) order[k] = k;
+    std::vector<BinNode> nodes(1);
+    nodes.reserve(2 * (size_t)n);
+    nodes[0].count = n;
+    std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};  // (node, depth)
+    while (!todo.empty()) {
+        const auto [x, depth] = todo.back();
+        todo.pop_back();
+        const uint32_t first = nodes[x].first, cnt = nodes[x].count;
+        Bounds box = empty_bounds(), cb = empty_bounds();
+        for (uint32_t k = first; k < first + cnt; ++k) {
+            box = merge(box, tb[order[k]]);
+            for (int a = 0; a < 3; ++a) {
+                cb.lo[a] = mn(cb.lo[a], c[order[k]][a]);
+                cb.hi[a] = mx(cb.hi[a], c[order[k]][a]);
+            }
+        }
+        nodes[x].box = box;
+        if (cnt == 1) continue;
+        int axis = -1, split = 0;
+        if (depth < SAH_DEPTH) {
+            double best = std::numeric_limits<double>::infinity();
+            for (int a = 0; a < 3; ++a) {
+                const double lo = cb.lo[a], ext = (double)cb.hi[a] - lo;
+                if (!(ext > 0.0)) continue;
+                const double scale = SAH_BINS / ext;
+                Bounds bb[SAH_BINS];
+                uint32_t bn[SAH_BINS] = {};
+                for (auto& b : bb) b = empty_bounds();
+                for (uint32_t k = first; k < first + cnt; ++k) {
+                    const int i = std::min(SAH_BINS - 1, (int)(((double)c[order[k]][a] - lo) * scale));
+                    bb[i] = merge(bb[i], tb[order[k]]);
+                    bn[i]++;
+                }
+                double right_area[SAH_BINS];
+                uint32_t right_n[SAH_BINS];
+                Bounds acc = empty_bounds();
+                uint32_t m = 0;
+                for (int i = SAH_BINS - 1; i > 0; --i) {
+                    acc = merge(acc, bb[i]);
+                    m += bn[i];
+                    right_area[i] = area2(acc);
+                    right_n[i] = m;
+                }
+                acc = empty_bounds();
+                m = 0;
+                for (int i = 1; i < SAH_BINS; ++i) {
+                    acc = merge(acc, bb[i - 1]);
+                    m += bn[i - 1];
+                    if (m == 0 || right_n[i] == 0) continue;
+                    const double cost = area2(acc) * m + right_area[i] * right_n[i];
+                    if (cost < best) {
+                        best = cost;
+                        axis = a;
+                        split = i;
+                    }
+                }
+            }
+        }
+        uint32_t mid;
+        if (axis >= 0) {
+            const double lo = cb.lo[axis], scale = SAH_BINS / ((double)cb.hi[axis] - lo);
+            auto it = std::partition(order.begin() + first, order.begin() + first + cnt, [&](uint32_t k) {
+                return std::min(SAH_BINS - 1, (int)(((double)c[k][axis] - lo) * scale)) < split;
+            });
+            mid = (uint32_t)(it - (order.begin() + first));
+        } else {
+            // no usable SAH split (equal centres, or past SAH_DEPTH): the median along the widest axis
+            int a = 0;
+            for (int b = 1; b < 3; ++b)
+                if (cb.hi[b] - cb.lo[b] > cb.hi[a] - cb.lo[a]) a = b;
+            mid = cnt / 2;
+            std::stable_sort(order.begin() + first, order.begin() + first + cnt,
+                             [&](uint32_t p, uint32_t q) { return c[p][a] < c[q][a]; });
+        }
+        const uint32_t l = (uint32_t)nodes.size();
+        nodes.resize(nodes.size() + 2);
+        nodes[l].first = first;
+        nodes[l].count = mid;
+        nodes[l + 1].first = first + mid;
+        nodes[l + 1].count = cnt - mid;
+        nodes[x].left = l;
+        nodes[x].right = l + 1;
+        todo.push_back({l + 1, depth + 1});
+        todo.push_back({l, depth + 1});
+    }
+    return nodes;
+}
+
+// The collapse of a binary tree into lists of at most 8 children (leaf tags from
+// `leaf_tag` for binary nodes taken whole as leaves, of at most `leaf_max`
+// primitives priced `leaf_price` (count) each per unit of area; list indices for
+// nodes). Returns the root tag: a leaf tag, or list 0.
+template <typename Tag, typename Price>
+uint32_t dp_collapse(const std::vector<BinNode>& nodes, uint32_t leaf_max, Price leaf_price, Tag leaf_tag,
+                     std::vector<std::vector<Cand>>& lists) {
+    struct Entry {
+        float cost[8];
+        int8_t how[8];  // i = 1: -1 leaf, -2 node; i > 1: -3 as i - 1, else k slots to the left child
+        int8_t k8;      // the node's own split of 8 slots
+    };
+    std::vector<Entry> dp(nodes.size());
+    // areas relative to the root's (boxes of any finite size keep finite costs)
+    const double root_area = area2(nodes[0].box), scale = root_area > 0.0 ? 1.0 / root_area : 1.0;
+    for (size_t x = nodes.size(); x-- > 0;) {
+        const BinNode& nd = nodes[x];
+        Entry& e = dp[x];
+        const double a = std::min(area2(nd.box) * scale, 1.0);
+        if (nd.left == 0) {
+            for (int i = 0; i < 8; ++i) {
+                e.cost[i] = (float)(a * leaf_price(1u));
+                e.how[i] = i == 0 ? -1 : -3;
+            }
+            e.k8 = 0;
+            continue;
+        }
+        const Entry &l = dp[nd.left], &r = dp[nd.right];
+        double d[9];
+        int dk[9];
+        for (int j = 2; j <= 8; ++j) {
+            d[j] = std::numeric_limits<double>::infinity();
+            dk[j] = 1;
+            for (int k = 1; k < j; ++k) {
+                const double s = (double)l.cost[k - 1] + (double)r.cost[j - k - 1];
+                if (s < d[j]) {
+                    d[j] = s;
+                    dk[j] = k;
+                }
+            }
+        }
+        const double as_node = a * DP_C_NODE + d[8];
+        const bool leaf = nd.count <= leaf_max && a * leaf_price(nd.count) <= as_node;
+        e.cost[0] = (float)(leaf ? a * leaf_price(nd.count) : as_node);
+        e.how[0] = leaf ? -1 : -2;
+        e.k8 = (int8_t)dk[8];
+        for (int i = 2; i <= 8; ++i) {
+            if (d[i] < (double)e.cost[i - 2]) {
+                e.cost[i - 1] = (float)d[i];
+                e.how[i - 1] = (int8_t)dk[i];
+            } else {
+                e.cost[i - 1] = e.cost[i - 2];
+                e.how[i - 1] = -3;
+            }
+        }
+    }
+    if (dp[0].how[0] == -1) return leaf_tag(nodes[0]);
+    // the binary nodes that fill node x's slots: its children's subtrees spread over k8 and 8 - k8
+    std::vector<uint32_t> slots;
+    auto gather = [&](auto&& self, uint32_t x, int i) -> void {
+        while (i > 1 && dp[x].how[i - 1] == -3) --i;
+        if (i == 1) {
+            slots.push_back(x);
+            return;
+        }
+        const int k = dp[x].how[i - 1];
+        self(self, nodes[x].left, k);
+        self(self, nodes[x].right, i - k);
+    };
+    std::vector<uint32_t> todo{0u}, list_of{0u};  // binary node -> its list
+    lists.assign(1, {});
+    for (size_t q = 0; q < todo.size(); ++q) {
+        const uint32_t x = todo[q], li = list_of[q];
+        slots.clear();
+        gather(gather, nodes[x].left, dp[x].k8);
+        gather(gather, nodes[x].right, 8 - dp[x].k8);
+        std::vector<Cand> kids;
+        for (uint32_t s : slots) {
+            Cand c;
+            c.box = nodes[s].box;
+            if (dp[s].how[0] == -1) {
+                c.child = leaf_tag(nodes[s]);
+            } else {
+                c.child = (uint32_t)lists.size();
+                lists.emplace_back();
+                todo.push_back(s);
+                list_of.push_back(c.child);
+            }
+            kids.push_back(c);
+        }
+        lists[li] = std::move(kids);
+    }
+    return 0u;
+}
+
+// Levels of the list tree under list 0.
+uint32_t list_levels(const std::vector<std::vector<Cand>>& lists) {
+    std::vector<uint32_t> level(lists.size(), 1u);
+    uint32_t deepest = 1;
+    for (size_t i = 0; i < lists.size(); ++i)  // a list's children come after it
+        for (const Cand& c : lists[i])
+            if (!(c.child & LEAF_BIT)) {
+                level[c.child] = level[i] + 1;
+                deepest = std::max(deepest, level[c.child]);
+            }
+    return deepest;
+}
 
 // Levels of the 8-wide tree under `tag` (0 for a leaf tag), counted without
 // recursion; stops once past `limit`.
@@ -574,6 +846,116 @@ uint32_t wide_levels(const std::vector<uint32_t>& nodes, uint32_t tag, uint32_t 
     return deepest;
 }
 
+// The CW8 tree of one mesh built from its triangles [tb, tb + n) of `out`, whose
+// records it reorders, into `nodes` (indices from 0). False with `err` empty when
+// the mesh's 4-wide tree (root tag `pr` in `plain`) does not hold each of those
+// triangles exactly once, a vertex is not finite, or the tree would pass
+// W8_MAX_LEVELS: the caller collapses that tree instead.
+bool rebuild_mesh(const HostLayout& plain, uint32_t pr, uint32_t tb, uint32_t n, HostLayout& out,
+                  std::vector<uint32_t>& nodes, uint32_t& root, std::string& err) {
+    if (n == 0 || (size_t)tb + n > out.verts.size() / TRI_FLOATS) return false;
+    for (size_t k = (size_t)TRI_FLOATS * tb; k < (size_t)TRI_FLOATS * (tb + n); ++k)
+        if (!std::isfinite(out.verts[k])) return false;
+    std::vector<uint8_t> seen(n, 0);
+    std::vector<uint32_t> todo{pr};
+    while (!todo.empty()) {
+        const uint32_t t = todo.back();
+        todo.pop_back();
+        if (t & LEAF_BIT) {
+            const uint32_t first = t & ((1u << LEAF_COUNT_SHIFT) - 1u), cnt = ((t >> LEAF_COUNT_SHIFT) & 0xfu) + 1u;
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                if (k < tb || k - tb >= n || seen[k - tb]) return false;
+                seen[k - tb] = 1;
+            }
+            continue;
+        }
+        if (((size_t)t + 1) * TOP_WORDS > plain.nodes.size()) return false;
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t c = plain.nodes[(size_t)t * TOP_WORDS + 24 + s];
+            if (c != EMPTY) todo.push_back(c);
+        }
+        if (todo.size() > 4 * (size_t)n + 64) return false;  // not a tree over these triangles
+    }
+    for (uint8_t s : seen)
+        if (!s) return false;
+    std::vector<uint32_t> order;
+    const std::vector<BinNode> bin = sah_binary(triangle_boxes(&out.verts[(size_t)TRI_FLOATS * tb], n), order);
+    std::vector<std::vector<Cand>> lists;
+    const uint32_t rt = dp_collapse(
+        bin, DP_LEAF_MAX, [](uint32_t cnt) { return DP_C_PAIR * ((cnt + 1) / 2); },
+        [&](const BinNode& x) { return LEAF_BIT | ((x.count - 1u) << LEAF_COUNT_SHIFT) | (tb + x.first); }, lists);
+    if (rt & LEAF_BIT) {
+        root = rt;
+    } else {
+        if (list_levels(lists) > W8_MAX_LEVELS) return false;
+        W8Builder w{nullptr, &nodes, &err};
+        w.lists = &lists;
+        if (!w.root(rt, root)) return false;
+    }
+    // triangle position tb + p now holds the triangle that was at tb + order[p]
+    auto permute = [&](auto& v, size_t width) {
+        std::vector<typename std::decay_t<decltype(v)>::value_type> tmp(v.begin() + width * tb,
+                                                                        v.begin() + width * (tb + n));
+        for (uint32_t p = 0; p < n; ++p)
+            std::copy(tmp.begin() + width * order[p], tmp.begin() + width * (order[p] + 1),
+                      v.begin() + width * (tb + p));
+    };
+    permute(out.tris, TRI_FLOATS);
+    permute(out.tri_ids, TRI_ID_WORDS);
+    permute(out.verts, TRI_FLOATS);
+    return true;
+}
+
+// The CW8 top level built from the instances' world boxes (record words 12-17),
+// one instance a leaf. False with `err` empty when the 4-wide top level does not
+// reach each instance exactly once, a box is not finite, or the tree would pass
+// W8_MAX_LEVELS: the caller collapses the 4-wide top level instead.
+bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
+    const uint32_t count = plain.instance_count;
+    if (count == 0) return false;
+    std::vector<uint8_t> seen(count, 0);
+    std::vector<uint32_t> todo{plain.top_root};
+    size_t visits = 0;
+    while (!todo.empty()) {
+        const uint32_t t = todo.back();
+        todo.pop_back();
+        if (t & LEAF_BIT) {
+            const uint32_t i = t & ~LEAF_BIT;
+            if (i >= count || seen[i]) return false;
+            seen[i] = 1;
+            continue;
+        }
+        if (((size_t)t + 1) * TOP_WORDS > plain.top.size() || ++visits > 4 * (size_t)count + 64) return false;
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t c = plain.top[(size_t)t * TOP_WORDS + 24 + s];
+            if (c != EMPTY) todo.push_back(c);
+        }
+    }
+    for (uint8_t s : seen)
+        if (!s) return false;
+    std::vector<Bounds> boxes(count);
+    for (uint32_t i = 0; i < count; ++i)
+        for (int a = 0; a < 3; ++a) {
+            boxes[i].lo[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 12 + a]);
+            boxes[i].hi[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 15 + a]);
+            if (!std::isfinite(boxes[i].lo[a]) || !std::isfinite(boxes[i].hi[a])) return false;
+        }
+    std::vector<uint32_t> order;
+    const std::vector<BinNode> bin = sah_binary(boxes, order);
+    std::vector<std::vector<Cand>> lists;
+    const uint32_t rt = dp_collapse(
+        bin, 1, [](uint32_t) { return DP_C_INST; }, [&](const BinNode& x) { return LEAF_BIT | order[x.first]; },
+        lists);
+    if (rt & LEAF_BIT) {
+        out.top_root = rt;
+        return true;
+    }
+    if (list_levels(lists) > W8_MAX_LEVELS) return false;
+    W8Builder w{nullptr, &out.top, &err};
+    w.lists = &lists;
+    return w.root(rt, out.top_root);
+}
+
 }  // namespace
 
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
@@ -583,34 +965,83 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     out.tri_ids = plain.tri_ids;
     out.verts = plain.verts;
     out.instance_count = plain.instance_count;
-    // BLAS: every mesh root reachable from an instance record, collapsed once.
-    W8Builder b{&plain.nodes, &out.nodes, &err};
-    b.tris = &out;
-    std::vector<std::pair<uint32_t, uint32_t>> done;  // plain root -> cw8 root
+    // BLAS: every mesh root reachable from an instance record, built once -- from
+    // its triangles where the mesh's 4-wide tree holds each of them once, else
+    // collapsed from that tree. Meshes are built on threads, each into nodes of
+    // its own (each moves only its own triangles), then placed one after another.
+    struct MeshJob {
+        uint32_t pr, tb, n;
+        std::vector<uint32_t> nodes;
+        uint32_t root = EMPTY;
+        std::string err;
+    };
+    std::vector<MeshJob> jobs;
+    std::unordered_map<uint32_t, uint32_t> job_of;  // plain root -> job
+    const size_t total_tris = out.verts.size() / TRI_FLOATS;
     for (uint32_t i = 0; i < plain.instance_count; ++i) {
-        uint32_t pr = plain.inst[(size_t)i * INSTANCE_WORDS + 18];
-        uint32_t nr = EMPTY;
-        for (auto& d : done)
-            if (d.first == pr) nr = d.second;
-        if (nr == EMPTY) {
-            if (wide_levels(plain.nodes, pr, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS) {
-                err = "CW8 relayout: a mesh tree deeper than 16 levels";
-                return false;
-            }
-            if (!b.root(pr, nr)) return false;
-            if (w8_levels(out.nodes, nr, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
-                err = "CW8 relayout: a mesh tree deeper than 16 levels";
-                return false;
-            }
-            done.push_back({pr, nr});
-        }
-        out.inst[(size_t)i * INSTANCE_WORDS + 18] = nr;
+        const uint32_t pr = plain.inst[(size_t)i * INSTANCE_WORDS + 18];
+        if (job_of.count(pr)) continue;
+        const uint32_t mesh = plain.inst[(size_t)i * INSTANCE_WORDS + 21];
+        const uint32_t tb = plain.inst[(size_t)i * INSTANCE_WORDS + 20];
+        const size_t te = (size_t)mesh + 1 < plain.mesh_base.size() / 3 ? plain.mesh_base[3 * (mesh + 1) + 2]
+                                                                         : total_tris;
+        job_of.emplace(pr, (uint32_t)jobs.size());
+        jobs.push_back(MeshJob{pr, tb, (uint32_t)(te - tb), {}, EMPTY, {}});
     }
-    // Top level: the reference's 4-wide float top level collapsed the same way
-    // (instance leaves keep their LEAF | instance tags).
+    auto build_one = [&](MeshJob& j) {
+        if (rebuild_mesh(plain, j.pr, j.tb, j.n, out, j.nodes, j.root, j.err) || !j.err.empty()) return;
+        if (wide_levels(plain.nodes, j.pr, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS) {
+            j.err = "CW8 relayout: a mesh tree deeper than 16 levels";
+            return;
+        }
+        W8Builder b{&plain.nodes, &j.nodes, &j.err};
+        b.tris = &out;
+        if (b.root(j.pr, j.root) && w8_levels(j.nodes, j.root, W8_MAX_LEVELS) > W8_MAX_LEVELS)
+            j.err = "CW8 relayout: a mesh tree deeper than 16 levels";
+    };
+    {
+        const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 16));
+        std::vector<std::thread> pool;
+        std::atomic_uint next{0};
+        for (uint32_t t = 0; t < std::min<size_t>(threads, jobs.size()); ++t)
+            pool.emplace_back([&]() {
+                for (uint32_t k; (k = next.fetch_add(1)) < jobs.size();) {
+                    try {
+                        build_one(jobs[k]);
+                    } catch (const std::bad_alloc&) {
+                        jobs[k].err = "CW8 relayout: out of host memory";
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    std::vector<uint32_t> roots(jobs.size());
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        MeshJob& j = jobs[k];
+        if (!j.err.empty()) {
+            err = j.err;
+            return false;
+        }
+        // the mesh's node indices (its children's tags and its root) move by its first node
+        const uint32_t base = (uint32_t)(out.nodes.size() / W8_WORDS);
+        for (size_t w = 0; w < j.nodes.size(); w += W8_WORDS)
+            for (size_t s = 16; s < 24; ++s)
+                if (!(j.nodes[w + s] & LEAF_BIT)) j.nodes[w + s] += base;
+        out.nodes.insert(out.nodes.end(), j.nodes.begin(), j.nodes.end());
+        roots[k] = (j.root & LEAF_BIT) ? j.root : j.root + base;
+        std::vector<uint32_t>().swap(j.nodes);
+    }
+    for (uint32_t i = 0; i < plain.instance_count; ++i)
+        out.inst[(size_t)i * INSTANCE_WORDS + 18] = roots[job_of.at(plain.inst[(size_t)i * INSTANCE_WORDS + 18])];
+    // Top level: built from the instances' world boxes where the 4-wide top level
+    // reaches each instance once, else that top level collapsed the same way as
+    // the meshes' trees (instance leaves keep their LEAF | instance tags).
     out.top_on = plain.top_on;
     out.top_root = plain.top_root;
-    if (plain.top_on) {
+    if (plain.top_on && !rebuild_top(plain, out, err)) {
+        if (!err.empty()) return false;
+        out.top.clear();
+        out.top_root = plain.top_root;
         W8Builder t{&plain.top, &out.top, &err};
         if (wide_levels(plain.top, plain.top_root, 7 * W8_MAX_LEVELS) > 7 * W8_MAX_LEVELS) {
             err = "CW8 relayout: a top level deeper than 16 levels";

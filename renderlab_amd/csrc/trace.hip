@@ -574,18 +574,22 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                 nxt = NONE;
             }
         }
-        // ---- the wave runs one kind of step at a time: whichever most lanes want
-        // (1 triangle pair, 2 instance entry, 3 node). The others wait a step; each
-        // lane's own sequence of steps is unchanged.
+        // ---- the wave runs one kind of step at a time (1 triangle pair, 2 instance
+        // entry, 3 node), chosen by how many lanes want each. The others wait a step;
+        // each lane's own sequence of steps is unchanged.
         // The 8-wide layouts enter an instance and visit its BLAS root in one node step
         // (kind 3 covers both), so instance entries share the node steps' iterations;
         // each lane's sequence of visits is unchanged. (PLAIN measured slower merged.)
+        // Their node step costs about twice a pair step, so they take node steps only
+        // while twice as many lanes want one as want a pair (the measured optimum on
+        // the 4M-triangle frame); PLAIN takes whichever most lanes want.
         constexpr bool MERGE = W8;
+        constexpr uint32_t TRI_WEIGHT = W8 ? 2u : 1u;
         const int kind = fin ? 0 : (tri_left > 0 ? 1 : ((nxt & LEAF_BIT) && !MERGE ? 2 : 3));
         const uint32_t n_tri = (uint32_t)__popcll(__ballot(kind == 1));
-        const uint32_t n_inst = (uint32_t)__popcll(__ballot(kind == 2));
+        const uint32_t n_inst = MERGE ? 0u : (uint32_t)__popcll(__ballot(kind == 2));
         const uint32_t n_node = (uint32_t)__popcll(__ballot(kind == 3));
-        const int pick = (n_node >= n_tri && n_node >= n_inst) ? 3 : (n_tri >= n_inst ? 1 : 2);
+        const int pick = (n_node >= n_tri * TRI_WEIGHT && n_node >= n_inst) ? 3 : (n_tri >= n_inst ? 1 : 2);
         // CW8 speculation (after Aila and Laine, "Understanding the Efficiency of Ray
         // Traversal on GPUs", HPG 2009): while the wave runs node steps, a lane whose
         // leaf still has triangles to test takes a node step too -- its pending node,

@@ -68,9 +68,12 @@ def vis_bits(vis, n):
     return ((vis[np.arange(n) // 32] >> (np.arange(n) % 32)) & 1).astype(bool)
 
 
-def chain_words(depth):
+def chain_words(depth, hole=False):
     """One instance of a mesh whose 4-wide tree is a chain `depth` nodes deep: node k holds leaf k
-    and node k - 1 (node 0 holds leaves 0 and `depth`); every child box is [0, 255]^3."""
+    and node k - 1 (node 0 holds leaves 0 and `depth`); every child box is [0, 255]^3. With `hole`,
+    leaf 0 holds triangle 1 as leaf 1 does, and no leaf triangle 0: a tree that does not hold each
+    of its mesh's triangles once, whose shape the CW8 relayout keeps (it builds other meshes' trees
+    again from their triangles)."""
     tris = depth + 1
     pos = np.zeros((3 * tris, 3), np.float32)
     for t in range(tris):
@@ -83,6 +86,8 @@ def chain_words(depth):
         nodes[k, 8:10] = rq.EMPTY
         nodes[k, 13:16] = 0xFFFF  # upper codes 255 for children 0 and 1
     leaves = np.array([[3 * t, 3] for t in range(tris)], np.uint32)
+    if hole:
+        leaves[0, 0] = 3
     rec = np.zeros(32, np.uint32)
     rec[[0, 5, 10]] = np.float32(1.0).view(np.uint32)  # identity rows
     rec[15:18] = np.float32(256.0).view(np.uint32)      # world box [0, 256]^3

@@ -75,11 +75,11 @@ def test_plain_boxes_are_the_reference_decode(kind, seed, meshes, tris):
 
 
 @pytest.mark.parametrize("kind,seed,meshes,tris", [(0, 1, 0, 0), (1, 5, 60, 20000)])
-def test_cw8_leaves_split_plain_leaves_and_their_boxes_enclose_their_triangles(kind, seed, meshes, tris):
-    """CW8 keeps PLAIN's triangles, each moved at most within its PLAIN leaf's range (its triangle
-    record, side record and vertices together); each CW8 leaf is a contiguous part of one PLAIN leaf,
-    and the CW8 leaves under a mesh root cover the PLAIN leaves' triangles exactly once; every CW8
-    child box encloses its leaf's vertices (inner children: their subtrees' leaves)."""
+def test_cw8_leaves_cover_their_mesh_and_their_boxes_enclose_their_triangles(kind, seed, meshes, tris):
+    """CW8 keeps PLAIN's triangles, each moved at most within its mesh's range (its triangle
+    record, side record and vertices together); the CW8 leaves under a mesh root, at most 8
+    triangles each, cover the mesh's triangles exactly once; every CW8 child box encloses its
+    leaf's vertices."""
     sc = rq.Scene.generate(kind, seed, meshes, tris)
     pn, cn = sc.layout_image("plain", 2), sc.layout_image("cw8", 2)
     pi, ci = sc.layout_image("plain", 0).reshape(-1, 32), sc.layout_image("cw8", 0).reshape(-1, 32)
@@ -116,26 +116,24 @@ def test_cw8_leaves_split_plain_leaves_and_their_boxes_enclose_their_triangles(k
                     stack.append(c)
                     continue
                 a, b = span(c)
+                assert b - a <= 8
                 got.append((a, b))
-                home = [(pa, pb) for pa, pb in plain_spans if pa <= a and b <= pb]
-                assert len(home) == 1  # inside one PLAIN leaf
                 v = cv[a:b].reshape(-1, 3).view(np.float32)
                 assert (lo <= v.min(axis=0)).all() and (hi >= v.max(axis=0)).all()
                 checked += 1
-        covered = np.zeros(max(b for _, b in plain_spans), int)
+        m0, m1 = min(a for a, _ in plain_spans), max(b for _, b in plain_spans)
+        covered = np.zeros(m1, int)
         for a, b in got:
             covered[a:b] += 1
-        for a, b in plain_spans:
-            assert (covered[a:b] == 1).all()
-            # the PLAIN leaf's triangles, moved only among themselves (rows moved whole)
-            assert np.array_equal(np.unique(crow[a:b], axis=0), np.unique(prow[a:b], axis=0))
+        assert (covered[:m0] == 0).all() and (covered[m0:m1] == 1).all()
+        # the mesh's triangles, moved only among themselves (rows moved whole)
+        assert np.array_equal(np.unique(crow[m0:m1], axis=0), np.unique(prow[m0:m1], axis=0))
         if i > 40:
             break
     assert checked > 0
 
 
-def test_cw8_top_level_reaches_every_instance_once():
-    sc = rq.Scene.generate(0, 2)
+def check_cw8_top_level(sc):
     meta = sc.layout_image("cw8", 5)
     top = sc.layout_image("cw8", 1)
     assert meta[0] == 1
@@ -156,6 +154,59 @@ def test_cw8_top_level_reaches_every_instance_once():
                 assert (lo <= blo).all() and (hi >= bhi).all()
             stack.append(c)
     assert (seen == 1).all()
+
+
+def test_cw8_top_level_reaches_every_instance_once():
+    check_cw8_top_level(rq.Scene.generate(0, 2))
+
+
+def check_cw8_mesh_leaves(sc):
+    """Every CW8 mesh tree's leaves hold 1 to 8 triangles inside the triangle records, and each
+    leaf's box encloses its triangles' vertices."""
+    inst, nodes = sc.layout_image("cw8", 0).reshape(-1, 32), sc.layout_image("cw8", 2)
+    verts = sc.layout_image("cw8", 6).reshape(-1, 9)
+    for root in {int(r) for r in inst[:, 18]}:
+        stack = [root]
+        while stack:
+            t = stack.pop()
+            if t & LEAF:
+                continue
+            for c, lo, hi in w8_children(nodes, t):
+                if not c & LEAF:
+                    stack.append(c)
+                    continue
+                first, n = leaf_ranges(c)
+                assert 1 <= n <= 8 and first + n <= len(verts)
+                v = verts[first:first + n].reshape(-1, 3).view(np.float32)
+                assert (lo <= v.min(axis=0)).all() and (hi >= v.max(axis=0)).all()
+
+
+def test_cw8_trees_over_boxes_of_any_finite_size():
+    """Meshes spanning 1 to 1e36 (a strip of triangles at x = 1.5^k) and far from the origin: the
+    trees built from their triangles and the instances' boxes price areas relative to their root, so
+    their costs stay finite -- the top level still reaches every instance once and every leaf keeps
+    at most 8 triangles."""
+    g = np.random.default_rng(3)
+    meshes = []
+    for base, n in ((1.5, 210), (1.2, 450), (2.0, 120)):
+        xs = base ** np.arange(n)
+        pos = np.array([[x, 0, 0] for x in xs] + [[x + 0.01, 0, 0] for x in xs] + [[x, 0.01, 0] for x in xs],
+                       np.float32)
+        idx = np.stack([np.arange(n), np.arange(n) + n, np.arange(n) + 2 * n], 1).reshape(-1).astype(np.uint32)
+        meshes.append((pos, idx, 0))
+    far = (g.normal(size=(3000, 3)) * 1e3 + [3e6, -2e6, 5e5]).astype(np.float32)
+    meshes.append((far, np.arange(3000, dtype=np.uint32), 0))
+    soup = g.normal(size=(3 * 2000, 3)).astype(np.float32)
+    meshes.append((soup, np.arange(3 * 2000, dtype=np.uint32), 1))
+    inst = []
+    for k in range(48):
+        m = np.eye(4, dtype=np.float32)
+        m[:3, :3] *= g.uniform(0.5, 2.0)
+        m[:3, 3] = g.uniform(-20, 20, 3)
+        inst.append(dict(model=m, mesh=k % len(meshes)))
+    sc = rq.Scene.build(meshes, inst)
+    check_cw8_top_level(sc)
+    check_cw8_mesh_leaves(sc)
 
 
 def far_tiny_scene():
@@ -229,15 +280,26 @@ def test_plain_triangle_records_hold_vertex_and_edges():
 
 def test_cw8_rejects_trees_deeper_than_16_levels():
     """The CW8 kernel pushes without a stack bound check: 7 entries a level for at most 16 levels
-    fit its stack. A chain collapses 7 levels into one 8-wide node, so 100 levels give 15 and 120
-    give 18: the relayout refuses the second, and PLAIN still takes it."""
-    ok = rq.Scene.from_gi_words(chain_words(100))
+    fit its stack. A mesh whose tree does not hold each of its triangles once keeps that tree's
+    shape: a chain collapses 7 levels into one 8-wide node, so 100 levels give 15 and 120 give 18,
+    and the relayout refuses the second (PLAIN still takes it). A chain over all of its mesh's
+    triangles is built again from them, a few levels deep."""
+    ok = rq.Scene.from_gi_words(chain_words(100, hole=True))
     nodes = ok.layout_image("cw8", 2).reshape(-1, 24)
     assert len(nodes) == 15
-    deep = rq.Scene.from_gi_words(chain_words(120))
+    deep = rq.Scene.from_gi_words(chain_words(120, hole=True))
     assert deep.layout_image("plain", 2).size == 120 * 32
     with pytest.raises(rq.InvalidInput, match="deeper than 16"):
         deep.layout_image("cw8", 2)
+    rebuilt = rq.Scene.from_gi_words(chain_words(120))
+    nodes = rebuilt.layout_image("cw8", 2).reshape(-1, 24)
+    root = int(rebuilt.layout_image("cw8", 0)[18])
+    levels, todo = 0, [(root, 1)]
+    while todo:
+        n, lv = todo.pop()
+        levels = max(levels, lv)
+        todo += [(int(c), lv + 1) for c in nodes[n, 16:24] if not int(c) & LEAF]
+    assert 2 <= levels <= 4
 
 
 def test_external_words_must_hold_finite_trees_inside_their_arrays():
