@@ -544,8 +544,9 @@ struct W8Builder {
 // ---- CW8 mesh trees built from the triangles
 // The 4-wide trees of the words are the reference builder's; the CW8 layout
 // needs only their triangles. A mesh's CW8 tree is built again from them: a
-// binary tree by binned SAH over the triangles' box centres (32 bins, every
-// axis), then collapsed into nodes of at most 8 children by the dynamic program
+// binary tree by SAH over the triangles' box centres (32 bins on every axis;
+// every split position of the sorted centres for 256 triangles or fewer), then
+// collapsed into nodes of at most 8 children by the dynamic program
 // of Ylitie, Karras and Laine (HPG 2017, sec. 3.1), which finds the collapse of
 // least SAH cost. C(n, i) is the cost of binary node n spread over at most i
 // child slots:
@@ -557,6 +558,7 @@ struct W8Builder {
 // costs about twice a pair step on the kernel's measured instruction counts.
 // The mesh's triangles are reordered so each leaf is a contiguous range.
 constexpr int SAH_BINS = 32;
+constexpr uint32_t SAH_SWEEP_MAX = 256;  // smaller nodes: every split position (1.7% on the 4M frame)
 constexpr uint32_t SAH_DEPTH = 64;     // deeper splits take the median (bounds the depth)
 constexpr uint32_t DP_LEAF_MAX = 8;
 // C_PAIR = C_NODE: a pair step and a node step each take a wave iteration; the
@@ -622,7 +624,37 @@ std::vector<BinNode> sah_binary(const std::vector<Bounds>& tb, std::vector<uint3
         nodes[x].box = box;
         if (cnt == 1) continue;
         int axis = -1, split = 0;
-        if (depth < SAH_DEPTH) {
+        uint32_t sweep_mid = 0;
+        if (depth < SAH_DEPTH && cnt <= SAH_SWEEP_MAX) {
+            // every split position of the primitives sorted by centre, on each axis
+            double best = std::numeric_limits<double>::infinity();
+            std::vector<uint32_t> tmp(order.begin() + first, order.begin() + first + cnt), best_order;
+            std::vector<double> right(cnt);
+            for (int a = 0; a < 3; ++a) {
+                std::sort(tmp.begin(), tmp.end(), [&](uint32_t p, uint32_t q) {
+                    return c[p][a] < c[q][a] || (c[p][a] == c[q][a] && p < q);
+                });
+                Bounds acc = empty_bounds();
+                for (uint32_t k = cnt; k-- > 1;) {
+                    acc = merge(acc, tb[tmp[k]]);
+                    right[k] = area2(acc) * (cnt - k);
+                }
+                acc = empty_bounds();
+                bool better = false;
+                for (uint32_t k = 1; k < cnt; ++k) {
+                    acc = merge(acc, tb[tmp[k - 1]]);
+                    const double cost = area2(acc) * k + right[k];
+                    if (cost < best) {
+                        best = cost;
+                        axis = a;
+                        sweep_mid = k;
+                        better = true;
+                    }
+                }
+                if (better) best_order = tmp;
+            }
+            if (axis >= 0) std::copy(best_order.begin(), best_order.end(), order.begin() + first);
+        } else if (depth < SAH_DEPTH) {
             double best = std::numeric_limits<double>::infinity();
             for (int a = 0; a < 3; ++a) {
                 const double lo = cb.lo[a], ext = (double)cb.hi[a] - lo;
@@ -662,7 +694,9 @@ std::vector<BinNode> sah_binary(const std::vector<Bounds>& tb, std::vector<uint3
             }
         }
         uint32_t mid;
-        if (axis >= 0) {
+        if (axis >= 0 && cnt <= SAH_SWEEP_MAX) {
+            mid = sweep_mid;
+        } else if (axis >= 0) {
             const double lo = cb.lo[axis], scale = SAH_BINS / ((double)cb.hi[axis] - lo);
             auto it = std::partition(order.begin() + first, order.begin() + first + cnt, [&](uint32_t k) {
                 return std::min(SAH_BINS - 1, (int)(((double)c[k][axis] - lo) * scale)) < split;
