@@ -60,6 +60,22 @@ def profiled_traffic(config, layout, launch):
     return None
 
 
+def profiled_issue(config, layout, launch):
+    """The launch's VALU issue rate as a fraction of the chip's (SQ_INSTS_VALU over 1024 SIMDs x 2.4 GHz /
+    4 cycles for its duration), measured by rocprofv3, from the committed profile summary; None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", PROFILES.get(config, ""))
+    if not os.path.isfile(path):
+        return None
+    pat = re.compile(r"^VALU issue (\w+) (\w+): .* = ([0-9.]+) of ")
+    with open(path) as f:
+        for line in f:
+            m = pat.match(line.strip().lstrip("- "))
+            if m and m.group(1) == layout and m.group(2) == launch:
+                return float(m.group(3))
+    return None
+
+
 class Frame:
     """Device buffers of one frame and the launches of one step."""
 
@@ -778,7 +794,8 @@ def main():
                 "lane_use": round((stats["node_steps"] + stats["tri_steps"] + stats["inst_steps"]) /
                                   max(64 * stats["iterations"], 1), 3),
                 "live_lanes": round(stats["live_lanes"] / max(64 * stats["iterations"], 1), 3),
-                "steps_per_ray": {k: round(stats[f"{k}_steps"] / max(dom_rays, 1), 2) for k in ("node", "tri", "inst")}}
+                "steps_per_ray": {k: round(stats[f"{k}_steps"] / max(dom_rays, 1), 2) for k in ("node", "tri", "inst")},
+                "valu_issue_frac": profiled_issue(args.config, args.layout, kind_of)}
 
     cpu = None
     if not args.no_cpu_baseline:

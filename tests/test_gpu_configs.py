@@ -6,7 +6,9 @@
   4M triangles, 3840x2160 primary + shadow + 2 GI bounces (28.5M rays). Every
   ray of the frame is traced in the REF, PLAIN and CW8 layouts and by the
   oracle: REF and PLAIN records equal the oracle's bit for bit, CW8 differs
-  only on ties, visibility bits equal.
+  only on ties, visibility bits equal. Every CW8 closest hit is then
+  recomputed in FP64 (rq_verify_fp64): where the hit is well conditioned its t
+  is within 16 ulps of the FP64 answer.
 - c3: 8 resident scenes of 100K-4M triangles and 256 lognormal batches
   (4K-2M rays; 40% primary, 30% shadow, 30% GI; 32M rays) in one persistent
   launch: every batch equal to its own launch and to the oracle on every
@@ -92,6 +94,15 @@ def test_full_frame_at_scale(cfg):
     scene.upload(("ref", "plain", "cw8"))
     passes, keep = frame_rays(scene, 1, w, h, n_gi)
     check_frame(scene, passes)
+    for name, rays, n, mode in passes:
+        if mode != rq.CLOSEST:
+            continue
+        hits = rq.DeviceBuffer(n * 48)
+        scene.trace("cw8", rays, n, mode, hits=hits)
+        v = scene.verify_fp64(rays, hits, n)
+        print(f"{name}: fp64 {v}", flush=True)
+        assert v["checked"] > n // 10 and v["conditioned"] > 0.9 * v["checked"], name
+        assert v["max_ulps_conditioned"] < 16, name
 
 
 def test_c3_eight_scenes_and_256_batches_in_one_launch():
