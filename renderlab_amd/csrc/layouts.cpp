@@ -27,6 +27,7 @@
 #include <cmath>
 #include <atomic>
 #include <limits>
+#include <map>
 #include <thread>
 #include <tuple>
 #include <unordered_map>
@@ -990,6 +991,89 @@ bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
     return w.root(rt, out.top_root);
 }
 
+// The CW8 tree of one reflection mesh (node base `base`, root `root`, triangle
+// records at word `tb`) built from its triangles. Its triangle records are
+// appended to `out.inst` (the words) in the new leaf order; `new_tb` is their
+// first word. Each triangle's box holds a, a + e1 and a + e2 computed exactly,
+// widened by 2e-5 of its edges for rs_trace's barycentric slack (4e-6) and by
+// 2^-20 of its coordinates, rounded outward. False with `err` empty when the
+// tree does not hold each of the triangles [0, n) once (n its largest leaf end),
+// a value is not finite, the words would pass 2^32, or the tree would pass
+// W8_MAX_LEVELS: the caller collapses the reflection tree instead.
+bool rebuild_rs_mesh(const std::vector<uint32_t>& words, uint32_t base, uint32_t root, uint32_t tb, HostLayout& out,
+                     uint32_t& cw8_root, uint32_t& new_tb, std::string& err) {
+    if (root & LEAF_BIT) return false;
+    std::vector<uint8_t> seen;
+    std::vector<uint32_t> todo{root};
+    size_t visits = 0;
+    while (!todo.empty()) {
+        const uint32_t t = todo.back();
+        todo.pop_back();
+        if (t & LEAF_BIT) {
+            const uint32_t first = (t & ~LEAF_BIT) >> 4, cnt = t & 15u;
+            if (seen.size() < (size_t)first + cnt) seen.resize((size_t)first + cnt, 0);
+            for (uint32_t k = first; k < first + cnt; ++k) {
+                if (seen[k]) return false;
+                seen[k] = 1;
+            }
+            continue;
+        }
+        const size_t at = (size_t)base + (size_t)t * RS_NODE_WORDS;
+        if (at + RS_NODE_WORDS > words.size() || ++visits > (size_t)1 << 27) return false;
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t c = words[at + 6 + s];
+            if (c != EMPTY) todo.push_back(c);
+        }
+    }
+    const uint32_t n = (uint32_t)seen.size();
+    if (n == 0 || n >= MAX_FIRST_TRI || (size_t)tb + (size_t)RS_TRI_WORDS * n > words.size()) return false;
+    for (uint8_t s : seen)
+        if (!s) return false;
+    if (out.inst.size() + (size_t)RS_TRI_WORDS * n >= ((size_t)1 << 32)) return false;
+    std::vector<Bounds> boxes(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t* w = &words[(size_t)tb + (size_t)RS_TRI_WORDS * k];
+        double lo[3], hi[3], e = 0.0, m = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const double p = bits_f(w[a]), e1 = bits_f(w[3 + a]), e2 = bits_f(w[6 + a]);
+            if (!std::isfinite(p) || !std::isfinite(e1) || !std::isfinite(e2)) return false;
+            lo[a] = std::min({p, p + e1, p + e2});
+            hi[a] = std::max({p, p + e1, p + e2});
+            e = std::max({e, std::fabs(e1), std::fabs(e2)});
+            m = std::max({m, std::fabs(lo[a]), std::fabs(hi[a])});
+        }
+        const double pad = 2e-5 * e + std::ldexp(m, -20);
+        for (int a = 0; a < 3; ++a) {
+            float l = (float)(lo[a] - pad), h = (float)(hi[a] + pad);
+            if ((double)l > lo[a] - pad) l = std::nextafter(l, -std::numeric_limits<float>::infinity());
+            if ((double)h < hi[a] + pad) h = std::nextafter(h, std::numeric_limits<float>::infinity());
+            if (!std::isfinite(l) || !std::isfinite(h)) return false;
+            boxes[k].lo[a] = l;
+            boxes[k].hi[a] = h;
+        }
+    }
+    std::vector<uint32_t> order;
+    const std::vector<BinNode> bin = sah_binary(boxes, order);
+    std::vector<std::vector<Cand>> lists;
+    const uint32_t rt = dp_collapse(
+        bin, DP_LEAF_MAX, [](uint32_t cnt) { return DP_C_PAIR * ((cnt + 1) / 2); },
+        [&](const BinNode& x) { return LEAF_BIT | (x.first << 4) | x.count; }, lists);
+    if (rt & LEAF_BIT) {
+        cw8_root = rt;
+    } else {
+        if (list_levels(lists) > W8_MAX_LEVELS) return false;
+        W8Builder w{nullptr, &out.nodes, &err};
+        w.lists = &lists;
+        if (!w.root(rt, cw8_root)) return false;
+    }
+    new_tb = (uint32_t)out.inst.size();
+    for (uint32_t p = 0; p < n; ++p) {
+        const size_t src = (size_t)tb + (size_t)RS_TRI_WORDS * order[p];
+        out.inst.insert(out.inst.end(), words.begin() + src, words.begin() + src + RS_TRI_WORDS);
+    }
+    return true;
+}
+
 }  // namespace
 
 bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, HostLayout& out, std::string& err) {
@@ -1098,15 +1182,37 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
     const uint32_t count = words.empty() ? 0u : words[0];
     out.inst = words;
     out.instance_count = count;
-    // Each mesh tree (node base, root) as 32-word f32 nodes -- the decode rs_trace makes
-    // (origin + step * code, unfused) -- then collapsed into CW8 nodes. RS leaf tags
-    // (LEAF | first triangle << 4 | count) pass through unchanged.
+    // Each mesh's CW8 tree is built from its triangle records where its tree holds
+    // each of them once (the records then follow the words in the new leaf order,
+    // and the instance records point there). Otherwise each mesh tree (node base,
+    // root) is decoded as rs_trace decodes it (origin + step * code, unfused) into
+    // 32-word f32 nodes and collapsed into CW8 nodes; RS leaf tags (LEAF | first
+    // triangle << 4 | count) pass through unchanged.
     std::vector<uint32_t> plain;
     W8Builder b{&plain, &out.nodes, &err};
     std::unordered_map<uint64_t, uint32_t> done;  // node base << 32 | root -> CW8 root tag
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> rebuilt;  // -> (root, tb)
     for (uint32_t i = 0; i < count; ++i) {
         const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
         const uint32_t base = words[r + 18], root = words[r + 20];
+        {
+            const auto k3 = std::make_tuple(base, root, words[r + 19]);
+            auto rb = rebuilt.find(k3);
+            uint32_t nr = EMPTY, ntb = 0;
+            if (rb != rebuilt.end()) {
+                nr = rb->second.first;
+                ntb = rb->second.second;
+            } else if (rebuild_rs_mesh(words, base, root, words[r + 19], out, nr, ntb, err)) {
+                rebuilt.emplace(k3, std::make_pair(nr, ntb));
+            } else if (!err.empty()) {
+                return false;
+            }
+            if (nr != EMPTY) {
+                out.inst[r + 19] = ntb;
+                out.inst[r + 20] = nr;
+                continue;
+            }
+        }
         const uint64_t key = (uint64_t)base << 32 | root;
         auto it = done.find(key);
         uint32_t cw8_root = EMPTY;

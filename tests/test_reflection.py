@@ -264,10 +264,12 @@ def rs_node_children(words, base, n):
     return out
 
 
-def test_rs_cw8_relayout_encloses_every_leaf_once():
-    """The RS_CW8 image: each record's root becomes a CW8 tree reaching exactly the reflection
-    tree's leaves (their tags unchanged), every CW8 box enclosing the rs_trace box of each leaf
-    below it; the CW8 top level reaches every instance once; the rest of the words is unchanged."""
+def test_rs_cw8_relayout_covers_every_triangle_once():
+    """The RS_CW8 image: the words are kept but for each record's root (word 20) and triangle base
+    (word 19); each mesh's tree is built again from its triangle records, which follow the words in
+    the new leaf order (the mesh's own rows, moved whole). Each record's CW8 tree reaches each of its
+    mesh's triangles exactly once, every CW8 box on the way to a leaf encloses the leaf's triangles
+    (a, a + e1, a + e2), and the CW8 top level reaches every instance once."""
     from test_layouts import w8_children
     meshes, inst, _ = random_reflection_scene(3, instances=60)
     sc = rq.Scene.build(meshes, inst, reflection=True)
@@ -278,8 +280,9 @@ def test_rs_cw8_relayout_encloses_every_leaf_once():
     meta = sc.layout_image("rs_cw8", 5)
     count = int(w[0])
     keep = np.ones(len(w), bool)
+    keep[RS_HEADER + RS_REC * np.arange(count) + 19] = False
     keep[RS_HEADER + RS_REC * np.arange(count) + 20] = False
-    assert len(img) == len(w) and np.array_equal(img[keep], w[keep])
+    assert len(img) > len(w) and np.array_equal(img[:len(w)][keep], w[keep])
 
     def rs_leaves(base, tag):
         if tag & rq.LEAF_BIT:
@@ -291,6 +294,9 @@ def test_rs_cw8_relayout_encloses_every_leaf_once():
             else:
                 out.update(rs_leaves(base, c))
         return out
+
+    def rows(words, tb, first, n):
+        return words[tb + 12 * first: tb + 12 * (first + n)].reshape(-1, 12)
 
     def cw8_leaves(tag):
         """(leaf tag, [CW8 boxes on the path to it]) under `tag`."""
@@ -304,13 +310,20 @@ def test_rs_cw8_relayout_encloses_every_leaf_once():
 
     for i in range(count):
         r = RS_HEADER + RS_REC * i
+        old_tb, new_tb = int(w[r + 19]), int(img[r + 19])
         want = rs_leaves(int(w[r + 18]), int(w[r + 20]))
+        n = max(((t & ~rq.LEAF_BIT) >> 4) + (t & 15) for t in want)
         got = cw8_leaves(int(img[r + 20]))
-        tags = [t for t, _ in got]
-        assert sorted(tags) == sorted(want) and len(set(tags)) == len(tags)
-        for t, path in got:  # every CW8 box on the way down encloses the leaf's rs_trace box
-            for lo, hi in path:
-                assert (lo <= want[t][0]).all() and (hi >= want[t][1]).all()
+        covered = np.zeros(n, int)
+        for t, path in got:
+            first, cnt = (t & ~rq.LEAF_BIT & 0xFFFFFFFF) >> 4, t & 15
+            covered[first:first + cnt] += 1
+            tri = rows(img, new_tb, first, cnt).view(np.float32).astype(np.float64)
+            pts = np.concatenate([tri[:, 0:3], tri[:, 0:3] + tri[:, 3:6], tri[:, 0:3] + tri[:, 6:9]])
+            for lo, hi in path:  # every CW8 box on the way down encloses the leaf's triangles
+                assert (lo <= pts.min(axis=0)).all() and (hi >= pts.max(axis=0)).all()
+        assert (covered == 1).all()
+        assert np.array_equal(np.unique(rows(img, new_tb, 0, n), axis=0), np.unique(rows(w, old_tb, 0, n), axis=0))
 
     assert meta[0] == 1  # 60 instances: a top level
     def top_leaves(tag):
