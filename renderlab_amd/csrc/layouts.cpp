@@ -992,16 +992,15 @@ bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
 }
 
 // The CW8 tree of one reflection mesh (node base `base`, root `root`, triangle
-// records at word `tb`) built from its triangles. Its triangle records are
-// appended to `out.inst` (the words) in the new leaf order; `new_tb` is their
-// first word. Each triangle's box holds a, a + e1 and a + e2 computed exactly,
+// records at word `tb`) built from its triangles into `nodes` (indices from 0),
+// its triangle records copied to `tris` in the new leaf order. Each triangle's box holds a, a + e1 and a + e2 computed exactly,
 // widened by 2e-5 of its edges for rs_trace's barycentric slack (4e-6) and by
 // 2^-20 of its coordinates, rounded outward. False with `err` empty when the
 // tree does not hold each of the triangles [0, n) once (n its largest leaf end),
-// a value is not finite, the words would pass 2^32, or the tree would pass
-// W8_MAX_LEVELS: the caller collapses the reflection tree instead.
-bool rebuild_rs_mesh(const std::vector<uint32_t>& words, uint32_t base, uint32_t root, uint32_t tb, HostLayout& out,
-                     uint32_t& cw8_root, uint32_t& new_tb, std::string& err) {
+// a value is not finite, or the tree would pass W8_MAX_LEVELS: the caller
+// collapses the reflection tree instead.
+bool rebuild_rs_mesh(const std::vector<uint32_t>& words, uint32_t base, uint32_t root, uint32_t tb,
+                     std::vector<uint32_t>& nodes, std::vector<uint32_t>& tris, uint32_t& cw8_root, std::string& err) {
     if (root & LEAF_BIT) return false;
     std::vector<uint8_t> seen;
     std::vector<uint32_t> todo{root};
@@ -1029,7 +1028,6 @@ bool rebuild_rs_mesh(const std::vector<uint32_t>& words, uint32_t base, uint32_t
     if (n == 0 || n >= MAX_FIRST_TRI || (size_t)tb + (size_t)RS_TRI_WORDS * n > words.size()) return false;
     for (uint8_t s : seen)
         if (!s) return false;
-    if (out.inst.size() + (size_t)RS_TRI_WORDS * n >= ((size_t)1 << 32)) return false;
     std::vector<Bounds> boxes(n);
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t* w = &words[(size_t)tb + (size_t)RS_TRI_WORDS * k];
@@ -1062,14 +1060,14 @@ bool rebuild_rs_mesh(const std::vector<uint32_t>& words, uint32_t base, uint32_t
         cw8_root = rt;
     } else {
         if (list_levels(lists) > W8_MAX_LEVELS) return false;
-        W8Builder w{nullptr, &out.nodes, &err};
+        W8Builder w{nullptr, &nodes, &err};
         w.lists = &lists;
         if (!w.root(rt, cw8_root)) return false;
     }
-    new_tb = (uint32_t)out.inst.size();
+    tris.reserve((size_t)RS_TRI_WORDS * n);
     for (uint32_t p = 0; p < n; ++p) {
         const size_t src = (size_t)tb + (size_t)RS_TRI_WORDS * order[p];
-        out.inst.insert(out.inst.end(), words.begin() + src, words.begin() + src + RS_TRI_WORDS);
+        tris.insert(tris.end(), words.begin() + src, words.begin() + src + RS_TRI_WORDS);
     }
     return true;
 }
@@ -1191,27 +1189,69 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
     std::vector<uint32_t> plain;
     W8Builder b{&plain, &out.nodes, &err};
     std::unordered_map<uint64_t, uint32_t> done;  // node base << 32 | root -> CW8 root tag
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> rebuilt;  // -> (root, tb)
+    // the rebuilds, one per (node base, root, triangle base), on threads
+    struct RsJob {
+        uint32_t base, root, tb;
+        std::vector<uint32_t> nodes, tris;
+        uint32_t cw8_root = EMPTY, new_tb = 0;
+        bool ok = false, placed = false;
+        std::string err;
+    };
+    std::vector<RsJob> jobs;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t> job_of;
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
+        const auto k3 = std::make_tuple(words[r + 18], words[r + 20], words[r + 19]);
+        if (job_of.emplace(k3, (uint32_t)jobs.size()).second)
+            jobs.push_back(RsJob{words[r + 18], words[r + 20], words[r + 19], {}, {}, EMPTY, 0, false, false, {}});
+    }
+    {
+        const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 16));
+        std::vector<std::thread> pool;
+        std::atomic_uint next{0};
+        for (uint32_t t = 0; t < std::min<size_t>(threads, jobs.size()); ++t)
+            pool.emplace_back([&]() {
+                for (uint32_t k; (k = next.fetch_add(1)) < jobs.size();) {
+                    RsJob& j = jobs[k];
+                    try {
+                        j.ok = rebuild_rs_mesh(words, j.base, j.root, j.tb, j.nodes, j.tris, j.cw8_root, j.err);
+                    } catch (const std::bad_alloc&) {
+                        j.err = "RS CW8 relayout: out of host memory";
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
     for (uint32_t i = 0; i < count; ++i) {
         const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
         const uint32_t base = words[r + 18], root = words[r + 20];
-        {
-            const auto k3 = std::make_tuple(base, root, words[r + 19]);
-            auto rb = rebuilt.find(k3);
-            uint32_t nr = EMPTY, ntb = 0;
-            if (rb != rebuilt.end()) {
-                nr = rb->second.first;
-                ntb = rb->second.second;
-            } else if (rebuild_rs_mesh(words, base, root, words[r + 19], out, nr, ntb, err)) {
-                rebuilt.emplace(k3, std::make_pair(nr, ntb));
-            } else if (!err.empty()) {
-                return false;
+        RsJob& j = jobs[job_of.at(std::make_tuple(base, root, words[r + 19]))];
+        if (!j.err.empty()) {
+            err = j.err;
+            return false;
+        }
+        if (j.ok) {
+            if (!j.placed) {
+                // its node indices move by the first node it lands on; its records follow the words
+                const uint32_t nb = (uint32_t)(out.nodes.size() / W8_WORDS);
+                for (size_t w = 0; w < j.nodes.size(); w += W8_WORDS)
+                    for (size_t s = 16; s < 24; ++s)
+                        if (!(j.nodes[w + s] & LEAF_BIT)) j.nodes[w + s] += nb;
+                if (!(j.cw8_root & LEAF_BIT)) j.cw8_root += nb;
+                if (out.inst.size() + j.tris.size() >= ((size_t)1 << 32)) {
+                    err = "RS CW8 relayout: words past 2^32";
+                    return false;
+                }
+                j.new_tb = (uint32_t)out.inst.size();
+                out.nodes.insert(out.nodes.end(), j.nodes.begin(), j.nodes.end());
+                out.inst.insert(out.inst.end(), j.tris.begin(), j.tris.end());
+                std::vector<uint32_t>().swap(j.nodes);
+                std::vector<uint32_t>().swap(j.tris);
+                j.placed = true;
             }
-            if (nr != EMPTY) {
-                out.inst[r + 19] = ntb;
-                out.inst[r + 20] = nr;
-                continue;
-            }
+            out.inst[r + 19] = j.new_tb;
+            out.inst[r + 20] = j.cw8_root;
+            continue;
         }
         const uint64_t key = (uint64_t)base << 32 | root;
         auto it = done.find(key);
