@@ -369,7 +369,7 @@ rq_status rq_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32
    callers that cannot allocate device memory (code under HIP graph capture,
    tight serving loops, harnesses that forbid hipMalloc). The ray cursor and
    the stack spill area come from one static device array of
-   rq_trace_noalloc_scratch_bytes() = 112 MiB + 256 B per GPU (held until the
+   rq_trace_noalloc_scratch_bytes() = 112 MiB + 8 KiB per GPU (held until the
    process ends) instead of the stream's own memory, and the persistent grid
    is capped to the 131,072 lanes that array holds; the ray count is not
    limited. RQ_REORDER is refused (RQ_ERR_UNSUPPORTED): its sort space grows

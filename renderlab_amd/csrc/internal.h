@@ -209,8 +209,14 @@ void set_error(const std::string& msg);
 
 // The traversal's per-lane stack spill area: words per lane of a persistent grid.
 constexpr uint32_t TRACE_SPILL_WORDS = 224;
+// The persistent kernels' ray cursor: the launch's 64-ray chunks are dealt
+// round robin to CURSOR_PARTS partitions, each with its own claim counter,
+// CURSOR_STRIDE words (256 bytes) apart so that no two share a cache line.
+constexpr uint32_t CURSOR_PARTS = 32;
+constexpr uint32_t CURSOR_STRIDE = 64;
+constexpr size_t CURSOR_BYTES = (size_t)CURSOR_PARTS * CURSOR_STRIDE * sizeof(uint32_t);
 // Device memory a trace launch uses in place of its stream's own (no allocation):
-// the ray cursor (one word, cleared by the launch) and the spill area of `lanes` lanes.
+// the ray cursor (CURSOR_BYTES, cleared by the launch) and the spill area of `lanes` lanes.
 struct FixedScratch {
     uint32_t* counter;
     uint32_t* spill;

@@ -6,9 +6,10 @@
 // the per-lane stack spill area of its grid. This entry point takes both from
 // one file-scope device array instead:
 //
-//   scratch = 256 bytes (the cursor, cleared by every launch with a fill
-//             kernel) + NOALLOC_LANES lanes x TRACE_SPILL_WORDS words,
-//             = 256 B + 131,072 x 896 B = 112 MiB + 256 B,
+//   scratch = CURSOR_BYTES (the ray cursor: 32 partition counters 256 bytes
+//             apart, cleared by every launch with a fill kernel)
+//             + NOALLOC_LANES lanes x TRACE_SPILL_WORDS words,
+//             = 8 KiB + 131,072 x 896 B = 112 MiB + 8 KiB,
 //
 // which every GPU the library runs this entry point on holds until the process
 // ends. The grid is capped to NOALLOC_LANES lanes (512 workgroups of 256, two
@@ -25,8 +26,7 @@
 namespace {
 
 constexpr size_t NOALLOC_LANES = 131072;
-constexpr size_t CURSOR_BYTES = 256;
-constexpr size_t SCRATCH_BYTES = CURSOR_BYTES + NOALLOC_LANES * rq::TRACE_SPILL_WORDS * sizeof(uint32_t);
+constexpr size_t SCRATCH_BYTES = rq::CURSOR_BYTES + NOALLOC_LANES * rq::TRACE_SPILL_WORDS * sizeof(uint32_t);
 
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
@@ -68,7 +68,7 @@ extern "C" rq_status rq_trace_noalloc(rq_scene* scene, rq_layout layout, const r
     if (st != RQ_OK) return st;
     rq::FixedScratch f;
     f.counter = reinterpret_cast<uint32_t*>(base);
-    f.spill = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(base) + CURSOR_BYTES);
+    f.spill = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(base) + rq::CURSOR_BYTES);
     f.lanes = NOALLOC_LANES;
     return rq::launch_trace(scene, layout, rays, count, mode, hits, vis_bits, stream, nullptr, nullptr, &f);
 }

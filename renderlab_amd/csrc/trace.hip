@@ -251,7 +251,6 @@ struct TraceArgs {
     uint32_t n_batches;
     uint32_t count, refill_min;
     uint32_t speculate;  // CW8: lanes with triangles pending take node steps with the wave
-    uint32_t first;      // rays handed out without the cursor (64 per wave of the grid); the cursor counts from here
     uint32_t drain;      // once the rays run out, every live lane takes its step each iteration
 };
 
@@ -316,9 +315,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     uint32_t m_nodes = 0, m_leaves = 0, m_pos = 0, m_idx = 0;
     uint32_t tri_next = 0, tri_left = 0;  // the current leaf's untested triangles
     uint32_t nxt = NONE;                  // the next item to visit (the top of the stack, in a register)
-    bool exhausted = false;
-    bool first_take = true;  // the first refill takes rays [64 gwave, 64 gwave + 64)
-    const uint32_t gwave = blockIdx.x * (uint32_t)(BLOCK / 64) + tid / 64u;
+    bool exhausted = false;  // the wave found no rays left to claim
     unsigned long long nb = 0;
     // COUNT: [1] wave iterations, [2] node, [3] triangle-pair, [4] instance
     // steps (lane sums), [5] refill events (per wave)
@@ -327,6 +324,58 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     // step, summed over wave iterations (wave-uniform)
     unsigned long long st_live = 0, st_want_node = 0, st_want_tri = 0, st_want_inst = 0;
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
+
+    // ---- the ray cursor. The rays [0, count) are dealt out in chunks of 64
+    // (chunk c = rays [64c, 64c + 64)); chunk c belongs to partition c mod
+    // CURSOR_PARTS, and each partition has its own counter of claimed chunks. Claims
+    // thus spread over CURSOR_PARTS addresses instead of queueing on one (an
+    // address takes one atomic at a time: a single cursor made every refill wait
+    // its turn), while all partitions advance through the rays together, so the
+    // grid still works on one narrow window of the ray order. Ray sets come in
+    // screen order, so neighbouring rays visit the same nodes, and a narrow
+    // window keeps those in L2 (contiguous partitions, one window each, measured
+    // 20% slower). Wave w of the grid starts on chunk w without an atomic, then
+    // claims one chunk at a time from its home partition, w mod CURSOR_PARTS, and
+    // hands the rays to its idle lanes over the next refills. A wave whose
+    // partition runs dry moves on to one that still holds chunks and is exhausted
+    // when none does.
+    // The cursor state is wave-uniform and kept in scalar registers (readfirstlane).
+    constexpr uint32_t PARTS = CURSOR_PARTS;
+    static_assert(PARTS <= 32, "a 32-bit mask of empty partitions; one lane per partition when probing");
+    const uint32_t gwave = blockIdx.x * (uint32_t)(BLOCK / 64) + __builtin_amdgcn_readfirstlane(tid) / 64u;
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    const uint64_t chunks = ((uint64_t)count + 63u) / 64u;
+    uint32_t part = gwave % PARTS;
+    uint32_t dead = 0u;  // partitions this wave found empty
+    // partition p's chunks below `waves` are first slices: its counter counts from there
+    auto first_round = [&](uint32_t p) -> uint64_t { return p < waves ? (waves - 1u - p) / PARTS + 1u : 0ull; };
+    // the wave's claimed rays not yet handed out: [res_next, res_end)
+    // (64-bit bounds compared by hand: no 64-bit min overload that stays integer)
+    uint32_t res_next = 64ull * gwave < count ? 64u * gwave : count;
+    uint32_t res_end = (uint64_t)res_next + 64ull < count ? res_next + 64u : count;
+    // Claims the wave's next chunk from its current partition. A partition whose
+    // claim fails has no chunks left: the wave moves on to the next partition it
+    // has not found empty (and tries it at its next refill), and is exhausted once
+    // it has found them all empty.
+    auto claim = [&]() {
+        uint32_t old = 0;
+        if (lane == 0) old = atomicAdd(a.counter + part * CURSOR_STRIDE, 1u);
+        old = __builtin_amdgcn_readfirstlane(old);  // every lane is active here: lane 0's value
+        const uint64_t c = (first_round(part) + old) * PARTS + part;
+        if (c < chunks) {
+            res_next = (uint32_t)(64ull * c);
+            res_end = 64ull * c + 64ull < count ? res_next + 64u : count;
+            return;
+        }
+        dead |= 1u << part;
+        const uint32_t open = ~dead & (PARTS == 32 ? ~0u : (1u << PARTS) - 1u);
+        if (open == 0u) {
+            exhausted = true;
+            return;
+        }
+        const uint32_t after = part < 31u ? open & (~0u << (part + 1u)) : 0u;
+        part = (uint32_t)__builtin_ctz(after ? after : open);
+    };
     SceneView v = a.sv;
     bool any_hit = AM == 1;
     rq_hit* out_hits = a.hits;
@@ -444,22 +493,16 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         }
         if (COUNT && refill_now && !exhausted) st_refill++;
         if (!exhausted && refill_now) {
-            const uint32_t n = (uint32_t)__popcll(idle);
-            uint64_t base;
-            if (first_take) {
-                // every lane is idle at the start: the wave's own 64 rays, no atomic
-                // (thousands of waves starting at once would queue on one cursor)
-                base = (uint64_t)gwave * 64u;
-                first_take = false;
-            } else {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(a.counter, n);
-                base = (uint64_t)a.first + __shfl(b, 0);
-            }
-            if (base + n >= count) exhausted = true;
-            if (ray == NONE && base + lane_rank(idle) < count) {
+            // hand the wave's claimed rays to its idle lanes in lane order (claiming a
+            // chunk first if none is left); lanes left over wait for the next refill
+            if (res_next == res_end) claim();
+            const uint32_t k = min((uint32_t)__popcll(idle), res_end - res_next);
+            const uint32_t rank = lane_rank(idle);
+            const uint32_t mine = res_next + rank;
+            res_next += k;
+            if (ray == NONE && rank < k) {
                 {
-                    uint32_t r = (uint32_t)(base + lane_rank(idle));
+                    uint32_t r = mine;
                     if (!BATCHED && a.order) {
                         r = a.order[r];
                         if (COUNT) nb += 4;
@@ -1520,7 +1563,7 @@ rq_status res_for(hipStream_t s, StreamRes** out) {
         }
     StreamRes r;
     r.stream = s;
-    HIP_TRY(hipMalloc(&r.counter, 256), "hipMalloc counter");
+    HIP_TRY(hipMalloc(&r.counter, CURSOR_BYTES), "hipMalloc counter");
     g_ctx.res.push_back(r);
     *out = &g_ctx.res.back();
     return RQ_OK;
@@ -1899,7 +1942,7 @@ rq_status rq::launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays
         counter = res->counter;
         spill = res->spill;
     }
-    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, counter, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(dev::fill_u32, dim3(16), dim3(256), 0, s, counter, 0u, (uint64_t)(CURSOR_BYTES / 4));
 
     dev::TraceArgs a;
     std::memset(&a, 0, sizeof a);
@@ -1914,7 +1957,6 @@ rq_status rq::launch_trace(rq_scene* scene, rq_layout layout, const rq_ray* rays
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
     a.drain = t.drain;
-    a.first = (uint32_t)(grid * k->block);
     a.bytes = bytes;
     if (reorder && count >= REORDER_MIN) {
         // keys, indices, sorted keys, order, then the sort's own space
@@ -2019,7 +2061,7 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
     StreamRes* res = nullptr;
     if ((st = res_for(s, &res)) != RQ_OK) return st;
     if ((st = ensure_spill(*res, (size_t)(grid * k->block))) != RQ_OK) return st;
-    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(dev::fill_u32, dim3(16), dim3(256), 0, s, res->counter, 0u, (uint64_t)(CURSOR_BYTES / 4));
     a.sv = scene_view(scene, layout);
     a.count = (uint32_t)total;
     a.counter = res->counter;
@@ -2027,7 +2069,6 @@ extern "C" rq_status rq_trace_ranges(rq_scene* scene, rq_layout layout, const rq
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = t.speculate;
     a.drain = t.drain;
-    a.first = (uint32_t)(grid * k->block);
     k->rfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "trace ranges launch");
     return RQ_OK;
@@ -2104,7 +2145,7 @@ rq_status rq::trace_batches_impl(rq_scene* const* scenes, uint32_t scene_count, 
     HIP_TRY(hipMemcpyAsync(tb, table.data(), tbytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync batch table");
     size_t poff = (tbytes + 255) & ~(size_t)255;
     HIP_TRY(hipMemcpyAsync(tb + poff, prefix.data(), pbytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync prefix");
-    hipLaunchKernelGGL(dev::fill_u32, dim3(1), dim3(64), 0, s, res->counter, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(dev::fill_u32, dim3(16), dim3(256), 0, s, res->counter, 0u, (uint64_t)(CURSOR_BYTES / 4));
     dev::TraceArgs a;
     std::memset(&a, 0, sizeof a);
     a.count = (uint32_t)total;
@@ -2116,7 +2157,6 @@ rq_status rq::trace_batches_impl(rq_scene* const* scenes, uint32_t scene_count, 
     a.refill_min = t.refill ? t.refill : 1u;
     a.speculate = speculate < 0 ? t.speculate : (uint32_t)speculate;
     a.drain = t.drain;
-    a.first = (uint32_t)(grid * k->block);
     k->bfn[layout](dim3((unsigned)grid), dim3(k->block), s, a);
     HIP_TRY(hipGetLastError(), "batched trace launch");
     // The table is read by the kernel: the next call on this stream may overwrite it only after.

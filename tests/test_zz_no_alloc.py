@@ -28,7 +28,7 @@ def both(scene, layout, rays, n, mode):
 
 @pytest.mark.parametrize("layout", ["ref", "plain", "cw8"])
 def test_noalloc_trace_equals_rq_trace_on_two_calls_in_a_row(layout):
-    assert rq.lib.rq_trace_noalloc_scratch_bytes() == 256 + 131072 * 224 * 4
+    assert rq.lib.rq_trace_noalloc_scratch_bytes() == 32 * 256 + 131072 * 224 * 4
     scene = rq.Scene.generate(0, 1)
     scene.upload((layout,))
     _, prim, _, shadow, gis = gpu_workload(scene, 0, 1, 640, 480, 1, layout)
