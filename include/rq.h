@@ -429,9 +429,12 @@ rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* 
 /* The same launch with its step statistics: [0] bytes, [1] wave iterations,
    [2] node steps, [3] triangle-pair steps, [4] instance entries, [5] refills;
    summed over wave iterations: [6] lanes holding a live ray, [7] [8] [9]
-   lanes wanting a node / triangle-pair / instance step. */
+   lanes wanting a node / triangle-pair / instance step; [10] [11] wave
+   iterations running node / triangle-pair steps, [12] node iterations in which
+   some lane entered an instance, [13] speculative node steps, [14] drain
+   iterations, [15] top-level node steps. */
 rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
-                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[10]);
+                               uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[16]);
 
 /* Launch parameters of the traversal kernels (the autotuner's knobs). */
 typedef struct rq_tuning {

@@ -941,8 +941,67 @@ bool rebuild_mesh(const HostLayout& plain, uint32_t pr, uint32_t tb, uint32_t n,
     return true;
 }
 
-// The CW8 top level built from the instances' world boxes (record words 12-17),
-// one instance a leaf. False with `err` empty when the 4-wide top level does not
+// The world box of each instance's own vertices, in place of the box of its
+// mesh box's transformed corners (record words 12-17), which for a rotated mesh
+// is up to about 1.7 times wider on each axis: a top-level leaf box only culls.
+// The vertices (the triangle records `out.verts` of the instance's mesh) go to
+// world space in f64 through the inverse of the record's world-to-object rows
+// (words 0-11); the box is padded as the record's own box is (src/gi_bvh.rs
+// instance_words: 1e-5 (1 + max |coordinate|) per axis), which is far wider
+// than the f32 roundings of the object-space ray a hit is found with, then
+// rounded outward to f32 and clipped to the record box (the kernel tests that
+// box exactly at instance entry anyway). Instances of meshes above
+// TIGHT_MAX_TRIS triangles, or with a singular or non-finite transform, keep
+// the record box.
+constexpr uint32_t TIGHT_MAX_TRIS = 1u << 16;
+void tighten_instance_boxes(const HostLayout& plain, const HostLayout& out, std::vector<Bounds>& boxes) {
+    const size_t total_tris = out.verts.size() / TRI_FLOATS;
+    for (uint32_t i = 0; i < (uint32_t)boxes.size(); ++i) {
+        const uint32_t* r = &out.inst[(size_t)i * INSTANCE_WORDS];
+        const uint32_t mesh = plain.inst[(size_t)i * INSTANCE_WORDS + 21];
+        const size_t tb = plain.inst[(size_t)i * INSTANCE_WORDS + 20];
+        const size_t te = (size_t)mesh + 1 < plain.mesh_base.size() / 3 ? plain.mesh_base[3 * (mesh + 1) + 2]
+                                                                         : total_tris;
+        if (te <= tb || te > total_tris || te - tb > TIGHT_MAX_TRIS) continue;
+        // world = R^-1 (object - T), R the rows' 3x3 and T their fourth column
+        double R[3][3], T[3];
+        for (int a = 0; a < 3; ++a) {
+            for (int b = 0; b < 3; ++b) R[a][b] = bits_f(r[4 * a + b]);
+            T[a] = bits_f(r[4 * a + 3]);
+        }
+        const double c00 = R[1][1] * R[2][2] - R[1][2] * R[2][1], c01 = R[1][2] * R[2][0] - R[1][0] * R[2][2],
+                     c02 = R[1][0] * R[2][1] - R[1][1] * R[2][0];
+        const double det = R[0][0] * c00 + R[0][1] * c01 + R[0][2] * c02;
+        if (!std::isfinite(det) || std::fabs(det) < 1e-30) continue;
+        const double inv[3][3] = {
+            {c00 / det, (R[0][2] * R[2][1] - R[0][1] * R[2][2]) / det, (R[0][1] * R[1][2] - R[0][2] * R[1][1]) / det},
+            {c01 / det, (R[0][0] * R[2][2] - R[0][2] * R[2][0]) / det, (R[0][2] * R[1][0] - R[0][0] * R[1][2]) / det},
+            {c02 / det, (R[0][1] * R[2][0] - R[0][0] * R[2][1]) / det, (R[0][0] * R[1][1] - R[0][1] * R[1][0]) / det}};
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t k = TRI_FLOATS * tb; k < TRI_FLOATS * te; k += 3) {
+            const double p[3] = {out.verts[k] - T[0], out.verts[k + 1] - T[1], out.verts[k + 2] - T[2]};
+            for (int a = 0; a < 3; ++a) {
+                const double w = inv[a][0] * p[0] + inv[a][1] * p[1] + inv[a][2] * p[2];
+                lo[a] = std::min(lo[a], w);
+                hi[a] = std::max(hi[a], w);
+            }
+        }
+        Bounds t;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const double pad = 1e-5 * (1.0 + std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+            t.lo[a] = std::nextafter((float)(lo[a] - pad), -INFINITY);
+            t.hi[a] = std::nextafter((float)(hi[a] + pad), INFINITY);
+            t.lo[a] = std::max(t.lo[a], boxes[i].lo[a]);
+            t.hi[a] = std::min(t.hi[a], boxes[i].hi[a]);
+            ok = ok && std::isfinite(t.lo[a]) && std::isfinite(t.hi[a]) && t.lo[a] <= t.hi[a];
+        }
+        if (ok) boxes[i] = t;
+    }
+}
+
+// The CW8 top level built from the instances' world boxes (record words 12-17,
+// tightened to the instances' vertices), one instance a leaf. False with `err` empty when the 4-wide top level does not
 // reach each instance exactly once, a box is not finite, or the tree would pass
 // W8_MAX_LEVELS: the caller collapses the 4-wide top level instead.
 bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
@@ -975,6 +1034,7 @@ bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
             boxes[i].hi[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 15 + a]);
             if (!std::isfinite(boxes[i].lo[a]) || !std::isfinite(boxes[i].hi[a])) return false;
         }
+    tighten_instance_boxes(plain, out, boxes);
     std::vector<uint32_t> order;
     const std::vector<BinNode> bin = sah_binary(boxes, order);
     std::vector<std::vector<Cand>> lists;

@@ -323,6 +323,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     // [6] live lanes, [7] [8] [9] lanes wanting a node / triangle-pair / instance
     // step, summed over wave iterations (wave-uniform)
     unsigned long long st_live = 0, st_want_node = 0, st_want_tri = 0, st_want_inst = 0;
+    // [10] [11] wave iterations running node / triangle-pair steps, [12] node
+    // iterations in which some lane entered an instance, [13] speculative node
+    // steps (lane sums), [14] drain iterations, [15] top-level node steps (lane sums)
+    unsigned long long st_node_it = 0, st_tri_it = 0, st_inst_it = 0, st_spec = 0, st_drain = 0, st_top = 0;
+    bool entered = false;  // COUNT: the lane entered an instance this iteration
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
 
     // ---- the ray cursor. The rays [0, count) are dealt out in chunks of 64
@@ -735,7 +740,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             // an instance (a top-level or flat-loop leaf; speculation takes nodes only) or a node
             bool node_step = !(item & LEAF_BIT);
             if (!node_step) {
-                if (COUNT) st_inst++;
+                if (COUNT) {
+                    st_inst++;
+                    entered = true;
+                }
                 // gi_instance prologue (:100-113)
                 const uint32_t i = item & ~LEAF_BIT;
                 uint32_t rw[20];
@@ -804,7 +812,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     }
                 }
             }
-            if (node_step && COUNT) st_node++;
+            if (node_step && COUNT) {
+                st_node++;
+                st_spec += spec;
+                st_top += !in_blas;
+            }
             if (!node_step) {
             } else if (LAYOUT == REF || LAYOUT == RS) {
                 // RS: rs_trace's top level and BLAS nodes are the same records; its
@@ -1024,6 +1036,18 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             }
             }
         }
+        if (COUNT) {
+            const bool node_lane = (kind >= 2 && (kind == pick || dual)) || spec;
+            const bool tri_lane = kind == 1 && (pick == 1 || dual);
+            const uint64_t nm = __ballot(node_lane), tm = __ballot(tri_lane), im = __ballot(entered);
+            if (lane == (uint32_t)__ffsll((unsigned long long)__ballot(true)) - 1u) {
+                st_node_it += nm != 0ull;
+                st_tri_it += tm != 0ull;
+                st_inst_it += im != 0ull;
+                st_drain += dual;
+            }
+            entered = false;
+        }
         if (fin) done = true;
     }
     if (COUNT) {
@@ -1035,6 +1059,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             st_want_node += __shfl_xor(st_want_node, off);
             st_want_tri += __shfl_xor(st_want_tri, off);
             st_want_inst += __shfl_xor(st_want_inst, off);
+            st_node_it += __shfl_xor(st_node_it, off);
+            st_tri_it += __shfl_xor(st_tri_it, off);
+            st_inst_it += __shfl_xor(st_inst_it, off);
+            st_spec += __shfl_xor(st_spec, off);
+            st_drain += __shfl_xor(st_drain, off);
+            st_top += __shfl_xor(st_top, off);
         }
         if (lane == 0) {
             atomicAdd(a.bytes, nb);
@@ -1047,6 +1077,12 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
             atomicAdd(a.bytes + 7, st_want_node);
             atomicAdd(a.bytes + 8, st_want_tri);
             atomicAdd(a.bytes + 9, st_want_inst);
+            atomicAdd(a.bytes + 10, st_node_it);
+            atomicAdd(a.bytes + 11, st_tri_it);
+            atomicAdd(a.bytes + 12, st_inst_it);
+            atomicAdd(a.bytes + 13, st_spec);
+            atomicAdd(a.bytes + 14, st_drain);
+            atomicAdd(a.bytes + 15, st_top);
         }
     }
 }
@@ -2200,29 +2236,29 @@ extern "C" rq_status rq_set_batch_tuning(rq_layout layout, const rq_tuning* t) {
 }
 
 extern "C" rq_status rq_trace_count_stats(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
-                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[10]) {
+                                          uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t stats[16]) {
     if (!scene || !stats || !layout_ok(scene, layout) || (mode & ~RQ_REORDER) > 2 || (count && !rays) ||
         !scene->dev[layout].ready) {
         set_error("rq_trace_count_bytes: bad argument");
         return RQ_ERR_INVALID_INPUT;
     }
     unsigned long long* acc = nullptr;
-    HIP_TRY(hipMalloc(&acc, 10 * sizeof(unsigned long long)), "hipMalloc count");
-    HIP_TRY(hipMemset(acc, 0, 10 * sizeof(unsigned long long)), "hipMemset count");
+    HIP_TRY(hipMalloc(&acc, 16 * sizeof(unsigned long long)), "hipMalloc count");
+    HIP_TRY(hipMemset(acc, 0, 16 * sizeof(unsigned long long)), "hipMemset count");
     rq_status st = launch_trace(scene, layout, rays, count, mode, hits, vis_bits, nullptr, acc);
-    unsigned long long host[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long host[16] = {};
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(host, acc, sizeof host, hipMemcpyDeviceToHost);
     (void)hipFree(acc);
     if (st != RQ_OK) return st;
     if (e != hipSuccess) return hip_fail(e, "count");
-    for (int k = 0; k < 10; ++k) stats[k] = host[k];
+    for (int k = 0; k < 16; ++k) stats[k] = host[k];
     return RQ_OK;
 }
 
 extern "C" rq_status rq_trace_count_bytes(rq_scene* scene, rq_layout layout, const rq_ray* rays, uint32_t count,
                                           uint32_t mode, rq_hit* hits, uint32_t* vis_bits, uint64_t* bytes) {
-    uint64_t stats[10];
+    uint64_t stats[16];
     rq_status st = rq_trace_count_stats(scene, layout, rays, count, mode, hits, vis_bits, stats);
     if (st == RQ_OK) *bytes = stats[0];
     return st;

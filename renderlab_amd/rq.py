@@ -761,10 +761,12 @@ class Scene:
         return n.value
 
     def count_stats(self, layout, rays, count, mode, hits=None, vis=None):
-        """Step statistics of one launch: bytes, wave iterations, node / triangle-pair / instance steps, refills."""
+        """Step statistics of one launch (rq_trace_count_stats): bytes, wave iterations, node / triangle-pair /
+        instance steps, refills, per-iteration lane sums, iterations by step kind, speculative and top-level node
+        steps, drain iterations."""
         def p(x):
             return None if x is None else (x.ptr if isinstance(x, DeviceBuffer) else x)
-        out = np.zeros(10, np.uint64)
+        out = np.zeros(16, np.uint64)
         own = None
         if hits is None and vis is None:
             own = DeviceBuffer(max(int(count), 1) * HIT_DTYPE.itemsize)
@@ -775,7 +777,8 @@ class Scene:
         check(lib.rq_trace_count_stats(self._h, LAYOUTS.get(layout, layout), p(rays), int(count), int(mode), p(hits),
                                        p(vis), _ptr(out)))
         return dict(zip(["bytes", "iterations", "node_steps", "tri_steps", "inst_steps", "refills", "live_lanes",
-                         "want_node", "want_tri", "want_inst"],
+                         "want_node", "want_tri", "want_inst", "node_iterations", "tri_iterations",
+                         "inst_iterations", "spec_steps", "drain_iterations", "top_steps"],
                         (int(x) for x in out)))
 
     def hit_record(self, hit):

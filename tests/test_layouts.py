@@ -133,13 +133,35 @@ def test_cw8_leaves_cover_their_mesh_and_their_boxes_enclose_their_triangles(kin
     assert checked > 0
 
 
+def instance_world_vertices(sc, i):
+    """The vertices of instance i's triangles (through its CW8 tree's leaves) taken to world space in f64
+    by inverting the record's world-to-object rows."""
+    inst, nodes = sc.layout_image("cw8", 0).reshape(-1, 32), sc.layout_image("cw8", 2)
+    verts = sc.layout_image("cw8", 6).reshape(-1, 9).view(np.float32)
+    spans, stack = [], [int(inst[i, 18])]
+    while stack:
+        t = stack.pop()
+        if t & LEAF:
+            spans.append(leaf_ranges(t))
+            continue
+        stack.extend(c for c, _, _ in w8_children(nodes, t))
+    v = np.concatenate([verts[a:a + n] for a, n in spans]).reshape(-1, 3).astype(np.float64)
+    rows = sc.words[16 + 32 * i:16 + 32 * i + 12].view(np.float32).astype(np.float64).reshape(3, 4)
+    return (np.linalg.inv(rows[:, :3]) @ (v - rows[:, 3]).T).T
+
+
 def check_cw8_top_level(sc):
+    """Every instance is one leaf of the CW8 top level; each leaf box encloses the instance's vertices in
+    world space with a margin far above the f32 roundings of an object-space ray, and, for instances whose
+    transform rotates the mesh, is tighter than the record's world box (the box of the transformed mesh-box
+    corners)."""
     meta = sc.layout_image("cw8", 5)
     top = sc.layout_image("cw8", 1)
     assert meta[0] == 1
     seen = np.zeros(int(meta[2]), int)
     stack = [int(meta[1])]
     w = sc.words
+    tighter = 0
     while stack:
         t = stack.pop()
         if t & LEAF:
@@ -149,15 +171,22 @@ def check_cw8_top_level(sc):
         for c, lo, hi in w8_children(top, t):
             if c & LEAF:
                 i = c & ~LEAF & 0xFFFFFFFF
+                wv = instance_world_vertices(sc, i)
+                margin = 5e-6 * (1.0 + np.abs(wv))
+                assert (lo <= wv - margin).all() and (hi >= wv + margin).all()
                 r = 16 + 32 * i
                 blo, bhi = w[r + 12:r + 15].view(np.float32), w[r + 15:r + 18].view(np.float32)
-                assert (lo <= blo).all() and (hi >= bhi).all()
+                tighter += bool(np.prod(hi - lo, dtype=np.float64) < 0.9 * np.prod(bhi - blo, dtype=np.float64))
             stack.append(c)
     assert (seen == 1).all()
+    return tighter
 
 
 def test_cw8_top_level_reaches_every_instance_once():
-    check_cw8_top_level(rq.Scene.generate(0, 2))
+    # rotated boxes: the transformed corners are the vertices, so only the 8 spheres' boxes tighten
+    assert check_cw8_top_level(rq.Scene.generate(0, 2)) >= 1
+    # rotated lognormal clusters: nearly every leaf box is tighter than the record box
+    assert check_cw8_top_level(rq.Scene.generate(1, 3, 300, 30_000)) > 150
 
 
 def check_cw8_mesh_leaves(sc):
