@@ -941,6 +941,64 @@ bool rebuild_mesh(const HostLayout& plain, uint32_t pr, uint32_t tb, uint32_t n,
     return true;
 }
 
+// A CW8 top level over instance boxes, one instance a leaf (LEAF | instance):
+// binned SAH, then the cost-optimal 8-wide collapse. False with `err` empty when
+// the tree would pass W8_MAX_LEVELS.
+bool top_from_boxes(const std::vector<Bounds>& boxes, std::vector<uint32_t>& top, uint32_t& root, std::string& err) {
+    std::vector<uint32_t> order;
+    const std::vector<BinNode> bin = sah_binary(boxes, order);
+    std::vector<std::vector<Cand>> lists;
+    const uint32_t rt = dp_collapse(
+        bin, 1, [](uint32_t) { return DP_C_INST; }, [&](const BinNode& x) { return LEAF_BIT | order[x.first]; },
+        lists);
+    if (rt & LEAF_BIT) {
+        root = rt;
+        return true;
+    }
+    if (list_levels(lists) > W8_MAX_LEVELS) return false;
+    W8Builder w{nullptr, &top, &err};
+    w.lists = &lists;
+    return w.root(rt, root);
+}
+
+// The world-to-object rows (3 x 4, record words 0-11) inverted in f64: world =
+// inv (object - T). False for a singular or non-finite transform.
+bool inverse_rows(const uint32_t* r, double inv[3][3], double T[3]) {
+    double R[3][3];
+    for (int a = 0; a < 3; ++a) {
+        for (int b = 0; b < 3; ++b) R[a][b] = bits_f(r[4 * a + b]);
+        T[a] = bits_f(r[4 * a + 3]);
+    }
+    const double c00 = R[1][1] * R[2][2] - R[1][2] * R[2][1], c01 = R[1][2] * R[2][0] - R[1][0] * R[2][2],
+                 c02 = R[1][0] * R[2][1] - R[1][1] * R[2][0];
+    const double det = R[0][0] * c00 + R[0][1] * c01 + R[0][2] * c02;
+    if (!std::isfinite(det) || std::fabs(det) < 1e-30) return false;
+    const double m[3][3] = {
+        {c00, R[0][2] * R[2][1] - R[0][1] * R[2][2], R[0][1] * R[1][2] - R[0][2] * R[1][1]},
+        {c01, R[0][0] * R[2][2] - R[0][2] * R[2][0], R[0][2] * R[1][0] - R[0][0] * R[1][2]},
+        {c02, R[0][1] * R[2][0] - R[0][0] * R[2][1], R[0][0] * R[1][1] - R[0][1] * R[1][0]}};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            inv[a][b] = m[a][b] / det;
+            if (!std::isfinite(inv[a][b])) return false;
+        }
+    return true;
+}
+
+// Pads a world box of an instance's geometry (f64, lo <= hi) as the record's own
+// box is padded (src/gi_bvh.rs instance_words: 1e-5 (1 + max |coordinate|) per
+// axis), rounds it outward to f32 and clips it to the record box `rec`; false when
+// the result is empty or not finite (the record box is kept then).
+bool padded_world_box(const double lo[3], const double hi[3], const Bounds& rec, Bounds& out) {
+    for (int a = 0; a < 3; ++a) {
+        const double pad = 1e-5 * (1.0 + std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+        out.lo[a] = std::max(std::nextafter((float)(lo[a] - pad), -INFINITY), rec.lo[a]);
+        out.hi[a] = std::min(std::nextafter((float)(hi[a] + pad), INFINITY), rec.hi[a]);
+        if (!std::isfinite(out.lo[a]) || !std::isfinite(out.hi[a]) || out.lo[a] > out.hi[a]) return false;
+    }
+    return true;
+}
+
 // The world box of each instance's own vertices, in place of the box of its
 // mesh box's transformed corners (record words 12-17), which for a rotated mesh
 // is up to about 1.7 times wider on each axis: a top-level leaf box only culls.
@@ -963,20 +1021,8 @@ void tighten_instance_boxes(const HostLayout& plain, const HostLayout& out, std:
         const size_t te = (size_t)mesh + 1 < plain.mesh_base.size() / 3 ? plain.mesh_base[3 * (mesh + 1) + 2]
                                                                          : total_tris;
         if (te <= tb || te > total_tris || te - tb > TIGHT_MAX_TRIS) continue;
-        // world = R^-1 (object - T), R the rows' 3x3 and T their fourth column
-        double R[3][3], T[3];
-        for (int a = 0; a < 3; ++a) {
-            for (int b = 0; b < 3; ++b) R[a][b] = bits_f(r[4 * a + b]);
-            T[a] = bits_f(r[4 * a + 3]);
-        }
-        const double c00 = R[1][1] * R[2][2] - R[1][2] * R[2][1], c01 = R[1][2] * R[2][0] - R[1][0] * R[2][2],
-                     c02 = R[1][0] * R[2][1] - R[1][1] * R[2][0];
-        const double det = R[0][0] * c00 + R[0][1] * c01 + R[0][2] * c02;
-        if (!std::isfinite(det) || std::fabs(det) < 1e-30) continue;
-        const double inv[3][3] = {
-            {c00 / det, (R[0][2] * R[2][1] - R[0][1] * R[2][2]) / det, (R[0][1] * R[1][2] - R[0][2] * R[1][1]) / det},
-            {c01 / det, (R[0][0] * R[2][2] - R[0][2] * R[2][0]) / det, (R[0][2] * R[1][0] - R[0][0] * R[1][2]) / det},
-            {c02 / det, (R[0][1] * R[2][0] - R[0][0] * R[2][1]) / det, (R[0][0] * R[1][1] - R[0][1] * R[1][0]) / det}};
+        double inv[3][3], T[3];
+        if (!inverse_rows(r, inv, T)) continue;
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (size_t k = TRI_FLOATS * tb; k < TRI_FLOATS * te; k += 3) {
             const double p[3] = {out.verts[k] - T[0], out.verts[k + 1] - T[1], out.verts[k + 2] - T[2]};
@@ -987,16 +1033,49 @@ void tighten_instance_boxes(const HostLayout& plain, const HostLayout& out, std:
             }
         }
         Bounds t;
-        bool ok = true;
-        for (int a = 0; a < 3; ++a) {
-            const double pad = 1e-5 * (1.0 + std::max(std::fabs(lo[a]), std::fabs(hi[a])));
-            t.lo[a] = std::nextafter((float)(lo[a] - pad), -INFINITY);
-            t.hi[a] = std::nextafter((float)(hi[a] + pad), INFINITY);
-            t.lo[a] = std::max(t.lo[a], boxes[i].lo[a]);
-            t.hi[a] = std::min(t.hi[a], boxes[i].hi[a]);
-            ok = ok && std::isfinite(t.lo[a]) && std::isfinite(t.hi[a]) && t.lo[a] <= t.hi[a];
+        if (padded_world_box(lo, hi, boxes[i], t)) boxes[i] = t;
+    }
+}
+
+// The reflection records' version: the world box of each instance's triangle
+// records (a, e1, e2 at word `tb[i]`, `n[i]` of them; n 0 keeps the record box),
+// each vertex a, a + e1, a + e2 taken to world space in f64 and widened by its
+// triangle's own box slack (2e-5 of its longest edge and 2^-20 of its largest
+// coordinate, the bound rs_trace's barycentric slack needs, as the mesh trees
+// use) scaled by the inverse's row sums, then padded and clipped as above.
+void tighten_rs_boxes(const std::vector<uint32_t>& words, const std::vector<uint32_t>& tb,
+                      const std::vector<uint32_t>& n, std::vector<Bounds>& boxes) {
+    for (uint32_t i = 0; i < (uint32_t)boxes.size(); ++i) {
+        if (n[i] == 0 || n[i] > TIGHT_MAX_TRIS || (size_t)tb[i] + (size_t)RS_TRI_WORDS * n[i] > words.size()) continue;
+        double inv[3][3], T[3];
+        if (!inverse_rows(&words[RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS], inv, T)) continue;
+        double rows[3];
+        for (int a = 0; a < 3; ++a) rows[a] = std::fabs(inv[a][0]) + std::fabs(inv[a][1]) + std::fabs(inv[a][2]);
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        bool finite = true;
+        for (uint32_t k = 0; k < n[i] && finite; ++k) {
+            const uint32_t* w = &words[(size_t)tb[i] + (size_t)RS_TRI_WORDS * k];
+            double v[3][3], e = 0.0, m = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double p = bits_f(w[a]), e1 = bits_f(w[3 + a]), e2 = bits_f(w[6 + a]);
+                v[0][a] = p;
+                v[1][a] = p + e1;
+                v[2][a] = p + e2;
+                e = std::max({e, std::fabs(e1), std::fabs(e2)});
+                m = std::max({m, std::fabs(p), std::fabs(p + e1), std::fabs(p + e2)});
+            }
+            const double slack = 2e-5 * e + std::ldexp(m, -20);
+            finite = std::isfinite(slack);
+            for (int c = 0; c < 3; ++c)
+                for (int a = 0; a < 3; ++a) {
+                    const double x = inv[a][0] * (v[c][0] - T[0]) + inv[a][1] * (v[c][1] - T[1]) +
+                                     inv[a][2] * (v[c][2] - T[2]);
+                    lo[a] = std::min(lo[a], x - slack * rows[a]);
+                    hi[a] = std::max(hi[a], x + slack * rows[a]);
+                }
         }
-        if (ok) boxes[i] = t;
+        Bounds t;
+        if (finite && padded_world_box(lo, hi, boxes[i], t)) boxes[i] = t;
     }
 }
 
@@ -1035,20 +1114,7 @@ bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
             if (!std::isfinite(boxes[i].lo[a]) || !std::isfinite(boxes[i].hi[a])) return false;
         }
     tighten_instance_boxes(plain, out, boxes);
-    std::vector<uint32_t> order;
-    const std::vector<BinNode> bin = sah_binary(boxes, order);
-    std::vector<std::vector<Cand>> lists;
-    const uint32_t rt = dp_collapse(
-        bin, 1, [](uint32_t) { return DP_C_INST; }, [&](const BinNode& x) { return LEAF_BIT | order[x.first]; },
-        lists);
-    if (rt & LEAF_BIT) {
-        out.top_root = rt;
-        return true;
-    }
-    if (list_levels(lists) > W8_MAX_LEVELS) return false;
-    W8Builder w{nullptr, &out.top, &err};
-    w.lists = &lists;
-    return w.root(rt, out.top_root);
+    return top_from_boxes(boxes, out.top, out.top_root, err);
 }
 
 // The CW8 tree of one reflection mesh (node base `base`, root `root`, triangle
@@ -1253,7 +1319,7 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
     struct RsJob {
         uint32_t base, root, tb;
         std::vector<uint32_t> nodes, tris;
-        uint32_t cw8_root = EMPTY, new_tb = 0;
+        uint32_t cw8_root = EMPTY, new_tb = 0, n = 0;
         bool ok = false, placed = false;
         std::string err;
     };
@@ -1263,7 +1329,7 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
         const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
         const auto k3 = std::make_tuple(words[r + 18], words[r + 20], words[r + 19]);
         if (job_of.emplace(k3, (uint32_t)jobs.size()).second)
-            jobs.push_back(RsJob{words[r + 18], words[r + 20], words[r + 19], {}, {}, EMPTY, 0, false, false, {}});
+            jobs.push_back(RsJob{words[r + 18], words[r + 20], words[r + 19], {}, {}, EMPTY, 0, 0, false, false, {}});
     }
     {
         const uint32_t threads = std::max(1u, std::min<uint32_t>(std::thread::hardware_concurrency(), 16));
@@ -1282,6 +1348,7 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
             });
         for (auto& th : pool) th.join();
     }
+    std::vector<uint32_t> rec_tb(count, 0u), rec_n(count, 0u);  // rebuilt meshes' triangle records per record
     for (uint32_t i = 0; i < count; ++i) {
         const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
         const uint32_t base = words[r + 18], root = words[r + 20];
@@ -1303,6 +1370,7 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
                     return false;
                 }
                 j.new_tb = (uint32_t)out.inst.size();
+                j.n = (uint32_t)(j.tris.size() / RS_TRI_WORDS);
                 out.nodes.insert(out.nodes.end(), j.nodes.begin(), j.nodes.end());
                 out.inst.insert(out.inst.end(), j.tris.begin(), j.tris.end());
                 std::vector<uint32_t>().swap(j.nodes);
@@ -1311,6 +1379,8 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
             }
             out.inst[r + 19] = j.new_tb;
             out.inst[r + 20] = j.cw8_root;
+            rec_tb[i] = j.new_tb;
+            rec_n[i] = j.n;
             continue;
         }
         const uint64_t key = (uint64_t)base << 32 | root;
@@ -1370,10 +1440,36 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
         }
         out.inst[r + 20] = cw8_root;
     }
-    // Top level: the reference's 4-wide f32 nodes, collapsed the same way.
+    // Top level: built from the instances' world boxes (record words 12-17), each
+    // tightened to its mesh's triangle records where the mesh tree was rebuilt
+    // from them; rs_trace's closest hit is the least (t, instance, primitive)
+    // whatever the order the instances are visited in. Boxes that are not finite,
+    // or a tree past W8_MAX_LEVELS: the reference's 4-wide f32 nodes, collapsed
+    // the same way as the meshes' trees.
     out.top_on = count && words[1] != 0;
     out.top_root = count ? words[2] : 0u;
+    bool rebuilt = false;
     if (out.top_on) {
+        std::vector<Bounds> boxes(count);
+        bool finite = true;
+        for (uint32_t i = 0; i < count && finite; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const size_t r = RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS;
+                boxes[i].lo[a] = bits_f(out.inst[r + 12 + a]);
+                boxes[i].hi[a] = bits_f(out.inst[r + 15 + a]);
+                finite = finite && std::isfinite(boxes[i].lo[a]) && std::isfinite(boxes[i].hi[a]);
+            }
+        if (finite) {
+            tighten_rs_boxes(out.inst, rec_tb, rec_n, boxes);
+            rebuilt = top_from_boxes(boxes, out.top, out.top_root, err);
+            if (!rebuilt) {
+                if (!err.empty()) return false;
+                out.top.clear();
+                out.top_root = words[2];
+            }
+        }
+    }
+    if (out.top_on && !rebuilt) {
         const size_t at = words[1];
         const size_t n = std::min<size_t>(top_reserve(count), words.size() - std::min(words.size(), at));
         std::vector<uint32_t> top(words.begin() + at, words.begin() + at + n);

@@ -269,7 +269,8 @@ def test_rs_cw8_relayout_covers_every_triangle_once():
     (word 19); each mesh's tree is built again from its triangle records, which follow the words in
     the new leaf order (the mesh's own rows, moved whole). Each record's CW8 tree reaches each of its
     mesh's triangles exactly once, every CW8 box on the way to a leaf encloses the leaf's triangles
-    (a, a + e1, a + e2), and the CW8 top level reaches every instance once."""
+    (a, a + e1, a + e2), and the CW8 top level reaches every instance once, its leaf boxes enclosing the
+    instances' triangles in world space."""
     from test_layouts import w8_children
     meshes, inst, _ = random_reflection_scene(3, instances=60)
     sc = rq.Scene.build(meshes, inst, reflection=True)
@@ -308,11 +309,13 @@ def test_rs_cw8_relayout_covers_every_triangle_once():
                 got += [(t, [(lo, hi)] + path) for t, path in cw8_leaves(c)]
         return got
 
+    tris_of = {}
     for i in range(count):
         r = RS_HEADER + RS_REC * i
         old_tb, new_tb = int(w[r + 19]), int(img[r + 19])
         want = rs_leaves(int(w[r + 18]), int(w[r + 20]))
         n = max(((t & ~rq.LEAF_BIT) >> 4) + (t & 15) for t in want)
+        tris_of[i] = rows(img, new_tb, 0, n).view(np.float32).astype(np.float64)
         got = cw8_leaves(int(img[r + 20]))
         covered = np.zeros(n, int)
         for t, path in got:
@@ -334,3 +337,28 @@ def test_rs_cw8_relayout_covers_every_triangle_once():
             out += top_leaves(c)
         return out
     assert sorted(top_leaves(int(meta[1]))) == list(range(count))
+
+    # Each top-level leaf box encloses its instance's triangle vertices taken to world space in f64
+    # (the inverse of the record's world-to-object rows), with a margin far above the f32 roundings of
+    # an object-space ray; most are tighter than the record's world box.
+    def leaf_boxes(tag, out):
+        for c, lo, hi in w8_children(top, tag):
+            if c & rq.LEAF_BIT:
+                out[c & ~rq.LEAF_BIT] = (lo, hi)
+            else:
+                leaf_boxes(c, out)
+        return out
+    boxes = leaf_boxes(int(meta[1]), {})
+    tighter = 0
+    for i in range(count):
+        r = RS_HEADER + RS_REC * i
+        m = w[r:r + 12].view(np.float32).astype(np.float64).reshape(3, 4)
+        t = tris_of[i]
+        pts = np.concatenate([t[:, 0:3], t[:, 0:3] + t[:, 3:6], t[:, 0:3] + t[:, 6:9]])
+        wv = (np.linalg.inv(m[:, :3]) @ (pts - m[:, 3]).T).T
+        lo, hi = boxes[i]
+        margin = 5e-6 * (1.0 + np.abs(wv))
+        assert (lo <= wv - margin).all() and (hi >= wv + margin).all()
+        blo, bhi = w[r + 12:r + 15].view(np.float32), w[r + 15:r + 18].view(np.float32)
+        tighter += bool(np.prod(hi - lo, dtype=np.float64) < 0.9 * np.prod(bhi - blo, dtype=np.float64))
+    assert tighter > count // 2
