@@ -292,7 +292,7 @@ __device__ __forceinline__ void cas_desc(uint32_t& ka, uint32_t& va, uint32_t& k
 // triangles' own vertices (REF: through the indices; PLAIN/CW8: their vertex
 // records).
 template <int LAYOUT, int BLOCK, int S, int AM, bool COUNT = false, bool BATCHED = false, bool WT = false>
-__global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
+__global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kernel(TraceArgs a) {
     static_assert((S & (S - 1)) == 0, "stack ring is a power of two");
     // RSF: reflection words (records, triangles, rs_trace's rules); W8: 8-wide nodes.
     constexpr bool RSF = LAYOUT == RS || LAYOUT == RSW;
@@ -331,7 +331,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     const uint32_t count = a.count_ptr ? min(*a.count_ptr, a.count) : a.count;
 
     // ---- the ray cursor. The rays [0, count) are dealt out in chunks of 64
-    // (chunk c = rays [64c, 64c + 64)); chunk c belongs to partition c mod
+    // (chunk c = rays [64c, 64c + 64), or, in a launch of several ranges, each
+    // range's 64-ray chunks in turn); chunk c belongs to partition c mod
     // CURSOR_PARTS, and each partition has its own counter of claimed chunks. Claims
     // thus spread over CURSOR_PARTS addresses instead of queueing on one (an
     // address takes one atomic at a time: a single cursor made every refill wait
@@ -349,15 +350,45 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     static_assert(PARTS <= 32, "a 32-bit mask of empty partitions; one lane per partition when probing");
     const uint32_t gwave = blockIdx.x * (uint32_t)(BLOCK / 64) + __builtin_amdgcn_readfirstlane(tid) / 64u;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / 64);
-    const uint64_t chunks = ((uint64_t)count + 63u) / 64u;
+    // A launch of several ranges (AM 2) numbers each range's chunks on from the
+    // previous range's, so no chunk holds rays of two ranges: the rays of one
+    // refill then share their range, whose buffers are picked once per refill.
+    // cend[j]: the chunks of ranges 0..j.
+    uint32_t cend[4] = {0u, 0u, 0u, 0u};
+    uint64_t chunks;
+    if (AM == 2 && !BATCHED) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            c += (a.rstart[j + 1] - a.rstart[j] + 63u) / 64u;
+            cend[j] = c;
+        }
+        chunks = c;
+    } else {
+        chunks = ((uint64_t)count + 63u) / 64u;
+    }
     uint32_t part = gwave % PARTS;
     uint32_t dead = 0u;  // partitions this wave found empty
     // partition p's chunks below `waves` are first slices: its counter counts from there
     auto first_round = [&](uint32_t p) -> uint64_t { return p < waves ? (waves - 1u - p) / PARTS + 1u : 0ull; };
-    // the wave's claimed rays not yet handed out: [res_next, res_end)
-    // (64-bit bounds compared by hand: no 64-bit min overload that stays integer)
-    uint32_t res_next = 64ull * gwave < count ? 64u * gwave : count;
-    uint32_t res_end = (uint64_t)res_next + 64ull < count ? res_next + 64u : count;
+    // the wave's claimed rays not yet handed out: [res_next, res_end), all of range res_j
+    uint32_t res_next = 0u, res_end = 0u, res_j = 0u;
+    // the rays of chunk c < chunks (64-bit bounds compared by hand: there is no 64-bit
+    // min overload that stays integer)
+    auto take_chunk = [&](uint64_t c) {
+        if (AM == 2 && !BATCHED) {
+            res_j = (c >= cend[0] ? 1u : 0u) + (c >= cend[1] ? 1u : 0u) + (c >= cend[2] ? 1u : 0u);
+            const uint32_t first = res_j == 0 ? 0u : res_j == 1 ? cend[0] : res_j == 2 ? cend[1] : cend[2];
+            const uint32_t lo = res_j == 0 ? a.rstart[0] : res_j == 1 ? a.rstart[1] : res_j == 2 ? a.rstart[2] : a.rstart[3];
+            const uint32_t hi = res_j == 0 ? a.rstart[1] : res_j == 1 ? a.rstart[2] : res_j == 2 ? a.rstart[3] : a.rstart[4];
+            res_next = lo + 64u * ((uint32_t)c - first);
+            res_end = hi - res_next > 64u ? res_next + 64u : hi;
+        } else {
+            res_next = (uint32_t)(64ull * c);
+            res_end = 64ull * c + 64ull < count ? res_next + 64u : count;
+        }
+    };
+    if (gwave < chunks) take_chunk(gwave);
     // Claims the wave's next chunk from its current partition. A partition whose
     // claim fails has no chunks left: the wave moves on to the next partition it
     // has not found empty (and tries it at its next refill), and is exhausted once
@@ -368,8 +399,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
         old = __builtin_amdgcn_readfirstlane(old);  // every lane is active here: lane 0's value
         const uint64_t c = (first_round(part) + old) * PARTS + part;
         if (c < chunks) {
-            res_next = (uint32_t)(64ull * c);
-            res_end = 64ull * c + 64ull < count ? res_next + 64u : count;
+            take_chunk(c);
             return;
         }
         dead |= 1u << part;
@@ -516,8 +546,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
                     const rq_ray* src = a.rays + r;
                     out_idx = r;
                     if (AM == 2 && !BATCHED) {
-                        const uint32_t j = (r >= a.rstart[1] ? 1u : 0u) + (r >= a.rstart[2] ? 1u : 0u) +
-                                           (r >= a.rstart[3] ? 1u : 0u);
+                        const uint32_t j = res_j;  // every ray of this refill is of range res_j
                         const rq_ray* rb = j == 0 ? a.rrays[0] : j == 1 ? a.rrays[1] : j == 2 ? a.rrays[2] : a.rrays[3];
                         out_hits = j == 0 ? a.rhits[0] : j == 1 ? a.rhits[1] : j == 2 ? a.rhits[2] : a.rhits[3];
                         out_vis = j == 0 ? a.rvis[0] : j == 1 ? a.rvis[1] : j == 2 ? a.rvis[2] : a.rvis[3];
