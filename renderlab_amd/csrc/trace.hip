@@ -1490,7 +1490,7 @@ void default_tuning(rq_tuning& t, int layout, int any) {
     t.persistent = 1;
     t.waves_per_eu = 0;
     t.grid_per_cu = 0;
-    t.refill = 32u;  // refill threshold (lanes): the autotuner's pick on every measured workload
+    t.refill = 16u;  // refill threshold (lanes): the autotuner's pick for CW8 on the measured frames (PLAIN: 16 or 32)
     t.speculate = layout == RQ_LAYOUT_CW8_Q8 || layout == RQ_LAYOUT_RS_CW8_Q8 ? 1u : 0u;  // 2-3% on the measured frames
     t.drain = 1u;                                        // 0.4-1% faster on the measured frames
 }

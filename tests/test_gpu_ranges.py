@@ -5,7 +5,7 @@ range in the middle, and the four-range maximum."""
 import numpy as np
 import pytest
 
-from helpers import compare_exact, gpu_trace, gpu_workload, to_device
+from helpers import classify_mismatch, compare_exact, gpu_trace, gpu_workload, to_device
 from renderlab_amd import rq
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +54,23 @@ def test_ranges_equal_separate_launches(scene, passes, layout):
             continue
         h0, v0 = gpu_trace(scene, layout, rays, mode)
         if mode == rq.CLOSEST:
+            assert len(compare_exact(h, h0)) == 0
+        assert np.array_equal(v, v0)
+
+
+@pytest.mark.parametrize("layout", ["plain", "cw8"])
+def test_ranges_of_odd_sizes_equal_separate_launches(scene, passes, layout):
+    """Ranges of 1, 63, 65 and 130 rays: the cursor's 64-ray chunks stop at each range's end. CW8 closest
+    hits may differ only on equal-distance ties (its speculation follows the wave's other lanes)."""
+    p, s, g0, g1 = passes
+    specs = [(p[:1], rq.CLOSEST), (s[:63], rq.ANY), (g0[:65], rq.CLOSEST), (g1[:130], rq.ANY_OPAQUE)]
+    print(f"case: ranges {layout} sizes {[len(x) for x, _ in specs]}", flush=True)
+    got = run_ranges(scene, layout, specs)
+    for (rays, mode), (h, v) in zip(specs, got):
+        h0, v0 = gpu_trace(scene, layout, rays, mode)
+        if mode == rq.CLOSEST and layout == "cw8":
+            assert len(classify_mismatch(h, h0)[0]) == 0
+        elif mode == rq.CLOSEST:
             assert len(compare_exact(h, h0)) == 0
         assert np.array_equal(v, v0)
 
