@@ -360,7 +360,7 @@ def run_serving_forms(rq, layout):
     mixed = [run_serving(rq, layout, scenes, rate=r, n_jobs=int(min(600, max(60, 0.4 * r))), resolutions=res,
                          mixes=[0, 1], latency_budget_ms=16.0) for r in (50, 200, 500, 1000, 2000)]
     log(f"serving, mixed resolutions, 16 ms budget: {mixed}")
-    rates = (50, 100, 150, 200, 250, 300, 400)
+    rates = (50, 100, 200, 300, 400, 500, 600, 700)
     sweep = [run_serving(rq, layout, scenes, rate=r, n_jobs=int(max(200, 0.5 * r)), resolutions=[(1920, 1080)],
                          mixes=[1]) for r in rates]
     over = [s["rate_jobs_s"] for s in sweep if s["p99_ms"] >= 16.0]
