@@ -681,6 +681,21 @@ def hit_id_agreement(rq, scene, frame, layout, base="plain"):
     return out
 
 
+def gi_only(rq, scene, frame, layouts, stream, reps=3):
+    """The frame's incoherent GI bounce rays alone (its GI ranges in one launch, no primary or shadow rays),
+    timed with HIP events in each layout at that layout's tuned launch."""
+    from renderlab_amd import autotune
+    ranges = [r for r in frame.secondary_ranges() if r[2] == rq.CLOSEST]
+    n = sum(int(r[1]) for r in ranges)
+    out = {"rays": n}
+    for layout in layouts:
+        ms = autotune._time(lambda: scene.trace_ranges(layout, ranges, stream=stream), reps=reps, stream=stream)
+        out[layout] = {"ms": round(ms, 4), "mrays_s": round(n / ms / 1e3, 1)}
+    if "plain" in out and len(layouts) > 1:
+        out["quantized_speedup"] = round(out["plain"]["ms"] / out[layouts[0]]["ms"], 3)
+    return out
+
+
 def fp64_all(rq, scene, frame, layout):
     """rq_verify_fp64 over every closest hit of one frame (primary and each GI bounce) in `layout`."""
     n, n_gi = frame.n, frame.n_gi
@@ -836,6 +851,8 @@ def main():
             line["plain_f32"]["dominant_lane_use"] = round(
                 (plain_stats["node_steps"] + plain_stats["tri_steps"] + plain_stats["inst_steps"]) /
                 max(64 * plain_stats["iterations"], 1), 3)
+        line["gi_rays_only"] = gi_only(rq, scene, frame, layouts, stream)
+        log(f"GI bounce rays alone: {line['gi_rays_only']}")
         if not args.no_agreement:
             line["hit_id_agreement"] = hit_id_agreement(rq, scene, frame, args.layout)
             log(f"hit-ID agreement: {line['hit_id_agreement']}")
