@@ -97,10 +97,11 @@ __device__ __forceinline__ float tri_test(V3 o, V3 d, V3 a, V3 e1, V3 e2, float 
                                           float& v_out) {
     V3 p = cross3(d, e2);
     float det = dot3(e1, p);
+    V3 s = sub3(o, a);
+    const float su = dot3(s, p);  // before the det test, so that a's load is issued with e1's and e2's
     if (fabsf(det) < 1e-30f) return -1.0f;
     float inv = 1.0f / det;
-    V3 s = sub3(o, a);
-    float u = dot3(s, p) * inv;
+    float u = su * inv;
     if (u < 0.0f || u > 1.0f) return -1.0f;
     V3 q = cross3(s, e1);
     float v = dot3(d, q) * inv;
@@ -253,6 +254,16 @@ struct TraceArgs {
     uint32_t speculate;  // CW8: lanes with triangles pending take node steps with the wave
     uint32_t drain;      // once the rays run out, every live lane takes its step each iteration
 };
+
+// Marks the 20 words as used here: the loads that produce them are all issued
+// before this point, so they take one memory round trip together, instead of the
+// compiler sinking each group into the branch of the test that first reads it
+// (three round trips one after another at an instance entry).
+__device__ __forceinline__ void hold_words(const uint32_t (&w)[20]) {
+    asm volatile("" ::"v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]), "v"(w[4]), "v"(w[5]), "v"(w[6]), "v"(w[7]),
+                 "v"(w[8]), "v"(w[9]), "v"(w[10]), "v"(w[11]), "v"(w[12]), "v"(w[13]), "v"(w[14]), "v"(w[15]),
+                 "v"(w[16]), "v"(w[17]), "v"(w[18]), "v"(w[19]));
+}
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -484,17 +495,22 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 r0 = mk(bf(r[0]), bf(r[1]), bf(r[2]));
                 r1 = mk(bf(r[4]), bf(r[5]), bf(r[6]));
                 r2 = mk(bf(r[8]), bf(r[9]), bf(r[10]));
+                const uint32_t flags = r[19];
                 const float3* tp = reinterpret_cast<const float3*>(v.tris + (uint64_t)TRI_FLOATS * btri);
                 const float3 t1 = tp[1], t2 = tp[2];
                 e1 = mk(t1.x, t1.y, t1.z);
                 e2 = mk(t2.x, t2.y, t2.z);
                 const uint2* ids = reinterpret_cast<const uint2*>(v.tri_ids + (uint64_t)TRI_ID_WORDS * btri);
                 const uint2 i0 = ids[0], i1 = ids[1], i2 = ids[2];
+                // all of it in one memory round trip (the flags word would otherwise load
+                // inside the primitive-table branch, after the rest)
+                hold_words({fb(r0.x), fb(r0.y), fb(r0.z), fb(r1.x), fb(r1.y), fb(r1.z), fb(r2.x), fb(r2.y), fb(r2.z),
+                            flags, fb(e1.x), fb(e1.y), fb(e1.z), fb(e2.x), fb(e2.y), fb(e2.z), i0.x, i0.y, i1.x, i1.y});
                 tri0 = i0.x;
                 tri1 = i0.y;
                 tri2 = i1.x;
                 slot = i1.y;
-                if (!any_hit && (r[19] & 2u)) {
+                if (!any_hit && (flags & 2u)) {
                     tri0 = i2.x;
                     tri1 = 0;
                     tri2 = 0;
@@ -704,13 +720,43 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
         if (kind == 1 && (pick == 1 || dual)) {
             // ---- triangle step (:126-168): the next two triangles in leaf order
             const uint32_t k = tri_left >= 2 ? 2u : 1u;
+            // Both triangles' records (the second is the first again for a lone
+            // triangle, so every load stays inside the leaf) are loaded before the
+            // first test: one memory round trip for the pair, not one each.
+            const uint32_t tri2 = tri_next + (k == 2 ? 1u : 0u);
+            uint4 rsw[2][3];
+            float3 tw[2][3];
+            if (RSF) {
+                const uint4* tp0 = reinterpret_cast<const uint4*>(v.words + m_idx + (uint64_t)RS_TRI_WORDS * tri_next);
+                const uint4* tp1 = reinterpret_cast<const uint4*>(v.words + m_idx + (uint64_t)RS_TRI_WORDS * tri2);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    rsw[0][q] = tp0[q];
+                    rsw[1][q] = tp1[q];
+                }
+                hold_words({rsw[0][0].x, rsw[0][0].y, rsw[0][0].z, rsw[0][0].w, rsw[0][1].x, rsw[0][1].y, rsw[0][1].z,
+                            rsw[0][1].w, rsw[0][2].x, rsw[0][2].y, rsw[1][0].x, rsw[1][0].y, rsw[1][0].z, rsw[1][0].w,
+                            rsw[1][1].x, rsw[1][1].y, rsw[1][1].z, rsw[1][1].w, rsw[1][2].x, rsw[1][2].y});
+            } else if (LAYOUT != REF) {
+                const float* base = WT ? v.verts : v.tris;
+                const float3* tp0 = reinterpret_cast<const float3*>(base + (uint64_t)TRI_FLOATS * tri_next);
+                const float3* tp1 = reinterpret_cast<const float3*>(base + (uint64_t)TRI_FLOATS * tri2);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    tw[0][q] = tp0[q];
+                    tw[1][q] = tp1[q];
+                }
+                hold_words({fb(tw[0][0].x), fb(tw[0][0].y), fb(tw[0][0].z), fb(tw[0][1].x), fb(tw[0][1].y),
+                            fb(tw[0][1].z), fb(tw[0][2].x), fb(tw[0][2].y), fb(tw[0][2].z), fb(tw[1][0].x),
+                            fb(tw[1][0].y), fb(tw[1][0].z), fb(tw[1][1].x), fb(tw[1][1].y), fb(tw[1][1].z),
+                            fb(tw[1][2].x), fb(tw[1][2].y), fb(tw[1][2].z), 0u, 0u});
+            }
 #pragma unroll
             for (uint32_t j = 0; j < 2; ++j) {
                 if (j >= k || fin) break;
                 if (RSF) {
                     // rs_trace leaf (:118-141): words 0-2 corner a, 3-5 e1, 6-8 e2, 9 primitive
-                    const uint4* tp = reinterpret_cast<const uint4*>(v.words + m_idx + (uint64_t)RS_TRI_WORDS * (tri_next + j));
-                    const uint4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+                    const uint4 w0 = rsw[j][0], w1 = rsw[j][1], w2 = rsw[j][2];
                     if (COUNT) nb += 48;
                     float t, u, vv;
                     if (rs_tri_test(oo, od, mk(bf(w0.x), bf(w0.y), bf(w0.z)), mk(bf(w0.w), bf(w1.x), bf(w1.y)),
@@ -742,13 +788,11 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                         t = tri_test(oo, od, va, e1, e2, tmin, limit(cur), u, vv);
                     }
                 } else if (WT) {
-                    const float3* tp = reinterpret_cast<const float3*>(v.verts + (uint64_t)TRI_FLOATS * (tri_next + j));
-                    const float3 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    const float3 t0 = tw[j][0], t1 = tw[j][1], t2 = tw[j][2];
                     t = wt_tri_test(oo, od, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), tmin,
                                     limit(cur), u, vv);
                 } else {
-                    const float3* tp = reinterpret_cast<const float3*>(v.tris + (uint64_t)TRI_FLOATS * (tri_next + j));
-                    const float3 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    const float3 t0 = tw[j][0], t1 = tw[j][1], t2 = tw[j][2];
                     va = mk(t0.x, t0.y, t0.z);
                     e1 = mk(t1.x, t1.y, t1.z);
                     e2 = mk(t2.x, t2.y, t2.z);
@@ -806,6 +850,7 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                     }
                     if (COUNT) nb += 80;
                 }
+                hold_words(rw);
                 const bool skip = any_hit && (rw[19] & 1u);
                 const float lim = limit(i);
                 if (!skip && slab_near<WT>(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
