@@ -325,7 +325,8 @@ rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32
 /* Host image of a relayout (PLAIN or CW8), built on demand without a device:
    part 0 instance records, 1 top-level nodes, 2 BLAS nodes, 3 triangles,
    4 position indices, 5 {top_on, top_root, instances, 0}, 6 the triangles'
-   vertices (a, b, c: the watertight test's records). For inspection.
+   vertices (a, b, c: the watertight test's records), 7 (CW8) a copy of each
+   instance's BLAS root node (24 words, zero for a leaf root). For inspection.
    RS_CW8 (a reflection scene): part 0 the words with each record's root
    replaced by its CW8 root, 1 the CW8 top level, 2 the CW8 BLAS nodes, 5 as
    above; 3 and 4 are empty. */

@@ -163,7 +163,7 @@ extern "C" rq_status rq_scene_stats(const rq_scene* scene, uint64_t stats[8]) {
 extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
                                            size_t* bytes) {
     const bool rs = layout == RQ_LAYOUT_RS_CW8_Q8;
-    if (!scene || !data || !bytes || !((int)layout == 1 || (int)layout == 2 || rs) || part > 6 ||
+    if (!scene || !data || !bytes || !((int)layout == 1 || (int)layout == 2 || rs) || part > 7 ||
         scene->kind != (rs ? RQ_SCENE_REFLECTION : RQ_SCENE_GI)) {
         set_error("rq_scene_layout_image: bad argument");
         return RQ_ERR_INVALID_INPUT;
@@ -194,6 +194,7 @@ extern "C" rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, ui
         case 3: *data = h.tris.data(); *bytes = h.tris.size() * 4; break;
         case 4: *data = h.tri_ids.data(); *bytes = h.tri_ids.size() * 4; break;
         case 6: *data = h.verts.data(); *bytes = h.verts.size() * 4; break;
+        case 7: *data = h.roots.data(); *bytes = h.roots.size() * 4; break;
         default: {
             static thread_local uint32_t meta[4];
             meta[0] = h.top_on;

@@ -192,6 +192,7 @@ struct HostLayout {
     std::vector<float> tris;       // TRI_FLOATS per triangle: a, b - a, c - a
     std::vector<uint32_t> tri_ids; // TRI_ID_WORDS per triangle: ia, ib, ic, slot, primitive, 0
     std::vector<float> verts;      // TRI_FLOATS per triangle: the vertices a, b, c (watertight tests)
+    std::vector<uint32_t> roots;   // CW8: per instance a copy of its BLAS root node (24 words; zero for a leaf root)
     uint32_t top_root = 0;
     uint32_t top_on = 0;
     uint32_t instance_count = 0;

@@ -1297,6 +1297,15 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
     }
     if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
     if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);
+    // Each instance's BLAS root node, copied beside the records: the kernel loads an
+    // instance's root together with its record instead of after it.
+    out.roots.assign((size_t)std::max(out.instance_count, 1u) * W8_WORDS, 0u);
+    for (uint32_t i = 0; i < out.instance_count; ++i) {
+        const uint32_t root = out.inst[(size_t)i * INSTANCE_WORDS + 18];
+        if (root & LEAF_BIT) continue;
+        std::copy(out.nodes.begin() + (size_t)root * W8_WORDS, out.nodes.begin() + ((size_t)root + 1) * W8_WORDS,
+                  out.roots.begin() + (size_t)i * W8_WORDS);
+    }
     (void)words;
     return true;
 }

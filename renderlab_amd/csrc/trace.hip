@@ -219,6 +219,7 @@ struct SceneView {
     const float* tris;      // TRI_FLOATS a triangle: a, b - a, c - a
     const uint32_t* tri_ids;  // TRI_ID_WORDS a triangle: ia, ib, ic, slot, primitive, 0
     const float* verts;     // TRI_FLOATS a triangle: a, b, c (the watertight test)
+    const uint32_t* roots;  // CW8: a copy of each instance's BLAS root node (24 words)
     uint32_t n_inst, top_on, top_root, pad;
 };
 
@@ -812,6 +813,16 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
             nxt = NONE;  // consumed (a speculating lane's pending node, or its stack top already popped)
             // an instance (a top-level or flat-loop leaf; speculation takes nodes only) or a node
             bool node_step = !(item & LEAF_BIT);
+            // CW8: the node this step tests is loaded first -- for an instance entry the
+            // copy of its BLAS root kept beside the records -- so its loads are in flight
+            // together with the instance record's (one memory round trip for the step).
+            uint4 cq0, cq1, cq2, cq3, cq4, cq5;
+            if (LAYOUT == CW8) {
+                const uint32_t* np = node_step ? (in_blas ? v.nodes : v.top) + (uint64_t)item * 24u
+                                               : v.roots + (uint64_t)(item & ~LEAF_BIT) * 24u;
+                const uint4* w4 = reinterpret_cast<const uint4*>(np);
+                cq0 = w4[0]; cq1 = w4[1]; cq2 = w4[2]; cq3 = w4[3]; cq4 = w4[4]; cq5 = w4[5];
+            }
             if (!node_step) {
                 if (COUNT) {
                     st_inst++;
@@ -1020,8 +1031,13 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 // reciprocal's 1 ulp, two products, the slack subtraction and the fma)
                 // stay below 7 * 2^-24 * M with M = |base| + 255 |scale|, so a slack
                 // of 2^-20 * M keeps culling conservative.
-                const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
-                uint4 q0 = w4[0], q1 = w4[1], q2 = w4[2], q3 = w4[3], q4 = w4[4], q5 = w4[5];
+                uint4 q0, q1, q2, q3, q4, q5;
+                if (LAYOUT == CW8) {
+                    q0 = cq0; q1 = cq1; q2 = cq2; q3 = cq3; q4 = cq4; q5 = cq5;
+                } else {
+                    const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
+                    q0 = w4[0]; q1 = w4[1]; q2 = w4[2]; q3 = w4[3]; q4 = w4[4]; q5 = w4[5];
+                }
                 if (COUNT) nb += 96;
                 // An empty slot holds an inverted box (low codes 255, high codes 0), which fails
                 // the slab test unless the ray's origin is about 2^19 node extents away (the
@@ -1839,8 +1855,9 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             if (h.tris.empty()) h.tris.assign(TRI_FLOATS, 0.0f);
             if (h.tri_ids.empty()) h.tri_ids.assign(TRI_ID_WORDS, 0u);
             if (h.verts.empty()) h.verts.assign(TRI_FLOATS, 0.0f);
+            if (h.roots.empty()) h.roots.assign(24, 0u);
             d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.tris) + padded(h.tri_ids) +
-                      padded(h.verts);
+                      padded(h.verts) + padded(h.roots);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
             d.off[0] = at; if ((s = upload(d.base, at, h.inst)) != RQ_OK) return s;
@@ -1849,6 +1866,7 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
             d.off[3] = at; if ((s = upload(d.base, at, h.tris)) != RQ_OK) return s;
             d.off[4] = at; if ((s = upload(d.base, at, h.tri_ids)) != RQ_OK) return s;
             d.off[5] = at; if ((s = upload(d.base, at, h.verts)) != RQ_OK) return s;
+            d.off[6] = at; if ((s = upload(d.base, at, h.roots)) != RQ_OK) return s;
             d.count[0] = h.instance_count;
             d.count[1] = h.top_on ? 1u : 0u;
             d.count[2] = h.top_root;
@@ -1968,6 +1986,7 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.tris = reinterpret_cast<const float*>(b + d.off[3]);
         v.tri_ids = reinterpret_cast<const uint32_t*>(b + d.off[4]);
         v.verts = reinterpret_cast<const float*>(b + d.off[5]);
+        v.roots = reinterpret_cast<const uint32_t*>(b + d.off[6]);
         v.n_inst = d.count[0];
         v.top_on = d.count[1] != 0;
         v.top_root = d.count[2];

@@ -690,7 +690,7 @@ class Scene:
         check(lib.rq_scene_upload(self._h, mask))
 
     def layout_image(self, layout, part):
-        """Host image of a relayout part (0 inst, 1 top, 2 nodes, 3 tris, 4 ids, 5 meta, 6 vertices) as u32
+        """Host image of a relayout part (0 inst, 1 top, 2 nodes, 3 tris, 4 ids, 5 meta, 6 vertices, 7 CW8 root copies) as u32
         words."""
         p = _P()
         n = _sz()

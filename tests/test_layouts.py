@@ -182,6 +182,19 @@ def check_cw8_top_level(sc):
     return tighter
 
 
+def test_cw8_root_copies_equal_each_instance_root_node():
+    """The kernel loads an instance's BLAS root from a copy beside the records (with the record, one
+    memory round trip): each copy is the instance's root node word for word, zero for a leaf root."""
+    for sc in (rq.Scene.generate(0, 2), rq.Scene.generate(1, 3, 300, 30_000)):
+        inst = sc.layout_image("cw8", 0).reshape(-1, 32)
+        nodes = sc.layout_image("cw8", 2).reshape(-1, 24)
+        roots = sc.layout_image("cw8", 7).reshape(-1, 24)
+        assert len(roots) == len(inst)
+        for i, r in enumerate(inst[:, 18]):
+            want = np.zeros(24, np.uint32) if r & LEAF else nodes[r]
+            assert np.array_equal(roots[i], want)
+
+
 def test_cw8_top_level_reaches_every_instance_once():
     # rotated boxes: the transformed corners are the vertices, so only the 8 spheres' boxes tighten
     assert check_cw8_top_level(rq.Scene.generate(0, 2)) >= 1
