@@ -318,7 +318,7 @@ rq_status rq_scene_device_bytes(const rq_scene* scene, rq_layout layout, size_t*
 /* Copies one part of an uploaded layout back to the host (no reference
    counterpart: a diagnostic for checking device images). PLAIN / CW8 parts as
    rq_scene_layout_image numbers them (0-4 and 6, 256-byte padded); RS_CW8 parts 0-2
-   likewise; REF / RS part 0 is the whole word buffer. *bytes = the part's size; nothing is copied when
+   likewise, and its part 3 the root copies (layout_image part 7); REF / RS part 0 is the whole word buffer. *bytes = the part's size; nothing is copied when
    `capacity` is smaller. */
 rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32_t part, void* dst,
                                  size_t capacity, size_t* bytes);
@@ -328,8 +328,8 @@ rq_status rq_scene_download_part(const rq_scene* scene, rq_layout layout, uint32
    vertices (a, b, c: the watertight test's records), 7 (CW8) a copy of each
    instance's BLAS root node (24 words, zero for a leaf root). For inspection.
    RS_CW8 (a reflection scene): part 0 the words with each record's root
-   replaced by its CW8 root, 1 the CW8 top level, 2 the CW8 BLAS nodes, 5 as
-   above; 3 and 4 are empty. */
+   replaced by its CW8 root, 1 the CW8 top level, 2 the CW8 BLAS nodes, 5 and 7
+   as above; 3 and 4 are empty. */
 rq_status rq_scene_layout_image(rq_scene* scene, rq_layout layout, uint32_t part, const void** data,
                                 size_t* bytes);
 void rq_scene_destroy(rq_scene* scene);

@@ -1491,6 +1491,14 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
     }
     if (out.top.empty()) out.top.assign(W8_WORDS, 0u);
     if (out.nodes.empty()) out.nodes.assign(W8_WORDS, 0u);
+    // each instance's BLAS root node copied beside the records, as for CW8
+    out.roots.assign((size_t)std::max(count, 1u) * W8_WORDS, 0u);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t root = out.inst[RS_HEADER_WORDS + (size_t)i * RS_RECORD_WORDS + 20];
+        if (root & LEAF_BIT) continue;
+        std::copy(out.nodes.begin() + (size_t)root * W8_WORDS, out.nodes.begin() + ((size_t)root + 1) * W8_WORDS,
+                  out.roots.begin() + (size_t)i * W8_WORDS);
+    }
     return true;
 }
 

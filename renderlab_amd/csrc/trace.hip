@@ -813,11 +813,11 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
             nxt = NONE;  // consumed (a speculating lane's pending node, or its stack top already popped)
             // an instance (a top-level or flat-loop leaf; speculation takes nodes only) or a node
             bool node_step = !(item & LEAF_BIT);
-            // CW8: the node this step tests is loaded first -- for an instance entry the
+            // 8-wide layouts: the node this step tests is loaded first -- for an instance entry the
             // copy of its BLAS root kept beside the records -- so its loads are in flight
             // together with the instance record's (one memory round trip for the step).
             uint4 cq0, cq1, cq2, cq3, cq4, cq5;
-            if (LAYOUT == CW8) {
+            if (W8) {
                 const uint32_t* np = node_step ? (in_blas ? v.nodes : v.top) + (uint64_t)item * 24u
                                                : v.roots + (uint64_t)(item & ~LEAF_BIT) * 24u;
                 const uint4* w4 = reinterpret_cast<const uint4*>(np);
@@ -1032,7 +1032,7 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 // stay below 7 * 2^-24 * M with M = |base| + 255 |scale|, so a slack
                 // of 2^-20 * M keeps culling conservative.
                 uint4 q0, q1, q2, q3, q4, q5;
-                if (LAYOUT == CW8) {
+                if (W8) {
                     q0 = cq0; q1 = cq1; q2 = cq2; q3 = cq3; q4 = cq4; q5 = cq5;
                 } else {
                     const uint4* w4 = reinterpret_cast<const uint4*>((blas ? v.nodes : v.top) + (uint64_t)item * 24u);
@@ -1831,13 +1831,15 @@ extern "C" rq_status rq_scene_upload(rq_scene* scene, uint32_t layout_mask) {
         DeviceLayout& d = scene->dev[l];
         if (l == RQ_LAYOUT_RS_CW8_Q8) {  // reflection words, CW8 top level, CW8 BLAS nodes
             HostLayout& h = scene->rs_cw8;
-            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes);
+            if (h.roots.empty()) h.roots.assign(24, 0u);
+            d.bytes = padded(h.inst) + padded(h.top) + padded(h.nodes) + padded(h.roots);
             HIP_TRY(hipMalloc(&d.base, d.bytes), "hipMalloc scene");
             uint64_t at = 0;
             d.off[0] = at; if ((s = upload(d.base, at, h.inst)) != RQ_OK) return s;
             d.off[1] = at; if ((s = upload(d.base, at, h.top)) != RQ_OK) return s;
             d.off[2] = at; if ((s = upload(d.base, at, h.nodes)) != RQ_OK) return s;
-            d.off[3] = d.off[4] = at;
+            d.off[3] = at; if ((s = upload(d.base, at, h.roots)) != RQ_OK) return s;
+            d.off[4] = at;
             d.count[0] = h.instance_count;
             d.count[1] = h.top_on ? 1u : 0u;
             d.count[2] = h.top_root;
@@ -1895,14 +1897,14 @@ extern "C" rq_status rq_scene_download_part(const rq_scene* scene, rq_layout lay
     if (layout == RQ_LAYOUT_PLAIN_F32 || layout == RQ_LAYOUT_CW8_Q8) {
         const uint32_t p = part == 6 ? 5u : part;  // part 6: the vertex records, stored after part 4
         at = d.off[p];
-        n = (p < 5 ? d.off[p + 1] : d.bytes) - at;
-    } else if (layout == RQ_LAYOUT_RS_CW8_Q8) {  // parts: words, top, nodes
-        if (part > 2) {
-            set_error("rq_scene_download_part: the RS CW8 layout has three parts");
+        n = d.off[p + 1] - at;  // part 6 (p 5) ends where the CW8 root copies start
+    } else if (layout == RQ_LAYOUT_RS_CW8_Q8) {  // parts: words, top, nodes, root copies
+        if (part > 3) {
+            set_error("rq_scene_download_part: the RS CW8 layout has four parts");
             return RQ_ERR_INVALID_INPUT;
         }
         at = d.off[part];
-        n = (part < 2 ? d.off[part + 1] : d.bytes) - at;
+        n = d.off[part + 1] - at;
     } else if (part != 0) {
         set_error("rq_scene_download_part: the word layouts have one part");
         return RQ_ERR_INVALID_INPUT;
@@ -1975,6 +1977,7 @@ dev::SceneView scene_view(const rq_scene* scene, rq_layout layout) {
         v.words = reinterpret_cast<const uint32_t*>(b + d.off[0]);
         v.top = reinterpret_cast<const uint32_t*>(b + d.off[1]);
         v.nodes = reinterpret_cast<const uint32_t*>(b + d.off[2]);
+        v.roots = reinterpret_cast<const uint32_t*>(b + d.off[3]);
         v.n_inst = d.count[0];
         v.top_on = d.count[1] != 0;
         v.top_root = d.count[2];

@@ -362,3 +362,13 @@ def test_rs_cw8_relayout_covers_every_triangle_once():
         blo, bhi = w[r + 12:r + 15].view(np.float32), w[r + 15:r + 18].view(np.float32)
         tighter += bool(np.prod(hi - lo, dtype=np.float64) < 0.9 * np.prod(bhi - blo, dtype=np.float64))
     assert tighter > count // 2
+
+    # The kernel loads an instance's mesh-tree root from a copy beside the records (part 7), one
+    # memory round trip with the record: each copy is that root node word for word, zero for a leaf.
+    roots = sc.layout_image("rs_cw8", 7).reshape(-1, 24)
+    nodes24 = nodes.reshape(-1, 24)
+    assert len(roots) == count
+    for i in range(count):
+        r = int(img[RS_HEADER + RS_REC * i + 20])
+        want = np.zeros(24, np.uint32) if r & rq.LEAF_BIT else nodes24[r]
+        assert np.array_equal(roots[i], want)
