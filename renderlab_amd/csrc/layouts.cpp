@@ -1083,6 +1083,22 @@ void tighten_rs_boxes(const std::vector<uint32_t>& words, const std::vector<uint
 // tightened to the instances' vertices), one instance a leaf. False with `err` empty when the 4-wide top level does not
 // reach each instance exactly once, a box is not finite, or the tree would pass
 // W8_MAX_LEVELS: the caller collapses the 4-wide top level instead.
+// The CW8 top level from the instances' world boxes (record words 12-17), each
+// tightened to its mesh's vertices. False (err empty) when a box is not finite.
+bool top_from_records(const HostLayout& plain, HostLayout& out, std::string& err) {
+    const uint32_t count = plain.instance_count;
+    if (count == 0) return false;
+    std::vector<Bounds> boxes(count);
+    for (uint32_t i = 0; i < count; ++i)
+        for (int a = 0; a < 3; ++a) {
+            boxes[i].lo[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 12 + a]);
+            boxes[i].hi[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 15 + a]);
+            if (!std::isfinite(boxes[i].lo[a]) || !std::isfinite(boxes[i].hi[a])) return false;
+        }
+    tighten_instance_boxes(plain, out, boxes);
+    return top_from_boxes(boxes, out.top, out.top_root, err);
+}
+
 bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
     const uint32_t count = plain.instance_count;
     if (count == 0) return false;
@@ -1106,15 +1122,7 @@ bool rebuild_top(const HostLayout& plain, HostLayout& out, std::string& err) {
     }
     for (uint8_t s : seen)
         if (!s) return false;
-    std::vector<Bounds> boxes(count);
-    for (uint32_t i = 0; i < count; ++i)
-        for (int a = 0; a < 3; ++a) {
-            boxes[i].lo[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 12 + a]);
-            boxes[i].hi[a] = bits_f(out.inst[(size_t)i * INSTANCE_WORDS + 15 + a]);
-            if (!std::isfinite(boxes[i].lo[a]) || !std::isfinite(boxes[i].hi[a])) return false;
-        }
-    tighten_instance_boxes(plain, out, boxes);
-    return top_from_boxes(boxes, out.top, out.top_root, err);
+    return top_from_records(plain, out, err);
 }
 
 // The CW8 tree of one reflection mesh (node base `base`, root `root`, triangle
@@ -1293,6 +1301,18 @@ bool build_cw8(const std::vector<uint32_t>& words, const HostLayout& plain, Host
         if (w8_levels(out.top, out.top_root, W8_MAX_LEVELS) > W8_MAX_LEVELS) {
             err = "CW8 relayout: a top level deeper than 16 levels";
             return false;
+        }
+    }
+    // Words without a top level (fewer than 40 instances): CW8 builds one all the same.
+    // The CW8 kernel does not test an instance's world box at entry -- the top-level leaf
+    // box it came through holds the box of the instance's triangles, quantized outward --
+    // so the top level is its instances' only cull.
+    if (!plain.top_on && plain.instance_count > 0) {
+        if (top_from_records(plain, out, err)) {
+            out.top_on = true;
+        } else {
+            if (!err.empty()) return false;
+            out.top.clear();  // not finite: the flat loop, every instance entered
         }
     }
     if (out.top.empty()) out.top.assign(W8_WORDS, 0u);

@@ -195,6 +195,30 @@ def test_cw8_root_copies_equal_each_instance_root_node():
             assert np.array_equal(roots[i], want)
 
 
+def test_cw8_has_a_top_level_whatever_the_instance_count():
+    """The CW8 kernel tests no instance's world box at entry: the top-level leaf box it came through
+    holds the box of the instance's triangles. So CW8 builds a top level even where the words have
+    none (fewer than 40 instances), reaching every instance once with leaf boxes that hold the
+    instances' world vertices; its records are PLAIN's but for the root (word 18)."""
+    g = np.random.default_rng(7)
+    pos = g.uniform(-1, 1, (30, 3)).astype(np.float32)
+    mesh = (pos, np.arange(30, dtype=np.uint32))
+    def shifted(k):  # row-major model matrix: a translation by 3k along x
+        m = np.eye(4)
+        m[0, 3] = 3.0 * k
+        return m
+
+    one = rq.Scene.build([mesh], [dict(model=shifted(0), mesh=0)])
+    sixty = rq.Scene.build([mesh], [dict(model=shifted(k), mesh=0) for k in range(60)], top_level=0)
+    for sc in (rq.Scene.generate(1, 3, 10, 2000), one, sixty):
+        assert sc.layout_image("plain", 5)[0] == 0 and sc.layout_image("cw8", 5)[0] == 1
+        pi, ci = sc.layout_image("plain", 0).reshape(-1, 32), sc.layout_image("cw8", 0).reshape(-1, 32)
+        other = np.ones(32, bool)
+        other[18] = False
+        assert np.array_equal(pi[:, other], ci[:, other])
+        check_cw8_top_level(sc)
+
+
 def test_cw8_top_level_reaches_every_instance_once():
     # rotated boxes: the transformed corners are the vertices, so only the 8 spheres' boxes tighten
     assert check_cw8_top_level(rq.Scene.generate(0, 2)) >= 1
