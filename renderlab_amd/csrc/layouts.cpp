@@ -1474,9 +1474,13 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
     // from them; rs_trace's closest hit is the least (t, instance, primitive)
     // whatever the order the instances are visited in. Boxes that are not finite,
     // or a tree past W8_MAX_LEVELS: the reference's 4-wide f32 nodes, collapsed
-    // the same way as the meshes' trees.
-    out.top_on = count && words[1] != 0;
-    out.top_root = count ? words[2] : 0u;
+    // the same way as the meshes' trees. Words without a top level get one all the
+    // same (the kernel tests no record box at instance entry for the 8-wide layouts:
+    // the top-level leaf box holds the instance's triangles); where it cannot be
+    // built, the flat loop enters every instance.
+    const bool had_top = count && words[1] != 0;
+    out.top_on = count > 0;
+    out.top_root = had_top ? words[2] : 0u;
     bool rebuilt = false;
     if (out.top_on) {
         std::vector<Bounds> boxes(count);
@@ -1494,9 +1498,10 @@ bool build_rs_cw8(const std::vector<uint32_t>& words, HostLayout& out, std::stri
             if (!rebuilt) {
                 if (!err.empty()) return false;
                 out.top.clear();
-                out.top_root = words[2];
+                out.top_root = had_top ? words[2] : 0u;
             }
         }
+        if (!rebuilt && !had_top) out.top_on = false;
     }
     if (out.top_on && !rebuilt) {
         const size_t at = words[1];

@@ -864,12 +864,12 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 hold_words(rw);
                 const bool skip = any_hit && (rw[19] & 1u);
                 const float lim = limit(i);
-                // The world-box test only culls. CW8 skips it: its instances are reached
-                // through top-level leaf boxes (the CW8 relayout builds a top level for any
-                // instance count) that hold the box of the instance's triangles, clipped to
-                // the record box and quantized outward, so the test would cull few rays and
-                // cost more than the node steps it saves.
-                const bool box_test = LAYOUT != CW8;
+                // The world-box test only culls. The 8-wide layouts skip it: their instances
+                // are reached through top-level leaf boxes (their relayouts build a top level
+                // for any instance count) that hold the box of the instance's triangles,
+                // clipped to the record box and quantized outward, so the test would cull few
+                // rays and cost more than the node steps it saves.
+                const bool box_test = !W8;
                 if (!skip && (!box_test || slab_near<WT>(wo, winv, mk(bf(rw[12]), bf(rw[13]), bf(rw[14])),
                                                          mk(bf(rw[15]), bf(rw[16]), bf(rw[17])), tmin, lim) <= lim)) {
                     V3 r0 = mk(bf(rw[0]), bf(rw[1]), bf(rw[2])), r1 = mk(bf(rw[4]), bf(rw[5]), bf(rw[6])),

@@ -209,3 +209,28 @@ def test_small_launches_match_oracle(scene_c0, workload_c0, count):
             bad = compare_exact(g, o)
         assert bad.size == 0, f"{layout}: {bad.size} records differ, first {bad[:5]}"
 
+
+
+def test_cw8_without_a_top_level_in_the_words():
+    """Words with no top level (forced off with 50 instances; a scene of 2 instances): CW8 builds one
+    all the same, since it tests no record box at instance entry. Closest hits agree with the
+    oracle within the tie rule, PLAIN's records equal it bit for bit, visibility bits are equal."""
+    from test_reflection import random_reflection_scene, random_rays
+    meshes, inst, g = random_reflection_scene(6)
+    rays = random_rays(g, 4000)
+    for name, sc in (("50 instances, flat words", rq.Scene.build(meshes, inst, top_level=0)),
+                     ("2 instances", rq.Scene.build(meshes, inst[:2]))):
+        assert sc.words[1] == 0
+        sc.upload(("plain", "cw8"))
+        assert sc.layout_image("cw8", 5)[0] == 1
+        o, _, _ = oracle.traverse(sc.words, rays, 0)
+        assert ((o["flags"] & 1) == 1).sum() > 50
+        print(f"case: cw8 and plain, {name} n={len(rays)}", flush=True)
+        p, _ = gpu_trace(sc, "plain", rays, rq.CLOSEST)
+        assert compare_exact(p, o).size == 0
+        c, _ = gpu_trace(sc, "cw8", rays, rq.CLOSEST)
+        bad, _ = classify_mismatch(c, o)
+        assert bad.size == 0, f"{name}: {bad.size} non-tie mismatches, first {bad[:5]}"
+        _, ov1, _ = oracle.traverse(sc.words, rays, 1, want_hits=False, want_vis=True)
+        _, cv = gpu_trace(sc, "cw8", rays, 1)
+        assert np.array_equal(cv, ov1)

@@ -99,3 +99,32 @@ def test_kind_and_layout_must_match(scenes):
         box.trace("cw8", d, 1, rq.CLOSEST)
     with pytest.raises(rq.InvalidInput):
         gi.trace("rs_cw8", d, 1, rq.CLOSEST)
+
+
+def test_rs_cw8_without_a_top_level_in_the_words():
+    """Words with no top level (forced off; and a scene of 2 instances): RS_CW8 builds one all the same,
+    since it tests no record box at instance entry. Closest records equal the oracle's bit for bit,
+    also with 50 instances stacked at one transform (the lower instance wins the equal-distance tie),
+    and visibility bits equal."""
+    from test_reflection import random_reflection_scene, random_rays, rays_of
+    meshes, inst, g = random_reflection_scene(4)
+    pos = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+    stack = [dict(model=np.eye(4), mesh=0, placement=9), dict(model=np.eye(4), mesh=0, placement=8)]
+    cases = [("random 50, flat words", rq.Scene.build(meshes, inst, top_level=0, reflection=True), random_rays(g, 3000)),
+             ("2 instances", rq.Scene.build(meshes, inst[:2], reflection=True), random_rays(g, 3000)),
+             ("50 stacked, flat words", rq.Scene.build([(pos, np.arange(3, dtype=np.uint32), 0)], stack * 25,
+                                                       top_level=0, reflection=True),
+              rays_of(np.c_[g.uniform(-0.3, 0.3, (500, 2)), np.full(500, 2.0)], [0, 0, -1]))]
+    for name, sc, r in cases:
+        assert sc.words[1] == 0
+        sc.upload(("rs_cw8",))
+        assert sc.layout_image("rs_cw8", 5)[0] == 1
+        print(f"case: rs_cw8 {name} n={len(r)}", flush=True)
+        h, _ = gpu_trace(sc, "rs_cw8", r, rq.CLOSEST)
+        o, _, _ = oracle.rs_traverse(sc.words, r, 0)
+        assert ((o["flags"] & 1) == 1).sum() > 50
+        assert len(compare_exact(h, o)) == 0
+        _, v = gpu_trace(sc, "rs_cw8", r, rq.ANY)
+        _, ov, _ = oracle.rs_traverse(sc.words, r, 1, want_hits=False, want_vis=True)
+        assert np.array_equal(v, ov)
+
