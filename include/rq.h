@@ -487,9 +487,8 @@ rq_status rq_gen_primary_tiled(const float inverse[16], uint32_t width, uint32_t
                                void* stream);
 /* Secondary rays of `count` primary rays and their closest hits: compacted
    in pixel order into shadow (one per lit hit) and gi (n_gi per hit, one
-   array per bounce, each compacted); counts (device, n_gi+1 words) receive
-   the lengths. src_pixel (may be NULL) receives each ray's pixel, array by
-   array. */
+   array per bounce of `count` rays each, each compacted); counts (device,
+   n_gi+1 words) receive the lengths. */
 rq_status rq_gen_secondary(const rq_ray* primary, const rq_hit* hits, uint32_t count,
                            const float sun[3], uint32_t seed, uint32_t n_gi, rq_ray* shadow,
                            rq_ray* gi, uint32_t* counts, void* stream);
