@@ -1052,8 +1052,9 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 const V3 scl = mk(bf((q0.w & 0xffu) << 23) * ri.x, bf(((q0.w >> 8) & 0xffu) << 23) * ri.y,
                                   bf(((q0.w >> 16) & 0xffu) << 23) * ri.z);
                 const V3 base = mk((bf(q0.x) - ro.x) * ri.x, (bf(q0.y) - ro.y) * ri.y, (bf(q0.z) - ro.z) * ri.z);
-                const float err = fmx(fmx(fabsf(base.x) + 255.0f * fabsf(scl.x), fabsf(base.y) + 255.0f * fabsf(scl.y)),
-                                      fabsf(base.z) + 255.0f * fabsf(scl.z)) * 9.53674316e-7f;  // 2^-20
+                const float err = fmx(fmx(__builtin_fmaf(255.0f, fabsf(scl.x), fabsf(base.x)),
+                                          __builtin_fmaf(255.0f, fabsf(scl.y), fabsf(base.y))),
+                                      __builtin_fmaf(255.0f, fabsf(scl.z), fabsf(base.z))) * 9.53674316e-7f;  // 2^-20
                 const V3 bn = mk(base.x - err, base.y - err, base.z - err);  // near planes
                 const V3 bfr = mk(base.x + err, base.y + err, base.z + err); // far planes
                 // planes: q1 = lo x (2 words), lo y; q2 = lo z, hi x; q3 = hi y, hi z
