@@ -280,7 +280,7 @@ std::vector<Cand> wide_children(const std::vector<uint32_t>& nodes, uint32_t nod
 }
 
 // Collapses a 4-wide tree into 8-wide nodes of 24 words (96 B):
-//   0-2 origin (f32), 3 exponents x|y<<8|z<<16 and valid-slot mask <<24,
+//   0-2 origin (f32), 3 exponents x|y<<8|z<<16 (signed bytes) and valid-slot mask <<24,
 //   4-15 byte planes (lo x, lo y, lo z, hi x, hi y, hi z; 8 codes, 2 words each),
 //   16-23 child tags (node index, leaf tag unchanged from the source); an empty
 //   slot holds an inverted box and a copy of the first child's tag.
@@ -522,8 +522,8 @@ struct W8Builder {
                 w[16 + s] = dup;
                 for (int a = 0; a < 3; ++a) w[4 + 2 * a + s / 4] |= 0xffu << (8 * (s % 4));
             }
-        w[3] = (uint32_t)(exps[0] + 127) | ((uint32_t)(exps[1] + 127) << 8) | ((uint32_t)(exps[2] + 127) << 16) |
-               (valid << 24);
+        w[3] = (uint32_t)(uint8_t)(int8_t)exps[0] | ((uint32_t)(uint8_t)(int8_t)exps[1] << 8) |
+               ((uint32_t)(uint8_t)(int8_t)exps[2] << 16) | (valid << 24);
         std::copy(w, w + W8_WORDS, out->begin() + (size_t)index * W8_WORDS);
         for (auto& t : todo)
             if (!fill(t.first, source_children(t.second))) return false;

@@ -1049,8 +1049,9 @@ __global__ __launch_bounds__(BLOCK, (COUNT || BATCHED) ? 1 : 5) void trace_kerne
                 // the slab test unless the ray's origin is about 2^19 node extents away (the
                 // slack then exceeds the node), and a copy of a real child's tag: visiting it is extra work, never a
                 // wrong answer. So no slot needs the valid mask.
-                const V3 scl = mk(bf((q0.w & 0xffu) << 23) * ri.x, bf(((q0.w >> 8) & 0xffu) << 23) * ri.y,
-                                  bf(((q0.w >> 16) & 0xffu) << 23) * ri.z);
+                // 2^e * (1/d) per axis, e a signed byte: exact, as a product with a power of two is
+                const V3 scl = mk(ldexpf(ri.x, (int8_t)(q0.w & 0xffu)), ldexpf(ri.y, (int8_t)((q0.w >> 8) & 0xffu)),
+                                  ldexpf(ri.z, (int8_t)((q0.w >> 16) & 0xffu)));
                 const V3 base = mk((bf(q0.x) - ro.x) * ri.x, (bf(q0.y) - ro.y) * ri.y, (bf(q0.z) - ro.z) * ri.z);
                 const float err = fmx(fmx(__builtin_fmaf(255.0f, fabsf(scl.x), fabsf(base.x)),
                                           __builtin_fmaf(255.0f, fabsf(scl.y), fabsf(base.y))),

@@ -27,7 +27,7 @@ def plain_children(nodes, i):
 def w8_children(nodes, i):
     w = nodes[i * 24:(i + 1) * 24]
     p = w[:3].view(np.float32)
-    sc = [np.float32(2.0) ** (int((w[3] >> (8 * a)) & 0xFF) - 127) for a in range(3)]
+    sc = [np.float32(2.0) ** int(np.uint8((w[3] >> (8 * a)) & 0xFF).view(np.int8)) for a in range(3)]
     valid = int(w[3] >> 24)
     first = int(w[16 + ((valid & -valid).bit_length() - 1)])
     out = []
@@ -305,7 +305,7 @@ def test_cw8_codes_decode_exactly(which):
         origin = w[:, :3].view(np.float32).astype(np.float64)
         q = np.arange(256, dtype=np.float64)
         for a in range(3):
-            e = ((w[:, 3] >> (8 * a)) & 0xFF).astype(np.int64) - 127
+            e = ((w[:, 3] >> (8 * a)) & 0xFF).astype(np.uint8).view(np.int8).astype(np.int64)
             assert (e >= -126).all()
             dec = origin[:, a:a + 1] + np.ldexp(1.0, e)[:, None] * q[None, :]  # exact in f64
             assert np.array_equal(dec.astype(np.float32).astype(np.float64), dec)
