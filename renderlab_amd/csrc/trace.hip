@@ -66,9 +66,10 @@ __device__ __forceinline__ float safe_inv1(float d) {
 __device__ __forceinline__ V3 safe_inverse(V3 d) { return mk(safe_inv1(d.x), safe_inv1(d.y), safe_inv1(d.z)); }
 // The same clamp with the hardware reciprocal (1 ulp): for box tests that
 // carry their own error slack (the CW8 layout), never for hit arithmetic.
+// |d| is raised to 1e-30 keeping d's sign (a zero's too: the near and far
+// planes are chosen by the reciprocal's sign, so either sign culls correctly).
 __device__ __forceinline__ float fast_inv1(float d) {
-    float tiny = d >= 0.0f ? 1e-30f : -1e-30f;
-    return __builtin_amdgcn_rcpf(fabsf(d) < 1e-30f ? tiny : d);
+    return __builtin_amdgcn_rcpf(copysignf(fmaxf(fabsf(d), 1e-30f), d));
 }
 __device__ __forceinline__ V3 fast_inverse(V3 d) { return mk(fast_inv1(d.x), fast_inv1(d.y), fast_inv1(d.z)); }
 
