@@ -234,3 +234,33 @@ def test_cw8_without_a_top_level_in_the_words():
         _, ov1, _ = oracle.traverse(sc.words, rays, 1, want_hits=False, want_vis=True)
         _, cv = gpu_trace(sc, "cw8", rays, 1)
         assert np.array_equal(cv, ov1)
+
+
+def test_axis_parallel_rays_with_signed_zeros(scene_c0):
+    """Rays along the axes whose other direction components are +0 or -0 (the 8-wide box test takes
+    the reciprocal of each component, clamped to 1e30 with the component's sign, and picks near and
+    far planes by that sign): CW8 agrees with the oracle within the tie rule, PLAIN bit for bit, and
+    visibility bits are equal."""
+    g = np.random.default_rng(11)
+    n = 6000
+    rays = np.zeros(n, rq.RAY_DTYPE)
+    rays["origin"] = g.uniform(-60, 60, (n, 3))
+    axis = g.integers(0, 3, n)
+    d = np.zeros((n, 3), np.float32)
+    d[np.arange(n), axis] = np.where(g.random(n) < 0.5, -1.0, 1.0)
+    zero_sign = g.random((n, 3)) < 0.5
+    d = np.where((d == 0) & zero_sign, np.float32(-0.0), d)
+    rays["direction"] = d
+    rays["tmax"] = 1e20
+    assert (np.signbit(rays["direction"]) & (rays["direction"] == 0)).any()
+    o, _, _ = oracle.traverse(scene_c0.words, rays, 0)
+    assert ((o["flags"] & 1) == 1).sum() > 200
+    print(f"case: axis-parallel rays n={n}", flush=True)
+    p, _ = gpu_trace(scene_c0, "plain", rays, rq.CLOSEST)
+    assert compare_exact(p, o).size == 0
+    c, _ = gpu_trace(scene_c0, "cw8", rays, rq.CLOSEST)
+    bad, _ = classify_mismatch(c, o)
+    assert bad.size == 0, f"{bad.size} non-tie mismatches, first {bad[:5]}"
+    _, ov, _ = oracle.traverse(scene_c0.words, rays, 1, want_hits=False, want_vis=True)
+    _, cv = gpu_trace(scene_c0, "cw8", rays, 1)
+    assert np.array_equal(cv, ov)
